@@ -1,0 +1,206 @@
+/* C ABI of the MI355X-native MIND simulation step (libmind).
+ *
+ * The library replaces the reference's device side and its CPU event loop:
+ * build! packing, initialize!, run!/step!, spike detection, event routing and
+ * delivery, the cable solve and the mechanism phases. A host layer (a
+ * stdlib-only Julia module calling these with ccall, or Python ctypes) keeps
+ * the reference's model-building API and flattens a model into
+ * mind_model_desc. All arrays are plain caller-owned host pointers; the
+ * library copies what it needs and owns every device allocation.
+ *
+ * Every function returns 0 on success or a MIND_E* code; mind_last_error()
+ * gives the message of the last failure on the calling thread.
+ * Calls on one mind_state must be serialised by the caller; distinct states
+ * are independent.
+ */
+#ifndef MIND_H
+#define MIND_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. 3, 5, 6 and 8 keep the reference's device error-word meaning
+ * (src/runtime/utils/step.jl:183-189). */
+enum {
+  MIND_OK = 0,
+  MIND_EARG = 1,      /* invalid argument (reference: ArgumentError) */
+  MIND_EHIP = 2,      /* HIP runtime failure or missing device */
+  MIND_EOOM = 3,      /* device allocation failed */
+  MIND_EEVENT = 5,    /* invalid self-event delay or target */
+  MIND_ESINGULAR = 6, /* singular or non-finite cable system */
+  MIND_ENONFINITE = 8,/* non-finite numerical state */
+  MIND_EQUEUE = 9,    /* event ring or spike buffer capacity exceeded */
+  MIND_EUNSUPPORTED = 10 /* model feature outside the device path */
+};
+
+/* Builtin mechanisms (reference mod/{hh,pas,ExpSyn,Exp2Syn,IClamp,NetStim}.mod). */
+enum {
+  MIND_MECH_HH = 0,
+  MIND_MECH_PAS = 1,
+  MIND_MECH_EXPSYN = 2,
+  MIND_MECH_EXP2SYN = 3,
+  MIND_MECH_ICLAMP = 4,
+  MIND_MECH_NETSTIM = 5,
+  MIND_MECH_COUNT = 6
+};
+
+/* Field columns of each builtin mechanism (parameters, states, assigned). */
+enum { HH_GNABAR, HH_GKBAR, HH_GL, HH_EL, HH_M, HH_H, HH_N, HH_ENA, HH_EK, HH_GNA, HH_GK,
+       HH_INA, HH_IK, HH_IL, HH_MINF, HH_HINF, HH_NINF, HH_MTAU, HH_HTAU, HH_NTAU, HH_NFIELDS };
+enum { PAS_G, PAS_E, PAS_I, PAS_NFIELDS };
+enum { EXPSYN_TAU, EXPSYN_E, EXPSYN_G, EXPSYN_I, EXPSYN_NFIELDS };
+enum { EXP2SYN_TAU1, EXP2SYN_TAU2, EXP2SYN_E, EXP2SYN_A, EXP2SYN_B, EXP2SYN_G, EXP2SYN_I,
+       EXP2SYN_FACTOR, EXP2SYN_NFIELDS };
+enum { ICLAMP_DEL, ICLAMP_DUR, ICLAMP_AMP, ICLAMP_I, ICLAMP_NFIELDS };
+enum { NETSTIM_INTERVAL, NETSTIM_NUMBER, NETSTIM_START, NETSTIM_NOISE, NETSTIM_EVENT,
+       NETSTIM_ON, NETSTIM_ISPIKE, NETSTIM_NFIELDS };
+
+/* A flattened MicroModel (src/model/micro/model.jl:35-50) plus its sections
+ * (src/model/micro/morphology.jl:17-29). Section indices are local to their
+ * morphology; parent -1 marks a root. Parameter overrides of an insertion or
+ * point process are (field, value) pairs in param_field/param_value, indexed
+ * by the *_param_offsets CSR (length count+1). */
+typedef struct mind_model_desc {
+  double celsius;
+  int32_t n_morphologies;
+  const int64_t* morph_offsets; /* n_morphologies+1, into the section arrays */
+  const int32_t* sec_parent;
+  const double* sec_parent_x;
+  const double* sec_child_x;
+  const int32_t* sec_nseg;
+  const double* sec_length;   /* um */
+  const double* sec_diameter; /* um */
+  const double* sec_Ra;       /* ohm cm */
+  const double* sec_cm;       /* uF/cm2 */
+  const int64_t* sec_point_offsets; /* NULL, or n_sections+1 into points (pt3d) */
+  const double* points;       /* x, y, z, diameter per sample */
+
+  int32_t n_cells;
+  const int32_t* cell_morphology;
+  const int64_t* cell_gid;
+  const double* cell_voltage; /* initial membrane potential, mV */
+
+  /* Density insertions in insertion order (reference insert!, model.jl:153). */
+  int64_t n_density;
+  const int32_t* dens_cell;
+  const int32_t* dens_section;
+  const int32_t* dens_mech;
+  const int64_t* dens_param_offsets;
+
+  /* Point processes in creation order (reference add_pointprocess!, model.jl:172-194);
+   * cell -1 marks an artificial cell without a location. */
+  int64_t n_points;
+  const int32_t* pt_mech;
+  const int32_t* pt_cell;
+  const int32_t* pt_section;
+  const double* pt_x;
+  const int64_t* pt_param_offsets;
+
+  const int32_t* param_field; /* shared pool for density and point overrides */
+  const double* param_value;
+
+  /* Spike sources (reference register_spike_source!, model.jl:286-365): kind 0 is a
+   * membrane-voltage threshold detector at (cell, section, x); kind 1 is the
+   * net_event output of point process `point`. */
+  int64_t n_sources;
+  const int32_t* src_kind;
+  const int32_t* src_cell;
+  const int32_t* src_section;
+  const double* src_x;
+  const double* src_threshold;
+  const int32_t* src_point;
+  const int64_t* src_sid;
+
+  /* NetCons (reference connect!, model.jl:379-400) in creation order. */
+  int64_t n_connections;
+  const int32_t* con_source; /* source index */
+  const int32_t* con_target; /* point-process index */
+  const double* con_weight;  /* uS */
+  const double* con_delay;   /* ms */
+} mind_model_desc;
+
+/* Kernel families for the micro step. */
+enum {
+  MIND_METHOD_PLAIN = 0,   /* one cell per thread, node state in HBM */
+  MIND_METHOD_BATCHED = 1  /* cells per workgroup, node state staged in LDS */
+};
+
+/* build! keyword arguments (src/model/build.jl:120-131) plus device choices. */
+typedef struct mind_build_options {
+  double dt;              /* dt_micro, ms (reference default 0.025) */
+  uint64_t seed;          /* RNG seed (reference default 1) */
+  int32_t method;         /* MIND_METHOD_* */
+  int32_t cells_per_group;/* batched: cells per workgroup, 0 = autotune */
+  int32_t window_steps;   /* steps per device window, 0 = longest the minimum delay allows */
+  int64_t event_capacity; /* events per delivery-step ring slot, 0 = automatic */
+  int32_t device;         /* HIP device ordinal */
+} mind_build_options;
+
+typedef struct mind_state mind_state;
+
+/* Record selectors (reference voltage_record / mechanism_record, recording.jl:4-76).
+ * kind 0: membrane voltage at (cell, section, x);
+ * kind 1: field of point process `point`;
+ * kind 2: field of density mechanism `mech` at (cell, section, x). */
+typedef struct mind_record {
+  int32_t kind;
+  int32_t cell;
+  int32_t section;
+  double x;
+  int32_t point;
+  int32_t mech;
+  int32_t field;
+} mind_record;
+
+/* SimulationResult (src/runtime/recording.jl:78-87): samples at every micro step
+ * boundary of the run, traces row-major [n_records][n_samples], spikes sorted by
+ * (time, sid). Arrays are owned by the library until mind_result_free. */
+typedef struct mind_result {
+  int64_t n_samples;
+  double* time;
+  int64_t n_records;
+  double* traces;
+  int64_t n_spikes;
+  double* spike_times;
+  int64_t* spike_sids;
+  double run_seconds; /* wall time of the step loop */
+} mind_result;
+
+/* Pack and upload a model. Replaces build!/pack_micro (src/model/build.jl:120-220,
+ * src/model/micro/model.jl:420-902) and allocate_simulation
+ * (src/runtime/initialize.jl:4-250). */
+int mind_build(const mind_model_desc* model, const mind_build_options* options,
+               mind_state** out);
+
+/* Reset clocks, queues and state and run INITIAL blocks, time-zero delivery and
+ * current evaluation (src/runtime/initialize.jl:270-429). */
+int mind_initialize(mind_state* state);
+
+/* Advance to an absolute stop time that is an integer number of steps ahead
+ * (src/runtime/step.jl:4-74). */
+int mind_run(mind_state* state, double stop, const mind_record* records, int64_t n_records,
+             int32_t record_spikes, mind_result* out);
+
+/* One micro step (src/runtime/step.jl:77-86). */
+int mind_step(mind_state* state);
+
+void mind_result_free(mind_result* result);
+void mind_state_destroy(mind_state* state);
+
+/* Introspection used by the host layer and by tests. */
+int mind_node_count(const mind_state* state, int64_t* nodes, int64_t* compartments);
+int mind_read_voltage(mind_state* state, double* out, int64_t n); /* canonical node order */
+int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out, int64_t n);
+int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows);
+int mind_time(const mind_state* state, double* t, int64_t* step);
+int mind_window_steps(const mind_state* state, int32_t* steps);
+const char* mind_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,440 @@
+// Packing of mind_model_desc into the device layout (see layout.h).
+//
+// Geometry follows the reference's compartmentalisation: a root node of area
+// 100 um^2 with no capacitance, one node per segment holding the area of its
+// two half-segments, and a zero-capacitance endpoint per section
+// (src/model/micro/morphology.jl:406-484). Half-segment area, axial resistance
+// and mean diameter are integrated piecewise-linearly over the pt3d samples
+// with a cursor carried across half-segments
+// (src/model/micro/utils/morphology.jl:28-63). Compiled with -ffp-contract=off:
+// these numbers feed a bit-exact comparison.
+#include "layout.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <numeric>
+
+namespace mind {
+
+int field_count(int mech) {
+  static const int n[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS,
+                                         EXP2SYN_NFIELDS, ICLAMP_NFIELDS, NETSTIM_NFIELDS};
+  return n[mech];
+}
+
+bool mech_is_point(int mech) { return mech >= MIND_MECH_EXPSYN; }
+
+void field_defaults(int mech, double* f) {
+  for (int i = 0; i < field_count(mech); ++i) f[i] = 0.0;
+  switch (mech) {
+    case MIND_MECH_HH:  // hh.mod PARAMETER block; ena/ek from the default ions
+      f[HH_GNABAR] = 0.12; f[HH_GKBAR] = 0.036; f[HH_GL] = 0.0003; f[HH_EL] = -54.3;
+      f[HH_ENA] = 50.0; f[HH_EK] = -77.0;
+      break;
+    case MIND_MECH_PAS: f[PAS_G] = 0.001; f[PAS_E] = -70.0; break;
+    case MIND_MECH_EXPSYN: f[EXPSYN_TAU] = 0.1; break;
+    case MIND_MECH_EXP2SYN: f[EXP2SYN_TAU1] = 0.1; f[EXP2SYN_TAU2] = 10.0; break;
+    case MIND_MECH_NETSTIM:
+      f[NETSTIM_INTERVAL] = 10.0; f[NETSTIM_NUMBER] = 10.0; f[NETSTIM_START] = 50.0;
+      break;
+    default: break;
+  }
+}
+
+namespace {
+
+struct Sample { double x, y, z, d; };
+
+// Walks the pt3d polyline of one section, integrating area, axial resistance
+// and diameter between successive boundaries.
+class GeometryCursor {
+ public:
+  GeometryCursor(const std::vector<Sample>& s, double Ra) : s_(s), Ra_(Ra), arc_(s.size(), 0.0) {
+    for (size_t i = 1; i < s.size(); ++i) {
+      const double dx = s[i].x - s[i - 1].x, dy = s[i].y - s[i - 1].y, dz = s[i].z - s[i - 1].z;
+      arc_[i] = arc_[i - 1] + sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    last_arc_ = arc_[0];
+    last_d_ = s[0].d;
+  }
+  double total() const { return arc_.back(); }
+
+  // Integrates from the previous boundary to `boundary`; returns area (um^2),
+  // resistance (Mohm) and the diameter integral (um^2).
+  void advance(double boundary, double& area, double& resistance, double& diam) {
+    double a = 0.0, r = 0.0, di = 0.0;
+    const size_t last = s_.size() - 1;
+    for (;;) {
+      const size_t nx = span_ + 1;
+      const bool stop = arc_[nx] > boundary || nx == last;
+      double arc, d;
+      if (stop) {
+        const double width = arc_[nx] - arc_[span_];
+        const double frac = fabs(width) < 1.0e-10 ? 1.0 : (boundary - arc_[span_]) / width;
+        arc = boundary;
+        d = (1.0 - frac) * s_[span_].d + frac * s_[nx].d;
+      } else {
+        arc = arc_[nx];
+        d = s_[nx].d;
+      }
+      const double step = arc - last_arc_;
+      di += (last_d_ + d) * step;
+      const double len = step > 1.0e-15 ? step : 1.0e-15;
+      const double cond = last_d_ * d / len;
+      r += 1.0 / (cond == 0.0 ? 1.0e-15 : cond);
+      const double dd = d - last_d_;
+      a += (last_d_ + d) * sqrt(len * len + 0.25 * (dd * dd));
+      last_arc_ = arc;
+      last_d_ = d;
+      if (stop) break;
+      span_ = nx;
+    }
+    area = 0.5 * M_PI * a;
+    resistance = 0.04 * Ra_ / M_PI * r;
+    diam = 0.5 * di;
+  }
+
+ private:
+  const std::vector<Sample>& s_;
+  double Ra_;
+  std::vector<double> arc_;
+  size_t span_ = 0;
+  double last_arc_, last_d_;
+};
+
+int32_t template_location(const Template& t, int sec, double x) {
+  if (x == 0.0) return t.sec_start[sec];
+  const std::vector<int32_t>& nodes = t.sec_nodes[sec];
+  const int64_t segs = (int64_t)nodes.size() - 1;
+  if (x == 1.0) return nodes[segs];
+  return nodes[std::min((int64_t)floor(x * segs), segs - 1)];
+}
+
+int32_t template_range(const Template& t, int sec, double x) {
+  const std::vector<int32_t>& nodes = t.sec_nodes[sec];
+  const int64_t segs = (int64_t)nodes.size() - 1;
+  return nodes[std::min((int64_t)floor(x * segs), segs - 1)];
+}
+
+std::string build_template(const mind_model_desc& d, int m, Template& t) {
+  const int64_t s0 = d.morph_offsets[m], s1 = d.morph_offsets[m + 1];
+  const int nsec = (int)(s1 - s0);
+  t.sec_nodes.assign(nsec, {});
+  t.sec_start.assign(nsec, -1);
+  // Parent-first section order, otherwise input order (utils/morphology.jl:77-104).
+  std::vector<int> order;
+  order.reserve(nsec);
+  std::vector<uint8_t> state(nsec, 0);
+  std::vector<int> chain;
+  for (int start = 0; start < nsec; ++start) {
+    int s = start;
+    while (state[s] != 2) {
+      if (state[s] == 1) return "morphology contains a cycle";
+      state[s] = 1;
+      chain.push_back(s);
+      const int p = d.sec_parent[s0 + s];
+      if (p < 0) break;
+      if (p >= nsec) return "unknown parent section";
+      s = p;
+    }
+    for (auto it = chain.rbegin(); it != chain.rend(); ++it) {
+      state[*it] = 2;
+      order.push_back(*it);
+    }
+    chain.clear();
+  }
+  auto node = [&](int32_t parent, double area, double g, double c) {
+    t.parent.push_back(parent);
+    t.area.push_back(area);
+    t.axial.push_back(g);
+    t.cap.push_back(c);
+    return (int32_t)t.parent.size() - 1;
+  };
+  for (int s : order) {
+    const int64_t i = s0 + s;
+    const int nseg = d.sec_nseg[i];
+    if (nseg <= 0 || !(d.sec_Ra[i] > 0) || !(d.sec_cm[i] >= 0))
+      return "invalid segmentation or biophysics";
+    std::vector<Sample> pts;
+    if (d.sec_point_offsets && d.sec_point_offsets[i + 1] - d.sec_point_offsets[i] >= 2) {
+      for (int64_t p = d.sec_point_offsets[i]; p < d.sec_point_offsets[i + 1]; ++p)
+        pts.push_back({d.points[4 * p], d.points[4 * p + 1], d.points[4 * p + 2], d.points[4 * p + 3]});
+    } else {
+      pts = {{0.0, 0.0, 0.0, d.sec_diameter[i]}, {d.sec_length[i], 0.0, 0.0, d.sec_diameter[i]}};
+    }
+    GeometryCursor geo(pts, d.sec_Ra[i]);
+    if (!(geo.total() > 0.0)) return "section has zero arc length";
+    const int p = d.sec_parent[i];
+    int32_t attach;
+    if (p < 0) {
+      attach = node(-1, 100.0, 0.0, 0.0);
+    } else {
+      if (t.sec_nodes[p].empty()) return "unknown parent section";
+      const double px = d.sec_parent_x[i];
+      if (!(px >= 0.0 && px <= 1.0)) return "parent_x must be in [0, 1]";
+      attach = template_location(t, p, px);
+    }
+    const bool flipped = d.sec_child_x[i] == 1.0;
+    std::vector<int32_t>& nodes = t.sec_nodes[s];
+    nodes.assign(nseg + 1, -1);
+    const double h = geo.total() / nseg;
+    int32_t prev = attach;
+    double carry = 0.0;  // right half-segment resistance of the previous segment
+    for (int k = 1; k <= nseg; ++k) {
+      double aL, rL, dL, aR, rR, dR;
+      geo.advance((k - 0.5) * h, aL, rL, dL);
+      geo.advance(k * h, aR, rR, dR);
+      const double area = aL + aR;
+      const int32_t n = node(prev, area, 1.0 / (carry + rL), d.sec_cm[i] * area * 1.0e-5);
+      nodes[flipped ? nseg - k : k - 1] = n;
+      prev = n;
+      carry = rR;
+    }
+    const int32_t end = node(prev, 100.0, 1.0 / carry, 0.0);
+    t.sec_start[s] = flipped ? end : attach;
+    nodes[nseg] = flipped ? attach : end;
+    t.compartments += nseg;
+  }
+  return "";
+}
+
+}  // namespace
+
+int64_t Layout::location_node(int32_t cell, int32_t section, double x) const {
+  return cell_off[cell] + template_location(templates[cell_template[cell]], section, x);
+}
+
+int64_t Layout::range_node(int32_t cell, int32_t section, double x) const {
+  return cell_off[cell] + template_range(templates[cell_template[cell]], section, x);
+}
+
+std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) {
+  L = Layout();
+  L.dt = dt;
+  L.seed = seed;
+  L.celsius = d.celsius;
+  if (!(dt > 0.0) || !isfinite(dt)) return "dt_micro must be positive";
+  // 1. Morphology templates.
+  L.templates.resize(d.n_morphologies);
+  for (int m = 0; m < d.n_morphologies; ++m) {
+    std::string err = build_template(d, m, L.templates[m]);
+    if (!err.empty()) return err;
+  }
+  // 2. Cells as contiguous node blocks.
+  L.ncells = d.n_cells;
+  L.cell_template.assign(d.cell_morphology, d.cell_morphology + d.n_cells);
+  L.cell_off.assign(d.n_cells + 1, 0);
+  for (int c = 0; c < d.n_cells; ++c) {
+    const int m = d.cell_morphology[c];
+    if (m < 0 || m >= d.n_morphologies) return "cell has an unknown morphology";
+    L.cell_off[c + 1] = L.cell_off[c] + (int64_t)L.templates[m].parent.size();
+    L.compartments += L.templates[m].compartments;
+  }
+  const int64_t N = L.cell_off[d.n_cells];
+  if (N >= INT32_MAX) return "packed node count exceeds Int32 range";
+  L.nodes = N;
+  L.parent.resize(N); L.area.resize(N); L.cap.resize(N); L.axial.resize(N); L.v_init.resize(N);
+  for (int c = 0; c < d.n_cells; ++c) {
+    const Template& t = L.templates[L.cell_template[c]];
+    const int64_t o = L.cell_off[c];
+    for (size_t i = 0; i < t.parent.size(); ++i) {
+      L.parent[o + i] = t.parent[i] < 0 ? -1 : (int32_t)(o + t.parent[i]);
+      L.area[o + i] = t.area[i];
+      L.cap[o + i] = t.cap[i];
+      L.axial[o + i] = t.axial[i];
+      L.v_init[o + i] = d.cell_voltage[c];
+    }
+  }
+  // axial diagonal: own conductance, then children in discretization order
+  L.adiag.assign(N, 0.0);
+  L.child_off.assign(N + 1, 0);
+  for (int64_t i = 0; i < N; ++i)
+    if (L.parent[i] >= 0) {
+      L.adiag[i] += L.axial[i];
+      L.adiag[L.parent[i]] += L.axial[i];
+      L.child_off[L.parent[i] + 1] += 1;
+    }
+  for (int64_t i = 0; i < N; ++i) L.child_off[i + 1] += L.child_off[i];
+  L.child.resize(L.child_off[N]);
+  {
+    std::vector<int64_t> cur(L.child_off.begin(), L.child_off.end() - 1);
+    for (int64_t i = 0; i < N; ++i)
+      if (L.parent[i] >= 0) L.child[cur[L.parent[i]]++] = (int32_t)i;
+  }
+  // 3. Mechanism groups: first use over density insertions, then point processes.
+  for (int k = 0; k < MIND_MECH_COUNT; ++k) L.group_of_mech[k] = -1;
+  struct Pending { int64_t p0, p1; };
+  std::vector<std::vector<Pending>> params;
+  auto group_of = [&](int mech) {
+    if (L.group_of_mech[mech] < 0) {
+      L.group_of_mech[mech] = (int)L.groups.size();
+      L.groups.emplace_back();
+      L.groups.back().mech = mech;
+      params.emplace_back();
+    }
+    return L.group_of_mech[mech];
+  };
+  std::vector<std::vector<uint8_t>> seen(MIND_MECH_COUNT);
+  for (int64_t k = 0; k < d.n_density; ++k) {
+    const int mech = d.dens_mech[k];
+    if (mech < 0 || mech >= MIND_MECH_COUNT || mech_is_point(mech))
+      return "cell insertion requires a distributed micro mechanism";
+    const int c = d.dens_cell[k];
+    if (c < 0 || c >= d.n_cells) return "density insertion on an unknown cell";
+    const Template& t = L.templates[L.cell_template[c]];
+    const int sec = d.dens_section[k];
+    if (sec < 0 || sec >= (int)t.sec_nodes.size()) return "unknown section";
+    const int g = group_of(mech);
+    if (seen[mech].empty()) seen[mech].assign(N, 0);
+    const std::vector<int32_t>& nodes = t.sec_nodes[sec];
+    for (size_t j = 0; j + 1 < nodes.size(); ++j) {
+      const int64_t n = L.cell_off[c] + nodes[j];
+      if (seen[mech][n]) continue;
+      seen[mech][n] = 1;
+      L.groups[g].node.push_back((int32_t)n);
+      L.groups[g].point.push_back(-1);
+      L.groups[g].logical.push_back((uint64_t)n + 1);
+      params[g].push_back({d.dens_param_offsets[k], d.dens_param_offsets[k + 1]});
+    }
+  }
+  L.point_group.assign(d.n_points, -1);
+  L.point_row.assign(d.n_points, -1);
+  L.point_cell.assign(d.n_points, -1);
+  for (int64_t p = 0; p < d.n_points; ++p) {
+    const int mech = d.pt_mech[p];
+    if (mech < 0 || mech >= MIND_MECH_COUNT || !mech_is_point(mech))
+      return "point creation requires a point process";
+    const int g = group_of(mech);
+    int64_t n = -1;
+    const int c = d.pt_cell[p];
+    if (mech == MIND_MECH_NETSTIM) {
+      if (c >= 0) return "NetStim is an artificial cell without a location";
+    } else {
+      if (c < 0 || c >= d.n_cells) return "point process needs a cell location";
+      const Template& t = L.templates[L.cell_template[c]];
+      if (d.pt_section[p] < 0 || d.pt_section[p] >= (int)t.sec_nodes.size()) return "unknown section";
+      if (!(d.pt_x[p] >= 0.0 && d.pt_x[p] <= 1.0)) return "section location must be in [0, 1]";
+      n = L.location_node(c, d.pt_section[p], d.pt_x[p]);
+      L.point_cell[p] = c;
+    }
+    L.groups[g].node.push_back((int32_t)n);
+    L.groups[g].point.push_back((int32_t)p);
+    L.groups[g].logical.push_back((uint64_t)(p + 1) + (uint64_t)N);
+    L.point_group[p] = g;
+    params[g].push_back({d.pt_param_offsets[p], d.pt_param_offsets[p + 1]});
+  }
+  // Sort rows by node (stable) and fill parameter columns.
+  for (size_t g = 0; g < L.groups.size(); ++g) {
+    MechGroup& G = L.groups[g];
+    const int64_t rows = (int64_t)G.node.size();
+    G.rows = rows;
+    std::vector<int64_t> perm(rows);
+    std::iota(perm.begin(), perm.end(), 0);
+    if (G.mech != MIND_MECH_NETSTIM)
+      std::stable_sort(perm.begin(), perm.end(),
+                       [&](int64_t a, int64_t b) { return G.node[a] < G.node[b]; });
+    MechGroup H;
+    H.mech = G.mech;
+    H.rows = rows;
+    H.node.resize(rows); H.point.resize(rows); H.logical.resize(rows);
+    const int nf = field_count(G.mech);
+    double def[32];
+    field_defaults(G.mech, def);
+    H.data.resize((size_t)nf * rows);
+    for (int f = 0; f < nf; ++f) std::fill(H.data.begin() + f * rows, H.data.begin() + (f + 1) * rows, def[f]);
+    for (int64_t r = 0; r < rows; ++r) {
+      const int64_t o = perm[r];
+      H.node[r] = G.node[o];
+      H.point[r] = G.point[o];
+      H.logical[r] = G.logical[o];
+      for (int64_t q = params[g][o].p0; q < params[g][o].p1; ++q) {
+        const int f = d.param_field[q];
+        if (f < 0 || f >= nf) return "unknown mechanism field";
+        H.data[(size_t)f * rows + r] = d.param_value[q];
+      }
+      if (H.point[r] >= 0) L.point_row[H.point[r]] = (int32_t)r;
+    }
+    if (G.mech != MIND_MECH_NETSTIM) {
+      H.cell_rows.assign(L.ncells + 1, 0);
+      int64_t r = 0;
+      for (int c = 0; c < L.ncells; ++c) {
+        H.cell_rows[c] = r;
+        while (r < rows && H.node[r] < L.cell_off[c + 1]) ++r;
+      }
+      H.cell_rows[L.ncells] = rows;
+      H.run_start.assign(rows, 1);
+      for (int64_t q = 1; q < rows; ++q) H.run_start[q] = H.node[q] != H.node[q - 1];
+    }
+    L.groups[g] = std::move(H);
+  }
+  // 4. Sources, detectors (one per voltage source, in source order) and fanout.
+  L.nsources = d.n_sources;
+  L.source_sid.assign(d.src_sid, d.src_sid + d.n_sources);
+  L.source_point.assign(d.n_sources, -1);
+  std::vector<int32_t> det_of_cell_count(L.ncells + 1, 0);
+  for (int64_t s = 0; s < d.n_sources; ++s) {
+    if (d.src_kind[s] == 0) {
+      const int c = d.src_cell[s];
+      if (c < 0 || c >= L.ncells) return "detector on an unknown cell";
+      if (!isfinite(d.src_threshold[s])) return "invalid spike detector";
+      L.det_node.push_back((int32_t)L.location_node(c, d.src_section[s], d.src_x[s]));
+      L.det_source.push_back((int32_t)s);
+      L.det_cell.push_back(c);
+      L.det_threshold.push_back(d.src_threshold[s]);
+      det_of_cell_count[c + 1] += 1;
+    } else {
+      const int p = d.src_point[s];
+      if (p < 0 || p >= d.n_points) return "event source on an unknown point process";
+      L.source_point[s] = p;
+    }
+  }
+  L.cell_det_off.assign(L.ncells + 1, 0);
+  for (int c = 0; c < L.ncells; ++c) L.cell_det_off[c + 1] = L.cell_det_off[c] + det_of_cell_count[c + 1];
+  L.cell_det.resize(L.det_node.size());
+  {
+    std::vector<int64_t> cur(L.cell_det_off.begin(), L.cell_det_off.end() - 1);
+    for (size_t k = 0; k < L.det_node.size(); ++k) L.cell_det[cur[L.det_cell[k]]++] = (int32_t)k;
+  }
+  L.fanout.assign(d.n_sources + 1, 0);
+  for (int64_t e = 0; e < d.n_connections; ++e) {
+    if (d.con_source[e] < 0 || d.con_source[e] >= d.n_sources) return "connection from an unknown source";
+    if (d.con_target[e] < 0 || d.con_target[e] >= d.n_points) return "connection to an unknown point";
+    const int tm = d.pt_mech[d.con_target[e]];
+    if (tm != MIND_MECH_EXPSYN && tm != MIND_MECH_EXP2SYN && tm != MIND_MECH_NETSTIM)
+      return "target has no NET_RECEIVE block";
+    if (!(d.con_delay[e] >= 0.0) || !isfinite(d.con_delay[e]) || !isfinite(d.con_weight[e]))
+      return "connection weight and delay must be finite, delay nonnegative";
+    L.fanout[d.con_source[e] + 1] += 1;
+  }
+  for (int64_t s = 0; s < d.n_sources; ++s) L.fanout[s + 1] += L.fanout[s];
+  L.edge_target.resize(d.n_connections);
+  L.edge_delay.resize(d.n_connections);
+  L.edge_weight.resize(d.n_connections);
+  {
+    std::vector<int64_t> cur(L.fanout.begin(), L.fanout.end() - 1);
+    for (int64_t e = 0; e < d.n_connections; ++e) {
+      const int64_t k = cur[d.con_source[e]]++;
+      L.edge_target[k] = d.con_target[e];
+      L.edge_delay[k] = d.con_delay[e];
+      L.edge_weight[k] = d.con_weight[e];
+    }
+  }
+  L.min_delay = INFINITY;
+  L.max_delay = 0.0;
+  for (double x : L.edge_delay) {
+    L.min_delay = std::min(L.min_delay, x);
+    L.max_delay = std::max(L.max_delay, x);
+  }
+  const int gn = L.group_of_mech[MIND_MECH_NETSTIM];
+  if (gn >= 0) {
+    const MechGroup& G = L.groups[gn];
+    L.netstim_source.assign(G.rows, -1);
+    for (int64_t s = 0; s < d.n_sources; ++s)
+      if (L.source_point[s] >= 0 && L.point_group[L.source_point[s]] == gn)
+        L.netstim_source[L.point_row[L.source_point[s]]] = (int32_t)s;
+  }
+  return "";
+}
+
+}  // namespace mind

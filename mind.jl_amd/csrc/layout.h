@@ -1,0 +1,82 @@
+// Host-side packing of a flattened MicroModel into the structure-of-arrays
+// layout the HIP kernels read.
+//
+// Node order: cells are contiguous blocks in cell order; inside a cell, nodes
+// follow the discretization order of the reference (parents before children,
+// src/model/micro/morphology.jl:406-484). Mechanism rows of a group are sorted
+// by node (stable), so every cell owns a contiguous row range per group
+// (src/model/micro/model.jl:533-555); artificial cells keep creation order.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mind.h"
+
+namespace mind {
+
+struct MechGroup {
+  int mech = 0;
+  int64_t rows = 0;
+  std::vector<int32_t> node;      // packed node, -1 for artificial cells
+  std::vector<int32_t> point;     // point-process index, -1 for density rows
+  std::vector<uint64_t> logical;  // logical identity (Random123 key)
+  std::vector<double> data;       // field-major: data[field * rows + row]
+  std::vector<int64_t> cell_rows; // rows of cell c: [cell_rows[c], cell_rows[c+1])
+  std::vector<uint8_t> run_start; // point groups: row starts a run of equal nodes
+};
+
+struct Template {  // one discretized morphology
+  std::vector<int32_t> parent;  // local, -1 for the root
+  std::vector<double> area, cap, axial;
+  std::vector<std::vector<int32_t>> sec_nodes;  // nseg + 1 nodes per section
+  std::vector<int32_t> sec_start;
+  int64_t compartments = 0;
+};
+
+struct Layout {
+  double dt = 0.025, celsius = 6.3;
+  uint64_t seed = 1;
+  int32_t ncells = 0;
+  int64_t nodes = 0, compartments = 0;
+  std::vector<Template> templates;
+  std::vector<int32_t> cell_template;
+  std::vector<int64_t> cell_off;  // ncells + 1
+  std::vector<int32_t> parent;    // global index, -1 for roots
+  std::vector<double> area, cap, axial, adiag, v_init;
+  std::vector<int64_t> child_off;  // children CSR, increasing index
+  std::vector<int32_t> child;
+
+  std::vector<MechGroup> groups;   // canonical order: first use, densities first
+  int group_of_mech[MIND_MECH_COUNT];
+  std::vector<int32_t> point_group, point_row, point_cell;
+
+  int64_t nsources = 0;
+  std::vector<int64_t> source_sid;
+  std::vector<int32_t> source_point;  // point index of an event source, -1 for detectors
+  std::vector<int32_t> det_node, det_source, det_cell;
+  std::vector<double> det_threshold;
+  std::vector<int64_t> cell_det_off;  // detectors of cell c (indices into det_*)
+  std::vector<int32_t> cell_det;
+
+  std::vector<int64_t> fanout;  // nsources + 1
+  std::vector<int32_t> edge_target;
+  std::vector<double> edge_delay, edge_weight;
+  double min_delay = 0.0, max_delay = 0.0;
+
+  std::vector<int32_t> netstim_source;  // per NetStim row: its source index or -1
+
+  int64_t location_node(int32_t cell, int32_t section, double x) const;
+  int64_t range_node(int32_t cell, int32_t section, double x) const;
+};
+
+// Returns an empty string on success, else the error message (ArgumentError).
+std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& out);
+
+int field_count(int mech);
+void field_defaults(int mech, double* values);
+bool mech_is_point(int mech);
+
+}  // namespace mind

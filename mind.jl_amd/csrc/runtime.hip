@@ -1,0 +1,1238 @@
+// MI355X runtime of the MIND simulation step: device state, kernels and the C ABI.
+//
+// Execution model. The reference runs ~12 launches per micro step with a
+// blocking device->host read of the outbox every step and a CPU binary heap
+// for events (src/runtime/step.jl:91-151, src/runtime/events.jl:4-165). Here
+// the whole loop stays on the device and advances in *windows* of W steps,
+// W*dt <= min NetCon delay - dt/2: no event produced inside a window can be
+// due inside it, so cells are independent for W steps. Per window:
+//   gather   : events due in the window -> per-cell lists sorted by
+//              (delivery call, target, time, ticket)
+//   cells    : W steps of detect / deliver / nrn_cur / cable / nrn_state /
+//              record / deliver for every cell (plain or batched kernel)
+//   exchange : NetStim events of the window in heap order, detector spikes
+//              compacted by wave ballot + prefix sum, heap tickets re-derived
+//              in emission order, NetCon fanout routed into a ring of
+//              per-step delivery buckets.
+// The three launches of a full window are captured once in a hipGraph and
+// replayed; the clock lives in device memory.
+//
+// Event order contract (the reference's CPU heap, src/runtime/utils/events.jl:7-62):
+// a target applies its events in (time, ticket) order, tickets being
+// insertion order: per step, first the detector spikes of that step in source
+// order (each routed over its CSR edges in edge order, events.jl:34-53), then
+// NetStim events of the midpoint delivery, then of the end-of-step delivery,
+// in the heap's (time, ticket) order (events.jl:92-101, 160-162).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "layout.h"
+#include "mech.h"
+
+namespace mind {
+
+// Event in the delivery ring and in per-cell window lists.
+struct QEvent {
+  double t;
+  uint64_t ticket;
+  int64_t call;  // delivery call: 2k = midpoint of step k, 2k+1 = end of step k
+  int32_t edge;
+  int32_t pad;
+};
+
+struct Clock {
+  int64_t step;
+  uint64_t ticket;  // routed events so far
+  uint64_t ns_seq;  // NetStim self-event insertion order
+};
+
+struct GroupDev {
+  int32_t mech;
+  int32_t point;  // 1 for point processes
+  int64_t rows;
+  double* data;
+  const int32_t* node;
+  const int64_t* cell_rows;
+  const uint8_t* run_start;
+};
+
+struct Emit {  // one spike emission: source index and spike time
+  int64_t key;
+  double t;
+  int32_t source;
+  int32_t pad;
+};
+
+struct Dev {
+  double dt, celsius;
+  uint64_t seed;
+  int32_t ncells;
+  int64_t nodes;
+  const int64_t* cell_off;
+  const int32_t* parent;
+  const double* area;
+  const double* cdt;  // capacitance / dt
+  const double* axial;
+  const double* adiag;
+  double* v;
+  double* rhs;
+  double* diag;
+  const int64_t* child_off;
+  const int32_t* child;
+  int32_t ngroups;
+  const GroupDev* g;  // device array of ngroups descriptors
+  int32_t ns_group;  // group index of NetStim or -1
+  const int32_t* point_group;
+  const int32_t* point_row;
+  const int32_t* point_cell;
+  int64_t ndet;
+  const int32_t* det_node;
+  const double* det_thr;
+  uint8_t* det_above;
+  const int64_t* cell_det_off;
+  const int32_t* cell_det;
+  const int32_t* det_source;
+  uint8_t* det_flags;  // [window step][detector]
+  const int64_t* fanout;
+  const int32_t* edge_target;
+  const double* edge_delay;
+  const double* edge_weight;
+  const int64_t* source_sid;
+  // NetStim rows
+  int64_t ns_rows;
+  Stream* ns_stream;
+  double* ns_t;
+  double* ns_flag;
+  uint64_t* ns_seq;
+  int32_t* ns_edge;
+  int32_t* ns_valid;
+  const int32_t* ns_source;
+  // delivery ring
+  int64_t ring_slots, ring_cap;
+  QEvent* ring;
+  unsigned long long* ring_count;
+  // window lists
+  int64_t win_cap;
+  QEvent* win;
+  int64_t* cell_seg_off;
+  unsigned long long* cell_cursor;
+  // emissions
+  int64_t emit_cap;
+  Emit* emit_det;
+  Emit* emit_ns;
+  Emit* emit_all;
+  int64_t* emit_base;
+  // spike recording
+  int64_t spike_cap;
+  double* spike_t;
+  int64_t* spike_sid;
+  unsigned long long* spike_count;
+  int32_t record_spikes;
+  // value recording
+  int64_t nrec;
+  const int32_t* rec_kind;   // 0 voltage, 1 field
+  const int32_t* rec_group;
+  const int32_t* rec_field;
+  const int64_t* rec_index;  // node or row
+  const int64_t* cell_rec_off;
+  const int32_t* cell_rec;
+  double* rec_buf;
+  int64_t rec_stride;
+  int64_t rec_start;
+  Clock* clock;
+  int32_t* error;
+};
+
+__device__ inline void raise_error(int32_t* error, int32_t code) {
+  // Keep the first, most specific failure (atomic max like the reference's error word).
+  atomicMax(error, code);
+}
+
+__host__ __device__ inline double call_cutoff(int64_t j, double dt) {
+  const int64_t k = j >= 0 ? j / 2 : (j - 1) / 2;  // floor(j / 2)
+  const int64_t phase = j - 2 * k;
+  return phase == 0 ? (double)k * dt + 0.5 * dt : (double)(k + 1) * dt;
+}
+
+// First delivery call j >= jmin whose cutoff reaches t.
+__device__ inline int64_t call_of(double t, int64_t jmin, double dt) {
+  double est = floor(t / dt);
+  int64_t j = jmin;
+  if (est * 2.0 - 2.0 > (double)jmin) j = (int64_t)est * 2 - 2;
+  while (t > call_cutoff(j, dt)) ++j;
+  return j;
+}
+
+__device__ inline int64_t emit_key(int64_t j) { return j < 0 ? -1 : 3 * (j / 2) + 1 + (j & 1); }
+
+__device__ inline Cols cols(const GroupDev& g) { return Cols{g.data, g.rows}; }
+
+// ------------------------------------------------------------ initialization
+__global__ void k_init_nodes(Dev d, const double* v_init) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.nodes) d.v[i] = v_init[i];
+  if (i < d.ndet) d.det_above[i] = 0;
+}
+
+__global__ void k_init_group(Dev d, int gi) {
+  const GroupDev& G = d.g[gi];
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= G.rows) return;
+  if (G.mech == MIND_MECH_NETSTIM) {
+    // initialize_random_kernel! (cuda/micro.jl:79-91): keys from the logical id
+    // and the NetStim source digest, then the INITIAL block.
+    return;
+  }
+  mech_initial(G.mech, cols(G), r, d.v[G.node[r]], d.celsius);
+}
+
+__global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= d.ns_rows) return;
+  const GroupDev& G = d.g[d.ns_group];
+  Stream s;
+  s.counter = 0;
+  s.key[0] = (uint32_t)logical[r];
+  s.key[1] = logical[r] >> 32;
+  s.key[2] = (uint64_t)(uint32_t)digest;
+  stream_setseq(s, d.seed, 0.0);
+  const NetStimOut o = netstim_initial(cols(G), r, s, d.seed);
+  d.ns_stream[r] = s;
+  d.ns_valid[r] = 0;
+  if (o.send) {
+    if (!(o.send_delay >= 0.0) || !mind_isfinite(o.send_delay)) {
+      raise_error(d.error, MIND_EEVENT);
+      return;
+    }
+    d.ns_t[r] = 0.0 + o.send_delay;
+    d.ns_flag[r] = o.send_flag;
+    d.ns_seq[r] = (uint64_t)r + 1;  // INITIAL flush orders NetStims by rank
+    d.ns_edge[r] = 0;
+    d.ns_valid[r] = 1;
+  }
+}
+
+// micro_currents!(0.0) at the end of initialize!: sets the assigned current fields.
+__global__ void k_currents_fields(Dev d, int gi, double t) {
+  const GroupDev& G = d.g[gi];
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= G.rows) return;
+  double value, shifted;
+  mech_current(G.mech, cols(G), r, d.v[G.node[r]], t, value, shifted, true);
+}
+
+__global__ void k_record_sample(Dev d, int64_t sample) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.nrec) return;
+  const int64_t idx = d.rec_index[q];
+  const double x = d.rec_kind[q] == 0 ? d.v[idx] : d.g[d.rec_group[q]].data[d.rec_field[q] * d.g[d.rec_group[q]].rows + idx];
+  d.rec_buf[q * d.rec_stride + sample] = x;
+}
+
+// ------------------------------------------------------------ shared pieces
+__device__ inline void record_cell(const Dev& d, int64_t c, int64_t sample) {
+  for (int64_t q = d.cell_rec_off[c]; q < d.cell_rec_off[c + 1]; ++q) {
+    const int32_t rec = d.cell_rec[q];
+    const int64_t idx = d.rec_index[rec];
+    const double x = d.rec_kind[rec] == 0
+                         ? d.v[idx]
+                         : d.g[d.rec_group[rec]].data[d.rec_field[rec] * d.g[d.rec_group[rec]].rows + idx];
+    d.rec_buf[rec * d.rec_stride + sample] = x;
+  }
+}
+
+// Applies this cell's events of delivery call j; returns the new cursor.
+__device__ inline int64_t deliver_cell(const Dev& d, int64_t seg, int64_t end, int64_t j) {
+  while (seg < end && d.win[seg].call == j) {
+    const int32_t e = d.win[seg].edge;
+    const int32_t target = d.edge_target[e];
+    const GroupDev& G = d.g[d.point_group[target]];
+    synapse_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
+    ++seg;
+  }
+  return seg;
+}
+
+__device__ inline void detect_cell(const Dev& d, int64_t c, int s) {
+  for (int64_t q = d.cell_det_off[c]; q < d.cell_det_off[c + 1]; ++q) {
+    const int32_t k = d.cell_det[q];
+    const bool cur = d.v[d.det_node[k]] > d.det_thr[k];
+    d.det_flags[(int64_t)s * d.ndet + k] = (cur && !d.det_above[k]) ? 1 : 0;
+    d.det_above[k] = cur;
+  }
+}
+
+// ------------------------------------------------------------ plain kernel
+// One cell per thread; every per-node array lives in HBM. This is the
+// reference's one-instance-per-thread execution shape, kept as the baseline
+// the batched kernel is measured against.
+__global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.ncells) return;
+  const int64_t k0 = d.clock->step;
+  const int64_t n0 = d.cell_off[c], n1 = d.cell_off[c + 1];
+  int64_t seg = d.cell_seg_off[c];
+  const int64_t seg_end = d.cell_seg_off[c + 1];
+  const double dt = d.dt;
+  double* v = d.v;
+  double* rhs = d.rhs;
+  double* diag = d.diag;
+  for (int s = 0; s < n; ++s) {
+    const int64_t k = k0 + s;
+    const double t = (double)k * dt;
+    detect_cell(d, c, s);
+    seg = deliver_cell(d, seg, seg_end, 2 * k);
+    // nrn_cur at t + dt/2
+    const double tc = t + 0.5 * dt;
+    for (int64_t i = n0; i < n1; ++i) { rhs[i] = 0.0; diag[i] = 0.0; }
+    for (int gi = 0; gi < d.ngroups; ++gi) {
+      const GroupDev& G = d.g[gi];
+      if (G.mech == MIND_MECH_NETSTIM) continue;
+      const Cols col = cols(G);
+      for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r) {
+        const int32_t node = G.node[r];
+        double value, shifted;
+        mech_current(G.mech, col, r, v[node], tc, value, shifted, true);
+        const double scale = G.point ? 1.0 : d.area[node] * 0.01;
+        rhs[node] -= scale * value;
+        diag[node] += scale * (shifted - value) / 0.001;
+      }
+    }
+    // cable assembly: own axial current first, children in increasing index
+    for (int64_t i = n0; i < n1; ++i) {
+      diag[i] = d.cdt[i] + diag[i] + d.adiag[i];
+      const int32_t p = d.parent[i];
+      if (p >= 0) {
+        const double cur = d.axial[i] * (v[p] - v[i]);
+        rhs[i] += cur;
+        rhs[p] -= cur;
+      }
+    }
+    // Hines elimination (children before parents) and back substitution
+    bool bad = false;
+    for (int64_t i = n1 - 1; i >= n0; --i) {
+      const double piv = diag[i];
+      bad |= (piv == 0.0) || !mind_isfinite(piv);
+      const int32_t p = d.parent[i];
+      if (p >= 0) {
+        const double g = d.axial[i];
+        const double f = g / piv;
+        diag[p] -= f * g;
+        rhs[p] += f * rhs[i];
+      }
+    }
+    if (bad) raise_error(d.error, MIND_ESINGULAR);
+    for (int64_t i = n0; i < n1; ++i) {
+      double x = rhs[i];
+      const int32_t p = d.parent[i];
+      if (p >= 0) x += d.axial[i] * rhs[p];
+      x = x / diag[i];
+      rhs[i] = x;
+      v[i] += x;
+    }
+    // nrn_state at t + dt
+    for (int gi = 0; gi < d.ngroups; ++gi) {
+      const GroupDev& G = d.g[gi];
+      if (G.mech != MIND_MECH_HH && G.mech != MIND_MECH_EXP2SYN && G.mech != MIND_MECH_EXPSYN) continue;
+      const Cols col = cols(G);
+      for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r)
+        mech_advance(G.mech, col, r, v[G.node[r]], dt, d.celsius);
+    }
+    if (d.nrec) record_cell(d, c, k + 1 - d.rec_start);
+    seg = deliver_cell(d, seg, seg_end, 2 * k + 1);
+  }
+}
+
+// ------------------------------------------------------------ block helpers
+constexpr int kBlock = 1024;
+
+// Exclusive scan of one value per thread over the block; returns the prefix
+// and writes the block total to *total.
+__device__ inline int64_t block_scan(int64_t x, int64_t* lds, int64_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t incl = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) lds[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t acc = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      const int64_t t = lds[w];
+      lds[w] = acc;
+      acc += t;
+    }
+    lds[kBlock / 64] = acc;
+  }
+  __syncthreads();
+  const int64_t r = lds[wave] + incl - x;
+  *total = lds[kBlock / 64];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------ exchange
+struct HeapItem { double t; uint64_t seq; int32_t row; };
+constexpr int kHeapCap = 2048;
+
+__device__ inline bool heap_less(const HeapItem& a, const HeapItem& b) {
+  return a.t < b.t || (a.t == b.t && a.seq < b.seq);
+}
+
+__device__ void heap_push(HeapItem* h, int& n, HeapItem x) {
+  int i = n++;
+  while (i > 0) {
+    const int p = (i - 1) >> 1;
+    if (!heap_less(x, h[p])) break;
+    h[i] = h[p];
+    i = p;
+  }
+  h[i] = x;
+}
+
+__device__ HeapItem heap_pop(HeapItem* h, int& n) {
+  const HeapItem top = h[0];
+  const HeapItem last = h[--n];
+  int i = 0;
+  while (2 * i + 1 < n) {
+    int c = 2 * i + 1;
+    if (c + 1 < n && heap_less(h[c + 1], h[c])) ++c;
+    if (!heap_less(h[c], last)) break;
+    h[i] = h[c];
+    i = c;
+  }
+  if (n > 0) h[i] = last;
+  return top;
+}
+
+// One workgroup. n == 0 processes the time-zero delivery of initialize!.
+__global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
+  __shared__ HeapItem heap[kHeapCap];
+  __shared__ int64_t scan[kBlock / 64 + 1];
+  __shared__ int heap_n;
+  __shared__ int64_t n_ns, n_det;
+  const int tid = threadIdx.x;
+  const int64_t k0 = d.clock->step;
+  const int64_t call_lo = n == 0 ? -1 : 2 * k0;
+  const int64_t call_hi = n == 0 ? -1 : 2 * (k0 + n) - 1;
+  const double dt = d.dt;
+  const double cut_hi = call_cutoff(call_hi, dt);
+  // A. NetStim events of the window, in heap order (time, insertion order).
+  if (tid == 0) { heap_n = 0; n_ns = 0; }
+  __syncthreads();
+  for (int64_t r = tid; r < d.ns_rows; r += kBlock) {
+    if (d.ns_valid[r] && d.ns_t[r] <= cut_hi) {
+      const int slot = atomicAdd(&heap_n, 1);
+      if (slot < kHeapCap) heap[slot] = HeapItem{d.ns_t[r], d.ns_seq[r], (int32_t)r};
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && d.ns_rows > 0) {
+    int hn = heap_n;
+    if (hn > kHeapCap) {
+      raise_error(d.error, MIND_EUNSUPPORTED);
+      hn = 0;
+    }
+    // heapify
+    int m = 0;
+    for (int i = 0; i < hn; ++i) heap_push(heap, m, heap[i]);
+    const GroupDev& G = d.g[d.ns_group];
+    const Cols col = cols(G);
+    int64_t count = 0;
+    uint64_t seq = d.clock->ns_seq;
+    while (m > 0) {
+      const HeapItem it = heap_pop(heap, m);
+      const int32_t r = it.row;
+      const double t = it.t;
+      const double flag = d.ns_flag[r];
+      const int32_t edge = d.ns_edge[r];
+      d.ns_valid[r] = 0;
+      const int64_t j = call_of(t, call_lo, dt);
+      const double w = edge == 0 ? 0.0 : d.edge_weight[edge - 1];
+      Stream s = d.ns_stream[r];
+      const NetStimOut o = netstim_receive(col, r, s, d.seed, t, w, flag);
+      d.ns_stream[r] = s;
+      if (o.spiked && d.ns_source[r] >= 0) {
+        if (count < d.emit_cap) d.emit_ns[count] = Emit{emit_key(j), o.spike_time, d.ns_source[r], 0};
+        else raise_error(d.error, MIND_EQUEUE);
+        ++count;
+      }
+      if (o.send) {
+        if (!(o.send_delay >= 0.0) || !mind_isfinite(o.send_delay)) {
+          raise_error(d.error, MIND_EEVENT);
+          continue;
+        }
+        const double tn = t + o.send_delay;
+        d.ns_t[r] = tn;
+        d.ns_flag[r] = o.send_flag;
+        d.ns_seq[r] = ++seq;
+        d.ns_edge[r] = edge;
+        d.ns_valid[r] = 1;
+        if (tn <= cut_hi) {
+          if (m < kHeapCap) heap_push(heap, m, HeapItem{tn, seq, r});
+          else raise_error(d.error, MIND_EUNSUPPORTED);
+        }
+      }
+    }
+    d.clock->ns_seq = seq;
+    n_ns = count < d.emit_cap ? count : d.emit_cap;
+  }
+  __syncthreads();
+  // B. Detector spikes of each window step, compacted in source order by
+  //    wave ballot and a block prefix sum.
+  int64_t ndet_total = 0;
+  for (int s = 0; s < n; ++s) {
+    const int64_t key = 3 * (k0 + s);
+    const double ts = (double)(k0 + s) * dt + 1.0e-10;
+    for (int64_t base = 0; base < d.ndet; base += kBlock) {
+      const int64_t q = base + tid;
+      const int flag = q < d.ndet ? d.det_flags[(int64_t)s * d.ndet + q] : 0;
+      const unsigned long long ballot = __ballot(flag);
+      const int lane = tid & 63;
+      const int wave_rank = __popcll(ballot & ((1ull << lane) - 1ull));
+      if (lane == 0) scan[tid >> 6] = __popcll(ballot);
+      __syncthreads();
+      if (tid == 0) {
+        int64_t acc = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+          const int64_t x = scan[w];
+          scan[w] = acc;
+          acc += x;
+        }
+        scan[kBlock / 64] = acc;
+      }
+      __syncthreads();
+      if (flag) {
+        const int64_t pos = ndet_total + scan[tid >> 6] + wave_rank;
+        if (pos < d.emit_cap) d.emit_det[pos] = Emit{key, ts, d.det_source[q], 0};
+        else raise_error(d.error, MIND_EQUEUE);
+      }
+      ndet_total += scan[kBlock / 64];
+      __syncthreads();
+    }
+  }
+  if (tid == 0) n_det = ndet_total < d.emit_cap ? ndet_total : d.emit_cap;
+  __syncthreads();
+  // C. Merge both emission lists by key (they never share a key).
+  const int64_t nd = n_det, ns = n_ns;
+  for (int64_t i = tid; i < nd + ns; i += kBlock) {
+    Emit e;
+    int64_t pos;
+    if (i < nd) {
+      e = d.emit_det[i];
+      int64_t lo = 0, hi = ns;  // count of NetStim emissions with key < e.key
+      while (lo < hi) { const int64_t mid = (lo + hi) >> 1; if (d.emit_ns[mid].key < e.key) lo = mid + 1; else hi = mid; }
+      pos = i + lo;
+    } else {
+      e = d.emit_ns[i - nd];
+      int64_t lo = 0, hi = nd;
+      while (lo < hi) { const int64_t mid = (lo + hi) >> 1; if (d.emit_det[mid].key < e.key) lo = mid + 1; else hi = mid; }
+      pos = (i - nd) + lo;
+    }
+    d.emit_all[pos] = e;
+  }
+  __syncthreads();
+  // D. Tickets: prefix sum of fanout sizes in emission order.
+  const int64_t ne = nd + ns;
+  int64_t running = 0;
+  for (int64_t base = 0; base < ne; base += kBlock) {
+    const int64_t i = base + tid;
+    const int64_t f = i < ne ? d.fanout[d.emit_all[i].source + 1] - d.fanout[d.emit_all[i].source] : 0;
+    int64_t total;
+    const int64_t pre = block_scan(f, scan, &total);
+    if (i < ne) d.emit_base[i] = running + pre;
+    running += total;
+  }
+  if (tid == 0) d.emit_base[ne] = running;
+  __syncthreads();
+  // E. Route every edge of every emission into its delivery bucket.
+  const uint64_t ticket0 = d.clock->ticket;
+  for (int64_t q = tid; q < running; q += kBlock) {
+    int64_t lo = 0, hi = ne;  // emission m with base[m] <= q < base[m+1]
+    while (hi - lo > 1) { const int64_t mid = (lo + hi) >> 1; if (d.emit_base[mid] <= q) lo = mid; else hi = mid; }
+    const Emit& em = d.emit_all[lo];
+    const int64_t e = d.fanout[em.source] + (q - d.emit_base[lo]);
+    const double T = em.t + d.edge_delay[e];
+    if (!mind_isfinite(T)) { raise_error(d.error, MIND_ENONFINITE); continue; }
+    const int64_t j = call_of(T, call_hi + 1, dt);
+    if (T <= cut_hi) { raise_error(d.error, MIND_EUNSUPPORTED); continue; }
+    const int64_t step = j / 2;
+    if (step - (k0 + n) >= d.ring_slots - 1) { raise_error(d.error, MIND_EQUEUE); continue; }
+    const int64_t b = step % d.ring_slots;
+    const unsigned long long pos = atomicAdd(&d.ring_count[b], 1ull);
+    if ((int64_t)pos >= d.ring_cap) { raise_error(d.error, MIND_EQUEUE); continue; }
+    d.ring[b * d.ring_cap + (int64_t)pos] = QEvent{T, ticket0 + (uint64_t)q + 1, j, (int32_t)e, 0};
+  }
+  // F. Spike recording.
+  if (d.record_spikes) {
+    for (int64_t i = tid; i < ne; i += kBlock) {
+      const unsigned long long pos = atomicAdd(d.spike_count, 1ull);
+      if ((int64_t)pos < d.spike_cap) {
+        d.spike_t[pos] = d.emit_all[i].t;
+        d.spike_sid[pos] = d.source_sid[d.emit_all[i].source];
+      } else {
+        raise_error(d.error, MIND_EQUEUE);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    d.clock->ticket = ticket0 + (uint64_t)running;
+    d.clock->step = k0 + n;
+  }
+}
+
+// ------------------------------------------------------------ gather
+__device__ inline bool win_less(const QEvent& a, const QEvent& b, const int32_t* edge_target) {
+  if (a.call != b.call) return a.call < b.call;
+  const int32_t ta = edge_target[a.edge], tb = edge_target[b.edge];
+  if (ta != tb) return ta < tb;
+  if (a.t != b.t) return a.t < b.t;
+  return a.ticket < b.ticket;
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
+  __shared__ int64_t scan[kBlock / 64 + 1];
+  const int tid = threadIdx.x;
+  const int64_t k1 = d.clock->step;
+  // 1. count events per cell
+  for (int64_t c = tid; c < d.ncells; c += kBlock) d.cell_cursor[c] = 0;
+  __syncthreads();
+  for (int s = 0; s < n; ++s) {
+    const int64_t b = (k1 + s) % d.ring_slots;
+    const int64_t cnt = (int64_t)d.ring_count[b];
+    for (int64_t i = tid; i < cnt; i += kBlock) {
+      const int32_t cell = d.point_cell[d.edge_target[d.ring[b * d.ring_cap + i].edge]];
+      atomicAdd(&d.cell_cursor[cell], 1ull);
+    }
+  }
+  __syncthreads();
+  // 2. exclusive scan over cells
+  int64_t running = 0;
+  for (int64_t base = 0; base < d.ncells; base += kBlock) {
+    const int64_t c = base + tid;
+    const int64_t x = c < d.ncells ? (int64_t)d.cell_cursor[c] : 0;
+    int64_t total;
+    const int64_t pre = block_scan(x, scan, &total);
+    if (c < d.ncells) {
+      d.cell_seg_off[c] = running + pre;
+      d.cell_cursor[c] = (unsigned long long)(running + pre);
+    }
+    running += total;
+  }
+  if (tid == 0) d.cell_seg_off[d.ncells] = running;
+  __syncthreads();
+  if (running > d.win_cap) {
+    if (tid == 0) raise_error(d.error, MIND_EQUEUE);
+    for (int64_t c = tid; c <= d.ncells; c += kBlock) d.cell_seg_off[c] = 0;
+    return;
+  }
+  // 3. scatter
+  for (int s = 0; s < n; ++s) {
+    const int64_t b = (k1 + s) % d.ring_slots;
+    const int64_t cnt = (int64_t)d.ring_count[b];
+    for (int64_t i = tid; i < cnt; i += kBlock) {
+      const QEvent ev = d.ring[b * d.ring_cap + i];
+      const int32_t cell = d.point_cell[d.edge_target[ev.edge]];
+      const unsigned long long pos = atomicAdd(&d.cell_cursor[cell], 1ull);
+      d.win[pos] = ev;
+    }
+  }
+  __syncthreads();
+  // 4. deterministic order inside each cell (insertion sort; lists are short)
+  for (int64_t c = tid; c < d.ncells; c += kBlock) {
+    const int64_t a = d.cell_seg_off[c], z = d.cell_seg_off[c + 1];
+    for (int64_t i = a + 1; i < z; ++i) {
+      const QEvent x = d.win[i];
+      int64_t j = i - 1;
+      while (j >= a && win_less(x, d.win[j], d.edge_target)) {
+        d.win[j + 1] = d.win[j];
+        --j;
+      }
+      d.win[j + 1] = x;
+    }
+  }
+  __syncthreads();
+  for (int s = tid; s < n; s += kBlock) d.ring_count[(k1 + s) % d.ring_slots] = 0;
+}
+
+
+// ============================================================ host side
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) return fail(MIND_EHIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+// Source digest of the NetStim mechanism: the first 16 hex digits of the
+// SHA-256 of the reference's mod/NetStim.mod, which seeds its Random123 key
+// (src/compiler/ir.jl:316-324, src/runtime/initialize.jl:309).
+constexpr uint64_t kNetStimDigest = 0xa186833b569ff3b2ull;
+
+}  // namespace
+
+struct State {
+  Layout L;
+  Dev d{};
+  int device = 0;
+  int method = MIND_METHOD_PLAIN;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocations;
+  std::vector<GroupDev> groups_host;
+  std::vector<double*> group_initial;  // parameter snapshot restored by initialize
+  double* v_init = nullptr;
+  uint64_t* ns_logical = nullptr;
+  int32_t wmax = 1;
+  int64_t step = 0;
+  bool initialized = false;
+  hipGraphExec_t graph = nullptr;
+  int32_t graph_steps = 0;
+  // batched kernel plan
+  int64_t batch_groups = 0;
+  int32_t cells_per_group = 0;
+  std::vector<void*> run_allocations;
+
+  ~State() {
+    if (graph) hipGraphExecDestroy(graph);
+    for (void* p : run_allocations) hipFree(p);
+    for (void* p : allocations) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MIND_EOOM, "device allocation failed");
+    allocations.push_back(p);
+    *out = static_cast<T*>(p);
+    return 0;
+  }
+
+  template <typename T>
+  int upload(T** out, const std::vector<T>& host) {
+    int rc = alloc(out, host.size());
+    if (rc) return rc;
+    if (!host.empty())
+      HIP_TRY(hipMemcpy(*out, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+  }
+};
+
+int launch_window(State& S, int32_t n);
+
+namespace {
+
+int setup(State& S, const mind_build_options& o) {
+  Layout& L = S.L;
+  Dev& d = S.d;
+  S.device = o.device;
+  S.method = o.method;
+  HIP_TRY(hipSetDevice(o.device));
+  HIP_TRY(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  d.dt = L.dt;
+  d.celsius = L.celsius;
+  d.seed = L.seed;
+  d.ncells = L.ncells;
+  d.nodes = L.nodes;
+  int rc = 0;
+  std::vector<double> cdt(L.nodes);
+  for (int64_t i = 0; i < L.nodes; ++i) cdt[i] = L.cap[i] / L.dt;
+  {
+    int64_t* p64; int32_t* p32; double* pd;
+    if ((rc = S.upload(&p64, L.cell_off))) return rc; d.cell_off = p64;
+    if ((rc = S.upload(&p32, L.parent))) return rc; d.parent = p32;
+    if ((rc = S.upload(&pd, L.area))) return rc; d.area = pd;
+    if ((rc = S.upload(&pd, cdt))) return rc; d.cdt = pd;
+    if ((rc = S.upload(&pd, L.axial))) return rc; d.axial = pd;
+    if ((rc = S.upload(&pd, L.adiag))) return rc; d.adiag = pd;
+    if ((rc = S.upload(&pd, L.v_init))) return rc; S.v_init = pd;
+    if ((rc = S.alloc(&d.v, L.nodes))) return rc;
+    if ((rc = S.alloc(&d.rhs, L.nodes))) return rc;
+    if ((rc = S.alloc(&d.diag, L.nodes))) return rc;
+    if ((rc = S.upload(&p64, L.child_off))) return rc; d.child_off = p64;
+    if ((rc = S.upload(&p32, L.child))) return rc; d.child = p32;
+  }
+  // mechanism groups
+  d.ngroups = (int32_t)L.groups.size();
+  d.ns_group = L.group_of_mech[MIND_MECH_NETSTIM];
+  S.groups_host.resize(L.groups.size());
+  S.group_initial.resize(L.groups.size());
+  for (size_t gi = 0; gi < L.groups.size(); ++gi) {
+    const MechGroup& G = L.groups[gi];
+    GroupDev& H = S.groups_host[gi];
+    H.mech = G.mech;
+    H.point = mech_is_point(G.mech) ? 1 : 0;
+    H.rows = G.rows;
+    if ((rc = S.alloc(&H.data, G.data.size()))) return rc;
+    if ((rc = S.upload(&S.group_initial[gi], G.data))) return rc;
+    int32_t* p32; int64_t* p64; uint8_t* p8;
+    if ((rc = S.upload(&p32, G.node))) return rc; H.node = p32;
+    if ((rc = S.upload(&p64, G.cell_rows))) return rc; H.cell_rows = p64;
+    if ((rc = S.upload(&p8, G.run_start))) return rc; H.run_start = p8;
+  }
+  {
+    GroupDev* pg;
+    if ((rc = S.upload(&pg, S.groups_host))) return rc;
+    d.g = pg;
+  }
+  {
+    int32_t* p32;
+    if ((rc = S.upload(&p32, L.point_group))) return rc; d.point_group = p32;
+    if ((rc = S.upload(&p32, L.point_row))) return rc; d.point_row = p32;
+    if ((rc = S.upload(&p32, L.point_cell))) return rc; d.point_cell = p32;
+  }
+  // detectors
+  d.ndet = (int64_t)L.det_node.size();
+  {
+    int32_t* p32; int64_t* p64; double* pd;
+    if ((rc = S.upload(&p32, L.det_node))) return rc; d.det_node = p32;
+    if ((rc = S.upload(&pd, L.det_threshold))) return rc; d.det_thr = pd;
+    if ((rc = S.alloc(&d.det_above, d.ndet))) return rc;
+    if ((rc = S.upload(&p64, L.cell_det_off))) return rc; d.cell_det_off = p64;
+    if ((rc = S.upload(&p32, L.cell_det))) return rc; d.cell_det = p32;
+    if ((rc = S.upload(&p32, L.det_source))) return rc; d.det_source = p32;
+  }
+  // window length: no event produced inside a window may be due inside it
+  const double min_delay = L.edge_delay.empty() ? INFINITY : L.min_delay;
+  int64_t wmax = 1000;
+  if (std::isfinite(min_delay)) wmax = (int64_t)floor(min_delay / L.dt - 0.5);
+  if (o.window_steps > 0) wmax = std::min<int64_t>(wmax, o.window_steps);
+  if (wmax < 1)
+    return fail(MIND_EUNSUPPORTED,
+                "NetCon delays shorter than 1.5 dt are not supported on the device path");
+  S.wmax = (int32_t)wmax;
+  if ((rc = S.alloc(&d.det_flags, (size_t)d.ndet * wmax))) return rc;
+  // edges
+  {
+    int64_t* p64; int32_t* p32; double* pd;
+    if ((rc = S.upload(&p64, L.fanout))) return rc; d.fanout = p64;
+    if ((rc = S.upload(&p32, L.edge_target))) return rc; d.edge_target = p32;
+    if ((rc = S.upload(&pd, L.edge_delay))) return rc; d.edge_delay = pd;
+    if ((rc = S.upload(&pd, L.edge_weight))) return rc; d.edge_weight = pd;
+    if ((rc = S.upload(&p64, L.source_sid))) return rc; d.source_sid = p64;
+  }
+  // connections must target synapses of cells (NetStim inputs are not on the device path)
+  for (int32_t t : L.edge_target)
+    if (L.point_cell[t] < 0)
+      return fail(MIND_EUNSUPPORTED, "connections into artificial cells are not supported on the device path");
+  // NetStim rows
+  d.ns_rows = d.ns_group >= 0 ? L.groups[d.ns_group].rows : 0;
+  if ((rc = S.alloc(&d.ns_stream, d.ns_rows))) return rc;
+  if ((rc = S.alloc(&d.ns_t, d.ns_rows))) return rc;
+  if ((rc = S.alloc(&d.ns_flag, d.ns_rows))) return rc;
+  if ((rc = S.alloc(&d.ns_seq, d.ns_rows))) return rc;
+  if ((rc = S.alloc(&d.ns_edge, d.ns_rows))) return rc;
+  if ((rc = S.alloc(&d.ns_valid, d.ns_rows))) return rc;
+  {
+    int32_t* p32;
+    if ((rc = S.upload(&p32, L.netstim_source))) return rc; d.ns_source = p32;
+    std::vector<uint64_t> logical = d.ns_group >= 0 ? L.groups[d.ns_group].logical : std::vector<uint64_t>();
+    if ((rc = S.upload(&S.ns_logical, logical))) return rc;
+  }
+  // delivery ring: one bucket per step up to the longest delay past a window
+  const int64_t E = (int64_t)L.edge_target.size();
+  d.ring_slots = (int64_t)ceil(L.max_delay / L.dt) + wmax + 4;
+  d.ring_cap = o.event_capacity > 0 ? o.event_capacity
+                                    : std::min<int64_t>(std::max<int64_t>(E + 1024, 4096), 1 << 20);
+  if ((rc = S.alloc(&d.ring, (size_t)d.ring_slots * d.ring_cap))) return rc;
+  if ((rc = S.alloc(&d.ring_count, d.ring_slots))) return rc;
+  HIP_TRY(hipMemset(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots));
+  d.win_cap = std::min<int64_t>(d.ring_cap * wmax, (int64_t)1 << 26);
+  if ((rc = S.alloc(&d.win, d.win_cap))) return rc;
+  if ((rc = S.alloc(&d.cell_seg_off, L.ncells + 1))) return rc;
+  if ((rc = S.alloc(&d.cell_cursor, L.ncells + 1))) return rc;
+  HIP_TRY(hipMemset(d.cell_seg_off, 0, sizeof(int64_t) * (L.ncells + 1)));
+  d.emit_cap = d.ndet * wmax + 64 * d.ns_rows + 1024;
+  if ((rc = S.alloc(&d.emit_det, d.emit_cap))) return rc;
+  if ((rc = S.alloc(&d.emit_ns, d.emit_cap))) return rc;
+  if ((rc = S.alloc(&d.emit_all, 2 * d.emit_cap))) return rc;
+  if ((rc = S.alloc(&d.emit_base, 2 * d.emit_cap + 1))) return rc;
+  d.spike_cap = 1 << 22;
+  if ((rc = S.alloc(&d.spike_t, d.spike_cap))) return rc;
+  if ((rc = S.alloc(&d.spike_sid, d.spike_cap))) return rc;
+  if ((rc = S.alloc(&d.spike_count, 1))) return rc;
+  if ((rc = S.alloc(&d.clock, 1))) return rc;
+  if ((rc = S.alloc(&d.error, 1))) return rc;
+  HIP_TRY(hipMemset(d.error, 0, sizeof(int32_t)));
+  return 0;
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+int check_device_error(State& S) {
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  int32_t code = 0;
+  HIP_TRY(hipMemcpy(&code, S.d.error, sizeof(code), hipMemcpyDeviceToHost));
+  if (code == 0) return 0;
+  S.initialized = false;
+  switch (code) {
+    case MIND_EEVENT: return fail(code, "invalid self-event delay or target");
+    case MIND_ESINGULAR: return fail(code, "singular or non-finite cable system");
+    case MIND_ENONFINITE: return fail(code, "non-finite numerical state");
+    case MIND_EQUEUE: return fail(code, "event ring, window list or spike buffer capacity exceeded");
+    case MIND_EUNSUPPORTED:
+      return fail(code, "an event fell due inside the window that produced it, or too many NetStims were due in one window");
+    default: return fail(code, "device error " + std::to_string(code));
+  }
+}
+
+}  // namespace
+
+int batched_setup(State& S, int32_t cells_per_group);
+int batched_launch(State& S, int32_t n);
+
+int launch_window(State& S, int32_t n) {
+  Dev& d = S.d;
+  hipLaunchKernelGGL(k_gather, dim3(1), dim3(kBlock), 0, S.stream, d, n);
+  if (S.method == MIND_METHOD_BATCHED) {
+    int rc = batched_launch(S, n);
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL(k_plain, dim3(blocks_for(d.ncells, 64)), dim3(64), 0, S.stream, d, n);
+  }
+  hipLaunchKernelGGL(k_exchange, dim3(1), dim3(kBlock), 0, S.stream, d, n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mind
+
+using mind::State;
+using mind::fail;
+
+extern "C" {
+
+const char* mind_last_error(void) { return mind::g_error.c_str(); }
+
+int mind_build(const mind_model_desc* model, const mind_build_options* options, mind_state** out) {
+  if (!model || !options || !out) return fail(MIND_EARG, "null argument");
+  State* S = new State();
+  std::string err = mind::pack(*model, options->dt, options->seed, S->L);
+  if (!err.empty()) {
+    delete S;
+    return fail(MIND_EARG, err);
+  }
+  int rc = mind::setup(*S, *options);
+  if (!rc && options->method == MIND_METHOD_BATCHED) rc = mind::batched_setup(*S, options->cells_per_group);
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  *out = reinterpret_cast<mind_state*>(S);
+  return 0;
+}
+
+void mind_state_destroy(mind_state* state) { delete reinterpret_cast<State*>(state); }
+
+int mind_initialize(mind_state* state) {
+  State& S = *reinterpret_cast<State*>(state);
+  mind::Dev& d = S.d;
+  HIP_TRY(hipSetDevice(S.device));
+  S.initialized = false;
+  S.step = 0;
+  HIP_TRY(hipMemsetAsync(d.error, 0, sizeof(int32_t), S.stream));
+  mind::Clock clock{0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(d.clock, &clock, sizeof(clock), hipMemcpyHostToDevice, S.stream));
+  HIP_TRY(hipMemsetAsync(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots, S.stream));
+  for (size_t gi = 0; gi < S.groups_host.size(); ++gi)
+    HIP_TRY(hipMemcpyAsync(S.groups_host[gi].data, S.group_initial[gi],
+                           sizeof(double) * S.L.groups[gi].data.size(), hipMemcpyDeviceToDevice, S.stream));
+  hipLaunchKernelGGL(mind::k_init_nodes, dim3(mind::blocks_for(std::max(d.nodes, d.ndet), 256)), dim3(256), 0,
+                     S.stream, d, S.v_init);
+  for (int gi = 0; gi < d.ngroups; ++gi) {
+    if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
+    hipLaunchKernelGGL(mind::k_init_group, dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0,
+                       S.stream, d, gi);
+  }
+  if (d.ns_rows > 0) {
+    hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
+                       S.ns_logical, mind::kNetStimDigest);
+    // the netstim sequence counter continues after the INITIAL insertions
+    mind::Clock c2{0, 0, (uint64_t)d.ns_rows};
+    HIP_TRY(hipMemcpyAsync(d.clock, &c2, sizeof(c2), hipMemcpyHostToDevice, S.stream));
+  }
+  // Spikes of the time-zero delivery are kept for the first run (step.jl:47-49).
+  d.record_spikes = 1;
+  HIP_TRY(hipMemsetAsync(d.spike_count, 0, sizeof(unsigned long long), S.stream));
+  // time-zero delivery (NetStim events at t <= 0), then currents at t = 0
+  hipLaunchKernelGGL(mind::k_exchange, dim3(1), dim3(mind::kBlock), 0, S.stream, d, 0);
+  for (int gi = 0; gi < d.ngroups; ++gi) {
+    if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
+    hipLaunchKernelGGL(mind::k_currents_fields, dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0,
+                       S.stream, d, gi, 0.0);
+  }
+  HIP_TRY(hipGetLastError());
+  d.record_spikes = 0;
+  int rc = mind::check_device_error(S);
+  if (rc) return rc;
+  S.initialized = true;
+  return 0;
+}
+
+static int resolve_records(State& S, const mind_record* recs, int64_t n, std::vector<int32_t>& kind,
+                           std::vector<int32_t>& group, std::vector<int32_t>& field, std::vector<int64_t>& index,
+                           std::vector<int32_t>& cell) {
+  const mind::Layout& L = S.L;
+  for (int64_t q = 0; q < n; ++q) {
+    const mind_record& r = recs[q];
+    if (r.kind == 0) {
+      if (r.cell < 0 || r.cell >= L.ncells) return fail(MIND_EARG, "record on an unknown cell");
+      kind.push_back(0); group.push_back(-1); field.push_back(-1);
+      index.push_back(L.location_node(r.cell, r.section, r.x));
+      cell.push_back(r.cell);
+    } else if (r.kind == 1) {
+      if (r.point < 0 || r.point >= (int32_t)L.point_group.size()) return fail(MIND_EARG, "unknown point process");
+      if (L.point_cell[r.point] < 0)
+        return fail(MIND_EUNSUPPORTED, "records of artificial-cell fields are not supported on the device path");
+      const int g = L.point_group[r.point];
+      if (r.field < 0 || r.field >= mind::field_count(L.groups[g].mech)) return fail(MIND_EARG, "unknown field");
+      kind.push_back(1); group.push_back(g); field.push_back(r.field);
+      index.push_back(L.point_row[r.point]);
+      cell.push_back(L.point_cell[r.point]);
+    } else if (r.kind == 2) {
+      if (r.mech < 0 || r.mech >= MIND_MECH_COUNT || L.group_of_mech[r.mech] < 0)
+        return fail(MIND_EARG, "mechanism not in model");
+      const int g = L.group_of_mech[r.mech];
+      const int64_t node = L.range_node(r.cell, r.section, r.x);
+      const std::vector<int32_t>& nodes = L.groups[g].node;
+      auto it = std::lower_bound(nodes.begin(), nodes.end(), (int32_t)node);
+      if (it == nodes.end() || *it != node)
+        return fail(MIND_EARG, "location must contain exactly one matching density mechanism");
+      if (r.field < 0 || r.field >= mind::field_count(r.mech)) return fail(MIND_EARG, "unknown field");
+      kind.push_back(1); group.push_back(g); field.push_back(r.field);
+      index.push_back(it - nodes.begin());
+      cell.push_back(r.cell);
+    } else {
+      return fail(MIND_EARG, "unknown record kind");
+    }
+  }
+  return 0;
+}
+
+int mind_run(mind_state* state, double stop, const mind_record* records, int64_t n_records,
+             int32_t record_spikes, mind_result* out) {
+  if (!state || !out) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  mind::Dev& d = S.d;
+  HIP_TRY(hipSetDevice(S.device));
+  if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  const double dt = S.L.dt;
+  const double now = (double)S.step * dt;
+  const double duration = stop - now;
+  if (!std::isfinite(duration)) return fail(MIND_EARG, "stop time must be finite");
+  const int64_t count = (int64_t)nearbyint(duration / dt);
+  const double big = std::max(std::max(fabs(stop), fabs(now)), dt);
+  const double tol = 64.0 * (nextafter(big, INFINITY) - big);
+  if (!(count >= 0 && fabs(duration - (double)count * dt) <= tol))
+    return fail(MIND_EARG, "stop time must be an integer number of steps after the current time");
+  // records
+  std::vector<int32_t> kind, group, field, cell;
+  std::vector<int64_t> index;
+  int rc = resolve_records(S, records, n_records, kind, group, field, index, cell);
+  if (rc) return rc;
+  for (void* p : S.run_allocations) hipFree(p);
+  S.run_allocations.clear();
+  auto run_upload = [&](auto** dst, const auto& host) -> int {
+    using T = typename std::remove_reference<decltype(host)>::type::value_type;
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(1, host.size()) * sizeof(T)) != hipSuccess)
+      return fail(MIND_EOOM, "device allocation failed");
+    S.run_allocations.push_back(p);
+    if (!host.empty() && hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(MIND_EHIP, "record upload failed");
+    *dst = static_cast<T*>(p);
+    return 0;
+  };
+  const int64_t samples = count + 1;
+  std::vector<int64_t> cell_rec_off(S.L.ncells + 1, 0);
+  std::vector<int32_t> cell_rec(n_records);
+  for (int64_t q = 0; q < n_records; ++q) cell_rec_off[cell[q] + 1] += 1;
+  for (int c = 0; c < S.L.ncells; ++c) cell_rec_off[c + 1] += cell_rec_off[c];
+  {
+    std::vector<int64_t> cur(cell_rec_off.begin(), cell_rec_off.end() - 1);
+    for (int64_t q = 0; q < n_records; ++q) cell_rec[cur[cell[q]]++] = (int32_t)q;
+  }
+  {
+    int32_t* p32; int64_t* p64;
+    if ((rc = run_upload(&p32, kind))) return rc; d.rec_kind = p32;
+    if ((rc = run_upload(&p32, group))) return rc; d.rec_group = p32;
+    if ((rc = run_upload(&p32, field))) return rc; d.rec_field = p32;
+    if ((rc = run_upload(&p64, index))) return rc; d.rec_index = p64;
+    if ((rc = run_upload(&p64, cell_rec_off))) return rc; d.cell_rec_off = p64;
+    if ((rc = run_upload(&p32, cell_rec))) return rc; d.cell_rec = p32;
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<int64_t>(1, n_records * samples) * sizeof(double)) != hipSuccess)
+      return fail(MIND_EOOM, "device allocation failed");
+    S.run_allocations.push_back(p);
+    d.rec_buf = static_cast<double*>(p);
+  }
+  d.nrec = n_records;
+  d.rec_stride = samples;
+  d.rec_start = S.step;
+  d.record_spikes = record_spikes ? 1 : 0;
+  if (S.step != 0 || !record_spikes)
+    HIP_TRY(hipMemsetAsync(d.spike_count, 0, sizeof(unsigned long long), S.stream));
+  if (S.graph) {  // kernel arguments changed: rebuild the window graph
+    hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+  }
+  if (n_records)
+    hipLaunchKernelGGL(mind::k_record_sample, dim3(mind::blocks_for(n_records, 256)), dim3(256), 0, S.stream, d,
+                       (int64_t)0);
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t left = count;
+  while (left > 0) {
+    const int32_t n = (int32_t)std::min<int64_t>(left, S.wmax);
+    if (n == S.wmax) {
+      if (!S.graph) {
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+        rc = mind::launch_window(S, n);
+        hipError_t e = hipStreamEndCapture(S.stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(MIND_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIP_TRY(hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+      }
+      HIP_TRY(hipGraphLaunch(S.graph, S.stream));
+    } else {
+      rc = mind::launch_window(S, n);
+      if (rc) return rc;
+    }
+    left -= n;
+    S.step += n;
+  }
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  rc = mind::check_device_error(S);
+  d.record_spikes = 0;
+  if (rc) return rc;
+  if (S.graph) {  // the graph holds this run's record buffers
+    hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+  }
+  // results
+  out->n_samples = samples;
+  out->time = (double*)malloc(sizeof(double) * samples);
+  for (int64_t j = 0; j < samples; ++j) out->time[j] = (double)(d.rec_start + j) * dt;
+  out->n_records = n_records;
+  out->traces = (double*)malloc(sizeof(double) * std::max<int64_t>(1, n_records * samples));
+  if (n_records)
+    HIP_TRY(hipMemcpy(out->traces, d.rec_buf, sizeof(double) * n_records * samples, hipMemcpyDeviceToHost));
+  unsigned long long ns = 0;
+  HIP_TRY(hipMemcpy(&ns, d.spike_count, sizeof(ns), hipMemcpyDeviceToHost));
+  if ((int64_t)ns > d.spike_cap) ns = d.spike_cap;
+  std::vector<double> st(ns);
+  std::vector<int64_t> ss(ns);
+  if (ns) {
+    HIP_TRY(hipMemcpy(st.data(), d.spike_t, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ss.data(), d.spike_sid, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
+  }
+  std::vector<int64_t> order(ns);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return st[a] < st[b] || (st[a] == st[b] && ss[a] < ss[b]);
+  });
+  out->n_spikes = (int64_t)ns;
+  out->spike_times = (double*)malloc(sizeof(double) * std::max<unsigned long long>(1, ns));
+  out->spike_sids = (int64_t*)malloc(sizeof(int64_t) * std::max<unsigned long long>(1, ns));
+  for (unsigned long long i = 0; i < ns; ++i) {
+    out->spike_times[i] = st[order[i]];
+    out->spike_sids[i] = ss[order[i]];
+  }
+  out->run_seconds = std::chrono::duration<double>(t1 - t0).count();
+  return 0;
+}
+
+int mind_step(mind_state* state) {
+  State& S = *reinterpret_cast<State*>(state);
+  HIP_TRY(hipSetDevice(S.device));
+  if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  S.d.nrec = 0;
+  S.d.record_spikes = 0;
+  if (S.graph) {
+    hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+  }
+  int rc = mind::launch_window(S, 1);
+  if (rc) return rc;
+  S.step += 1;
+  return mind::check_device_error(S);
+}
+
+void mind_result_free(mind_result* r) {
+  if (!r) return;
+  free(r->time);
+  free(r->traces);
+  free(r->spike_times);
+  free(r->spike_sids);
+  memset(r, 0, sizeof(*r));
+}
+
+int mind_node_count(const mind_state* state, int64_t* nodes, int64_t* compartments) {
+  const State& S = *reinterpret_cast<const State*>(state);
+  *nodes = S.L.nodes;
+  *compartments = S.L.compartments;
+  return 0;
+}
+
+int mind_read_voltage(mind_state* state, double* out, int64_t n) {
+  State& S = *reinterpret_cast<State*>(state);
+  if (n != S.L.nodes) return fail(MIND_EARG, "size mismatch");
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  HIP_TRY(hipMemcpy(out, S.d.v, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows) {
+  const State& S = *reinterpret_cast<const State*>(state);
+  if (mech < 0 || mech >= MIND_MECH_COUNT || S.L.group_of_mech[mech] < 0) return fail(MIND_EARG, "mechanism not in model");
+  *rows = S.L.groups[S.L.group_of_mech[mech]].rows;
+  return 0;
+}
+
+int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out, int64_t n) {
+  State& S = *reinterpret_cast<State*>(state);
+  if (mech < 0 || mech >= MIND_MECH_COUNT || S.L.group_of_mech[mech] < 0) return fail(MIND_EARG, "mechanism not in model");
+  const int g = S.L.group_of_mech[mech];
+  const int64_t rows = S.L.groups[g].rows;
+  if (n != rows || field < 0 || field >= mind::field_count(mech)) return fail(MIND_EARG, "size or field mismatch");
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  HIP_TRY(hipMemcpy(out, S.groups_host[g].data + (size_t)field * rows, sizeof(double) * rows, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mind_time(const mind_state* state, double* t, int64_t* step) {
+  const State& S = *reinterpret_cast<const State*>(state);
+  *step = S.step;
+  *t = (double)S.step * S.L.dt;
+  return 0;
+}
+
+int mind_window_steps(const mind_state* state, int32_t* steps) {
+  *steps = reinterpret_cast<const State*>(state)->wmax;
+  return 0;
+}
+
+}  // extern "C"

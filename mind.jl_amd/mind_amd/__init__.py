@@ -1,0 +1,13 @@
+"""Host layer of the MI355X-native MIND simulation step.
+
+Mirrors the reference's model-building API (MicroModel, Section,
+add_population, insert, add_pointprocess, register_spike_source, connect,
+build, initialize, run, step, *_record) over libmind's C ABI.
+"""
+from .mechanisms import Mechanism, builtin  # noqa: F401
+from .model import (  # noqa: F401
+    Cell, MicroModel, Morphology, PointProcess, Section, SimulationResult, SpikeSource,
+    add_pointprocess, add_population, build, connect, destroy, initialize, insert,
+    mechanism_record, read_field, read_voltage, register_spike_source, run, step,
+    voltage_record,
+)

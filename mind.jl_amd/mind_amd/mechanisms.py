@@ -1,0 +1,58 @@
+"""Builtin mechanism table: names, ids and field columns.
+
+Mirrors the reference's `builtin(:name)` (src/compiler/frontend.jl:58-72) for
+the six mechanisms of the simulation step. Field order is the column order of
+the packed mechanism data (include/mind.h).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+
+@dataclass(frozen=True)
+class Mechanism:
+    name: str
+    id: int
+    kind: str  # "density", "point" or "artificial"
+    fields: tuple
+    has_net_receive: bool = False
+    index: dict = field(default_factory=dict, compare=False, hash=False)
+
+    def field_index(self, name: str) -> int:
+        try:
+            return self.fields.index(name)
+        except ValueError:
+            raise ValueError(f"mechanism {self.name} has no field {name}") from None
+
+
+_TABLE = {
+    "hh": Mechanism(
+        "hh", 0, "density",
+        ("gnabar", "gkbar", "gl", "el", "m", "h", "n", "ena", "ek", "gna", "gk", "ina", "ik",
+         "il", "minf", "hinf", "ninf", "mtau", "htau", "ntau"),
+    ),
+    "pas": Mechanism("pas", 1, "density", ("g", "e", "i")),
+    "ExpSyn": Mechanism("ExpSyn", 2, "point", ("tau", "e", "g", "i"), True),
+    "Exp2Syn": Mechanism(
+        "Exp2Syn", 3, "point", ("tau1", "tau2", "e", "A", "B", "g", "i", "factor"), True
+    ),
+    "IClamp": Mechanism("IClamp", 4, "point", ("del", "dur", "amp", "i")),
+    "NetStim": Mechanism(
+        "NetStim", 5, "artificial",
+        ("interval", "number", "start", "noise", "event", "on", "ispike"), True,
+    ),
+}
+
+BY_ID = {m.id: m for m in _TABLE.values()}
+
+
+def builtin(name) -> Mechanism:
+    """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim)."""
+    key = str(name).lstrip(":")
+    if key not in _TABLE:
+        raise ValueError(f"unknown mechanism {key}; load its MOD with load_mod")
+    return _TABLE[key]
+
+
+def resolve(mechanism) -> Mechanism:
+    return mechanism if isinstance(mechanism, Mechanism) else builtin(mechanism)

@@ -1,0 +1,524 @@
+"""Model-building API of MIND.jl's micro layer, flattened into the C ABI.
+
+Names and argument meanings follow the reference (Julia's `!` suffix dropped):
+MicroModel (src/model/micro/model.jl:35-113), Section (morphology.jl:17-79),
+add_population (model.jl:217-283), insert (model.jl:153-169), add_pointprocess
+(model.jl:172-194), register_spike_source (model.jl:286-365, SID rules of
+utils/model.jl:124-224), connect (model.jl:379-400), build (build.jl:120-220),
+initialize (initialize.jl:256-267), run/step (step.jl:4-86) and the record
+selectors (recording.jl:4-76). Model state lives in libmind; this module only
+validates arguments and packs arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .mechanisms import Mechanism, resolve
+
+
+class Section:
+    """A cable section; `child_x` end (0 or 1) attaches at the parent's `parent_x`.
+
+    Length and diameter in um, Ra in ohm*cm, cm in uF/cm2 (morphology.jl:31-79).
+    """
+
+    __slots__ = ("name", "label", "parent", "parent_x", "child_x", "nseg", "length",
+                 "diameter", "points", "Ra", "cm")
+
+    def __init__(self, name, label="dend", parent=None, parent_x=1.0, child_x=0.0, nseg=1,
+                 length=100.0, diameter=1.0, points=(), Ra=100.0, cm=1.0):
+        points = [tuple(map(float, p)) for p in points]
+        if len(points) >= 2:
+            length = float(sum(np.linalg.norm(np.subtract(points[i][:3], points[i - 1][:3]))
+                               for i in range(1, len(points))))
+            diameter = points[0][3]
+        if not (isinstance(nseg, (int, np.integer)) and nseg > 0):
+            raise ValueError("nseg must be a positive integer")
+        if not 0.0 <= parent_x <= 1.0:
+            raise ValueError("parent_x must be in [0, 1]")
+        if child_x not in (0, 1):
+            raise ValueError("child_x must be 0 or 1")
+        if not all(np.isfinite([length, diameter, Ra, cm, parent_x])):
+            raise ValueError("non-finite section parameter")
+        if not (length > 0 and (diameter > 0 or (points and diameter == 0)) and Ra > 0
+                and cm >= 0):
+            raise ValueError("invalid section geometry or biophysics")
+        self.name = str(name)
+        self.label = str(label)
+        self.parent = None if parent is None else str(parent)
+        self.parent_x = float(parent_x)
+        self.child_x = float(child_x)
+        self.nseg = int(nseg)
+        self.length = float(length)
+        self.diameter = float(diameter)
+        self.points = points
+        self.Ra = float(Ra)
+        self.cm = float(cm)
+
+
+class Morphology:
+    """Array form of a section list, shared by every cell of one population."""
+
+    def __init__(self, names, labels, parent, parent_x, child_x, nseg, length, diameter, Ra, cm,
+                 points=None):
+        self.names = list(names)
+        self.labels = list(labels)
+        self.index = {n: i for i, n in enumerate(self.names)}
+        if len(self.index) != len(self.names):
+            raise ValueError("duplicate section name")
+        self.parent = np.asarray(parent, np.int32)
+        self.parent_x = np.asarray(parent_x, np.float64)
+        self.child_x = np.asarray(child_x, np.float64)
+        self.nseg = np.asarray(nseg, np.int32)
+        self.length = np.asarray(length, np.float64)
+        self.diameter = np.asarray(diameter, np.float64)
+        self.Ra = np.asarray(Ra, np.float64)
+        self.cm = np.asarray(cm, np.float64)
+        self.points = points  # None or list of per-section point lists
+        if np.any(self.nseg <= 0):
+            raise ValueError("nseg must be a positive integer")
+
+    @classmethod
+    def from_sections(cls, sections):
+        sections = list(sections)
+        names = [s.name for s in sections]
+        index = {n: i for i, n in enumerate(names)}
+        parent = []
+        for s in sections:
+            if s.parent is None:
+                parent.append(-1)
+            elif s.parent not in index:
+                raise ValueError(f"unknown parent section {s.parent}")
+            else:
+                parent.append(index[s.parent])
+        points = [s.points for s in sections] if any(len(s.points) >= 2 for s in sections) else None
+        return cls(names, [s.label for s in sections], parent, [s.parent_x for s in sections],
+                   [s.child_x for s in sections], [s.nseg for s in sections],
+                   [s.length for s in sections], [s.diameter for s in sections],
+                   [s.Ra for s in sections], [s.cm for s in sections], points)
+
+    def select(self, sections):
+        """Section indices for :all, a label, a name or a list of names."""
+        if isinstance(sections, str):
+            key = sections.lstrip(":")
+            if key == "all":
+                return list(range(len(self.names)))
+            if key in self.index:
+                return [self.index[key]]
+            picked = [i for i, label in enumerate(self.labels) if label == key]
+            if not picked:
+                raise ValueError(f"unknown section group {key}")
+            return picked
+        return [self.section_index(s) for s in sections]
+
+    def section_index(self, name) -> int:
+        key = str(name).lstrip(":")
+        if key not in self.index:
+            raise ValueError(f"unknown section {key}")
+        return self.index[key]
+
+
+@dataclass
+class Cell:
+    model: "MicroModel"
+    index: int
+    gid: int
+
+
+@dataclass
+class PointProcess:
+    model: "MicroModel"
+    index: int
+
+
+@dataclass
+class SpikeSource:
+    model: "MicroModel"
+    index: int
+    sid: int
+
+
+@dataclass
+class SimulationResult:
+    """SimulationResult (recording.jl:78-87) for micro runs."""
+
+    time: np.ndarray
+    traces: dict
+    spike_times: np.ndarray
+    spike_sources: np.ndarray
+    run_seconds: float = 0.0
+
+
+class MicroModel:
+    def __init__(self, celsius=6.3):
+        if not (np.isfinite(celsius) and celsius > -273.15):
+            raise ValueError("invalid temperature")
+        self.celsius = float(celsius)
+        self.morphologies: list[Morphology] = []
+        self.cell_morph: list[int] = []
+        self.cell_gid: list[int] = []
+        self.cell_voltage: list[float] = []
+        self.gids: set = set()
+        # density insertions: (cell, section, mech id, params)
+        self.density: list = []
+        self._density_seen: set = set()
+        # points: (mech, cell, section, x, params)
+        self.points: list = []
+        # sources: (kind, cell, section, x, threshold, point, sid)
+        self.sources: list = []
+        self.source_indices: dict = {}
+        self.point_sources: dict = {}
+        self.variable_sources: dict = {}
+        self.next_source_id = 0
+        # connections: (source, target, weight, delay)
+        self.connections: list = []
+        self.state = None
+        self.options = None
+
+    # -- cells ---------------------------------------------------------------
+    def add_population(self, sections, count, first_gid=None, voltage=-65.0):
+        """Create `count` cells sharing one morphology (model.jl:217-283)."""
+        if count < 0:
+            raise ValueError("population size must be nonnegative")
+        morph = sections if isinstance(sections, Morphology) else Morphology.from_sections(sections)
+        first_gid = len(self.cell_gid) if first_gid is None else int(first_gid)
+        first_cell = len(self.cell_gid)
+        if count == 0:
+            return []
+        self.morphologies.append(morph)
+        m = len(self.morphologies) - 1
+        cells = []
+        for i in range(count):
+            gid = first_gid + i
+            if gid in self.gids:
+                raise ValueError(f"duplicate cell gid {gid}")
+            value = voltage(i + 1) if callable(voltage) else voltage
+            if not np.isfinite(value):
+                raise ValueError("non-finite initial voltage")
+            self.gids.add(gid)
+            self.cell_morph.append(m)
+            self.cell_gid.append(gid)
+            self.cell_voltage.append(float(value))
+            cells.append(Cell(self, first_cell + i, gid))
+        return cells
+
+    def _morph(self, cell: Cell) -> Morphology:
+        return self.morphologies[self.cell_morph[cell.index]]
+
+    # -- mechanisms ----------------------------------------------------------
+    def _params(self, mech: Mechanism, params):
+        items = params.items() if isinstance(params, dict) else params
+        return [(mech.field_index(str(k)), float(v)) for k, v in items]
+
+    def insert(self, cell: Cell, mechanism, sections="all", params=None):
+        """Insert a density mechanism where absent (model.jl:153-169)."""
+        mech = resolve(mechanism)
+        if mech.kind != "density":
+            raise ValueError("cell insertion requires a distributed micro mechanism")
+        pv = self._params(mech, params or {})
+        for s in self._morph(cell).select(sections):
+            key = (cell.index, s, mech.id)
+            if key in self._density_seen:
+                continue
+            self._density_seen.add(key)
+            self.density.append((cell.index, s, mech.id, pv))
+        return cell
+
+    def add_pointprocess(self, target, mechanism, section="soma", x=0.5, params=None):
+        """Create a point process on a cell, or an artificial cell on the model."""
+        mech = resolve(mechanism)
+        if isinstance(target, MicroModel):
+            if mech.kind != "artificial":
+                raise ValueError(f"{mech.name} needs a cell location")
+            cell, sec, x = -1, -1, 0.0
+        else:
+            if mech.kind == "density":
+                raise ValueError("point creation requires a point process")
+            cell = target.index
+            sec = self._morph(target).section_index(section)
+            if not 0.0 <= x <= 1.0:
+                raise ValueError("section location must be in [0, 1]")
+        self.points.append((mech.id, cell, sec, float(x), self._params(mech, params or {})))
+        return PointProcess(self, len(self.points) - 1)
+
+    # -- sources and connections --------------------------------------------
+    def _new_source(self, entry, sid, key):
+        identity = self.next_source_id if sid is None else int(sid)
+        if identity < 0:
+            raise ValueError("spike-source IDs must be nonnegative")
+        if identity in self.source_indices:
+            raise ValueError(f"duplicate spike-source ID {identity}")
+        self.sources.append(entry + (identity,))
+        index = len(self.sources) - 1
+        self.source_indices[identity] = index
+        if key is not None:
+            self.variable_sources[key] = index
+        self.next_source_id = max(self.next_source_id, identity + 1)
+        return SpikeSource(self, index, identity)
+
+    def register_spike_source(self, target, sid=None, section="soma", x=0.5, threshold=10.0):
+        """Voltage detector on a cell, or the net_event output of a point process."""
+        if isinstance(target, PointProcess):
+            existing = self.point_sources.get(target.index)
+            if existing is not None:
+                return SpikeSource(self, existing, self.sources[existing][-1])
+            src = self._new_source((1, -1, -1, 0.0, 10.0, target.index), sid, None)
+            self.point_sources[target.index] = src.index
+            return src
+        if not (0.0 <= x <= 1.0 and np.isfinite(threshold)):
+            raise ValueError("invalid spike detector")
+        morph = self._morph(target)
+        s = morph.section_index(section)
+        # endpoint aliases follow the parent attachment (utils/model.jl:171-190)
+        name_idx, coord = s, float(x)
+        while coord == morph.child_x[name_idx] and morph.parent[name_idx] >= 0:
+            name_idx, coord = int(morph.parent[name_idx]), float(morph.parent_x[name_idx])
+        nseg = int(morph.nseg[name_idx])
+        segment = (0 if coord == 0.0 else nseg + 1) if coord in (0.0, 1.0) else \
+            min(int(np.floor(coord * nseg)) + 1, nseg)
+        key = ("voltage", target.index, name_idx, segment)
+        existing = self.variable_sources.get(key)
+        if existing is not None:
+            return SpikeSource(self, existing, self.sources[existing][-1])
+        return self._new_source((0, target.index, s, float(x), float(threshold), -1), sid, key)
+
+    def connect(self, source, target: PointProcess, weight=1.0, delay=0.0):
+        """One NetCon from a source (or a point's net_event) to a point process."""
+        from .mechanisms import BY_ID
+        mech = BY_ID[self.points[target.index][0]]
+        if not mech.has_net_receive:
+            raise ValueError("target has no NET_RECEIVE block")
+        if not (np.isfinite(weight) and np.isfinite(delay) and delay >= 0):
+            raise ValueError("connection weight and delay must be finite, delay nonnegative")
+        if isinstance(source, PointProcess):
+            source = self.register_spike_source(source)
+        self.connections.append((source.index, target.index, float(weight), float(delay)))
+        return len(self.connections)
+
+    # -- flattening ----------------------------------------------------------
+    def descriptor(self):
+        """Flatten into a mind_model_desc; returns (desc, keepalive)."""
+        keep = {}
+
+        def arr(name, values, dtype):
+            a = np.ascontiguousarray(np.asarray(values, dtype=dtype))
+            if a.size == 0:
+                a = np.zeros(1, dtype=dtype)
+            keep[name] = a
+            return a
+
+        morphs = self.morphologies
+        counts = [len(m.names) for m in morphs]
+        offsets = np.zeros(len(morphs) + 1, np.int64)
+        offsets[1:] = np.cumsum(counts) if counts else []
+
+        def cat(attr, dtype):
+            if not morphs:
+                return np.zeros(0, dtype)
+            return np.concatenate([getattr(m, attr) for m in morphs]).astype(dtype)
+
+        d = _abi.ModelDesc()
+        d.celsius = self.celsius
+        d.n_morphologies = len(morphs)
+        d.morph_offsets = _abi.as_ptr(arr("mo", offsets, np.int64), C.c_int64)
+        d.sec_parent = _abi.as_ptr(arr("sp", cat("parent", np.int32), np.int32), C.c_int32)
+        d.sec_parent_x = _abi.as_ptr(arr("spx", cat("parent_x", np.float64), np.float64), C.c_double)
+        d.sec_child_x = _abi.as_ptr(arr("scx", cat("child_x", np.float64), np.float64), C.c_double)
+        d.sec_nseg = _abi.as_ptr(arr("sn", cat("nseg", np.int32), np.int32), C.c_int32)
+        d.sec_length = _abi.as_ptr(arr("sl", cat("length", np.float64), np.float64), C.c_double)
+        d.sec_diameter = _abi.as_ptr(arr("sd", cat("diameter", np.float64), np.float64), C.c_double)
+        d.sec_Ra = _abi.as_ptr(arr("sra", cat("Ra", np.float64), np.float64), C.c_double)
+        d.sec_cm = _abi.as_ptr(arr("scm", cat("cm", np.float64), np.float64), C.c_double)
+        if any(m.points is not None for m in morphs):
+            pts, poff = [], [0]
+            for m in morphs:
+                for i in range(len(m.names)):
+                    p = m.points[i] if m.points is not None else []
+                    p = p if len(p) >= 2 else []
+                    pts.extend(p)
+                    poff.append(poff[-1] + len(p))
+            d.sec_point_offsets = _abi.as_ptr(arr("po", poff, np.int64), C.c_int64)
+            d.points = _abi.as_ptr(arr("pts", np.asarray(pts, np.float64).reshape(-1), np.float64),
+                                   C.c_double)
+        d.n_cells = len(self.cell_gid)
+        d.cell_morphology = _abi.as_ptr(arr("cm_", self.cell_morph, np.int32), C.c_int32)
+        d.cell_gid = _abi.as_ptr(arr("cg", self.cell_gid, np.int64), C.c_int64)
+        d.cell_voltage = _abi.as_ptr(arr("cv", self.cell_voltage, np.float64), C.c_double)
+
+        pf, pv = [], []
+
+        def params_csr(entries, getter):
+            off = [0]
+            for e in entries:
+                for f, v in getter(e):
+                    pf.append(f)
+                    pv.append(v)
+                off.append(len(pf))
+            return off
+
+        d.n_density = len(self.density)
+        d.dens_cell = _abi.as_ptr(arr("dc", [e[0] for e in self.density], np.int32), C.c_int32)
+        d.dens_section = _abi.as_ptr(arr("ds", [e[1] for e in self.density], np.int32), C.c_int32)
+        d.dens_mech = _abi.as_ptr(arr("dm", [e[2] for e in self.density], np.int32), C.c_int32)
+        d.dens_param_offsets = _abi.as_ptr(
+            arr("dpo", params_csr(self.density, lambda e: e[3]), np.int64), C.c_int64)
+        d.n_points = len(self.points)
+        d.pt_mech = _abi.as_ptr(arr("pm", [e[0] for e in self.points], np.int32), C.c_int32)
+        d.pt_cell = _abi.as_ptr(arr("pc", [e[1] for e in self.points], np.int32), C.c_int32)
+        d.pt_section = _abi.as_ptr(arr("ps", [e[2] for e in self.points], np.int32), C.c_int32)
+        d.pt_x = _abi.as_ptr(arr("px", [e[3] for e in self.points], np.float64), C.c_double)
+        d.pt_param_offsets = _abi.as_ptr(
+            arr("ppo", params_csr(self.points, lambda e: e[4]), np.int64), C.c_int64)
+        d.param_field = _abi.as_ptr(arr("pf", pf, np.int32), C.c_int32)
+        d.param_value = _abi.as_ptr(arr("pv", pv, np.float64), C.c_double)
+        S = self.sources
+        d.n_sources = len(S)
+        d.src_kind = _abi.as_ptr(arr("sk", [e[0] for e in S], np.int32), C.c_int32)
+        d.src_cell = _abi.as_ptr(arr("sc", [e[1] for e in S], np.int32), C.c_int32)
+        d.src_section = _abi.as_ptr(arr("ss", [e[2] for e in S], np.int32), C.c_int32)
+        d.src_x = _abi.as_ptr(arr("sx", [e[3] for e in S], np.float64), C.c_double)
+        d.src_threshold = _abi.as_ptr(arr("st", [e[4] for e in S], np.float64), C.c_double)
+        d.src_point = _abi.as_ptr(arr("spt", [e[5] for e in S], np.int32), C.c_int32)
+        d.src_sid = _abi.as_ptr(arr("ssid", [e[6] for e in S], np.int64), C.c_int64)
+        K = self.connections
+        d.n_connections = len(K)
+        d.con_source = _abi.as_ptr(arr("ks", [e[0] for e in K], np.int32), C.c_int32)
+        d.con_target = _abi.as_ptr(arr("kt", [e[1] for e in K], np.int32), C.c_int32)
+        d.con_weight = _abi.as_ptr(arr("kw", [e[2] for e in K], np.float64), C.c_double)
+        d.con_delay = _abi.as_ptr(arr("kd", [e[3] for e in K], np.float64), C.c_double)
+        return d, keep
+
+    # -- records -------------------------------------------------------------
+    def record_struct(self, spec):
+        r = _abi.Record()
+        r.kind, r.cell, r.section, r.x, r.point, r.mech, r.field = spec
+        return r
+
+
+# -- module-level API mirroring the reference's exported functions -----------
+def add_population(model, sections, count, **kw):
+    return model.add_population(sections, count, **kw)
+
+
+def insert(cell, mechanism, sections="all", params=None):
+    return cell.model.insert(cell, mechanism, sections, params)
+
+
+def add_pointprocess(target, mechanism, section="soma", x=0.5, params=None):
+    model = target if isinstance(target, MicroModel) else target.model
+    return model.add_pointprocess(target, mechanism, section, x, params)
+
+
+def register_spike_source(target, **kw):
+    return target.model.register_spike_source(target, **kw)
+
+
+def connect(source, target, weight=1.0, delay=0.0):
+    return target.model.connect(source, target, weight, delay)
+
+
+@dataclass
+class RecordSpec:
+    name: str
+    spec: tuple = field(default_factory=tuple)
+
+
+def voltage_record(model, cell, section="soma", x=0.5, name=None):
+    s = model._morph(cell).section_index(section)
+    return RecordSpec(name or f"v_{cell.gid}", (0, cell.index, s, float(x), -1, -1, -1))
+
+
+def mechanism_record(model, target, *args, name=None, section="soma", x=0.5):
+    """mechanism_record(model, point, field) or mechanism_record(model, cell, mech, field)."""
+    from .mechanisms import BY_ID
+    if isinstance(target, PointProcess):
+        (fname,) = args
+        mech = BY_ID[model.points[target.index][0]]
+        return RecordSpec(name or str(fname),
+                          (1, -1, -1, 0.0, target.index, mech.id, mech.field_index(str(fname))))
+    mechanism, fname = args
+    mech = resolve(mechanism)
+    s = model._morph(target).section_index(section)
+    return RecordSpec(name or f"{target.gid}.{fname}",
+                      (2, target.index, s, float(x), -1, mech.id, mech.field_index(str(fname))))
+
+
+def build(model, dt_micro=0.025, seed=1, method="plain", cells_per_group=0, window_steps=0,
+          event_capacity=0, device=0):
+    """Pack the model and allocate device state (build.jl:120-220 + allocate_simulation)."""
+    if not (np.isfinite(dt_micro) and dt_micro > 0):
+        raise ValueError("dt_micro must be positive")
+    L = _abi.lib()
+    if model.state is not None:
+        L.mind_state_destroy(model.state)
+        model.state = None
+    desc, keep = model.descriptor()
+    opts = _abi.BuildOptions()
+    opts.dt = float(dt_micro)
+    opts.seed = int(seed)
+    opts.method = {"plain": 0, "batched": 1}[method] if isinstance(method, str) else int(method)
+    opts.cells_per_group = int(cells_per_group)
+    opts.window_steps = int(window_steps)
+    opts.event_capacity = int(event_capacity)
+    opts.device = int(device)
+    handle = C.c_void_p()
+    _abi.check(L.mind_build(C.byref(desc), C.byref(opts), C.byref(handle)))
+    del keep
+    model.state = handle
+    model.options = opts
+    return model
+
+
+def initialize(model):
+    _abi.check(_abi.lib().mind_initialize(model.state))
+    return model
+
+
+def run(model, stop, records=(), record_spikes=True):
+    L = _abi.lib()
+    recs = (_abi.Record * max(1, len(records)))()
+    for i, r in enumerate(records):
+        recs[i] = model.record_struct(r.spec)
+    res = _abi.Result()
+    _abi.check(L.mind_run(model.state, float(stop), recs, len(records), int(bool(record_spikes)),
+                          C.byref(res)))
+    try:
+        time, traces, st, ss, secs = _abi.result_arrays(res)
+    finally:
+        L.mind_result_free(C.byref(res))
+    return SimulationResult(time, {r.name: traces[i] for i, r in enumerate(records)}, st, ss, secs)
+
+
+def step(model):
+    _abi.check(_abi.lib().mind_step(model.state))
+    return model
+
+
+def read_voltage(model):
+    L = _abi.lib()
+    n, s = C.c_int64(), C.c_int64()
+    _abi.check(L.mind_node_count(model.state, C.byref(n), C.byref(s)))
+    out = np.zeros(n.value)
+    _abi.check(L.mind_read_voltage(model.state, _abi.as_ptr(out, C.c_double), n.value))
+    return out
+
+
+def read_field(model, mechanism, fname):
+    mech = resolve(mechanism)
+    L = _abi.lib()
+    rows = C.c_int64()
+    _abi.check(L.mind_row_count(model.state, mech.id, C.byref(rows)))
+    out = np.zeros(rows.value)
+    _abi.check(L.mind_read_field(model.state, mech.id, mech.field_index(fname),
+                                 _abi.as_ptr(out, C.c_double), rows.value))
+    return out
+
+
+def destroy(model):
+    if model.state is not None:
+        _abi.lib().mind_state_destroy(model.state)
+        model.state = None

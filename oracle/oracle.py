@@ -1,0 +1,182 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to
+check and time the HIP path; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "mind.jl_amd"))
+from mind_amd import _abi  # noqa: E402
+
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle not built ({LIB})")
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        L.oracle_build.restype = C.c_void_p
+        L.oracle_build.argtypes = [P(_abi.ModelDesc), C.c_double, C.c_uint64]
+        L.oracle_free.argtypes = [C.c_void_p]
+        L.oracle_initialize.argtypes = [C.c_void_p]
+        L.oracle_run.argtypes = [C.c_void_p, C.c_double, P(_abi.Record), C.c_int64, C.c_int32,
+                                 P(_abi.Result)]
+        L.oracle_result_free.argtypes = [P(_abi.Result)]
+        L.oracle_read_voltage.argtypes = [C.c_void_p, P(C.c_double), C.c_int64]
+        L.oracle_read_field.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), C.c_int64]
+        L.oracle_node_count.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
+        L.oracle_geometry.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), P(C.c_double),
+                                      P(C.c_double), P(C.c_double)]
+        L.oracle_location_node.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                           C.c_int32, P(C.c_int64)]
+        L.oracle_point_row.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32)]
+        L.oracle_log_deliveries.argtypes = [C.c_void_p, C.c_int32]
+        L.oracle_delivery_log.restype = C.c_int64
+        L.oracle_delivery_log.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), P(C.c_int32),
+                                          C.c_int64]
+        L.oracle_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        L.oracle_uniform_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64,
+                                            P(C.c_double)]
+        L.oracle_hines.argtypes = [C.c_int64, P(C.c_int32), P(C.c_double), P(C.c_double),
+                                   P(C.c_double)]
+        L.oracle_exp.restype = C.c_double
+        L.oracle_exp.argtypes = [C.c_double]
+        L.oracle_log.restype = C.c_double
+        L.oracle_log.argtypes = [C.c_double]
+        L.oracle_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class OracleSim:
+    """The oracle's view of one MicroModel (same descriptor as libmind)."""
+
+    def __init__(self, model, dt=0.025, seed=1):
+        desc, keep = model.descriptor()
+        self.model = model
+        self.dt = dt
+        self.h = lib().oracle_build(C.byref(desc), float(dt), int(seed))
+        del keep
+        if not self.h:
+            raise RuntimeError("oracle build failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_free(self.h)
+            self.h = None
+
+    def initialize(self):
+        rc = lib().oracle_initialize(self.h)
+        if rc:
+            raise RuntimeError(f"oracle initialize failed [{rc}]: {lib().oracle_last_error()}")
+        return self
+
+    def run(self, stop, records=(), record_spikes=True):
+        recs = (_abi.Record * max(1, len(records)))()
+        for i, r in enumerate(records):
+            recs[i] = self.model.record_struct(r.spec)
+        res = _abi.Result()
+        rc = lib().oracle_run(self.h, float(stop), recs, len(records), int(bool(record_spikes)),
+                              C.byref(res))
+        if rc:
+            raise RuntimeError(f"oracle run failed [{rc}]: {lib().oracle_last_error().decode()}")
+        try:
+            time, traces, st, ss, secs = _abi.result_arrays(res)
+        finally:
+            lib().oracle_result_free(C.byref(res))
+        from mind_amd.model import SimulationResult
+        return SimulationResult(time, {r.name: traces[i] for i, r in enumerate(records)}, st, ss,
+                                secs)
+
+    def counts(self):
+        n, s = C.c_int64(), C.c_int64()
+        lib().oracle_node_count(self.h, C.byref(n), C.byref(s))
+        return n.value, s.value
+
+    def voltage(self):
+        n, _ = self.counts()
+        out = np.zeros(n)
+        lib().oracle_read_voltage(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        return out
+
+    def field(self, mech_id, field, rows):
+        out = np.zeros(rows)
+        rc = lib().oracle_read_field(self.h, mech_id, field,
+                                     out.ctypes.data_as(C.POINTER(C.c_double)), rows)
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return out
+
+    def geometry(self):
+        n, _ = self.counts()
+        parent = np.zeros(n, np.int32)
+        area, cap, axial, adiag = (np.zeros(n) for _ in range(4))
+        P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+        lib().oracle_geometry(self.h, P(parent, C.c_int32), P(area, C.c_double),
+                              P(cap, C.c_double), P(axial, C.c_double), P(adiag, C.c_double))
+        return dict(parent=parent, area=area, cap=cap, axial=axial, adiag=adiag)
+
+    def location_node(self, cell, section, x, range_=False):
+        out = C.c_int64()
+        lib().oracle_location_node(self.h, cell, section, float(x), int(range_), C.byref(out))
+        return out.value
+
+    def log_deliveries(self, on=True):
+        lib().oracle_log_deliveries(self.h, int(on))
+
+    def delivery_log(self, cap=1 << 22):
+        t = np.zeros(cap)
+        tk = np.zeros(cap, np.int64)
+        tg = np.zeros(cap, np.int32)
+        n = lib().oracle_delivery_log(self.h, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                      tk.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      tg.ctypes.data_as(C.POINTER(C.c_int32)), cap)
+        n = min(n, cap)
+        return t[:n], tk[:n], tg[:n]
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, o)
+    return list(o)
+
+
+def uniform_stream(seed, logical_id, digest, n):
+    out = np.zeros(n)
+    lib().oracle_uniform_stream(seed, logical_id, digest, n,
+                                out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def hines(parent, g, d, b):
+    parent = np.ascontiguousarray(parent, np.int32)
+    g = np.ascontiguousarray(g, np.float64)
+    d = np.array(d, np.float64)
+    b = np.array(b, np.float64)
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))  # noqa: E731
+    rc = lib().oracle_hines(len(parent), P(parent, C.c_int32), P(g, C.c_double),
+                            P(d, C.c_double), P(b, C.c_double))
+    if rc:
+        raise ZeroDivisionError("singular")
+    return b
+
+
+def exp(x):
+    return lib().oracle_exp(float(x))
+
+
+def log(x):
+    return lib().oracle_log(float(x))
