@@ -198,6 +198,12 @@ int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out,
 int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows);
 int mind_time(const mind_state* state, double* t, int64_t* step);
 int mind_window_steps(const mind_state* state, int32_t* steps);
+/* Batched method: node budget per workgroup, block size and workgroup count chosen by the autotuner. */
+int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int64_t* groups);
+/* Host-only packing query (no device needed): node and compartment counts and,
+ * when the arrays are non-NULL, the packed cable geometry in canonical node order. */
+int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int64_t* compartments,
+                    int32_t* parent, double* area, double* cap, double* axial, double* axial_diag);
 const char* mind_last_error(void);
 
 #ifdef __cplusplus

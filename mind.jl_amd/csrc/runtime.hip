@@ -34,146 +34,10 @@
 #include <string>
 #include <vector>
 
-#include "layout.h"
-#include "mech.h"
+#include "device.h"
+#include "state.h"
 
 namespace mind {
-
-// Event in the delivery ring and in per-cell window lists.
-struct QEvent {
-  double t;
-  uint64_t ticket;
-  int64_t call;  // delivery call: 2k = midpoint of step k, 2k+1 = end of step k
-  int32_t edge;
-  int32_t pad;
-};
-
-struct Clock {
-  int64_t step;
-  uint64_t ticket;  // routed events so far
-  uint64_t ns_seq;  // NetStim self-event insertion order
-};
-
-struct GroupDev {
-  int32_t mech;
-  int32_t point;  // 1 for point processes
-  int64_t rows;
-  double* data;
-  const int32_t* node;
-  const int64_t* cell_rows;
-  const uint8_t* run_start;
-};
-
-struct Emit {  // one spike emission: source index and spike time
-  int64_t key;
-  double t;
-  int32_t source;
-  int32_t pad;
-};
-
-struct Dev {
-  double dt, celsius;
-  uint64_t seed;
-  int32_t ncells;
-  int64_t nodes;
-  const int64_t* cell_off;
-  const int32_t* parent;
-  const double* area;
-  const double* cdt;  // capacitance / dt
-  const double* axial;
-  const double* adiag;
-  double* v;
-  double* rhs;
-  double* diag;
-  const int64_t* child_off;
-  const int32_t* child;
-  int32_t ngroups;
-  const GroupDev* g;  // device array of ngroups descriptors
-  int32_t ns_group;  // group index of NetStim or -1
-  const int32_t* point_group;
-  const int32_t* point_row;
-  const int32_t* point_cell;
-  int64_t ndet;
-  const int32_t* det_node;
-  const double* det_thr;
-  uint8_t* det_above;
-  const int64_t* cell_det_off;
-  const int32_t* cell_det;
-  const int32_t* det_source;
-  uint8_t* det_flags;  // [window step][detector]
-  const int64_t* fanout;
-  const int32_t* edge_target;
-  const double* edge_delay;
-  const double* edge_weight;
-  const int64_t* source_sid;
-  // NetStim rows
-  int64_t ns_rows;
-  Stream* ns_stream;
-  double* ns_t;
-  double* ns_flag;
-  uint64_t* ns_seq;
-  int32_t* ns_edge;
-  int32_t* ns_valid;
-  const int32_t* ns_source;
-  // delivery ring
-  int64_t ring_slots, ring_cap;
-  QEvent* ring;
-  unsigned long long* ring_count;
-  // window lists
-  int64_t win_cap;
-  QEvent* win;
-  int64_t* cell_seg_off;
-  unsigned long long* cell_cursor;
-  // emissions
-  int64_t emit_cap;
-  Emit* emit_det;
-  Emit* emit_ns;
-  Emit* emit_all;
-  int64_t* emit_base;
-  // spike recording
-  int64_t spike_cap;
-  double* spike_t;
-  int64_t* spike_sid;
-  unsigned long long* spike_count;
-  int32_t record_spikes;
-  // value recording
-  int64_t nrec;
-  const int32_t* rec_kind;   // 0 voltage, 1 field
-  const int32_t* rec_group;
-  const int32_t* rec_field;
-  const int64_t* rec_index;  // node or row
-  const int64_t* cell_rec_off;
-  const int32_t* cell_rec;
-  double* rec_buf;
-  int64_t rec_stride;
-  int64_t rec_start;
-  Clock* clock;
-  int32_t* error;
-};
-
-__device__ inline void raise_error(int32_t* error, int32_t code) {
-  // Keep the first, most specific failure (atomic max like the reference's error word).
-  atomicMax(error, code);
-}
-
-__host__ __device__ inline double call_cutoff(int64_t j, double dt) {
-  const int64_t k = j >= 0 ? j / 2 : (j - 1) / 2;  // floor(j / 2)
-  const int64_t phase = j - 2 * k;
-  return phase == 0 ? (double)k * dt + 0.5 * dt : (double)(k + 1) * dt;
-}
-
-// First delivery call j >= jmin whose cutoff reaches t.
-__device__ inline int64_t call_of(double t, int64_t jmin, double dt) {
-  double est = floor(t / dt);
-  int64_t j = jmin;
-  if (est * 2.0 - 2.0 > (double)jmin) j = (int64_t)est * 2 - 2;
-  while (t > call_cutoff(j, dt)) ++j;
-  return j;
-}
-
-__device__ inline int64_t emit_key(int64_t j) { return j < 0 ? -1 : 3 * (j / 2) + 1 + (j & 1); }
-
-__device__ inline Cols cols(const GroupDev& g) { return Cols{g.data, g.rows}; }
 
 // ------------------------------------------------------------ initialization
 __global__ void k_init_nodes(Dev d, const double* v_init) {
@@ -279,8 +143,8 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
   if (c >= d.ncells) return;
   const int64_t k0 = d.clock->step;
   const int64_t n0 = d.cell_off[c], n1 = d.cell_off[c + 1];
-  int64_t seg = d.cell_seg_off[c];
-  const int64_t seg_end = d.cell_seg_off[c + 1];
+  int64_t seg = d.seg_off[d.cell_bucket[c]];
+  const int64_t seg_end = d.seg_off[d.cell_bucket[c] + 1];
   const double dt = d.dt;
   double* v = d.v;
   double* rhs = d.rhs;
@@ -605,36 +469,37 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
   __shared__ int64_t scan[kBlock / 64 + 1];
   const int tid = threadIdx.x;
   const int64_t k1 = d.clock->step;
-  // 1. count events per cell
-  for (int64_t c = tid; c < d.ncells; c += kBlock) d.cell_cursor[c] = 0;
+  const int64_t nb = d.nbuckets;
+  // 1. count events per bucket
+  for (int64_t c = tid; c < nb; c += kBlock) d.seg_cursor[c] = 0;
   __syncthreads();
   for (int s = 0; s < n; ++s) {
     const int64_t b = (k1 + s) % d.ring_slots;
     const int64_t cnt = (int64_t)d.ring_count[b];
     for (int64_t i = tid; i < cnt; i += kBlock) {
       const int32_t cell = d.point_cell[d.edge_target[d.ring[b * d.ring_cap + i].edge]];
-      atomicAdd(&d.cell_cursor[cell], 1ull);
+      atomicAdd(&d.seg_cursor[d.cell_bucket[cell]], 1ull);
     }
   }
   __syncthreads();
-  // 2. exclusive scan over cells
+  // 2. exclusive scan over buckets
   int64_t running = 0;
-  for (int64_t base = 0; base < d.ncells; base += kBlock) {
+  for (int64_t base = 0; base < nb; base += kBlock) {
     const int64_t c = base + tid;
-    const int64_t x = c < d.ncells ? (int64_t)d.cell_cursor[c] : 0;
+    const int64_t x = c < nb ? (int64_t)d.seg_cursor[c] : 0;
     int64_t total;
     const int64_t pre = block_scan(x, scan, &total);
-    if (c < d.ncells) {
-      d.cell_seg_off[c] = running + pre;
-      d.cell_cursor[c] = (unsigned long long)(running + pre);
+    if (c < nb) {
+      d.seg_off[c] = running + pre;
+      d.seg_cursor[c] = (unsigned long long)(running + pre);
     }
     running += total;
   }
-  if (tid == 0) d.cell_seg_off[d.ncells] = running;
+  if (tid == 0) d.seg_off[nb] = running;
   __syncthreads();
   if (running > d.win_cap) {
     if (tid == 0) raise_error(d.error, MIND_EQUEUE);
-    for (int64_t c = tid; c <= d.ncells; c += kBlock) d.cell_seg_off[c] = 0;
+    for (int64_t c = tid; c <= nb; c += kBlock) d.seg_off[c] = 0;
     return;
   }
   // 3. scatter
@@ -644,14 +509,14 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
     for (int64_t i = tid; i < cnt; i += kBlock) {
       const QEvent ev = d.ring[b * d.ring_cap + i];
       const int32_t cell = d.point_cell[d.edge_target[ev.edge]];
-      const unsigned long long pos = atomicAdd(&d.cell_cursor[cell], 1ull);
+      const unsigned long long pos = atomicAdd(&d.seg_cursor[d.cell_bucket[cell]], 1ull);
       d.win[pos] = ev;
     }
   }
   __syncthreads();
-  // 4. deterministic order inside each cell (insertion sort; lists are short)
-  for (int64_t c = tid; c < d.ncells; c += kBlock) {
-    const int64_t a = d.cell_seg_off[c], z = d.cell_seg_off[c + 1];
+  // 4. deterministic order inside each bucket (insertion sort; lists are short)
+  for (int64_t c = tid; c < nb; c += kBlock) {
+    const int64_t a = d.seg_off[c], z = d.seg_off[c + 1];
     for (int64_t i = a + 1; i < z; ++i) {
       const QEvent x = d.win[i];
       int64_t j = i - 1;
@@ -666,10 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
   for (int s = tid; s < n; s += kBlock) d.ring_count[(k1 + s) % d.ring_slots] = 0;
 }
 
-
 // ============================================================ host side
-namespace {
-
 thread_local std::string g_error;
 
 int fail(int code, const std::string& msg) {
@@ -677,11 +539,9 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                         \
-  do {                                                                        \
-    hipError_t _e = (expr);                                                   \
-    if (_e != hipSuccess) return fail(MIND_EHIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
-  } while (0)
+State::~State() { release(); }
+
+namespace {
 
 // Source digest of the NetStim mechanism: the first 16 hex digits of the
 // SHA-256 of the reference's mod/NetStim.mod, which seeds its Random123 key
@@ -690,55 +550,21 @@ constexpr uint64_t kNetStimDigest = 0xa186833b569ff3b2ull;
 
 }  // namespace
 
-struct State {
-  Layout L;
-  Dev d{};
-  int device = 0;
-  int method = MIND_METHOD_PLAIN;
-  hipStream_t stream = nullptr;
-  std::vector<void*> allocations;
-  std::vector<GroupDev> groups_host;
-  std::vector<double*> group_initial;  // parameter snapshot restored by initialize
-  double* v_init = nullptr;
-  uint64_t* ns_logical = nullptr;
-  int32_t wmax = 1;
-  int64_t step = 0;
-  bool initialized = false;
-  hipGraphExec_t graph = nullptr;
-  int32_t graph_steps = 0;
-  // batched kernel plan
-  int64_t batch_groups = 0;
-  int32_t cells_per_group = 0;
-  std::vector<void*> run_allocations;
 
-  ~State() {
-    if (graph) hipGraphExecDestroy(graph);
-    for (void* p : run_allocations) hipFree(p);
-    for (void* p : allocations) hipFree(p);
-    if (stream) hipStreamDestroy(stream);
-  }
 
-  template <typename T>
-  int alloc(T** out, size_t count) {
-    void* p = nullptr;
-    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
-    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MIND_EOOM, "device allocation failed");
-    allocations.push_back(p);
-    *out = static_cast<T*>(p);
-    return 0;
-  }
-
-  template <typename T>
-  int upload(T** out, const std::vector<T>& host) {
-    int rc = alloc(out, host.size());
-    if (rc) return rc;
-    if (!host.empty())
-      HIP_TRY(hipMemcpy(*out, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
-    return 0;
-  }
-};
-
-int launch_window(State& S, int32_t n);
+// Event buckets of the window lists: S.cell_bucket maps each cell to one of nb buckets.
+int set_buckets(State& S, int64_t nb) {
+  Dev& d = S.d;
+  int rc;
+  int32_t* p32;
+  if ((rc = S.upload(&p32, S.cell_bucket))) return rc;
+  d.cell_bucket = p32;
+  d.nbuckets = nb;
+  if ((rc = S.alloc(&d.seg_off, nb + 1))) return rc;
+  if ((rc = S.alloc(&d.seg_cursor, nb + 1))) return rc;
+  HIP_TRY(hipMemset(d.seg_off, 0, sizeof(int64_t) * (nb + 1)));
+  return 0;
+}
 
 namespace {
 
@@ -859,9 +685,10 @@ int setup(State& S, const mind_build_options& o) {
   HIP_TRY(hipMemset(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots));
   d.win_cap = std::min<int64_t>(d.ring_cap * wmax, (int64_t)1 << 26);
   if ((rc = S.alloc(&d.win, d.win_cap))) return rc;
-  if ((rc = S.alloc(&d.cell_seg_off, L.ncells + 1))) return rc;
-  if ((rc = S.alloc(&d.cell_cursor, L.ncells + 1))) return rc;
-  HIP_TRY(hipMemset(d.cell_seg_off, 0, sizeof(int64_t) * (L.ncells + 1)));
+  // event buckets: one per cell (the batched plan regroups them)
+  S.cell_bucket.resize(L.ncells);
+  std::iota(S.cell_bucket.begin(), S.cell_bucket.end(), 0);
+  if ((rc = set_buckets(S, L.ncells))) return rc;
   d.emit_cap = d.ndet * wmax + 64 * d.ns_rows + 1024;
   if ((rc = S.alloc(&d.emit_det, d.emit_cap))) return rc;
   if ((rc = S.alloc(&d.emit_ns, d.emit_cap))) return rc;
@@ -897,9 +724,6 @@ int check_device_error(State& S) {
 }
 
 }  // namespace
-
-int batched_setup(State& S, int32_t cells_per_group);
-int batched_launch(State& S, int32_t n);
 
 int launch_window(State& S, int32_t n) {
   Dev& d = S.d;
@@ -1227,6 +1051,24 @@ int mind_time(const mind_state* state, double* t, int64_t* step) {
   const State& S = *reinterpret_cast<const State*>(state);
   *step = S.step;
   *t = (double)S.step * S.L.dt;
+  return 0;
+}
+
+int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int64_t* compartments,
+                    int32_t* parent, double* area, double* cap, double* axial, double* axial_diag) {
+  if (!model || !nodes || !compartments) return fail(MIND_EARG, "null argument");
+  mind::Layout L;
+  std::string err = mind::pack(*model, dt, 1, L);
+  if (!err.empty()) return fail(MIND_EARG, err);
+  *nodes = L.nodes;
+  *compartments = L.compartments;
+  if (parent) {
+    std::copy(L.parent.begin(), L.parent.end(), parent);
+    std::copy(L.area.begin(), L.area.end(), area);
+    std::copy(L.cap.begin(), L.cap.end(), cap);
+    std::copy(L.axial.begin(), L.axial.end(), axial);
+    std::copy(L.adiag.begin(), L.adiag.end(), axial_diag);
+  }
   return 0;
 }
 

@@ -1,0 +1,80 @@
+// Host-side state of one simulation (owned by the C ABI handle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device.h"
+
+namespace mind {
+
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) return fail(MIND_EHIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+struct BatchPlan;
+struct State;
+void batched_release(State& S);
+
+struct State {
+  Layout L;
+  Dev d{};
+  int device = 0;
+  int method = MIND_METHOD_PLAIN;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocations;
+  std::vector<GroupDev> groups_host;
+  std::vector<double*> group_initial;  // parameter snapshot restored by initialize
+  double* v_init = nullptr;
+  uint64_t* ns_logical = nullptr;
+  int32_t wmax = 1;
+  int64_t step = 0;
+  bool initialized = false;
+  hipGraphExec_t graph = nullptr;
+  int32_t graph_steps = 0;
+  // batched kernel plan (cell groups, chain units and level schedules)
+  BatchPlan* batch = nullptr;
+  std::vector<int32_t> cell_bucket;  // event bucket of each cell: the cell, or its group
+  std::vector<void*> run_allocations;
+
+  ~State();
+  void release() {
+    batched_release(*this);
+    if (graph) hipGraphExecDestroy(graph);
+    for (void* p : run_allocations) hipFree(p);
+    for (void* p : allocations) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MIND_EOOM, "device allocation failed");
+    allocations.push_back(p);
+    *out = static_cast<T*>(p);
+    return 0;
+  }
+
+  template <typename T>
+  int upload(T** out, const std::vector<T>& host) {
+    int rc = alloc(out, host.size());
+    if (rc) return rc;
+    if (!host.empty())
+      HIP_TRY(hipMemcpy(*out, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+  }
+};
+
+
+int launch_window(State& S, int32_t n);
+int batched_setup(State& S, int32_t cells_per_group);
+int batched_launch(State& S, int32_t n);
+
+}  // namespace mind

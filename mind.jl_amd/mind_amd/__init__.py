@@ -7,7 +7,7 @@ build, initialize, run, step, *_record) over libmind's C ABI.
 from .mechanisms import Mechanism, builtin  # noqa: F401
 from .model import (  # noqa: F401
     Cell, MicroModel, Morphology, PointProcess, Section, SimulationResult, SpikeSource,
-    add_pointprocess, add_population, build, connect, destroy, initialize, insert,
+    add_pointprocess, add_population, batch_plan, build, connect, destroy, initialize, insert,
     mechanism_record, read_field, read_voltage, register_spike_source, run, step,
-    voltage_record,
+    voltage_record, window_steps,
 )

@@ -518,6 +518,38 @@ def read_field(model, mechanism, fname):
     return out
 
 
+def pack_nodes(model, dt_micro=0.025):
+    """Packed cable geometry computed by libmind on the host (no device needed)."""
+    L = _abi.lib()
+    desc, keep = model.descriptor()
+    n, s = C.c_int64(), C.c_int64()
+    _abi.check(L.mind_pack_nodes(C.byref(desc), float(dt_micro), C.byref(n), C.byref(s),
+                                 None, None, None, None, None))
+    parent = np.zeros(n.value, np.int32)
+    area, cap, axial, adiag = (np.zeros(n.value) for _ in range(4))
+    _abi.check(L.mind_pack_nodes(C.byref(desc), float(dt_micro), C.byref(n), C.byref(s),
+                                 _abi.as_ptr(parent, C.c_int32), _abi.as_ptr(area, C.c_double),
+                                 _abi.as_ptr(cap, C.c_double), _abi.as_ptr(axial, C.c_double),
+                                 _abi.as_ptr(adiag, C.c_double)))
+    del keep
+    return dict(nodes=n.value, compartments=s.value, parent=parent, area=area, cap=cap,
+                axial=axial, adiag=adiag)
+
+
+def batch_plan(model):
+    """(node budget per workgroup, threads per workgroup, workgroups) of the batched kernel."""
+    L = _abi.lib()
+    b, t, g = C.c_int64(), C.c_int32(), C.c_int64()
+    _abi.check(L.mind_batch_plan(model.state, C.byref(b), C.byref(t), C.byref(g)))
+    return b.value, t.value, g.value
+
+
+def window_steps(model):
+    w = C.c_int32()
+    _abi.check(_abi.lib().mind_window_steps(model.state, C.byref(w)))
+    return w.value
+
+
 def destroy(model):
     if model.state is not None:
         _abi.lib().mind_state_destroy(model.state)

@@ -375,6 +375,7 @@ struct Sim {
   bool log_deliveries = false;
   std::vector<double> log_time;
   std::vector<int64_t> log_order;
+  std::vector<int64_t> log_edge;
   std::vector<int32_t> log_target;
 };
 
@@ -544,6 +545,7 @@ void receive(Sim& s, int gi, int row, const Event& ev) {
   if (s.log_deliveries) {
     s.log_time.push_back(t);
     s.log_order.push_back(ev.order);
+    s.log_edge.push_back(ev.edge);
     s.log_target.push_back(g.target[row] - 1);
   }
   switch (g.mech) {
@@ -1075,13 +1077,16 @@ int oracle_point_row(void* p, int32_t point, int32_t* row) {
 
 void oracle_log_deliveries(void* p, int32_t on) { static_cast<Sim*>(p)->log_deliveries = on != 0; }
 
-// Applied events in delivery order: time, heap ticket and target point.
-int64_t oracle_delivery_log(void* p, double* t, int64_t* ticket, int32_t* target, int64_t cap) {
+// Applied events in delivery order: time, heap ticket, edge (1-based CSR
+// index, 0 for self-events) and target point.
+int64_t oracle_delivery_log(void* p, double* t, int64_t* ticket, int64_t* edge, int32_t* target,
+                            int64_t cap) {
   Sim& s = *static_cast<Sim*>(p);
   const int64_t n = (int64_t)s.log_time.size();
   for (int64_t i = 0; i < n && i < cap; ++i) {
     t[i] = s.log_time[i];
     ticket[i] = s.log_order[i];
+    edge[i] = s.log_edge[i];
     target[i] = s.log_target[i];
   }
   return n;

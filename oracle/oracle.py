@@ -43,8 +43,8 @@ def lib():
         L.oracle_point_row.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32)]
         L.oracle_log_deliveries.argtypes = [C.c_void_p, C.c_int32]
         L.oracle_delivery_log.restype = C.c_int64
-        L.oracle_delivery_log.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), P(C.c_int32),
-                                          C.c_int64]
+        L.oracle_delivery_log.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), P(C.c_int64),
+                                          P(C.c_int32), C.c_int64]
         L.oracle_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         L.oracle_uniform_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64,
                                             P(C.c_double)]
@@ -136,14 +136,17 @@ class OracleSim:
         lib().oracle_log_deliveries(self.h, int(on))
 
     def delivery_log(self, cap=1 << 22):
+        """(time, ticket, edge, target point) of every applied event, in delivery order."""
         t = np.zeros(cap)
         tk = np.zeros(cap, np.int64)
+        ed = np.zeros(cap, np.int64)
         tg = np.zeros(cap, np.int32)
         n = lib().oracle_delivery_log(self.h, t.ctypes.data_as(C.POINTER(C.c_double)),
                                       tk.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      ed.ctypes.data_as(C.POINTER(C.c_int64)),
                                       tg.ctypes.data_as(C.POINTER(C.c_int32)), cap)
         n = min(n, cap)
-        return t[:n], tk[:n], tg[:n]
+        return t[:n], tk[:n], ed[:n], tg[:n]
 
 
 def philox(ctr, key):

@@ -1,0 +1,306 @@
+"""Pins of the CPU oracle against closed forms, published vectors and invariants.
+
+Each test fixes something the reference's own code determines (cited), using a
+source independent of the oracle: mpmath, numpy/scipy solvers, cable theory,
+Random123's published known-answer vectors, NEURON's published hh resting
+values, or orders computed by hand.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+import oracle
+from mind_amd import MicroModel, Section, configs, mechanism_record, voltage_record
+from mind_amd.model import pack_nodes
+
+mp.mp.dps = 60
+
+
+def ulp_error(got, exact):
+    exact_d = float(exact)
+    if exact_d == 0.0:
+        return abs(got)
+    return abs(got - exact_d) / math.ulp(exact_d)
+
+
+# --- elementary functions shared by the kernels and the oracle ---------------
+def test_exp_within_one_ulp():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-745, 709.7, 4000), rng.uniform(-1, 1, 2000),
+                         rng.uniform(-1e-6, 1e-6, 500), [0.0, 1.0, -1.0, 0.5 * math.log(2),
+                                                         700.0, -700.0, 1e-300]])
+    worst = max(ulp_error(oracle.exp(float(x)), mp.exp(mp.mpf(float(x)))) for x in xs)
+    assert worst <= 1.0, worst
+    assert oracle.exp(710.0) == math.inf and oracle.exp(-746.0) == 0.0
+    assert math.isnan(oracle.exp(math.nan))
+
+
+def test_log_within_one_ulp():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([10.0 ** rng.uniform(-300, 300, 4000), rng.uniform(0.5, 2.0, 2000),
+                         1.0 + rng.uniform(-1e-7, 1e-7, 200), [1.0, 2.0, 0.5, 5e-324, 1e308]])
+    worst = max(ulp_error(oracle.log(float(x)), mp.log(mp.mpf(float(x)))) for x in xs)
+    assert worst <= 1.0, worst
+    assert oracle.log(1.0) == 0.0 and oracle.log(0.0) == -math.inf
+    assert math.isnan(oracle.log(-1.0))
+
+
+# --- Random123 Philox4x32-10 (src/runtime/cuda/micro.jl:3-51) ------------------
+KAT = [  # Random123 kat_vectors, philox4x32 10 rounds
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert tuple(oracle.philox(ctr, key)) == expect
+
+
+def test_uniform_stream_layout():
+    """Counter block (ctr>>2, key3, id lo, id hi), key (seed lo, seed hi), (w+1)/(2^32+1)."""
+    seed, lid, digest = 0x1234_5678_9ABC, (7 << 32) | 99, 0xA186833B569FF3B2
+    u = oracle.uniform_stream(seed, lid, digest, 12)
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    for block in range(3):
+        words = oracle.philox((block, digest & 0xFFFFFFFF, lid & 0xFFFFFFFF, lid >> 32), key)
+        expect = [(w + 1.0) * (1.0 / 4294967297.0) for w in words]
+        assert list(u[4 * block:4 * block + 4]) == expect
+
+
+# --- Hines elimination (src/runtime/cuda/micro.jl:688-749) --------------------
+@pytest.mark.parametrize("seed", range(6))
+def test_hines_matches_dense_solve(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 200))
+    parent = np.array([-1] + [int(rng.integers(0, i)) for i in range(1, n)], np.int32)
+    g = rng.uniform(0.01, 5.0, n)
+    g[0] = 0.0
+    A = np.zeros((n, n))
+    for i in range(1, n):
+        A[i, parent[i]] = A[parent[i], i] = -g[i]
+    d = -A.sum(axis=1) + rng.uniform(0.1, 3.0, n)
+    A[np.arange(n), np.arange(n)] = d
+    b = rng.normal(size=n)
+    x = oracle.hines(parent, g, d, b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+
+
+# --- geometry (src/model/micro/morphology.jl:406-484) --------------------------
+def test_cylinder_geometry_closed_form():
+    L, dia, nseg, Ra, cm = 300.0, 3.0, 7, 150.0, 0.9
+    m = MicroModel()
+    m.add_population([Section("a", nseg=nseg, length=L, diameter=dia, Ra=Ra, cm=cm)], 1)
+    geo = oracle.OracleSim(m).geometry()
+    h = L / nseg
+    area = math.pi * dia * h
+    r_half = 0.04 * Ra * (h / 2) / (math.pi * dia * dia)
+    # root, nseg segment nodes, endpoint
+    assert geo["parent"].tolist() == [-1] + list(range(nseg + 1))
+    np.testing.assert_allclose(geo["area"], [100.0] + [area] * nseg + [100.0], rtol=1e-13)
+    np.testing.assert_allclose(geo["cap"], [0.0] + [cm * area * 1e-5] * nseg + [0.0], rtol=1e-13)
+    expect_g = [0.0, 1 / r_half] + [1 / (2 * r_half)] * (nseg - 1) + [1 / r_half]
+    np.testing.assert_allclose(geo["axial"], expect_g, rtol=1e-13)
+    adiag = np.zeros(nseg + 2)
+    for i in range(1, nseg + 2):
+        adiag[i] += expect_g[i]
+        adiag[i - 1] += expect_g[i]
+    np.testing.assert_allclose(geo["adiag"], adiag, rtol=1e-13)
+
+
+def test_tapered_pt3d_geometry():
+    """A frustum: lateral area pi (r1 + r2) s and resistance 4 Ra L/(pi d1 d2)."""
+    d1, d2, L = 4.0, 2.0, 50.0
+    m = MicroModel()
+    m.add_population([Section("t", nseg=1, points=[(0, 0, 0, d1), (L, 0, 0, d2)])], 1)
+    geo = oracle.OracleSim(m).geometry()
+    dm = (d1 + d2) / 2
+    s1 = math.hypot(L / 2, (d1 - dm) / 2)
+    s2 = math.hypot(L / 2, (dm - d2) / 2)
+    area = math.pi * ((d1 + dm) / 2 * s1 + (dm + d2) / 2 * s2)
+    np.testing.assert_allclose(geo["area"][1], area, rtol=1e-13)
+    r_left = 0.04 * 100.0 / math.pi * (L / 2) / (d1 * dm)
+    np.testing.assert_allclose(geo["axial"][1], 1 / r_left, rtol=1e-13)
+
+
+@pytest.mark.parametrize("which", ["single", "ball", "trees"])
+def test_product_packing_matches_oracle(which):
+    if which == "single":
+        m, _ = configs.single_cell()
+    elif which == "ball":
+        m, _ = configs.ball_stick_network(ncells=20, indegree=5)
+    else:
+        m, _ = configs.tree_network(ncells=25, median=150)
+    got = pack_nodes(m)
+    ref = oracle.OracleSim(m).geometry()
+    for k in ("parent", "area", "cap", "axial", "adiag"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+# --- membrane dynamics ---------------------------------------------------------
+def passive_cell(v0=-60.0, nseg=1, length=20.0, diameter=20.0):
+    m = MicroModel()
+    (cell,) = m.add_population([Section("soma", label="soma", nseg=nseg, length=length,
+                                        diameter=diameter)], 1, voltage=v0)
+    m.insert(cell, "pas", params={"g": 1e-4, "e": -65.0})
+    return m, cell
+
+
+def test_isopotential_decay_is_backward_euler():
+    """tau = cm/g = 10 ms; each step contracts v - e by 1/(1 + dt/tau)."""
+    m, cell = passive_cell()
+    rec = [voltage_record(m, cell)]
+    r = oracle.OracleSim(m).initialize().run(50.0, rec)
+    n = np.arange(len(r.time))
+    expect = -65.0 + 5.0 * (1.0 / (1.0 + 0.025 / 10.0)) ** n
+    np.testing.assert_allclose(r.traces["v_0"], expect, atol=1e-9)
+
+
+def test_rest_is_a_fixed_point():
+    m, cell = passive_cell(v0=-65.0, nseg=5, length=200.0, diameter=2.0)
+    sim = oracle.OracleSim(m).initialize()
+    sim.run(10.0)
+    assert np.all(sim.voltage() == -65.0)
+
+
+def test_cable_input_resistance():
+    """Passive soma + sealed finite cable: R_in = 1/(G_soma + tanh(L/lambda)/(r_a lambda))."""
+    m = MicroModel()
+    (cell,) = m.add_population(configs.ball_and_stick_sections(200), 1, voltage=-65.0)
+    m.insert(cell, "pas", params={"g": 1e-4, "e": -65.0})
+    amp = 0.05
+    m.add_pointprocess(cell, "IClamp", params={"amp": amp, "del": 0.0, "dur": 1e9})
+    rec = [voltage_record(m, cell)]
+    r = oracle.OracleSim(m).initialize().run(250.0, rec)
+    dv = r.traces["v_0"][-1] + 65.0
+    Rm, Ra, d, L = 1e4, 100.0, 2e-4, 0.1  # ohm cm2, ohm cm, cm, cm
+    lam = math.sqrt(d * Rm / (4 * Ra))
+    ra = 4 * Ra / (math.pi * d * d)  # ohm / cm
+    g_cable = math.tanh(L / lam) / (ra * lam)
+    g_soma = math.pi * 20e-4 * 20e-4 / Rm
+    r_in_mohm = 1.0 / (g_cable + g_soma) / 1e6
+    np.testing.assert_allclose(dv / amp, r_in_mohm, rtol=2e-3)
+
+
+def test_hh_resting_gates_match_neuron():
+    """NEURON's hh at -65 mV, 6.3 degC: m 0.0529325, h 0.596121, n 0.317677."""
+    m, info = configs.single_cell()
+    sim = oracle.OracleSim(m).initialize()
+    f = {k: sim.field(0, i, 1)[0] for k, i in (("m", 4), ("h", 5), ("n", 6))}
+    assert abs(f["m"] - 0.0529325) < 1e-6
+    assert abs(f["h"] - 0.596121) < 1e-6
+    assert abs(f["n"] - 0.317677) < 1e-6
+
+
+@pytest.mark.parametrize("v", [-90.0, -65.0, -55.0, -40.0, 0.0, 30.0])
+def test_hh_steady_state_against_mpmath(v):
+    """After many steps clamped by a huge passive load the gates sit at x_inf(v)."""
+    vv = mp.mpf(v)
+
+    def vtrap(x, y):
+        return x / (mp.e ** (x / y) - 1) if abs(x / y) >= 1e-6 else y * (1 - x / y / 2)
+
+    am, bm = mp.mpf("0.1") * vtrap(-(vv + 40), 10), 4 * mp.e ** (-(vv + 65) / 18)
+    ah, bh = mp.mpf("0.07") * mp.e ** (-(vv + 65) / 20), 1 / (mp.e ** (-(vv + 35) / 10) + 1)
+    an, bn = mp.mpf("0.01") * vtrap(-(vv + 55), 10), mp.mpf("0.125") * mp.e ** (-(vv + 65) / 80)
+    m = MicroModel()
+    (cell,) = m.add_population([Section("soma", label="soma", length=20.0, diameter=20.0)], 1,
+                               voltage=v)
+    m.insert(cell, "hh", params={"gnabar": 0.0, "gkbar": 0.0, "gl": 0.0})
+    sim = oracle.OracleSim(m).initialize()
+    for i, (a, b) in zip((4, 5, 6), ((am, bm), (ah, bh), (an, bn))):
+        assert abs(sim.field(0, i, 1)[0] - float(a / (a + b))) < 1e-14
+
+
+def test_exp2syn_peak_is_weight_and_decay_is_exact():
+    m, cell = passive_cell(v0=-65.0)
+    syn = m.add_pointprocess(cell, "Exp2Syn", params={"tau1": 0.5, "tau2": 5.0, "e": 0.0})
+    ns = m.add_pointprocess(m, "NetStim", params={"interval": 10.0, "number": 1, "start": 2.0,
+                                                  "noise": 0.0})
+    w = 3e-3
+    m.connect(ns, syn, weight=w, delay=1.0)
+    recs = [mechanism_record(m, syn, "g"), mechanism_record(m, syn, "B", name="B")]
+    r = oracle.OracleSim(m).initialize().run(40.0, recs)
+    g = r.traces["g"]
+    assert abs(g.max() - w) / w < 0.01  # factor normalises the peak to the weight
+    B = r.traces["B"]
+    k = int(np.argmax(B))
+    ratio = B[k + 10:k + 200] / B[k + 9:k + 199]
+    np.testing.assert_allclose(ratio, math.exp(-0.025 / 5.0), rtol=1e-13)
+
+
+def test_netstim_regular_train():
+    m = MicroModel()
+    ns = m.add_pointprocess(m, "NetStim", params={"interval": 7.0, "number": 5, "start": 3.0,
+                                                  "noise": 0.0})
+    m.register_spike_source(ns)
+    r = oracle.OracleSim(m).initialize().run(100.0)
+    assert r.spike_times.tolist() == [3.0, 10.0, 17.0, 24.0, 31.0]
+
+
+def test_netstim_poisson_statistics():
+    from scipy import stats
+
+    m = MicroModel()
+    ns = m.add_pointprocess(m, "NetStim", params={"interval": 2.0, "number": 1e9, "start": 0.0,
+                                                  "noise": 1.0})
+    m.register_spike_source(ns)
+    r = oracle.OracleSim(m).initialize().run(4000.0)
+    isi = np.diff(np.concatenate([[0.0], r.spike_times]))
+    assert abs(isi.mean() - 2.0) < 0.06
+    assert stats.kstest(isi, "expon", args=(0, 2.0)).pvalue > 1e-3
+
+
+def test_spike_detection_rule_matches_trace():
+    """A spike is the first step whose starting v exceeds threshold (time t + 1e-10)."""
+    m, info = configs.single_cell()
+    rec = [voltage_record(m, info["cells"][0])]
+    r = oracle.OracleSim(m).initialize().run(200.0, rec)
+    v = r.traces["v_0"]
+    above = v > -20.0
+    k = np.nonzero(above[:-1] & ~np.concatenate([[False], above[:-2]]))[0]
+    expect = k * 0.025 + 1e-10
+    assert np.array_equal(r.spike_times, expect)
+
+
+def test_equal_time_events_follow_source_then_edge_order():
+    """Two cells crossing threshold at the same step, equal delays into one synapse:
+    the heap tickets put source 0's edges first, each source's in creation order."""
+    m = MicroModel()
+    cells = m.add_population(configs.ball_and_stick_sections(10), 3)
+    srcs = []
+    for c in cells:
+        m.insert(c, "hh", sections="soma")
+        m.insert(c, "pas", sections="dend", params={"g": 1e-4, "e": -65.0})
+        m.add_pointprocess(c, "IClamp", params={"amp": 0.5, "del": 1.0, "dur": 5.0})
+        srcs.append(m.register_spike_source(c, threshold=-20.0))
+    syn = m.add_pointprocess(cells[2], "ExpSyn", section="dend", params={"tau": 2.0})
+    # edges: cell1 -> syn (w 1), cell0 -> syn (w 2), cell0 -> syn (w 3), cell1 -> syn (w 4)
+    for s, w in ((1, 1e-5), (0, 2e-5), (0, 3e-5), (1, 4e-5)):
+        m.connect(srcs[s], syn, weight=w, delay=2.0)
+    sim = oracle.OracleSim(m)
+    sim.log_deliveries(True)
+    sim.initialize()
+    r = sim.run(12.0)
+    t0 = r.spike_times[r.spike_sources == 0][0]
+    t1 = r.spike_times[r.spike_sources == 1][0]
+    assert t0 == r.spike_times[0]
+    assert t0 == t1  # identical cells spike at the same step
+    t, ticket, edge, target = sim.delivery_log()
+    first = t == t[0]
+    assert first.sum() == 4 and np.all(target == 3)
+    # CSR edges by source: source 0 holds creation #2, #3 (edges 1, 2), source 1
+    # holds #1, #4 (edges 3, 4); the heap delivers source 0's edges first.
+    assert edge[first].tolist() == [1, 2, 3, 4]
+    assert np.all(np.diff(ticket[first]) > 0)
+
+
+def test_run_rejects_fractional_stop():
+    m, _ = configs.single_cell()
+    sim = oracle.OracleSim(m).initialize()
+    with pytest.raises(RuntimeError, match="integer number of steps"):
+        sim.run(0.0301)
