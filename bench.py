@@ -1,0 +1,161 @@
+"""Benchmark of the MIND simulation step on one MI355X.
+
+Workload: 5000 seed-generated branched cells (compartment counts lognormal,
+median 400, clipped to 50..4000; about 2.7 M compartments and 5.4 M cable
+nodes), hh soma, pas dendrites, 20 Exp2Syn per cell, in-degree 50 recurrent
+NetCons and a 5 Hz Poisson NetStim per cell, dt 0.025 ms, FP64.
+
+A step is one micro step (dt) of the whole network: detection, delivery,
+nrn_cur, cable solve, nrn_state, recording. `value` is compartment-steps/s of
+the batched kernel with the model resident in HBM (build, autotune and
+initialize are outside the timed region); the plain one-cell-per-thread kernel
+is timed on the same model beside it.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes of its
+launch over its average duration measured with HIP events on the library's
+stream) and the CPU oracle timed on the host on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mind.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Compulsory bytes per step of the micro step's state, touched once:
+# node v r/w 16 + cap 8 + axial 8 + axial diagonal 8 + area 8 + parent 4;
+# pas g, e; hh 4 parameters + m, h, n r/w + ena, ek; Exp2Syn tau1, tau2, e,
+# factor + A, B r/w + node; IClamp; detector state.
+BYTES = {"nodes": 52, "pas": 16, "hh": 96, "exp2syn": 68, "iclamp": 28, "detectors": 2}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cells", type=int, default=5000)
+    ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded path is "
+              "design-only", file=sys.stderr)
+        sys.exit(2)
+
+    from mind_amd import build, configs, initialize, run, destroy
+    from mind_amd.model import (batch_plan, batch_work, kernel_time, set_profiling, synchronize,
+                                window_steps)
+
+    dt = 0.025
+    t0 = time.time()
+    model, info = configs.tree_network(ncells=args.cells)
+    log(f"generated model in {time.time() - t0:.1f}s; compartments {int(info['sizes'].sum())}")
+
+    def timed(method, steps, warmup, profile):
+        t1 = time.time()
+        build(model, dt_micro=dt, method=method)
+        initialize(model)
+        log(f"{method}: build+autotune+initialize {time.time() - t1:.1f}s")
+        if warmup:
+            run(model, warmup * dt)
+        set_profiling(model, profile)
+        synchronize(model)
+        t2 = time.perf_counter()
+        res = run(model, (warmup + steps) * dt)  # returns after its stream has drained
+        synchronize(model)
+        wall = time.perf_counter() - t2
+        kt = kernel_time(model) if profile else None
+        extra = {}
+        if method == "batched":
+            extra["plan"] = batch_plan(model)
+            extra["work"] = batch_work(model)
+        extra["window"] = window_steps(model)
+        extra["spikes"] = int(len(res.spike_times))
+        return wall, kt, extra
+
+    import ctypes as C
+
+    from mind_amd import _abi
+
+    wall, kt, extra = timed("batched", args.steps, args.warmup, True)
+    n, s = C.c_int64(), C.c_int64()
+    _abi.lib().mind_node_count(model.state, C.byref(n), C.byref(s))
+    nodes, comps = n.value, s.value
+    value = comps * args.steps / wall
+    launches, ksteps, kms = kt
+    work = extra["work"]
+    bytes_per_step = sum(BYTES[k] * work[k] for k in BYTES)
+    bytes_per_launch = bytes_per_step * (ksteps / launches)
+    avg_launch_s = kms / launches / 1e3
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    destroy(model)
+
+    plain = None
+    if not args.no_plain:
+        pw, _, pextra = timed("plain", args.plain_steps, 0, False)
+        plain = {"value": comps * args.plain_steps / pw, "steps": args.plain_steps,
+                 "ms_per_step": 1e3 * pw / args.plain_steps}
+        destroy(model)
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        sim = oracle.OracleSim(model, dt=dt).initialize()
+        r = sim.run(args.cpu_steps * dt, record_spikes=False)
+        cpu = {"value": comps * args.cpu_steps / r.run_seconds, "unit": "compartment-steps/s",
+               "cores": 1, "kind": "port",
+               "sample": f"the same {args.cells}-cell model, first {args.cpu_steps} steps "
+                         f"after initialize, oracle (C++ FP64, -O3 -ffp-contract=off), 1 thread"}
+
+    line = {
+        "metric": "compartment-steps/s (batched vs plain); macro sim-s/wall-s surrogate vs plain",
+        "value": value,
+        "unit": "compartment-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.cells} seed-generated branched cells (lognormal median 400 "
+                        "compartments), hh/pas/Exp2Syn, in-degree 50, 5 Hz NetStim; batched "
+                        "window kernel",
+            "cells": args.cells, "compartments": comps, "nodes": nodes, "dt_ms": dt,
+            "window_steps": extra["window"], "plan": extra["plan"],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_batched<0>", "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": kms / launches, "launches": launches,
+        },
+        "plain": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
+        "cpu_baseline": cpu,
+        "sim_seconds_per_wall_second": (args.steps * dt / 1e3) / wall,
+        "spikes_in_timed_run": extra["spikes"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

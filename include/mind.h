@@ -204,6 +204,17 @@ int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, 
  * when the arrays are non-NULL, the packed cable geometry in canonical node order. */
 int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int64_t* compartments,
                     int32_t* parent, double* area, double* cap, double* axial, double* axial_diag);
+/* Number of visible HIP devices (0 when none); no state needed. */
+int mind_device_count(int32_t* count);
+/* Wait until every launch of this state has completed. */
+int mind_synchronize(mind_state* state);
+/* Timing of the dominant cell-kernel launch with HIP events on the state's stream
+ * (batched: the launch of workgroups within the node budget; plain: the cell
+ * kernel). While profiling is on, windows launch without graph replay. */
+int mind_set_profiling(mind_state* state, int32_t on);
+int mind_kernel_time(const mind_state* state, int64_t* launches, int64_t* steps, double* total_ms);
+/* Work of that launch per step: nodes, pas rows, hh rows, Exp2Syn rows, IClamp rows, detectors. */
+int mind_batch_work(const mind_state* state, int64_t* counts);
 const char* mind_last_error(void);
 
 #ifdef __cplusplus

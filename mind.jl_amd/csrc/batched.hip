@@ -380,18 +380,36 @@ size_t lds_bytes(const BatchPlan& P, int mode) {
   return (mode == 0 || mode == 3) ? (size_t)nn * 24 : mode == 1 ? (size_t)nn * 16 : 0;
 }
 
+template <int MODE>
+void launch_list(State& S, const BatchPlan& P, int list, int32_t n, hipStream_t st) {
+  const unsigned ng = (unsigned)P.groups_by_mode[list].size();
+  if (ng)
+    hipLaunchKernelGGL(k_batched<MODE>, dim3(ng), dim3(P.threads), lds_bytes(P, list), st, S.d, P.b,
+                       P.dev_groups[list], n);
+}
+
+// The workgroups within the node budget launch on the main stream; oversized
+// cells (each a long critical path on one workgroup) run concurrently on a
+// second stream, joined before the exchange.
 int launch_cells(State& S, const BatchPlan& P, int32_t n) {
-  for (int m = 0; m < 4; ++m) {
-    const unsigned ng = (unsigned)P.groups_by_mode[m].size();
-    if (!ng) continue;
-    const size_t bytes = lds_bytes(P, m);
-    if (m == 0 || m == 3)
-      hipLaunchKernelGGL(k_batched<0>, dim3(ng), dim3(P.threads), bytes, S.stream, S.d, P.b, P.dev_groups[m], n);
-    else if (m == 1)
-      hipLaunchKernelGGL(k_batched<1>, dim3(ng), dim3(P.threads), bytes, S.stream, S.d, P.b, P.dev_groups[m], n);
-    else
-      hipLaunchKernelGGL(k_batched<2>, dim3(ng), dim3(P.threads), bytes, S.stream, S.d, P.b, P.dev_groups[m], n);
+  const bool side = !P.groups_by_mode[1].empty() || !P.groups_by_mode[2].empty() ||
+                    !P.groups_by_mode[3].empty();
+  if (side) {
+    HIP_TRY(hipEventRecord(S.fork, S.stream));
+    HIP_TRY(hipStreamWaitEvent(S.aux, S.fork, 0));
+    launch_list<2>(S, P, 2, n, S.aux);
+    launch_list<1>(S, P, 1, n, S.aux);
+    launch_list<0>(S, P, 3, n, S.aux);
+    HIP_TRY(hipEventRecord(S.join, S.aux));
   }
+  int rc = 0;
+  if (S.profiling && !P.groups_by_mode[0].empty() && (rc = timer_start(S))) return rc;
+  launch_list<0>(S, P, 0, n, S.stream);
+  if (S.profiling && !P.groups_by_mode[0].empty()) {
+    if ((rc = timer_stop(S))) return rc;
+    S.kernel_steps += n;
+  }
+  if (side) HIP_TRY(hipStreamWaitEvent(S.stream, S.join, 0));
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -411,6 +429,9 @@ int batched_launch(State& S, int32_t n) { return launch_cells(S, *S.batch, n); }
 // Chooses the node budget and block size by timing the window kernel on this
 // model (initialize! must follow, which restores every state the trials touched).
 int batched_setup(State& S, int32_t cells_per_group) {
+  HIP_TRY(hipStreamCreateWithFlags(&S.aux, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
   HIP_TRY(hipFuncSetAttribute((const void*)k_batched<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit));
   HIP_TRY(hipFuncSetAttribute((const void*)k_batched<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit));
   int64_t max_cell = 1;
@@ -472,6 +493,31 @@ int batched_setup(State& S, int32_t cells_per_group) {
   return 0;
 }
 
+// Nodes, pas/hh/Exp2Syn rows and detectors of the cells in LDS workgroups within
+// the node budget: the work of the dominant (mode 0) launch.
+int batched_dominant_work(const State& S, int64_t* counts) {
+  if (!S.batch) return fail(MIND_EARG, "state has no batched plan");
+  for (int i = 0; i < 6; ++i) counts[i] = 0;
+  const Layout& L = S.L;
+  std::vector<uint8_t> in0(L.ncells, 0);
+  std::vector<int32_t> c0(S.batch->ngroups), c1(S.batch->ngroups);
+  hipMemcpy(c0.data(), S.batch->b.grp_cell0, sizeof(int32_t) * c0.size(), hipMemcpyDeviceToHost);
+  hipMemcpy(c1.data(), S.batch->b.grp_cell1, sizeof(int32_t) * c1.size(), hipMemcpyDeviceToHost);
+  for (int32_t g : S.batch->groups_by_mode[0])
+    for (int32_t c = c0[g]; c < c1[g]; ++c) in0[c] = 1;
+  for (int32_t c = 0; c < L.ncells; ++c) {
+    if (!in0[c]) continue;
+    counts[0] += L.cell_off[c + 1] - L.cell_off[c];
+    counts[5] += L.cell_det_off[c + 1] - L.cell_det_off[c];
+    const int mechs[4] = {MIND_MECH_PAS, MIND_MECH_HH, MIND_MECH_EXP2SYN, MIND_MECH_ICLAMP};
+    for (int k = 0; k < 4; ++k) {
+      const int g = L.group_of_mech[mechs[k]];
+      if (g >= 0) counts[1 + k] += L.groups[g].cell_rows[c + 1] - L.groups[g].cell_rows[c];
+    }
+  }
+  return 0;
+}
+
 int batched_plan_info(const State& S, int64_t* budget, int32_t* threads, int64_t* groups) {
   if (!S.batch) return fail(MIND_EARG, "state has no batched plan");
   *budget = S.batch->budget;
@@ -484,4 +530,8 @@ int batched_plan_info(const State& S, int64_t* budget, int32_t* threads, int64_t
 
 extern "C" int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int64_t* groups) {
   return mind::batched_plan_info(*reinterpret_cast<const mind::State*>(state), budget, threads, groups);
+}
+
+extern "C" int mind_batch_work(const mind_state* state, int64_t* counts) {
+  return mind::batched_dominant_work(*reinterpret_cast<const mind::State*>(state), counts);
 }

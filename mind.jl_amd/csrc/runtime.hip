@@ -725,6 +725,25 @@ int check_device_error(State& S) {
 
 }  // namespace
 
+int timer_start(State& S) {
+  if (S.events_used + 2 > S.events.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      S.events.push_back(e);
+    }
+  }
+  HIP_TRY(hipEventRecord(S.events[S.events_used], S.stream));
+  return 0;
+}
+
+int timer_stop(State& S) {
+  HIP_TRY(hipEventRecord(S.events[S.events_used + 1], S.stream));
+  S.events_used += 2;
+  S.kernel_launches += 1;
+  return 0;
+}
+
 int launch_window(State& S, int32_t n) {
   Dev& d = S.d;
   hipLaunchKernelGGL(k_gather, dim3(1), dim3(kBlock), 0, S.stream, d, n);
@@ -732,7 +751,11 @@ int launch_window(State& S, int32_t n) {
     int rc = batched_launch(S, n);
     if (rc) return rc;
   } else {
+    int rc = 0;
+    if (S.profiling && (rc = timer_start(S))) return rc;
     hipLaunchKernelGGL(k_plain, dim3(blocks_for(d.ncells, 64)), dim3(64), 0, S.stream, d, n);
+    if (S.profiling && (rc = timer_stop(S))) return rc;
+    if (S.profiling) S.kernel_steps += n;
   }
   hipLaunchKernelGGL(k_exchange, dim3(1), dim3(kBlock), 0, S.stream, d, n);
   HIP_TRY(hipGetLastError());
@@ -928,7 +951,7 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
   int64_t left = count;
   while (left > 0) {
     const int32_t n = (int32_t)std::min<int64_t>(left, S.wmax);
-    if (n == S.wmax) {
+    if (n == S.wmax && !S.profiling) {
       if (!S.graph) {
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
@@ -949,6 +972,12 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
   }
   HIP_TRY(hipStreamSynchronize(S.stream));
   const auto t1 = std::chrono::steady_clock::now();
+  for (size_t e = 0; e + 1 < S.events_used; e += 2) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, S.events[e], S.events[e + 1]));
+    S.kernel_ms += ms;
+  }
+  S.events_used = 0;
   rc = mind::check_device_error(S);
   d.record_spikes = 0;
   if (rc) return rc;
@@ -1069,6 +1098,42 @@ int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int
     std::copy(L.axial.begin(), L.axial.end(), axial);
     std::copy(L.adiag.begin(), L.adiag.end(), axial_diag);
   }
+  return 0;
+}
+
+int mind_device_count(int32_t* count) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return 0;
+}
+
+int mind_synchronize(mind_state* state) {
+  State& S = *reinterpret_cast<State*>(state);
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  return 0;
+}
+
+int mind_set_profiling(mind_state* state, int32_t on) {
+  State& S = *reinterpret_cast<State*>(state);
+  S.profiling = on != 0;
+  S.kernel_ms = 0.0;
+  S.kernel_launches = 0;
+  S.kernel_steps = 0;
+  S.events_used = 0;
+  if (S.graph) {
+    (void)hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+  }
+  return 0;
+}
+
+int mind_kernel_time(const mind_state* state, int64_t* launches, int64_t* steps, double* total_ms) {
+  const State& S = *reinterpret_cast<const State*>(state);
+  *launches = S.kernel_launches;
+  *steps = S.kernel_steps;
+  *total_ms = S.kernel_ms;
   return 0;
 }
 

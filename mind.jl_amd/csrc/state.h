@@ -28,6 +28,8 @@ struct State {
   int device = 0;
   int method = MIND_METHOD_PLAIN;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;          // concurrent launches of oversized cells
+  hipEvent_t fork = nullptr, join = nullptr;
   std::vector<void*> allocations;
   std::vector<GroupDev> groups_host;
   std::vector<double*> group_initial;  // parameter snapshot restored by initialize
@@ -42,13 +44,24 @@ struct State {
   BatchPlan* batch = nullptr;
   std::vector<int32_t> cell_bucket;  // event bucket of each cell: the cell, or its group
   std::vector<void*> run_allocations;
+  // optional timing of the dominant cell-kernel launch (HIP events on `stream`)
+  bool profiling = false;
+  std::vector<hipEvent_t> events;  // start/stop pairs of the current run
+  size_t events_used = 0;
+  double kernel_ms = 0.0;
+  int64_t kernel_launches = 0;
+  int64_t kernel_steps = 0;
 
   ~State();
   void release() {
     batched_release(*this);
+    for (hipEvent_t e : events) hipEventDestroy(e);
     if (graph) hipGraphExecDestroy(graph);
     for (void* p : run_allocations) hipFree(p);
     for (void* p : allocations) hipFree(p);
+    if (fork) hipEventDestroy(fork);
+    if (join) hipEventDestroy(join);
+    if (aux) hipStreamDestroy(aux);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -74,6 +87,8 @@ struct State {
 
 
 int launch_window(State& S, int32_t n);
+int timer_start(State& S);
+int timer_stop(State& S);
 int batched_setup(State& S, int32_t cells_per_group);
 int batched_launch(State& S, int32_t n);
 

@@ -164,9 +164,20 @@ def lib():
     L.mind_pack_nodes.argtypes = [_p(ModelDesc), C.c_double, _p(C.c_int64), _p(C.c_int64),
                                   _p(C.c_int32), _p(C.c_double), _p(C.c_double), _p(C.c_double),
                                   _p(C.c_double)]
+    L.mind_device_count.argtypes = [_p(C.c_int32)]
+    L.mind_synchronize.argtypes = [C.c_void_p]
+    L.mind_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    L.mind_kernel_time.argtypes = [C.c_void_p, _p(C.c_int64), _p(C.c_int64), _p(C.c_double)]
+    L.mind_batch_work.argtypes = [C.c_void_p, _p(C.c_int64)]
     L.mind_last_error.restype = C.c_char_p
     _lib = L
     return L
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    lib().mind_device_count(C.byref(n))
+    return n.value
 
 
 def check(code: int):

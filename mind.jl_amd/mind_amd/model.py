@@ -544,6 +544,30 @@ def batch_plan(model):
     return b.value, t.value, g.value
 
 
+def synchronize(model):
+    _abi.check(_abi.lib().mind_synchronize(model.state))
+
+
+def set_profiling(model, on=True):
+    """Time the dominant cell-kernel launch with HIP events on the library's stream."""
+    _abi.check(_abi.lib().mind_set_profiling(model.state, int(bool(on))))
+
+
+def kernel_time(model):
+    """(launches, steps, total ms) of the timed cell-kernel launches."""
+    n, s, ms = C.c_int64(), C.c_int64(), C.c_double()
+    _abi.check(_abi.lib().mind_kernel_time(model.state, C.byref(n), C.byref(s), C.byref(ms)))
+    return n.value, s.value, ms.value
+
+
+def batch_work(model):
+    """Per-step work of the dominant batched launch: nodes, pas, hh, Exp2Syn, IClamp, detectors."""
+    out = (C.c_int64 * 6)()
+    _abi.check(_abi.lib().mind_batch_work(model.state, out))
+    keys = ("nodes", "pas", "hh", "exp2syn", "iclamp", "detectors")
+    return dict(zip(keys, list(out)))
+
+
 def window_steps(model):
     w = C.c_int32()
     _abi.check(_abi.lib().mind_window_steps(model.state, C.byref(w)))

@@ -14,12 +14,9 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
-    try:
-        import torch
+    from mind_amd import _abi
 
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    return _abi.device_count() > 0
 
 
 @pytest.fixture(scope="session")
