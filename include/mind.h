@@ -170,6 +170,47 @@ typedef struct mind_result {
   double run_seconds; /* wall time of the step loop */
 } mind_result;
 
+/* ---- Micro-macro coupling (reference MacroModel, src/model/macro/model.jl:55-303,
+ * advance_macro!, src/runtime/step.jl:156-204) ----------------------------------
+ * ROIs run the Wilson-Cowan REGION mechanism (RK4 at dt_macro); ROI pairs with a
+ * non-zero connectome weight run the linear delayed MACRO2MACRO projection
+ * (coupling_i = sum_j W_ij E_j(t - D_ij), delays rounded to macro steps); each
+ * MACRO2MICRO binding is an inhomogeneous Poisson source at rate_scale * E of
+ * its ROI, rekeyed per macro step, driving one point process; each ROI's
+ * MICRO2MACRO transform bins the spikes of its sources into a population rate. */
+enum { WC_TAU_E, WC_TAU_I, WC_C_EE, WC_C_EI, WC_C_IE, WC_C_II, WC_A_E, WC_THETA_E, WC_A_I,
+       WC_THETA_I, WC_P, WC_Q, WC_K_MICRO, WC_RATE_SCALE, WC_E0, WC_I0, WC_NPARAMS };
+
+typedef struct mind_macro_desc {
+  int32_t n_roi;
+  double dt_macro;            /* ms, an integer multiple of dt_micro */
+  const double* weights;      /* [n_roi * n_roi], W[target * n_roi + source] */
+  const double* delays;       /* [n_roi * n_roi], ms */
+  const double* wc_params;    /* [n_roi * WC_NPARAMS] */
+  const double* bin_ms;       /* per ROI: MICRO2MACRO bin width, multiple of dt_macro */
+  const int64_t* input_offsets;  /* n_roi + 1: MACRO2MICRO bindings of each ROI */
+  const int32_t* input_point;    /* target point process of each binding */
+  const double* input_weight;
+  const double* input_delay;
+  const int64_t* output_offsets; /* n_roi + 1: MICRO2MACRO spike sources of each ROI */
+  const int32_t* output_source;  /* voltage-detector source index (its node gives mean soma v) */
+} mind_macro_desc;
+
+/* Macro result: per macro step boundary, per ROI, E, I, population rate (Hz),
+ * coupling input and mean soma voltage of the ROI's sources, row-major
+ * [5][n_roi][n_samples]; micro spikes as in mind_result (input spikes carry
+ * the negative SIDs of their virtual sources). */
+typedef struct mind_macro_result {
+  int64_t n_samples;
+  double* time;
+  int32_t n_roi;
+  double* traces;
+  int64_t n_spikes;
+  double* spike_times;
+  int64_t* spike_sids;
+  double run_seconds;
+} mind_macro_result;
+
 /* Pack and upload a model. Replaces build!/pack_micro (src/model/build.jl:120-220,
  * src/model/micro/model.jl:420-902) and allocate_simulation
  * (src/runtime/initialize.jl:4-250). */
@@ -184,6 +225,14 @@ int mind_initialize(mind_state* state);
  * (src/runtime/step.jl:4-74). */
 int mind_run(mind_state* state, double stop, const mind_record* records, int64_t n_records,
              int32_t record_spikes, mind_result* out);
+
+/* Micro-macro model: the micro network plus its ROI graph (build!, build.jl:120-220,
+ * with pack_macro and pack_transforms). mind_initialize also seeds the ROI state and
+ * history (initialize.jl:390-424); mind_run_macro advances whole macro steps. */
+int mind_build_macro(const mind_model_desc* micro, const mind_macro_desc* macro,
+                     const mind_build_options* options, mind_state** out);
+int mind_run_macro(mind_state* state, double stop, int32_t record_spikes, mind_macro_result* out);
+void mind_macro_result_free(mind_macro_result* result);
 
 /* One micro step (src/runtime/step.jl:77-86). */
 int mind_step(mind_state* state);

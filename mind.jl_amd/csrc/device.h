@@ -19,8 +19,10 @@ struct QEvent {
 
 struct Clock {
   int64_t step;
-  uint64_t ticket;  // routed events so far
-  uint64_t ns_seq;  // NetStim self-event insertion order
+  uint64_t ticket;   // routed events so far
+  uint64_t ns_seq;   // NetStim self-event insertion order
+  int64_t emitted;   // spike emissions of the last exchange (in emit_all)
+  int64_t mstep;     // macro step
 };
 
 struct GroupDev {

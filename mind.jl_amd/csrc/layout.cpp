@@ -426,6 +426,8 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
     L.min_delay = std::min(L.min_delay, x);
     L.max_delay = std::max(L.max_delay, x);
   }
+  L.first_virtual = L.nsources;
+  L.min_micro_delay = L.min_delay;
   const int gn = L.group_of_mech[MIND_MECH_NETSTIM];
   if (gn >= 0) {
     const MechGroup& G = L.groups[gn];
@@ -433,6 +435,35 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
     for (int64_t s = 0; s < d.n_sources; ++s)
       if (L.source_point[s] >= 0 && L.point_group[L.source_point[s]] == gn)
         L.netstim_source[L.point_row[L.source_point[s]]] = (int32_t)s;
+  }
+  return "";
+}
+
+std::string pack_inputs(const mind_model_desc& d, const mind_macro_desc& m, Layout& L) {
+  const int64_t nb = m.input_offsets[m.n_roi];
+  for (int64_t b = 0; b < nb; ++b) {
+    const int32_t p = m.input_point[b];
+    if (p < 0 || p >= d.n_points) return "macro2micro target is not a point process";
+    const int tm = d.pt_mech[p];
+    if (tm != MIND_MECH_EXPSYN && tm != MIND_MECH_EXP2SYN) return "macro2micro target has no NET_RECEIVE";
+    if (L.point_cell[p] < 0) return "macro2micro target must sit on a cell";
+    if (!(m.input_delay[b] >= 0.0) || !isfinite(m.input_delay[b]) || !isfinite(m.input_weight[b]))
+      return "macro2micro weight and delay must be finite, delay nonnegative";
+    L.source_sid.push_back(-(b + 1));  // virtual SIDs (model.jl:622-628)
+    L.source_point.push_back(-1);
+    L.fanout.push_back(L.fanout.back() + 1);
+    L.edge_target.push_back(p);
+    L.edge_delay.push_back(m.input_delay[b]);
+    L.edge_weight.push_back(m.input_weight[b]);
+    L.max_delay = std::max(L.max_delay, m.input_delay[b]);
+  }
+  L.nsources += nb;
+  for (int64_t q = 0; q < m.output_offsets[m.n_roi]; ++q) {
+    const int32_t s = m.output_source[q];
+    if (s < 0 || s >= L.first_virtual) return "micro2macro source is not a spike source";
+    bool det = false;
+    for (size_t k = 0; k < L.det_source.size(); ++k) det |= L.det_source[k] == s;
+    if (!det) return "micro2macro sources must be voltage detectors";
   }
   return "";
 }

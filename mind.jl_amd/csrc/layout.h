@@ -67,6 +67,9 @@ struct Layout {
   double min_delay = 0.0, max_delay = 0.0;
 
   std::vector<int32_t> netstim_source;  // per NetStim row: its source index or -1
+  // MACRO2MICRO bindings are virtual sources [first_virtual, nsources), one edge each
+  int64_t first_virtual = 0;
+  double min_micro_delay = 0.0;  // over edges of real sources only
 
   int64_t location_node(int32_t cell, int32_t section, double x) const;
   int64_t range_node(int32_t cell, int32_t section, double x) const;
@@ -74,6 +77,8 @@ struct Layout {
 
 // Returns an empty string on success, else the error message (ArgumentError).
 std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& out);
+// Appends the virtual sources of MACRO2MICRO bindings (src/model/build.jl:261-295).
+std::string pack_inputs(const mind_model_desc& d, const mind_macro_desc& m, Layout& L);
 
 int field_count(int mech);
 void field_defaults(int mech, double* values);

@@ -453,6 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
   if (tid == 0) {
     d.clock->ticket = ticket0 + (uint64_t)running;
     d.clock->step = k0 + n;
+    d.clock->emitted = ne;
   }
 }
 
@@ -566,9 +567,7 @@ int set_buckets(State& S, int64_t nb) {
   return 0;
 }
 
-namespace {
-
-int setup(State& S, const mind_build_options& o) {
+int setup_state(State& S, const mind_build_options& o) {
   Layout& L = S.L;
   Dev& d = S.d;
   S.device = o.device;
@@ -639,7 +638,7 @@ int setup(State& S, const mind_build_options& o) {
     if ((rc = S.upload(&p32, L.det_source))) return rc; d.det_source = p32;
   }
   // window length: no event produced inside a window may be due inside it
-  const double min_delay = L.edge_delay.empty() ? INFINITY : L.min_delay;
+  const double min_delay = L.fanout[L.first_virtual] == 0 ? INFINITY : L.min_micro_delay;
   int64_t wmax = 1000;
   if (std::isfinite(min_delay)) wmax = (int64_t)floor(min_delay / L.dt - 0.5);
   if (o.window_steps > 0) wmax = std::min<int64_t>(wmax, o.window_steps);
@@ -704,7 +703,9 @@ int setup(State& S, const mind_build_options& o) {
   return 0;
 }
 
+namespace {
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+}  // namespace
 
 int check_device_error(State& S) {
   HIP_TRY(hipStreamSynchronize(S.stream));
@@ -723,7 +724,33 @@ int check_device_error(State& S) {
   }
 }
 
-}  // namespace
+// Copies the spikes recorded on the device and sorts them by (time, SID)
+// (collect_result, src/runtime/recording.jl:279-283).
+int collect_spikes(State& S, int64_t* n_out, double** times, int64_t** sids) {
+  Dev& d = S.d;
+  unsigned long long ns = 0;
+  HIP_TRY(hipMemcpy(&ns, d.spike_count, sizeof(ns), hipMemcpyDeviceToHost));
+  if ((int64_t)ns > d.spike_cap) ns = d.spike_cap;
+  std::vector<double> st(ns);
+  std::vector<int64_t> ss(ns);
+  if (ns) {
+    HIP_TRY(hipMemcpy(st.data(), d.spike_t, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ss.data(), d.spike_sid, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
+  }
+  std::vector<int64_t> order(ns);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return st[a] < st[b] || (st[a] == st[b] && ss[a] < ss[b]);
+  });
+  *n_out = (int64_t)ns;
+  *times = (double*)malloc(sizeof(double) * std::max<unsigned long long>(1, ns));
+  *sids = (int64_t*)malloc(sizeof(int64_t) * std::max<unsigned long long>(1, ns));
+  for (unsigned long long i = 0; i < ns; ++i) {
+    (*times)[i] = st[order[i]];
+    (*sids)[i] = ss[order[i]];
+  }
+  return 0;
+}
 
 int timer_start(State& S) {
   if (S.events_used + 2 > S.events.size()) {
@@ -779,7 +806,7 @@ int mind_build(const mind_model_desc* model, const mind_build_options* options, 
     delete S;
     return fail(MIND_EARG, err);
   }
-  int rc = mind::setup(*S, *options);
+  int rc = mind::setup_state(*S, *options);
   if (!rc && options->method == MIND_METHOD_BATCHED) rc = mind::batched_setup(*S, options->cells_per_group);
   if (rc) {
     delete S;
@@ -798,7 +825,7 @@ int mind_initialize(mind_state* state) {
   S.initialized = false;
   S.step = 0;
   HIP_TRY(hipMemsetAsync(d.error, 0, sizeof(int32_t), S.stream));
-  mind::Clock clock{0, 0, 0};
+  mind::Clock clock{0, 0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(d.clock, &clock, sizeof(clock), hipMemcpyHostToDevice, S.stream));
   HIP_TRY(hipMemsetAsync(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots, S.stream));
   for (size_t gi = 0; gi < S.groups_host.size(); ++gi)
@@ -815,7 +842,7 @@ int mind_initialize(mind_state* state) {
     hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
                        S.ns_logical, mind::kNetStimDigest);
     // the netstim sequence counter continues after the INITIAL insertions
-    mind::Clock c2{0, 0, (uint64_t)d.ns_rows};
+    mind::Clock c2{0, 0, (uint64_t)d.ns_rows, 0, 0};
     HIP_TRY(hipMemcpyAsync(d.clock, &c2, sizeof(c2), hipMemcpyHostToDevice, S.stream));
   }
   // Spikes of the time-zero delivery are kept for the first run (step.jl:47-49).
@@ -830,7 +857,8 @@ int mind_initialize(mind_state* state) {
   }
   HIP_TRY(hipGetLastError());
   d.record_spikes = 0;
-  int rc = mind::check_device_error(S);
+  int rc = mind::macro_initialize(S);
+  if (!rc) rc = mind::check_device_error(S);
   if (rc) return rc;
   S.initialized = true;
   return 0;
@@ -993,27 +1021,8 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
   out->traces = (double*)malloc(sizeof(double) * std::max<int64_t>(1, n_records * samples));
   if (n_records)
     HIP_TRY(hipMemcpy(out->traces, d.rec_buf, sizeof(double) * n_records * samples, hipMemcpyDeviceToHost));
-  unsigned long long ns = 0;
-  HIP_TRY(hipMemcpy(&ns, d.spike_count, sizeof(ns), hipMemcpyDeviceToHost));
-  if ((int64_t)ns > d.spike_cap) ns = d.spike_cap;
-  std::vector<double> st(ns);
-  std::vector<int64_t> ss(ns);
-  if (ns) {
-    HIP_TRY(hipMemcpy(st.data(), d.spike_t, sizeof(double) * ns, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(ss.data(), d.spike_sid, sizeof(int64_t) * ns, hipMemcpyDeviceToHost));
-  }
-  std::vector<int64_t> order(ns);
-  std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-    return st[a] < st[b] || (st[a] == st[b] && ss[a] < ss[b]);
-  });
-  out->n_spikes = (int64_t)ns;
-  out->spike_times = (double*)malloc(sizeof(double) * std::max<unsigned long long>(1, ns));
-  out->spike_sids = (int64_t*)malloc(sizeof(int64_t) * std::max<unsigned long long>(1, ns));
-  for (unsigned long long i = 0; i < ns; ++i) {
-    out->spike_times[i] = st[order[i]];
-    out->spike_sids[i] = ss[order[i]];
-  }
+  rc = mind::collect_spikes(S, &out->n_spikes, &out->spike_times, &out->spike_sids);
+  if (rc) return rc;
   out->run_seconds = std::chrono::duration<double>(t1 - t0).count();
   return 0;
 }

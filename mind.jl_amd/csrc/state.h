@@ -15,12 +15,14 @@ int fail(int code, const std::string& msg);
 #define HIP_TRY(expr)                                                                      \
   do {                                                                                     \
     hipError_t _e = (expr);                                                                \
-    if (_e != hipSuccess) return fail(MIND_EHIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    if (_e != hipSuccess) return ::mind::fail(MIND_EHIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
   } while (0)
 
 struct BatchPlan;
+struct MacroHost;
 struct State;
 void batched_release(State& S);
+void macro_release(State& S);
 
 struct State {
   Layout L;
@@ -42,6 +44,7 @@ struct State {
   int32_t graph_steps = 0;
   // batched kernel plan (cell groups, chain units and level schedules)
   BatchPlan* batch = nullptr;
+  MacroHost* macro = nullptr;
   std::vector<int32_t> cell_bucket;  // event bucket of each cell: the cell, or its group
   std::vector<void*> run_allocations;
   // optional timing of the dominant cell-kernel launch (HIP events on `stream`)
@@ -54,6 +57,7 @@ struct State {
 
   ~State();
   void release() {
+    macro_release(*this);
     batched_release(*this);
     for (hipEvent_t e : events) hipEventDestroy(e);
     if (graph) hipGraphExecDestroy(graph);
@@ -87,6 +91,11 @@ struct State {
 
 
 int launch_window(State& S, int32_t n);
+int setup_state(State& S, const mind_build_options& o);
+int check_device_error(State& S);
+int collect_spikes(State& S, int64_t* n, double** times, int64_t** sids);
+int macro_setup(State& S, const mind_macro_desc& md);
+int macro_initialize(State& S);
 int timer_start(State& S);
 int timer_stop(State& S);
 int batched_setup(State& S, int32_t cells_per_group);

@@ -148,3 +148,53 @@ def tree_network(ncells=5000, median=400, sigma=0.8, lo=50, hi=4000, synapses=20
             "interval": 1000.0 / rate_hz, "number": 1e9, "start": 0.0, "noise": 1.0})
         model.connect(ns, syns[t][drive[t]], weight=netstim_weight, delay=1.0)
     return model, dict(cells=cells, syns=syns, sizes=sizes, tstop=500.0)
+
+
+def micro_macro(n_roi=16, cells=256, nseg=200, seed=3, input_weight=2e-3, rate_scale=200.0,
+                k_micro=0.5, threshold=-20.0, wc=None):
+    """Wilson-Cowan ROIs coupled all-to-all (W U(0, 0.2), D U(2, 10) ms), each bound to a
+    population of ball-and-stick variants (hh soma, nseg-segment pas dendrite of
+    length U(800, 1200) um and diameter U(1.5, 2.5) um, one Exp2Syn at dend(0.1))
+    driven by an inhomogeneous Poisson input at rate_scale * E and read back as a
+    1 ms binned population rate."""
+    from . import macro as M
+
+    g = rng(seed, "connectivity")
+    W = g.uniform(0.0, 0.2, (n_roi, n_roi))
+    np.fill_diagonal(W, 0.0)
+    D = g.uniform(2.0, 10.0, (n_roi, n_roi))
+    mm = M.MacroModel(M.Connectivity(W, D))
+    micro = MicroModel()
+    gm = rng(seed, "morphology")
+    pops, syns, sources = [], [], []
+    for r in range(n_roi):
+        region = mm.roi(r)
+        params = dict(wc or {})
+        params.setdefault("k_micro", k_micro)
+        params.setdefault("rate_scale", rate_scale)
+        M.use_macro(region, "WilsonCowan", params=params,
+                    initial_state={"E": 0.05 + 0.2 * g.random(), "I": 0.05 + 0.1 * g.random()})
+        members = []
+        for c in range(cells):
+            L = gm.uniform(800.0, 1200.0)
+            d = gm.uniform(1.5, 2.5)
+            (cell,) = micro.add_population(ball_and_stick_sections(nseg, L, d), 1)
+            _biophysics(micro, cell)
+            syn = micro.add_pointprocess(cell, "Exp2Syn", section="dend", x=0.1,
+                                         params={"tau1": 0.5, "tau2": 5.0, "e": 0.0})
+            src = micro.register_spike_source(cell, section="soma", x=0.5, threshold=threshold)
+            members.append(cell)
+            syns.append(syn)
+            sources.append(src)
+        pops.append(members)
+        M.use_micro(region, micro, exposures=("rate",))
+    for r in range(n_roi):
+        for q in range(n_roi):
+            if r != q:
+                M.connect(mm.roi(q), mm.roi(r), "LinearDelay")
+    for r in range(n_roi):
+        for c in range(cells):
+            k = r * cells + c
+            M.macro2micro(mm.roi(r), "PoissonDrive", target=syns[k], weight=input_weight, delay=0.0)
+            M.micro2macro(mm.roi(r), "PopRate", source=sources[k], params={"bin": 1.0})
+    return mm, dict(populations=pops, syns=syns, sources=sources, tstop=2000.0)

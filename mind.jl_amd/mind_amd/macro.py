@@ -1,0 +1,295 @@
+"""Macro layer of MIND.jl, flattened into the C ABI.
+
+Mirrors Connectivity / MacroModel / roi / use_macro / use_micro / connect /
+micro2macro / macro2micro (src/model/macro/model.jl:4-162,
+src/model/build.jl:230-295) and run for macro models (src/runtime/step.jl:4-74,
+156-204). The MIND-role mechanisms are builtins of this project:
+
+  WilsonCowan   REGION: E/I populations, tau dE/dt = -E + S_E(c_EE E - c_EI I + P + u),
+                tau_I dI/dt = -I + S_I(c_IE E - c_II I + Q), S(x) = 1/(1+exp(-a(x-theta)))
+                - 1/(1+exp(a theta)), u = coupling + k_micro * rate / rate_scale; RK4 per
+                macro step.
+  LinearDelay   MACRO2MACRO: coupling_i = sum_j W_ij E_j(t - D_ij), D rounded to macro steps.
+  PoissonDrive  MACRO2MICRO: Poisson events at rate_scale * E (Hz) over each macro
+                interval into one point process.
+  PopRate       MICRO2MACRO: spikes of the ROI's sources per bin -> rate (Hz).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .model import MicroModel, PointProcess, SpikeSource
+
+WC_PARAMS = ("tau_e", "tau_i", "c_ee", "c_ei", "c_ie", "c_ii", "a_e", "theta_e", "a_i",
+             "theta_i", "P", "Q", "k_micro", "rate_scale", "E0", "I0")
+WC_DEFAULTS = dict(tau_e=10.0, tau_i=10.0, c_ee=16.0, c_ei=12.0, c_ie=15.0, c_ii=3.0, a_e=1.3,
+                   theta_e=4.0, a_i=2.0, theta_i=3.7, P=1.25, Q=0.0, k_micro=0.0,
+                   rate_scale=100.0, E0=0.1, I0=0.05)
+ROLES = {"WilsonCowan": "region", "LinearDelay": "macro2macro", "PoissonDrive": "macro2micro",
+         "PopRate": "micro2macro"}
+
+
+class MacroDesc(C.Structure):
+    _fields_ = [
+        ("n_roi", C.c_int32), ("dt_macro", C.c_double),
+        ("weights", C.POINTER(C.c_double)), ("delays", C.POINTER(C.c_double)),
+        ("wc_params", C.POINTER(C.c_double)), ("bin_ms", C.POINTER(C.c_double)),
+        ("input_offsets", C.POINTER(C.c_int64)), ("input_point", C.POINTER(C.c_int32)),
+        ("input_weight", C.POINTER(C.c_double)), ("input_delay", C.POINTER(C.c_double)),
+        ("output_offsets", C.POINTER(C.c_int64)), ("output_source", C.POINTER(C.c_int32)),
+    ]
+
+
+class MacroResultC(C.Structure):
+    _fields_ = [
+        ("n_samples", C.c_int64), ("time", C.POINTER(C.c_double)), ("n_roi", C.c_int32),
+        ("traces", C.POINTER(C.c_double)), ("n_spikes", C.c_int64),
+        ("spike_times", C.POINTER(C.c_double)), ("spike_sids", C.POINTER(C.c_int64)),
+        ("run_seconds", C.c_double),
+    ]
+
+
+@dataclass
+class Connectivity:
+    """Connectome: weights[target, source], delays[target, source] (ms), labels."""
+    weights: np.ndarray
+    delays: np.ndarray
+    labels: list = field(default_factory=list)
+
+    def __post_init__(self):
+        self.weights = np.asarray(self.weights, np.float64)
+        self.delays = np.asarray(self.delays, np.float64)
+        n = self.weights.shape[0]
+        if self.weights.shape != (n, n) or self.delays.shape != (n, n):
+            raise ValueError("connectivity matrices must be square and of equal size")
+        if not self.labels:
+            self.labels = [f"roi{i}" for i in range(n)]
+        if np.any(self.delays < 0) or not np.all(np.isfinite(self.delays)):
+            raise ValueError("delays must be finite and nonnegative")
+
+
+@dataclass
+class ROI:
+    model: "MacroModel"
+    index: int
+
+
+@dataclass
+class Micro2Macro:
+    model: "MacroModel"
+    index: int
+
+
+@dataclass
+class MacroResult:
+    time: np.ndarray
+    E: np.ndarray
+    I: np.ndarray
+    rate: np.ndarray
+    coupling: np.ndarray
+    mean_v: np.ndarray
+    spike_times: np.ndarray
+    spike_sources: np.ndarray
+    run_seconds: float
+
+
+class MacroModel:
+    def __init__(self, connectivity: Connectivity):
+        self.connectivity = connectivity
+        n = len(connectivity.labels)
+        self.regions = [None] * n
+        self.micro = None
+        self.micro_exposures = {}
+        self.projections = set()
+        self.inputs = []   # (roi, point index, weight, delay)
+        self.outputs = {}  # roi -> (list of source indices, bin ms)
+        self.state = None
+
+    def roi(self, label) -> ROI:
+        labels = self.connectivity.labels
+        index = label if isinstance(label, (int, np.integer)) else labels.index(str(label))
+        if not 0 <= index < len(labels):
+            raise ValueError(f"unknown ROI {label}")
+        return ROI(self, int(index))
+
+
+def _role(mechanism, role):
+    name = str(mechanism).lstrip(":")
+    if ROLES.get(name) != role:
+        raise ValueError(f"expected a MIND {role.upper()} mechanism, got {name}")
+    return name
+
+
+def roi(model: MacroModel, label) -> ROI:
+    return model.roi(label)
+
+
+def use_macro(region: ROI, mechanism, params=None, initial_state=None):
+    _role(mechanism, "region")
+    m = region.model
+    if m.regions[region.index] is not None:
+        raise ValueError("ROI already has a macro owner")
+    p = dict(WC_DEFAULTS)
+    for k, v in (params or {}).items():
+        if k not in p:
+            raise ValueError(f"WilsonCowan has no parameter {k}")
+        p[k] = float(v)
+    for k, v in (initial_state or {}).items():
+        p[{"E": "E0", "I": "I0"}[k]] = float(v)
+    m.regions[region.index] = p
+    return region
+
+
+def use_micro(region: ROI, micro: MicroModel, exposures=("rate",)):
+    m = region.model
+    if region.index in m.micro_exposures:
+        raise ValueError("ROI already has a micro owner")
+    if m.micro is not None and m.micro is not micro:
+        raise ValueError("ROIs in one simulation must share the same micro network")
+    m.micro = micro
+    m.micro_exposures[region.index] = list(exposures)
+    return region
+
+
+def connect(source: ROI, target: ROI, mechanism):
+    _role(mechanism, "macro2macro")
+    if source.model is not target.model:
+        raise ValueError("ROIs belong to different models")
+    source.model.projections.add((target.index, source.index))
+    return target
+
+
+def micro2macro(region: ROI, mechanism, source: SpikeSource, params=None):
+    _role(mechanism, "micro2macro")
+    m = region.model
+    if m.micro is None or source.model is not m.micro:
+        raise ValueError("spike source belongs to another micro network")
+    srcs, bin_ms = m.outputs.get(region.index, ([], float((params or {}).get("bin", 1.0))))
+    if source.index not in srcs:
+        srcs.append(source.index)
+    m.outputs[region.index] = (srcs, bin_ms)
+    return Micro2Macro(m, region.index)
+
+
+def macro2micro(region: ROI, mechanism, target: PointProcess, weight=1.0, delay=0.0):
+    _role(mechanism, "macro2micro")
+    m = region.model
+    if m.micro is None or target.model is not m.micro:
+        raise ValueError("point process belongs to another micro network")
+    if not (np.isfinite(weight) and np.isfinite(delay) and delay >= 0):
+        raise ValueError("weight and delay must be finite, delay nonnegative")
+    m.inputs.append((region.index, target.index, float(weight), float(delay)))
+    return target
+
+
+def macro_descriptor(model: MacroModel, dt_macro: float):
+    n = len(model.connectivity.labels)
+    for i in range(n):
+        if model.regions[i] is None:
+            raise ValueError(f"ROI {model.connectivity.labels[i]} has no REGION owner")
+    keep = {}
+
+    def arr(name, values, dtype):
+        a = np.ascontiguousarray(np.asarray(values, dtype))
+        if a.size == 0:
+            a = np.zeros(1, dtype)
+        keep[name] = a
+        return a
+
+    W = np.zeros((n, n))
+    for (t, s) in model.projections:
+        W[t, s] = model.connectivity.weights[t, s]
+    inputs = sorted(enumerate(model.inputs), key=lambda kv: (kv[1][0], kv[0]))
+    ioff = np.zeros(n + 1, np.int64)
+    for _, (r, _, _, _) in inputs:
+        ioff[r + 1] += 1
+    ioff = np.cumsum(ioff)
+    src, ooff, bins = [], [0], []
+    for r in range(n):
+        s_list, b = model.outputs.get(r, ([], 1.0))
+        src.extend(sorted(s_list))
+        ooff.append(len(src))
+        bins.append(b)
+    d = MacroDesc()
+    d.n_roi = n
+    d.dt_macro = float(dt_macro)
+    d.weights = _abi.as_ptr(arr("w", W.reshape(-1), np.float64), C.c_double)
+    d.delays = _abi.as_ptr(arr("d", model.connectivity.delays.reshape(-1), np.float64), C.c_double)
+    wc = [[model.regions[r][k] for k in WC_PARAMS] for r in range(n)]
+    d.wc_params = _abi.as_ptr(arr("wc", np.asarray(wc).reshape(-1), np.float64), C.c_double)
+    d.bin_ms = _abi.as_ptr(arr("bin", bins, np.float64), C.c_double)
+    d.input_offsets = _abi.as_ptr(arr("io", ioff, np.int64), C.c_int64)
+    d.input_point = _abi.as_ptr(arr("ip", [x[1][1] for x in inputs], np.int32), C.c_int32)
+    d.input_weight = _abi.as_ptr(arr("iw", [x[1][2] for x in inputs], np.float64), C.c_double)
+    d.input_delay = _abi.as_ptr(arr("id", [x[1][3] for x in inputs], np.float64), C.c_double)
+    d.output_offsets = _abi.as_ptr(arr("oo", ooff, np.int64), C.c_int64)
+    d.output_source = _abi.as_ptr(arr("os", src, np.int32), C.c_int32)
+    return d, keep
+
+
+def result_from_c(res, n_roi):
+    ns, nk = res.n_samples, res.n_spikes
+    time = np.ctypeslib.as_array(res.time, (ns,)).copy()
+    tr = np.ctypeslib.as_array(res.traces, (5 * n_roi * ns,)).reshape(5, n_roi, ns).copy()
+    st = np.ctypeslib.as_array(res.spike_times, (nk,)).copy() if nk else np.zeros(0)
+    ss = np.ctypeslib.as_array(res.spike_sids, (nk,)).copy() if nk else np.zeros(0, np.int64)
+    return MacroResult(time, tr[0], tr[1], tr[2], tr[3], tr[4], st, ss, res.run_seconds)
+
+
+def _lib():
+    L = _abi.lib()
+    if not getattr(L, "_macro_ready", False):
+        L.mind_build_macro.argtypes = [C.POINTER(_abi.ModelDesc), C.POINTER(MacroDesc),
+                                       C.POINTER(_abi.BuildOptions), C.POINTER(C.c_void_p)]
+        L.mind_run_macro.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.POINTER(MacroResultC)]
+        L.mind_macro_result_free.argtypes = [C.POINTER(MacroResultC)]
+        L.mind_macro_result_free.restype = None
+        L._macro_ready = True
+    return L
+
+
+def build(model: MacroModel, dt_micro=0.025, dt_macro=0.1, seed=1, method="batched",
+          cells_per_group=0, device=0):
+    """build! of a macro model: pack the micro network, the ROI graph and transforms."""
+    ratio = dt_macro / dt_micro
+    if abs(ratio - round(ratio)) > 1e-9 or round(ratio) < 1:
+        raise ValueError("dt_macro must be an integer multiple of dt_micro")
+    L = _lib()
+    if model.state is not None:
+        L.mind_state_destroy(model.state)
+        model.state = None
+    desc, keep = model.micro.descriptor()
+    mdesc, mkeep = macro_descriptor(model, dt_macro)
+    opts = _abi.BuildOptions()
+    opts.dt = float(dt_micro)
+    opts.seed = int(seed)
+    opts.method = {"plain": 0, "batched": 1}[method]
+    opts.cells_per_group = int(cells_per_group)
+    opts.window_steps = 0
+    opts.event_capacity = 0
+    opts.device = int(device)
+    h = C.c_void_p()
+    _abi.check(L.mind_build_macro(C.byref(desc), C.byref(mdesc), C.byref(opts), C.byref(h)))
+    del keep, mkeep
+    model.state = h
+    model.micro.state = h
+    return model
+
+
+def initialize(model: MacroModel):
+    _abi.check(_abi.lib().mind_initialize(model.state))
+    return model
+
+
+def run(model: MacroModel, stop, record_spikes=True) -> MacroResult:
+    L = _lib()
+    res = MacroResultC()
+    _abi.check(L.mind_run_macro(model.state, float(stop), int(bool(record_spikes)), C.byref(res)))
+    try:
+        return result_from_c(res, len(model.connectivity.labels))
+    finally:
+        L.mind_macro_result_free(C.byref(res))

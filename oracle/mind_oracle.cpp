@@ -371,6 +371,12 @@ struct Sim {
   bool recording_spikes = false;
   std::vector<double> spike_t;
   std::vector<int64_t> spike_src;
+  // micro-macro coupling: spikes of the current macro interval (source, time)
+  int64_t rank_total = 0;
+  bool collect_window = false;
+  std::vector<int32_t> window_src;
+  std::vector<double> window_t;
+  void* macro = nullptr;
   // delivery log for tests: (point, time, order) per applied event
   bool log_deliveries = false;
   std::vector<double> log_time;
@@ -424,6 +430,10 @@ void enqueue(Sim& s, Event e) {  // enqueue_event! (utils/events.jl:42-50)
 }
 
 void record_spike(Sim& s, int32_t source, double time) {
+  if (s.collect_window) {
+    s.window_src.push_back(source);
+    s.window_t.push_back(time);
+  }
   if (!s.recording_spikes) return;
   s.spike_t.push_back(time);
   s.spike_src.push_back(source);
@@ -973,6 +983,7 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
     }
     rank += (int64_t)max_id + 1;
   }
+  s.rank_total = rank;
   s.source_rank.resize(d->n_sources);
   for (int64_t k = 0; k < d->n_sources; ++k)
     s.source_rank[k] = s.source_target[k] == 0 ? rank + k + 1 : s.target_rank[s.source_target[k] - 1];
@@ -1238,5 +1249,319 @@ int oracle_hines(int64_t n, const int32_t* parent, const double* g, double* d, d
 
 double oracle_exp(double x) { return mind_exp(x); }
 double oracle_log(double x) { return mind_log(x); }
+
+}  // extern "C"
+
+// ================================================================ macro ======
+// Micro-macro coupling, restating advance_macro! (src/runtime/step.jl:156-204)
+// with the builtin MIND-role mechanisms of this project:
+//   REGION      Wilson-Cowan E/I populations, RK4 over one macro step;
+//   MACRO2MACRO coupling_i = sum_j W_ij E_j(t - D_ij) over the history ring
+//               (prepare_coupling_kernel! / macro_edge_kernel!, cuda/macro.jl:47-123);
+//   MACRO2MICRO inhomogeneous Poisson events at rate_scale * E, rekeyed per
+//               macro step (input_trigger_kernel!, cuda/macro.jl:272-301);
+//   MICRO2MACRO spike count of the ROI's sources per interval, published as a
+//               rate (Hz) when a bin closes.
+namespace {
+
+struct Macro {
+  int n = 0;
+  double dt_macro = 0.1;
+  int64_t ratio = 4;  // micro steps per macro step
+  std::vector<double> W, wc;           // W[target * n + source], wc[roi * WC_NPARAMS + k]
+  std::vector<int64_t> delay_steps;    // [target * n + source]
+  int64_t cap = 1;
+  std::vector<double> hist;            // [slot * n + roi] published E
+  std::vector<double> E, I, coupling, rate, acc, meanv;
+  std::vector<int64_t> bin_steps;
+  std::vector<int64_t> in_off;         // bindings of each ROI
+  std::vector<int32_t> in_roi;         // per binding
+  std::vector<uint64_t> stream_id;     // per binding
+  int64_t n_points = 0, first_virtual = 0;
+  std::vector<int64_t> out_off;
+  std::vector<int32_t> out_src, out_node;
+  std::vector<int32_t> src_roi;        // per source: ROI of the MICRO2MACRO that reads it, or -1
+  int64_t step = 0;
+  Rng rng;
+};
+
+// S(x) = 1/(1 + exp(-a (x - theta))) - 1/(1 + exp(a theta))
+double wc_sigmoid(double x, double a, double theta) {
+  return 1.0 / (1.0 + mind_exp(-a * (x - theta))) - 1.0 / (1.0 + mind_exp(a * theta));
+}
+
+void wc_deriv(const double* p, double e, double i, double u, double& de, double& di) {
+  de = (-e + wc_sigmoid(p[WC_C_EE] * e - p[WC_C_EI] * i + p[WC_P] + u, p[WC_A_E], p[WC_THETA_E])) /
+       p[WC_TAU_E];
+  di = (-i + wc_sigmoid(p[WC_C_IE] * e - p[WC_C_II] * i + p[WC_Q], p[WC_A_I], p[WC_THETA_I])) /
+       p[WC_TAU_I];
+}
+
+void wc_rk4(const double* p, double& e, double& i, double u, double h) {
+  double k1e, k1i, k2e, k2i, k3e, k3i, k4e, k4i;
+  wc_deriv(p, e, i, u, k1e, k1i);
+  wc_deriv(p, e + 0.5 * h * k1e, i + 0.5 * h * k1i, u, k2e, k2i);
+  wc_deriv(p, e + 0.5 * h * k2e, i + 0.5 * h * k2i, u, k3e, k3i);
+  wc_deriv(p, e + h * k3e, i + h * k3i, u, k4e, k4i);
+  e = e + h / 6.0 * (k1e + 2.0 * k2e + 2.0 * k3e + k4e);
+  i = i + h / 6.0 * (k1i + 2.0 * k2i + 2.0 * k3i + k4i);
+}
+
+// PoissonDrive NET_RECEIVE at the start of an exchange interval:
+//   if (rate > 0) { tn = start + negexp * 1000/rate
+//                   while (tn < stop) { net_event(tn); tn = tn + negexp * 1000/rate } }
+void trigger_inputs(Sim& s, Macro& M) {
+  const double start = (double)M.step * M.dt_macro, stop = (double)(M.step + 1) * M.dt_macro;
+  const int64_t slot = M.step % M.cap;
+  for (int64_t b = 0; b < (int64_t)M.in_roi.size(); ++b) {
+    const int roi = M.in_roi[b];
+    Rng r;
+    r.keys[0] = (uint32_t)(s.seed ^ M.stream_id[b]);
+    r.keys[1] = (uint32_t)(uint64_t)b;
+    r.keys[2] = (uint32_t)((s.seed >> 32) ^ (M.stream_id[b] >> 32) ^ (uint64_t)roi ^ 0ull);
+    rng_setseq(r, 0, (double)M.step);  // MACRO2MICRO streams use a zero Philox key
+    const double rate = M.wc[roi * WC_NPARAMS + WC_RATE_SCALE] * M.hist[slot * M.n + roi];
+    if (!(rate > 0.0)) continue;
+    const double scale = 1000.0 / rate;
+    double tn = start + -mind_log(rng_uniform(r, 0)) * 1.0 * scale;
+    while (tn < stop) {
+      net_event(s, (int)(M.n_points + b), tn, 0);
+      tn = tn + -mind_log(rng_uniform(r, 0)) * 1.0 * scale;
+    }
+  }
+  flush(s);
+}
+
+void macro_record(Macro& M, std::vector<double>& tr, int64_t samples, int64_t j) {
+  for (int r = 0; r < M.n; ++r) {
+    tr[(0 * (size_t)M.n + r) * samples + j] = M.E[r];
+    tr[(1 * (size_t)M.n + r) * samples + j] = M.I[r];
+    tr[(2 * (size_t)M.n + r) * samples + j] = M.rate[r];
+    tr[(3 * (size_t)M.n + r) * samples + j] = M.coupling[r];
+    tr[(4 * (size_t)M.n + r) * samples + j] = M.meanv[r];
+  }
+}
+
+void advance_macro(Sim& s, Macro& M) {
+  // 1. delayed coupling from the history ring
+  for (int i = 0; i < M.n; ++i) {
+    double c = 0.0;
+    for (int j = 0; j < M.n; ++j) {
+      const double w = M.W[(size_t)i * M.n + j];
+      if (w == 0.0) continue;
+      const int64_t slot = ((M.step - M.delay_steps[(size_t)i * M.n + j]) % M.cap + M.cap) % M.cap;
+      c = c + w * M.hist[slot * M.n + j];
+    }
+    M.coupling[i] = c;
+  }
+  // 2. MACRO2MICRO events for [start, stop)
+  s.window_src.clear();
+  s.window_t.clear();
+  s.collect_window = true;
+  trigger_inputs(s, M);
+  // 3. micro steps; micro samples are taken by the caller's recording
+  for (int64_t k = 0; k < M.ratio; ++k) {
+    advance_micro(s);
+    deliver(s, (double)s.step * s.dt);
+  }
+  s.collect_window = false;
+  // 4. MICRO2MACRO: spikes of each ROI's sources in [start, stop)
+  const double start = (double)M.step * M.dt_macro, stop = (double)(M.step + 1) * M.dt_macro;
+  std::vector<double> count(M.n, 0.0);
+  for (size_t q = 0; q < s.window_src.size(); ++q) {
+    const int r = M.src_roi[s.window_src[q]];
+    if (r >= 0 && s.window_t[q] >= start && s.window_t[q] < stop) count[r] += 1.0;
+  }
+  for (int r = 0; r < M.n; ++r) {
+    M.acc[r] = M.acc[r] + count[r];
+    if ((M.step + 1) % M.bin_steps[r] == 0) {
+      const double nsrc = (double)(M.out_off[r + 1] - M.out_off[r]);
+      const double bin = (double)M.bin_steps[r] * M.dt_macro;
+      M.rate[r] = nsrc > 0.0 ? M.acc[r] / (nsrc * bin) * 1000.0 : 0.0;
+      M.acc[r] = 0.0;
+    }
+    double sum = 0.0;
+    for (int64_t q = M.out_off[r]; q < M.out_off[r + 1]; ++q) sum += s.v[M.out_node[q]];
+    M.meanv[r] = M.out_off[r + 1] > M.out_off[r] ? sum / (double)(M.out_off[r + 1] - M.out_off[r]) : 0.0;
+  }
+  // 5. REGION advance over [start, stop)
+  for (int r = 0; r < M.n; ++r) {
+    const double* p = &M.wc[(size_t)r * WC_NPARAMS];
+    const double u = M.coupling[r] + p[WC_K_MICRO] * (M.rate[r] / p[WC_RATE_SCALE]);
+    wc_rk4(p, M.E[r], M.I[r], u, M.dt_macro);
+  }
+  // 6. publish the boundary into the history ring
+  M.step += 1;
+  for (int r = 0; r < M.n; ++r) M.hist[(M.step % M.cap) * M.n + r] = M.E[r];
+}
+
+}  // namespace
+
+extern "C" {
+
+void* oracle_build_macro(const mind_model_desc* d, const mind_macro_desc* m, double dt, uint64_t seed) {
+  Sim* sp = static_cast<Sim*>(oracle_build(d, dt, seed));
+  Sim& s = *sp;
+  Macro* M = new Macro();
+  s.macro = M;
+  M->n = m->n_roi;
+  M->dt_macro = m->dt_macro;
+  M->ratio = (int64_t)nearbyint(m->dt_macro / dt);
+  M->W.assign(m->weights, m->weights + (size_t)M->n * M->n);
+  M->wc.assign(m->wc_params, m->wc_params + (size_t)M->n * WC_NPARAMS);
+  M->delay_steps.resize((size_t)M->n * M->n);
+  int64_t maxd = 0;
+  for (size_t q = 0; q < M->delay_steps.size(); ++q) {
+    M->delay_steps[q] = (int64_t)nearbyint(m->delays[q] / m->dt_macro);  // pack_macro:225
+    if (M->W[q] != 0.0) maxd = std::max(maxd, M->delay_steps[q]);
+  }
+  M->cap = maxd + 1;
+  M->bin_steps.resize(M->n);
+  for (int r = 0; r < M->n; ++r) M->bin_steps[r] = std::max<int64_t>(1, (int64_t)nearbyint(m->bin_ms[r] / m->dt_macro));
+  // virtual sources of the MACRO2MICRO bindings (model.jl:622-628), after the real ones
+  const int64_t nb = m->input_offsets[M->n];
+  M->n_points = d->n_points;
+  M->first_virtual = d->n_sources;
+  M->in_off.assign(m->input_offsets, m->input_offsets + M->n + 1);
+  M->in_roi.resize(nb);
+  M->stream_id.resize(nb);
+  for (int r = 0; r < M->n; ++r)
+    for (int64_t b = m->input_offsets[r]; b < m->input_offsets[r + 1]; ++b) M->in_roi[b] = r;
+  // one random identity per (mechanism, ROI, parameters): all bindings of an ROI share it
+  std::vector<int64_t> roi_stream(M->n, -1);
+  int64_t next_id = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    if (roi_stream[M->in_roi[b]] < 0) roi_stream[M->in_roi[b]] = next_id++;
+    M->stream_id[b] = (uint64_t)roi_stream[M->in_roi[b]];
+  }
+  for (int64_t b = 0; b < nb; ++b) {
+    s.source_sid.push_back(-(b + 1));
+    s.source_target.push_back((int32_t)(d->n_points + b + 1));
+    s.target_sources.push_back({(int32_t)(d->n_sources + b)});
+    s.sequences.push_back(0);
+    s.target_rank.push_back(s.rank_total + b + 1);  // input rows rank after NetStims
+    s.fanout.push_back(s.fanout.back() + 1);
+    s.edge_target.push_back(m->input_point[b] + 1);
+    s.edge_delay.push_back(m->input_delay[b]);
+    s.edge_weight.push_back(m->input_weight[b]);
+  }
+  const int64_t rank_all = s.rank_total + nb;
+  s.n_sources = d->n_sources + nb;
+  s.source_rank.resize(s.n_sources);
+  for (int64_t k = 0; k < s.n_sources; ++k)
+    s.source_rank[k] = s.source_target[k] == 0 ? rank_all + k + 1 : s.target_rank[s.source_target[k] - 1];
+  M->out_off.assign(m->output_offsets, m->output_offsets + M->n + 1);
+  M->out_src.assign(m->output_source, m->output_source + m->output_offsets[M->n]);
+  M->src_roi.assign(s.n_sources, -1);
+  for (int r = 0; r < M->n; ++r)
+    for (int64_t q = M->out_off[r]; q < M->out_off[r + 1]; ++q) {
+      M->src_roi[M->out_src[q]] = r;
+      int64_t node = -1;
+      for (size_t k = 0; k < s.det_source.size(); ++k)
+        if (s.det_source[k] == M->out_src[q]) node = s.det_node[k];
+      M->out_node.push_back((int32_t)node);
+    }
+  return sp;
+}
+
+int oracle_initialize_macro(void* p) {
+  Sim& s = *static_cast<Sim*>(p);
+  Macro& M = *static_cast<Macro*>(s.macro);
+  int rc = oracle_initialize(p);
+  if (rc) return rc;
+  std::fill(s.sequences.begin(), s.sequences.end(), 0);
+  M.step = 0;
+  M.E.resize(M.n); M.I.resize(M.n);
+  M.coupling.assign(M.n, 0.0); M.rate.assign(M.n, 0.0); M.acc.assign(M.n, 0.0);
+  M.meanv.assign(M.n, 0.0);
+  M.hist.assign((size_t)M.cap * M.n, 0.0);
+  for (int r = 0; r < M.n; ++r) {
+    M.E[r] = M.wc[(size_t)r * WC_NPARAMS + WC_E0];
+    M.I[r] = M.wc[(size_t)r * WC_NPARAMS + WC_I0];
+    for (int64_t k = 0; k < M.cap; ++k) M.hist[k * M.n + r] = M.E[r];  // initialize_history_kernel!
+    double sum = 0.0;
+    for (int64_t q = M.out_off[r]; q < M.out_off[r + 1]; ++q) sum += s.v[M.out_node[q]];
+    M.meanv[r] = M.out_off[r + 1] > M.out_off[r] ? sum / (double)(M.out_off[r + 1] - M.out_off[r]) : 0.0;
+  }
+  return 0;
+}
+
+int oracle_run_macro(void* p, double stop, int32_t record_spikes, mind_macro_result* out) {
+  Sim& s = *static_cast<Sim*>(p);
+  Macro& M = *static_cast<Macro*>(s.macro);
+  if (!s.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  const double now = (double)M.step * M.dt_macro;
+  const double duration = stop - now;
+  const int64_t count = (int64_t)nearbyint(duration / M.dt_macro);
+  const double big = std::max(std::max(fabs(stop), fabs(now)), M.dt_macro);
+  const double tol = 64.0 * (nextafter(big, INFINITY) - big);
+  if (!(count >= 0 && fabs(duration - (double)count * M.dt_macro) <= tol))
+    return fail(MIND_EARG, "stop time must be an integer number of steps after the current time");
+  const int64_t samples = count + 1;
+  std::vector<double> tr((size_t)5 * M.n * samples);
+  s.recording_spikes = record_spikes != 0;
+  if (M.step != 0 || !s.recording_spikes) {
+    s.spike_t.clear();
+    s.spike_src.clear();
+  }
+  const int64_t start_step = M.step;
+  macro_record(M, tr, samples, 0);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int64_t j = 1; j <= count; ++j) {
+    advance_macro(s, M);
+    macro_record(M, tr, samples, j);
+    if (s.error) break;
+  }
+  flush(s);
+  const auto t1 = std::chrono::steady_clock::now();
+  s.recording_spikes = false;
+  if (s.error) {
+    s.initialized = false;
+    return fail(s.error, "macro run failed");
+  }
+  out->n_samples = samples;
+  out->n_roi = M.n;
+  out->time = (double*)malloc(sizeof(double) * samples);
+  for (int64_t j = 0; j < samples; ++j) out->time[j] = (double)(start_step + j) * M.dt_macro;
+  out->traces = (double*)malloc(sizeof(double) * tr.size());
+  memcpy(out->traces, tr.data(), sizeof(double) * tr.size());
+  const int64_t ns = (int64_t)s.spike_t.size();
+  std::vector<int64_t> order(ns);
+  for (int64_t i = 0; i < ns; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    const double ta = s.spike_t[a], tb = s.spike_t[b];
+    const int64_t ia = s.source_sid[s.spike_src[a]], ib = s.source_sid[s.spike_src[b]];
+    return ta < tb || (ta == tb && ia < ib);
+  });
+  out->n_spikes = ns;
+  out->spike_times = (double*)malloc(sizeof(double) * ns + 8);
+  out->spike_sids = (int64_t*)malloc(sizeof(int64_t) * ns + 8);
+  for (int64_t i = 0; i < ns; ++i) {
+    out->spike_times[i] = s.spike_t[order[i]];
+    out->spike_sids[i] = s.source_sid[s.spike_src[order[i]]];
+  }
+  out->run_seconds = std::chrono::duration<double>(t1 - t0).count();
+  return 0;
+}
+
+void oracle_macro_result_free(mind_macro_result* r) {
+  free(r->time);
+  free(r->traces);
+  free(r->spike_times);
+  free(r->spike_sids);
+  memset(r, 0, sizeof(*r));
+}
+
+void oracle_free_macro(void* p) {
+  Sim* s = static_cast<Sim*>(p);
+  delete static_cast<Macro*>(s->macro);
+  s->macro = nullptr;
+  delete s;
+}
+
+// Test hook: one RK4 step of the Wilson-Cowan REGION.
+void oracle_wc_rk4(const double* params, double* e, double* i, double u, double h) {
+  wc_rk4(params, *e, *i, u, h);
+}
 
 }  // extern "C"

@@ -183,3 +183,66 @@ def exp(x):
 
 def log(x):
     return lib().oracle_log(float(x))
+
+
+class OracleMacro:
+    """The oracle's view of one MacroModel (same descriptors as libmind)."""
+
+    def __init__(self, model, dt=0.025, dt_macro=0.1, seed=1):
+        from mind_amd import macro as M
+
+        L = _macro_lib()
+        self.model = model
+        d, k1 = model.micro.descriptor()
+        m, k2 = M.macro_descriptor(model, dt_macro)
+        self.h = L.oracle_build_macro(C.byref(d), C.byref(m), float(dt), int(seed))
+        del k1, k2
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_free_macro(self.h)
+            self.h = None
+
+    def initialize(self):
+        rc = lib().oracle_initialize_macro(self.h)
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return self
+
+    def run(self, stop, record_spikes=True):
+        from mind_amd import macro as M
+
+        res = M.MacroResultC()
+        rc = lib().oracle_run_macro(self.h, float(stop), int(bool(record_spikes)), C.byref(res))
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        try:
+            return M.result_from_c(res, len(self.model.connectivity.labels))
+        finally:
+            lib().oracle_macro_result_free(C.byref(res))
+
+
+def _macro_lib():
+    from mind_amd import macro as M
+
+    L = lib()
+    if not getattr(L, "_macro", False):
+        L.oracle_build_macro.restype = C.c_void_p
+        L.oracle_build_macro.argtypes = [C.POINTER(_abi.ModelDesc), C.POINTER(M.MacroDesc),
+                                         C.c_double, C.c_uint64]
+        L.oracle_initialize_macro.argtypes = [C.c_void_p]
+        L.oracle_run_macro.argtypes = [C.c_void_p, C.c_double, C.c_int32,
+                                       C.POINTER(M.MacroResultC)]
+        L.oracle_macro_result_free.argtypes = [C.POINTER(M.MacroResultC)]
+        L.oracle_free_macro.argtypes = [C.c_void_p]
+        L.oracle_wc_rk4.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_double, C.c_double]
+        L._macro = True
+    return L
+
+
+def wc_rk4(params, e, i, u, h):
+    p = (C.c_double * len(params))(*params)
+    ce, ci = C.c_double(e), C.c_double(i)
+    _macro_lib().oracle_wc_rk4(p, C.byref(ce), C.byref(ci), float(u), float(h))
+    return ce.value, ci.value

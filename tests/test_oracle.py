@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from mind_amd import MicroModel, Section, configs, mechanism_record, voltage_record
+from mind_amd import MicroModel, Section, configs, macro as M, mechanism_record, voltage_record
 from mind_amd.model import pack_nodes
 
 mp.mp.dps = 60
@@ -304,3 +304,65 @@ def test_run_rejects_fractional_stop():
     sim = oracle.OracleSim(m).initialize()
     with pytest.raises(RuntimeError, match="integer number of steps"):
         sim.run(0.0301)
+
+
+# --- micro-macro coupling --------------------------------------------------------
+def test_wilson_cowan_rk4_step_against_scipy():
+    from scipy.integrate import solve_ivp
+
+    from mind_amd.macro import WC_DEFAULTS, WC_PARAMS
+
+    p = dict(WC_DEFAULTS)
+    vec = [p[k] for k in WC_PARAMS]
+    u = 0.3
+
+    def S(x, a, th):
+        return 1 / (1 + np.exp(-a * (x - th))) - 1 / (1 + np.exp(a * th))
+
+    def f(t, y):
+        e, i = y
+        return [(-e + S(p["c_ee"] * e - p["c_ei"] * i + p["P"] + u, p["a_e"], p["theta_e"])) / p["tau_e"],
+                (-i + S(p["c_ie"] * e - p["c_ii"] * i + p["Q"], p["a_i"], p["theta_i"])) / p["tau_i"]]
+
+    def local_error(h, e, i):
+        exact = solve_ivp(f, (0, h), [e, i], rtol=1e-14, atol=1e-16, method="DOP853").y[:, -1]
+        return np.abs(np.array(oracle.wc_rk4(vec, e, i, u, h)) - exact), exact
+
+    e, i = 0.2, 0.1
+    for _ in range(20):
+        err, exact = local_error(0.1, e, i)
+        assert err.max() < 1e-9
+        e, i = exact
+    # fourth-order method: the local error scales as h^5
+    big, _ = local_error(0.8, 0.2, 0.1)
+    small, _ = local_error(0.4, 0.2, 0.1)
+    assert 20 < big.max() / small.max() < 45
+
+
+def test_macro_coupling_reads_the_delayed_history():
+    """coupling_0(t) = W_01 E_1(t - D_01): a delay of 3.3 ms is 33 macro steps."""
+    from mind_amd import MicroModel
+
+    W = np.array([[0.0, 0.15], [0.0, 0.0]])
+    D = np.array([[0.0, 3.3], [0.0, 0.0]])
+    mm = M.MacroModel(M.Connectivity(W, D))
+    for r in range(2):
+        M.use_macro(mm.roi(r), "WilsonCowan", initial_state={"E": 0.1 * (r + 1), "I": 0.05})
+    mm.micro = MicroModel()
+    M.connect(mm.roi(1), mm.roi(0), "LinearDelay")
+    r = oracle.OracleMacro(mm).initialize().run(20.0)
+    E1 = r.E[1]
+    c0 = r.coupling[0]
+    # coupling recorded after step s uses E_1 at boundary s - 33 (the initial value before 0)
+    for s in range(1, len(c0)):
+        src = E1[max(s - 1 - 33, 0)]
+        assert c0[s] == 0.15 * src
+
+
+def test_micro_macro_runs_and_is_reproducible():
+    mm, _ = configs.micro_macro(n_roi=3, cells=4, nseg=20, input_weight=5e-3)
+    a = oracle.OracleMacro(mm).initialize().run(30.0)
+    b = oracle.OracleMacro(mm).initialize().run(30.0)
+    assert np.array_equal(a.E, b.E) and np.array_equal(a.spike_times, b.spike_times)
+    assert a.E.shape == (3, 301)
+    assert np.all(np.isfinite(a.E)) and np.all(a.rate >= 0)
