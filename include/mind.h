@@ -249,6 +249,57 @@ int mind_time(const mind_state* state, double* t, int64_t* step);
 int mind_window_steps(const mind_state* state, int32_t* steps);
 /* Batched method: node budget per workgroup, block size and workgroup count chosen by the autotuner. */
 int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int64_t* groups);
+/* ---- Surrogate MLP (no counterpart in the reference) ----------------------------
+ * n_in -> hidden (x layers, GELU) -> n_out on BF16 MFMA with FP32 accumulation;
+ * FP32 master weights and Adam moments. Flat parameter order: per linear layer
+ * W (k x n, row-major) then b (n). */
+typedef struct mind_mlp mind_mlp;
+int mind_mlp_create(int32_t n_in, int32_t hidden, int32_t layers, int32_t n_out, int64_t max_batch,
+                    mind_mlp** out);
+void mind_mlp_destroy(mind_mlp* mlp);
+int mind_mlp_param_count(const mind_mlp* mlp, int64_t* n);
+int mind_mlp_set_params(mind_mlp* mlp, const float* params, int64_t n);
+int mind_mlp_get_params(mind_mlp* mlp, float* params, int64_t n);
+int mind_mlp_forward(mind_mlp* mlp, const float* x, int64_t batch, float* y);
+int mind_mlp_grad(mind_mlp* mlp, const float* x, const float* y, int64_t batch, float* loss, float* grads);
+int mind_mlp_train(mind_mlp* mlp, const float* X, const float* Y, int64_t n, int64_t batch, int32_t epochs,
+                   float lr, uint64_t seed, float* epoch_loss);
+
+/* ---- Surrogate micro-macro coupling (no counterpart in the reference) -----------
+ * The ROI graph of a mind_macro_desc (Wilson-Cowan RK4, delayed LinearDelay
+ * coupling; its MACRO2MICRO / MICRO2MACRO bindings are ignored) integrated in one
+ * persistent single-workgroup kernel with a trained MLP in place of every micro
+ * population. Each ROI's drive is rate_scale * E at the start of every macro step;
+ * at the end of every bin (bin_ms, the same for all ROIs) the network maps
+ * [the last `history` drive bins (Hz, oldest first; the first bin repeated before
+ * the run has that many), previous rate (Hz), previous mean V (mV)] to
+ * [rate (Hz, clamped at 0), mean soma V (mV)] for all ROIs at once (16 per MFMA
+ * tile). A bin whose raw inputs leave [x_lo, x_hi] is counted out of distribution;
+ * the caller then falls back to the plain micro-macro simulation. */
+typedef struct mind_surrogate_desc {
+  int32_t n_in, hidden, layers, n_out; /* n_in = history + 2, n_out = 2 */
+  int32_t history;
+  const float* params;                 /* flat order of mind_mlp */
+  int64_t n_params;
+  const float* x_mean;                 /* n_in: inputs are (x - x_mean) / x_std */
+  const float* x_std;
+  const float* y_mean;                 /* n_out: outputs are y * y_std + y_mean */
+  const float* y_std;
+  const float* x_lo;                   /* n_in: training range of the raw inputs */
+  const float* x_hi;
+  double v_rest;                       /* previous mean V before the first bin */
+} mind_surrogate_desc;
+typedef struct mind_surrogate mind_surrogate;
+int mind_build_surrogate(const mind_macro_desc* macro, const mind_surrogate_desc* net, int32_t device,
+                         mind_surrogate** out);
+/* Initial ROI state and history (as mind_initialize for the plain model), rate 0, V = v_rest. */
+int mind_initialize_surrogate(mind_surrogate* s);
+/* Advance whole macro steps to `stop`; the result holds no spikes. ood_windows receives the
+ * number of bins with out-of-range inputs and first_ood_time the end time of the first (-1 if none). */
+int mind_run_surrogate(mind_surrogate* s, double stop, mind_macro_result* out, int64_t* ood_windows,
+                       double* first_ood_time);
+void mind_surrogate_destroy(mind_surrogate* s);
+
 /* Host-only packing query (no device needed): node and compartment counts and,
  * when the arrays are non-NULL, the packed cable geometry in canonical node order. */
 int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int64_t* compartments,
