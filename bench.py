@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--macro-ms", type=float, default=2000.0)
+    ap.add_argument("--no-macro", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded path is "
@@ -122,6 +124,8 @@ def main():
                "sample": f"the same {args.cells}-cell model, first {args.cpu_steps} steps "
                          f"after initialize, oracle (C++ FP64, -O3 -ffp-contract=off), 1 thread"}
 
+    macro = None if args.no_macro else macro_leg(args.macro_ms)
+
     line = {
         "metric": "compartment-steps/s (batched vs plain); macro sim-s/wall-s surrogate vs plain",
         "value": value,
@@ -153,8 +157,55 @@ def main():
         "cpu_baseline": cpu,
         "sim_seconds_per_wall_second": (args.steps * dt / 1e3) / wall,
         "spikes_in_timed_run": extra["spikes"],
+        "macro": macro,
     }
     print(json.dumps(line), flush=True)
+
+
+def macro_leg(stop):
+    """16 Wilson-Cowan ROIs x 256 ball-and-stick cells (evaluation seed 1): the
+    detailed micro-macro run (batched micro kernel) against the surrogate run of
+    the same ROI graph, in simulated s per wall s of the run loop."""
+    import numpy as np
+
+    from mind_amd import configs, destroy
+    from mind_amd import macro as M
+    from mind_amd import surrogate as S
+
+    mm, _ = configs.micro_macro(seed=1)
+    t0 = time.time()
+    M.build(mm, seed=1, method="batched")
+    M.initialize(mm)
+    log(f"micro-macro: build+autotune+initialize {time.time() - t0:.1f}s")
+    plain = M.run(mm, stop, record_spikes=False)
+    log(f"micro-macro: {stop:.0f} ms in {plain.run_seconds:.2f}s")
+    destroy(mm)
+    mm.micro.state = None
+    out = {"workload": "16 Wilson-Cowan ROIs (RK4, dt 0.1 ms, delayed all-to-all coupling) x 256 "
+                       "200-compartment ball-and-stick cells, Poisson drive, 1 ms binned rate; seed 1",
+           "stop_ms": stop, "plain_sim_s_per_wall_s": stop / 1e3 / plain.run_seconds,
+           "plain_run_s": plain.run_seconds}
+    if not os.path.exists(S.WEIGHTS):
+        out["surrogate"] = None
+        return out
+    net = S.Network.load()
+    cs = S.CoupledSurrogate(mm, net).initialize()
+    sur = cs.run(stop)
+    cs.close()
+    rp = plain.rate[:, 10::10].mean(axis=1)
+    rs = sur.result.rate[:, 10::10].mean(axis=1)
+    err = np.abs(rs - rp) / rp
+    verr = np.abs(sur.result.mean_v[:, 10::10].mean(axis=1) - plain.mean_v[:, 10::10].mean(axis=1))
+    out.update({
+        "surrogate_sim_s_per_wall_s": stop / 1e3 / sur.result.run_seconds,
+        "surrogate_run_s": sur.result.run_seconds,
+        "speedup": plain.run_seconds / sur.result.run_seconds,
+        "rate_rel_err_max": float(err.max()), "rate_rel_err_mean": float(err.mean()),
+        "mean_v_abs_err_max_mV": float(verr.max()),
+        "ood_windows": sur.ood_windows,
+    })
+    log(f"surrogate: {stop:.0f} ms in {sur.result.run_seconds:.4f}s; rate error max {err.max():.4f}")
+    return out
 
 
 if __name__ == "__main__":

@@ -53,7 +53,7 @@ enum { HH_GNABAR, HH_GKBAR, HH_GL, HH_EL, HH_M, HH_H, HH_N, HH_ENA, HH_EK, HH_GN
        HH_INA, HH_IK, HH_IL, HH_MINF, HH_HINF, HH_NINF, HH_MTAU, HH_HTAU, HH_NTAU, HH_NFIELDS };
 enum { PAS_G, PAS_E, PAS_I, PAS_NFIELDS };
 enum { EXPSYN_TAU, EXPSYN_E, EXPSYN_G, EXPSYN_I, EXPSYN_NFIELDS };
-enum { EXP2SYN_TAU1, EXP2SYN_TAU2, EXP2SYN_E, EXP2SYN_A, EXP2SYN_B, EXP2SYN_G, EXP2SYN_I,
+enum { EXP2SYN_TAU1, EXP2SYN_TAU2, EXP2SYN_E, EXP2SYN_A, EXP2SYN_B, EXP2SYN_I, EXP2SYN_G,
        EXP2SYN_FACTOR, EXP2SYN_NFIELDS };
 enum { ICLAMP_DEL, ICLAMP_DUR, ICLAMP_AMP, ICLAMP_I, ICLAMP_NFIELDS };
 enum { NETSTIM_INTERVAL, NETSTIM_NUMBER, NETSTIM_START, NETSTIM_NOISE, NETSTIM_EVENT,
@@ -274,7 +274,7 @@ int mind_mlp_train(mind_mlp* mlp, const float* X, const float* Y, int64_t n, int
  * [the last `history` drive bins (Hz, oldest first; the first bin repeated before
  * the run has that many), previous rate (Hz), previous mean V (mV)] to
  * [rate (Hz, clamped at 0), mean soma V (mV)] for all ROIs at once (16 per MFMA
- * tile). A bin whose raw inputs leave [x_lo, x_hi] is counted out of distribution;
+ * tile), plus an optional linear skip map of the same normalised inputs. A bin whose raw inputs leave [x_lo, x_hi] is counted out of distribution;
  * the caller then falls back to the plain micro-macro simulation. */
 typedef struct mind_surrogate_desc {
   int32_t n_in, hidden, layers, n_out; /* n_in = history + 2, n_out = 2 */
@@ -287,6 +287,8 @@ typedef struct mind_surrogate_desc {
   const float* y_std;
   const float* x_lo;                   /* n_in: training range of the raw inputs */
   const float* x_hi;
+  const float* skip_w;                 /* NULL, or [n_in * n_out]: linear skip on the normalised */
+  const float* skip_b;                 /* inputs added to the network output (n_out) */
   double v_rest;                       /* previous mean V before the first bin */
 } mind_surrogate_desc;
 typedef struct mind_surrogate mind_surrogate;

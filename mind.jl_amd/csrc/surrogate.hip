@@ -26,6 +26,7 @@ constexpr int kThreads = 1024;  // 16 waves
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxWidth = 512;  // padded n_in and hidden
 constexpr int kMaxLayers = 8;
+constexpr int kMaxIn = 256;  // network inputs
 constexpr int64_t kStepsPerLaunch = 1 << 16;
 
 struct SurDev {
@@ -45,6 +46,7 @@ struct SurDev {
   const bf16x4* frag[kMaxLayers];
   const float* bias[kMaxLayers];
   const float *x_mean, *x_std, *y_mean, *y_std, *x_lo, *x_hi;
+  const float *skip_w, *skip_b;  // optional linear skip [n_in][n_out], [n_out]
   int32_t n_in;
   double* trace;  // [5][n][samples]
   int64_t samples, trace_start;
@@ -96,6 +98,7 @@ __device__ void mlp_rows(const SurDev& s, uint16_t (*act)[16][kMaxWidth + 4], fl
 __global__ void __launch_bounds__(kThreads) k_surrogate_run(SurDev s, int64_t count) {
   __shared__ uint16_t act[2][16][kMaxWidth + 4];
   __shared__ float out[16][16];
+  __shared__ float xin[16][kMaxIn];
   __shared__ int ood_row[16];
   const int tid = threadIdx.x;
   const int64_t m0 = s.clock[0];
@@ -145,6 +148,7 @@ __global__ void __launch_bounds__(kThreads) k_surrogate_run(SurDev s, int64_t co
             const float f = (float)raw;
             if (!(f >= s.x_lo[k] && f <= s.x_hi[k])) ood_row[row] = 1;
             x = (f - s.x_mean[k]) / s.x_std[k];
+            xin[row][k] = x;
           }
           act[0][row][k] = valid ? f32_to_bf16(x) : (uint16_t)0;
         }
@@ -152,8 +156,18 @@ __global__ void __launch_bounds__(kThreads) k_surrogate_run(SurDev s, int64_t co
         mlp_rows(s, act, out);
         if (tid < 16 && r0 + tid < n) {
           const int r = r0 + tid;
-          const float y0 = out[tid][0] * s.y_std[0] + s.y_mean[0];
-          const float y1 = out[tid][1] * s.y_std[1] + s.y_mean[1];
+          float o0 = out[tid][0], o1 = out[tid][1];
+          if (s.skip_w) {
+            float k0 = 0.0f, k1 = 0.0f;
+            for (int k = 0; k < s.n_in; ++k) {
+              k0 = k0 + xin[tid][k] * s.skip_w[k * 2 + 0];
+              k1 = k1 + xin[tid][k] * s.skip_w[k * 2 + 1];
+            }
+            o0 = o0 + (k0 + s.skip_b[0]);
+            o1 = o1 + (k1 + s.skip_b[1]);
+          }
+          const float y0 = o0 * s.y_std[0] + s.y_mean[0];
+          const float y1 = o1 * s.y_std[1] + s.y_mean[1];
           s.rate[r] = y0 > 0.0f ? (double)y0 : 0.0;
           s.meanv[r] = (double)y1;
           if (ood_row[tid]) {
@@ -248,7 +262,7 @@ int mind_build_surrogate(const mind_macro_desc* md, const mind_surrogate_desc* n
   if (n <= 0) return fail(MIND_EARG, "the macro model needs at least one ROI");
   if (nd->history < 1 || nd->n_in != nd->history + 2 || nd->n_out != 2 || nd->layers < 1 ||
       nd->layers + 1 > kMaxLayers || nd->hidden < 1 || MlpShape::pad(nd->hidden) > kMaxWidth ||
-      MlpShape::pad(nd->n_in) > kMaxWidth)
+      MlpShape::pad(nd->n_in) > kMaxWidth || nd->n_in > kMaxIn || !nd->skip_w != !nd->skip_b)
     return fail(MIND_EARG, "unsupported surrogate network shape");
   const MlpShape sh{nd->n_in, nd->hidden, nd->layers, nd->n_out};
   int64_t nparam = 0;
@@ -351,6 +365,13 @@ int mind_build_surrogate(const mind_macro_desc* md, const mind_surrogate_desc* n
   d.y_mean = pf;
   if ((rc = S->upload(&pf, nd->y_std, (size_t)nd->n_out))) return fin(rc);
   d.y_std = pf;
+  d.skip_w = d.skip_b = nullptr;
+  if (nd->skip_w) {
+    if ((rc = S->upload(&pf, nd->skip_w, (size_t)nd->n_in * nd->n_out))) return fin(rc);
+    d.skip_w = pf;
+    if ((rc = S->upload(&pf, nd->skip_b, (size_t)nd->n_out))) return fin(rc);
+    d.skip_b = pf;
+  }
   *out = reinterpret_cast<mind_surrogate*>(S);
   return 0;
 }

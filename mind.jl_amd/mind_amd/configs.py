@@ -150,7 +150,7 @@ def tree_network(ncells=5000, median=400, sigma=0.8, lo=50, hi=4000, synapses=20
     return model, dict(cells=cells, syns=syns, sizes=sizes, tstop=500.0)
 
 
-def micro_macro(n_roi=16, cells=256, nseg=200, seed=3, input_weight=2e-3, rate_scale=200.0,
+def micro_macro(n_roi=16, cells=256, nseg=200, seed=3, input_weight=5e-3, rate_scale=200.0,
                 k_micro=0.5, threshold=-20.0, wc=None):
     """Wilson-Cowan ROIs coupled all-to-all (W U(0, 0.2), D U(2, 10) ms), each bound to a
     population of ball-and-stick variants (hh soma, nseg-segment pas dendrite of

@@ -34,7 +34,7 @@ _TABLE = {
     "pas": Mechanism("pas", 1, "density", ("g", "e", "i")),
     "ExpSyn": Mechanism("ExpSyn", 2, "point", ("tau", "e", "g", "i"), True),
     "Exp2Syn": Mechanism(
-        "Exp2Syn", 3, "point", ("tau1", "tau2", "e", "A", "B", "g", "i", "factor"), True
+        "Exp2Syn", 3, "point", ("tau1", "tau2", "e", "A", "B", "i", "g", "factor"), True
     ),
     "IClamp": Mechanism("IClamp", 4, "point", ("del", "dur", "amp", "i")),
     "NetStim": Mechanism(
