@@ -306,6 +306,9 @@ void mind_surrogate_destroy(mind_surrogate* s);
  * when the arrays are non-NULL, the packed cable geometry in canonical node order. */
 int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int64_t* compartments,
                     int32_t* parent, double* area, double* cap, double* axial, double* axial_diag);
+/* How the mechanism phases were built: the NMODL lowering of the named MOD files
+ * (with their SHA-256 prefixes) or the hand lowering. No state needed. */
+const char* mind_build_info(void);
 /* Number of visible HIP devices (0 when none); no state needed. */
 int mind_device_count(int32_t* count);
 /* Wait until every launch of this state has completed. */

@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, cons
         const int32_t e = d.win[seg].edge;
         const int32_t target = d.edge_target[e];
         const GroupDev& G = d.g[d.point_group[target]];
-        synapse_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
+        phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
         ++seg;
       }
     }
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, cons
         int64_t q = r;
         do {  // a run of rows on one node, in row order
           double value, shifted;
-          mech_current(G.mech, col, q, v[li], tc, value, shifted, true);
+          phase_current(G.mech, col, q, v[li], tc, value, shifted, true);
           const double scale = G.point ? 1.0 : d.area[node] * 0.01;
           acc_r -= scale * value;
           acc_d += scale * (shifted - value) / 0.001;
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, cons
       const Cols col = cols(G);
       const int64_t r0 = G.cell_rows[c0], r1 = G.cell_rows[c1];
       for (int64_t r = r0 + tid; r < r1; r += T)
-        mech_advance(G.mech, col, r, v[G.node[r] - n0], dt, d.celsius);
+        phase_advance(G.mech, col, r, v[G.node[r] - n0], dt, d.celsius);
     }
     if (rec1 > rec0) {
       __syncthreads();
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, cons
         const int32_t e = d.win[seg].edge;
         const int32_t target = d.edge_target[e];
         const GroupDev& G = d.g[d.point_group[target]];
-        synapse_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
+        phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
         ++seg;
       }
     }

@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "layout.h"
-#include "mech.h"
+#include "mechanisms.h"
 
 namespace mind {
 

@@ -23,7 +23,7 @@
 #if defined(__HIPCC__)
 #define MIND_HD __host__ __device__ inline
 #else
-#define MIND_HD static inline
+#define MIND_HD inline
 #endif
 
 MIND_HD uint64_t mind_bits(double x) { return __builtin_bit_cast(uint64_t, x); }

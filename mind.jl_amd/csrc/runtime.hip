@@ -55,7 +55,7 @@ __global__ void k_init_group(Dev d, int gi) {
     // and the NetStim source digest, then the INITIAL block.
     return;
   }
-  mech_initial(G.mech, cols(G), r, d.v[G.node[r]], d.celsius);
+  phase_initial(G.mech, cols(G), r, d.v[G.node[r]], d.celsius);
 }
 
 __global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest) {
@@ -68,7 +68,7 @@ __global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest) 
   s.key[1] = logical[r] >> 32;
   s.key[2] = (uint64_t)(uint32_t)digest;
   stream_setseq(s, d.seed, 0.0);
-  const NetStimOut o = netstim_initial(cols(G), r, s, d.seed);
+  const NetStimOut o = phase_netstim_initial(cols(G), r, s, d.seed);
   d.ns_stream[r] = s;
   d.ns_valid[r] = 0;
   if (o.send) {
@@ -90,7 +90,7 @@ __global__ void k_currents_fields(Dev d, int gi, double t) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= G.rows) return;
   double value, shifted;
-  mech_current(G.mech, cols(G), r, d.v[G.node[r]], t, value, shifted, true);
+  phase_current(G.mech, cols(G), r, d.v[G.node[r]], t, value, shifted, true);
 }
 
 __global__ void k_record_sample(Dev d, int64_t sample) {
@@ -119,7 +119,7 @@ __device__ inline int64_t deliver_cell(const Dev& d, int64_t seg, int64_t end, i
     const int32_t e = d.win[seg].edge;
     const int32_t target = d.edge_target[e];
     const GroupDev& G = d.g[d.point_group[target]];
-    synapse_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
+    phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
     ++seg;
   }
   return seg;
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
       for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r) {
         const int32_t node = G.node[r];
         double value, shifted;
-        mech_current(G.mech, col, r, v[node], tc, value, shifted, true);
+        phase_current(G.mech, col, r, v[node], tc, value, shifted, true);
         const double scale = G.point ? 1.0 : d.area[node] * 0.01;
         rhs[node] -= scale * value;
         diag[node] += scale * (shifted - value) / 0.001;
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
       if (G.mech != MIND_MECH_HH && G.mech != MIND_MECH_EXP2SYN && G.mech != MIND_MECH_EXPSYN) continue;
       const Cols col = cols(G);
       for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r)
-        mech_advance(G.mech, col, r, v[G.node[r]], dt, d.celsius);
+        phase_advance(G.mech, col, r, v[G.node[r]], dt, d.celsius);
     }
     if (d.nrec) record_cell(d, c, k + 1 - d.rec_start);
     seg = deliver_cell(d, seg, seg_end, 2 * k + 1);
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
       const int64_t j = call_of(t, call_lo, dt);
       const double w = edge == 0 ? 0.0 : d.edge_weight[edge - 1];
       Stream s = d.ns_stream[r];
-      const NetStimOut o = netstim_receive(col, r, s, d.seed, t, w, flag);
+      const NetStimOut o = phase_netstim_receive(col, r, s, d.seed, t, w, flag);
       d.ns_stream[r] = s;
       if (o.spiked && d.ns_source[r] >= 0) {
         if (count < d.emit_cap) d.emit_ns[count] = Emit{emit_key(j), o.spike_time, d.ns_source[r], 0};
@@ -544,10 +544,6 @@ State::~State() { release(); }
 
 namespace {
 
-// Source digest of the NetStim mechanism: the first 16 hex digits of the
-// SHA-256 of the reference's mod/NetStim.mod, which seeds its Random123 key
-// (src/compiler/ir.jl:316-324, src/runtime/initialize.jl:309).
-constexpr uint64_t kNetStimDigest = 0xa186833b569ff3b2ull;
 
 }  // namespace
 
@@ -797,6 +793,8 @@ using mind::fail;
 extern "C" {
 
 const char* mind_last_error(void) { return mind::g_error.c_str(); }
+
+const char* mind_build_info(void) { return mind::kMechanismSource; }
 
 int mind_build(const mind_model_desc* model, const mind_build_options* options, mind_state** out) {
   if (!model || !options || !out) return fail(MIND_EARG, "null argument");

@@ -170,8 +170,14 @@ def lib():
     L.mind_kernel_time.argtypes = [C.c_void_p, _p(C.c_int64), _p(C.c_int64), _p(C.c_double)]
     L.mind_batch_work.argtypes = [C.c_void_p, _p(C.c_int64)]
     L.mind_last_error.restype = C.c_char_p
+    L.mind_build_info.restype = C.c_char_p
     _lib = L
     return L
+
+
+def build_info() -> str:
+    """How the library's mechanism phases were built (NMODL lowering of which MOD files)."""
+    return lib().mind_build_info().decode()
 
 
 def device_count() -> int:

@@ -23,4 +23,7 @@ def gpu_available() -> bool:
 def gpu():
     if not gpu_available():
         pytest.skip("no GPU")
+    from mind_amd import _abi
+
+    print(f"libmind: {_abi.LIB_PATH}; mechanisms: {_abi.build_info()}", flush=True)
     return True
