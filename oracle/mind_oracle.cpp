@@ -26,7 +26,7 @@
 //   its children's in increasing node index; Hines elimination visits nodes
 //   in decreasing index order (children before parents within a cell).
 //
-// Parity status (how each part is pinned, see tests/test_oracle_*.py):
+// Parity status (how each part is pinned, see tests/test_oracle.py):
 //   geometry       closed forms for cylinders
 //   Hines solve    dense numpy solve on random trees
 //   hh / pas / syn closed forms, textbook HH rates, mpmath
