@@ -51,7 +51,18 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--macro-ms", type=float, default=2000.0)
     ap.add_argument("--no-macro", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain streaming copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes as C
+
+        from mind_amd import _abi
+
+        g = C.c_double()
+        _abi.check(_abi.lib().mind_copy_bandwidth(0, args.copy_calibration, 20, C.byref(g)))
+        print(json.dumps({"copy_bytes": args.copy_calibration, "repeats": 20, "copy_gbps": g.value}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded path is "
               "design-only", file=sys.stderr)
@@ -149,7 +160,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_batched<0>", "bytes_per_launch": bytes_per_launch,
+            "kernel": "k_window<0>", "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": kms / launches, "launches": launches,
         },
         "plain": plain,

@@ -311,6 +311,10 @@ int mind_pack_nodes(const mind_model_desc* model, double dt, int64_t* nodes, int
 const char* mind_build_info(void);
 /* Number of visible HIP devices (0 when none); no state needed. */
 int mind_device_count(int32_t* count);
+/* Measurement aid, no counterpart in the reference: read+write GB/s of a plain
+ * streaming copy of `bytes` (a multiple of 32) repeated `repeats` times; the
+ * known-size kernel that calibrates HBM traffic counters. */
+int mind_copy_bandwidth(int32_t device, int64_t bytes, int32_t repeats, double* gbps);
 /* Wait until every launch of this state has completed. */
 int mind_synchronize(mind_state* state);
 /* Timing of the dominant cell-kernel launch with HIP events on the state's stream

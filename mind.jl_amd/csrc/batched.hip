@@ -1,20 +1,20 @@
 // Batched window kernel: a workgroup advances a group of consecutive cells
-// through a whole window with their node state (v, rhs, diag) staged in LDS.
+// through a whole window (W steps of the micro step) with its node state staged
+// in LDS as far as it fits.
 //
 // The Hines solve of the reference runs one warp per group of cells with
 // 32-lane "DHS" rounds and FP64 atomics into parents (src/runtime/cuda/micro.jl:688-749,
-// plan src/model/micro/model.jl:906-1077). Here each cell's tree is cut into
-// *chain units*: maximal runs of nodes whose parent has exactly one child,
-// starting from a node with zero or several children. Elimination processes
-// units level by level from the leaves (a unit's level is one more than the
-// highest level among its bottom node's child units), each thread walking one
-// unit bottom-up; a node *pulls* its children's contributions in decreasing
-// child index, which reproduces the CPU restatement's elimination order
-// exactly and needs no atomics. Back substitution walks the units top-down
-// by depth level. Workgroup barriers separate levels.
-//
-// Cells too large for LDS run with v (mode 1) or all node arrays (mode 2) in
-// HBM; the autotuner chooses the node budget per workgroup and the block size.
+// plan src/model/micro/model.jl:906-1077). Here a workgroup solves its cells
+// level by level: elimination visits nodes by height (a node's children are all
+// lower), each node *pulling* its children's contributions in decreasing child
+// index, which reproduces the CPU restatement's elimination order exactly and
+// needs no atomics; back substitution visits nodes by depth. One node per lane,
+// one barrier per level. The int16 parent / child links and the two node orders
+// are built on the host per group; v, rhs, diag and the axial conductances stay
+// in LDS for the whole window when they fit (k_window<0>), so the solve reads no
+// global memory. Cells too large for that keep v (1), v and the axial
+// conductances (2) or everything (3) in HBM. The autotuner chooses the node
+// budget per workgroup and the block size.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,85 +30,153 @@ namespace mind {
 struct BatchDev {
   const int32_t* grp_cell0;
   const int32_t* grp_cell1;
-  const int32_t* unit_start;  // into unit_nodes
-  const int32_t* unit_len;
-  const int32_t* unit_nodes;  // local node index, bottom first
-  const int32_t* tri_order;   // unit ids by (group, level)
-  const int32_t* tri_lvl;     // level offsets into tri_order
-  const int32_t* grp_tri;     // per group: first entry in tri_lvl
-  const int32_t* grp_ntri;    // per group: number of levels
-  const int32_t* bk_order;
-  const int32_t* bk_lvl;
-  const int32_t* grp_bk;
-  const int32_t* grp_nbk;
+  // per group one int16 block (local node ids): parent, lower and higher child,
+  // nodes by height, nodes by depth, and the level offsets of both orders
+  const int16_t* r_tab;
+  const int32_t* grp_rtab;
+  const int32_t* grp_nlh;  // levels by height
+  const int32_t* grp_nld;  // levels by depth
 };
+
+// Launch lists: 0 groups within the node budget, all in LDS (main stream); 1
+// single cells above the budget, all in LDS; 2 v in HBM; 3 v and axial in HBM;
+// 4 all in HBM (lists 1-4 on the second stream).
+constexpr int kLists = 5;
 
 struct BatchPlan {
   int64_t budget = 0;  // node budget per workgroup
   int64_t ngroups = 0;
   int threads = 256;
-  // launch lists: 0 groups within the budget (LDS), 1 v in HBM, 2 all in HBM,
-  // 3 single cells above the budget that still fit in LDS
-  std::vector<int32_t> groups_by_mode[4];
-  int32_t* dev_groups[4] = {nullptr, nullptr, nullptr, nullptr};
-  int64_t max_nodes[4] = {0, 0, 0, 0};
+  std::vector<int32_t> lists[kLists];
+  int32_t* dev_lists[kLists] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t lds[kLists] = {0, 0, 0, 0, 0};  // dynamic LDS bytes of each list
   BatchDev b{};
   std::vector<void*> allocations;
 };
 
 constexpr int kMaxThreads = 512;
-constexpr size_t kLdsLimit = 160 * 1024;
+constexpr int kMaxGroups = 8;    // mechanism groups cached in LDS (the six of the step)
+constexpr int kMaxDetLds = 256;  // detectors of a workgroup cached in LDS
+// dynamic LDS of a workgroup, leaving room for its static tables
+constexpr size_t kLdsResident = 152 * 1024;
 
-template <int MODE>
-__global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
+// int16 entries of a group's table block.
+__host__ __device__ inline int64_t rtab_len(int64_t nn, int64_t nlh, int64_t nld) { return 5 * nn + nlh + 1 + nld + 1; }
+// Dynamic LDS bytes of a workgroup in mode M: 4, 3, 2 or 0 doubles per node plus
+// the table block (modes 0-2).
+inline size_t window_bytes(int m, int64_t nn, int64_t nlh, int64_t nld) {
+  if (m == 3) return 0;
+  const int64_t dbl = m == 0 ? 4 : m == 1 ? 3 : 2;
+  return (size_t)(8 * dbl * nn + (2 * rtab_len(nn, nlh, nld) + 7) / 8 * 8);
+}
+
+// Delivery of the window-list events due at delivery call `call` (thread 0).
+__device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_end, int64_t call) {
+  while (seg < seg_end && d.win[seg].call == call) {
+    const int32_t e = d.win[seg].edge;
+    const int32_t target = d.edge_target[e];
+    const GroupDev& G = d.g[d.point_group[target]];
+    phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
+    ++seg;
+  }
+  return seg;
+}
+
+// M says where a workgroup's node state lives: 0 all in LDS (v, rhs, diag, axial
+// conductances, links and schedule), 1 v in HBM, 2 v and the axial conductances
+// in HBM, 3 everything in HBM (rhs and diag in the global scratch columns, the
+// links and schedule read from the plan).
+template <int M>
+__global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
   extern __shared__ double lds[];
   const int grp = groups[blockIdx.x];
   const int tid = threadIdx.x, T = blockDim.x;
   const int64_t c0 = b.grp_cell0[grp], c1 = b.grp_cell1[grp];
   const int64_t n0 = d.cell_off[c0], n1 = d.cell_off[c1];
   const int nn = (int)(n1 - n0);
-  double* rhs = MODE == 2 ? d.rhs + n0 : lds;
-  double* diag = MODE == 2 ? d.diag + n0 : lds + nn;
-  double* v = MODE == 0 ? lds + 2 * nn : d.v + n0;
-  if (MODE == 0)
-    for (int i = tid; i < nn; i += T) v[i] = d.v[n0 + i];
+  const int nlh = b.grp_nlh[grp], nld = b.grp_nld[grp];
+  const int ldsd = M == 0 ? 4 : M == 1 ? 3 : M == 2 ? 2 : 0;  // doubles per node in LDS
+  double* rhs = M == 3 ? d.rhs + n0 : lds;
+  double* diag = M == 3 ? d.diag + n0 : lds + nn;
+  double* v = M == 0 ? lds + 2 * nn : d.v + n0;
+  double* axl = M == 0 ? lds + 3 * nn : lds + 2 * nn;  // LDS copy (M <= 1)
+  const double* ax = M <= 1 ? axl : d.axial + n0;
+  const int tlen = (int)rtab_len(nn, nlh, nld);
+  const int16_t* rt = b.r_tab + b.grp_rtab[grp];
+  int16_t* tabl = reinterpret_cast<int16_t*>(lds + ldsd * nn);  // LDS copy (M <= 2)
+  const int16_t* tab = M == 3 ? rt : tabl;
+  const int16_t* par = tab;          // parent, -1 at a root
+  const int16_t* clo = par + nn;     // lower-index child, -1 if none
+  const int16_t* chi = clo + nn;     // higher-index child; -1 if fewer than two, -2 if more
+  const int16_t* oh = chi + nn;      // nodes by height
+  const int16_t* od = oh + nn;       // nodes by depth
+  const int16_t* lh = od + nn;       // nlh + 1 offsets into oh
+  const int16_t* ld = lh + nlh + 1;  // nld + 1 offsets into od
+  // per-window constants of the step's critical path: mechanism groups, their
+  // row ranges for this group's cells, and the detectors
+  __shared__ GroupDev sg[kMaxGroups];
+  __shared__ int64_t sr0[kMaxGroups], sr1[kMaxGroups];
+  __shared__ int32_t sdet_k[kMaxDetLds], sdet_n[kMaxDetLds];
+  __shared__ double sdet_thr[kMaxDetLds];
+  const int ngr = d.ngroups < kMaxGroups ? d.ngroups : kMaxGroups;
+  if (tid < ngr) {
+    sg[tid] = d.g[tid];
+    sr0[tid] = sg[tid].cell_rows[c0];
+    sr1[tid] = sg[tid].cell_rows[c1];
+  }
+  for (int i = tid; i < nn; i += T) {
+    if (M == 0) v[i] = d.v[n0 + i];
+    if (M <= 1) axl[i] = d.axial[n0 + i];
+  }
+  if (M <= 2)
+    for (int i = tid; i < tlen; i += T) tabl[i] = rt[i];
   const int64_t k0 = d.clock->step;
   const double dt = d.dt;
   const int32_t bucket = d.cell_bucket[c0];
   int64_t seg = d.seg_off[bucket];
   const int64_t seg_end = d.seg_off[bucket + 1];
   const int64_t det0 = d.cell_det_off[c0], det1 = d.cell_det_off[c1];
+  const int ndl = (int)(det1 - det0) <= kMaxDetLds ? (int)(det1 - det0) : 0;  // detectors cached in LDS
+  for (int q = tid; q < ndl; q += T) {
+    const int32_t kk = d.cell_det[det0 + q];
+    sdet_k[q] = kk;
+    sdet_n[q] = d.det_node[kk] - (int32_t)n0;
+    sdet_thr[q] = d.det_thr[kk];
+  }
   const int64_t rec0 = d.nrec ? d.cell_rec_off[c0] : 0, rec1 = d.nrec ? d.cell_rec_off[c1] : 0;
-  const int tri_base = b.grp_tri[grp], ntri = b.grp_ntri[grp];
-  const int bk_base = b.grp_bk[grp], nbk = b.grp_nbk[grp];
+  if (d.ngroups > kMaxGroups) raise_error(d.error, MIND_EUNSUPPORTED);
   __syncthreads();
   for (int s = 0; s < n; ++s) {
     const int64_t k = k0 + s;
     const double t = (double)k * dt;
     // threshold detection from the completed boundary
-    for (int64_t q = det0 + tid; q < det1; q += T) {
-      const int32_t kk = d.cell_det[q];
-      const bool cur = v[d.det_node[kk] - n0] > d.det_thr[kk];
-      d.det_flags[(int64_t)s * d.ndet + kk] = (cur && !d.det_above[kk]) ? 1 : 0;
-      d.det_above[kk] = cur;
-    }
-    if (tid == 0) {  // midpoint delivery
-      while (seg < seg_end && d.win[seg].call == 2 * k) {
-        const int32_t e = d.win[seg].edge;
-        const int32_t target = d.edge_target[e];
-        const GroupDev& G = d.g[d.point_group[target]];
-        phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
-        ++seg;
+    if (ndl) {
+      for (int q = tid; q < ndl; q += T) {
+        const int32_t kk = sdet_k[q];
+        const bool cur = v[sdet_n[q]] > sdet_thr[q];
+        d.det_flags[(int64_t)s * d.ndet + kk] = (cur && !d.det_above[kk]) ? 1 : 0;
+        d.det_above[kk] = cur;
+      }
+    } else {
+      for (int64_t q = det0 + tid; q < det1; q += T) {
+        const int32_t kk = d.cell_det[q];
+        const bool cur = v[d.det_node[kk] - n0] > d.det_thr[kk];
+        d.det_flags[(int64_t)s * d.ndet + kk] = (cur && !d.det_above[kk]) ? 1 : 0;
+        d.det_above[kk] = cur;
       }
     }
-    for (int i = tid; i < nn; i += T) { rhs[i] = 0.0; diag[i] = 0.0; }
+    if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k);  // midpoint delivery
+    for (int i = tid; i < nn; i += T) {
+      rhs[i] = 0.0;
+      diag[i] = 0.0;
+    }
     __syncthreads();
     // nrn_cur at t + dt/2, one mechanism group after another
     const double tc = t + 0.5 * dt;
-    for (int gi = 0; gi < d.ngroups; ++gi) {
-      const GroupDev& G = d.g[gi];
+    for (int gi = 0; gi < ngr; ++gi) {
+      const GroupDev& G = sg[gi];
       if (G.mech == MIND_MECH_NETSTIM) continue;
-      const int64_t r0 = G.cell_rows[c0], r1 = G.cell_rows[c1];
+      const int64_t r0 = sr0[gi], r1 = sr1[gi];
       if (r0 == r1) continue;
       const Cols col = cols(G);
       for (int64_t r = r0 + tid; r < r1; r += T) {
@@ -135,69 +203,84 @@ __global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, cons
       const int64_t gi = n0 + li;
       const double dg = d.cdt[gi] + diag[li] + d.adiag[gi];
       double bb = rhs[li];
-      const int32_t p = d.parent[gi];
+      const int p = par[li];
       const double vi = v[li];
-      if (p >= 0) bb += d.axial[gi] * (v[p - n0] - vi);
-      for (int64_t q = d.child_off[gi]; q < d.child_off[gi + 1]; ++q) {
-        const int32_t c = d.child[q];
-        bb -= d.axial[c] * (vi - v[c - n0]);
+      if (p >= 0) bb += ax[li] * (v[p] - vi);
+      const int h = chi[li];
+      if (h == -2) {
+        for (int64_t q = d.child_off[gi]; q < d.child_off[gi + 1]; ++q) {
+          const int c = d.child[q] - (int)n0;
+          bb -= ax[c] * (vi - v[c]);
+        }
+      } else {
+        const int l = clo[li];
+        if (l >= 0) bb -= ax[l] * (vi - v[l]);
+        if (h >= 0) bb -= ax[h] * (vi - v[h]);
       }
       diag[li] = dg;
       rhs[li] = bb;
     }
     __syncthreads();
-    // elimination, leaves to root
+    // elimination by height: a node pulls its children (lower levels) in
+    // decreasing index
     bool bad = false;
-    for (int L = 0; L < ntri; ++L) {
-      const int a = b.tri_lvl[tri_base + L], z = b.tri_lvl[tri_base + L + 1];
-      for (int ui = a + tid; ui < z; ui += T) {
-        const int u = b.tri_order[ui];
-        const int st = b.unit_start[u], len = b.unit_len[u];
-        for (int q = 0; q < len; ++q) {
-          const int li = b.unit_nodes[st + q];
+    for (int L = 0; L < nlh; ++L) {
+      const int z = lh[L + 1];
+      for (int i = lh[L] + tid; i < z; i += T) {
+        const int li = oh[i];
+        double dd = diag[li], bb = rhs[li];
+        const int h = chi[li];
+        if (h == -2) {
           const int64_t gi = n0 + li;
-          double dd = diag[li], bb = rhs[li];
           for (int64_t cq = d.child_off[gi + 1] - 1; cq >= d.child_off[gi]; --cq) {
-            const int32_t c = d.child[cq];
-            const int lc = c - (int)n0;
-            const double g = d.axial[c];
+            const int lc = d.child[cq] - (int)n0;
+            const double g = ax[lc];
             const double f = g / diag[lc];
             dd -= f * g;
             bb += f * rhs[lc];
           }
-          diag[li] = dd;
-          rhs[li] = bb;
-          bad |= (dd == 0.0) || !mind_isfinite(dd);
+        } else {
+          if (h >= 0) {
+            const double g = ax[h];
+            const double f = g / diag[h];
+            dd -= f * g;
+            bb += f * rhs[h];
+          }
+          const int l = clo[li];
+          if (l >= 0) {
+            const double g = ax[l];
+            const double f = g / diag[l];
+            dd -= f * g;
+            bb += f * rhs[l];
+          }
         }
+        diag[li] = dd;
+        rhs[li] = bb;
+        bad |= (dd == 0.0) || !mind_isfinite(dd);
       }
       __syncthreads();
     }
     if (bad) raise_error(d.error, MIND_ESINGULAR);
-    // back substitution, root to leaves; v += dv
-    for (int L = 0; L < nbk; ++L) {
-      const int a = b.bk_lvl[bk_base + L], z = b.bk_lvl[bk_base + L + 1];
-      for (int ui = a + tid; ui < z; ui += T) {
-        const int u = b.bk_order[ui];
-        const int st = b.unit_start[u], len = b.unit_len[u];
-        for (int q = len - 1; q >= 0; --q) {
-          const int li = b.unit_nodes[st + q];
-          const int64_t gi = n0 + li;
-          double x = rhs[li];
-          const int32_t p = d.parent[gi];
-          if (p >= 0) x += d.axial[gi] * rhs[p - n0];
-          x = x / diag[li];
-          rhs[li] = x;
-          v[li] += x;
-        }
+    // back substitution by depth, root to leaves; v += dv
+    for (int L = 0; L < nld; ++L) {
+      const int z = ld[L + 1];
+      for (int i = ld[L] + tid; i < z; i += T) {
+        const int li = od[i];
+        const int p = par[li];
+        double x = rhs[li];
+        if (p >= 0) x += ax[li] * rhs[p];
+        x = x / diag[li];
+        rhs[li] = x;
+        v[li] += x;
       }
       __syncthreads();
     }
     // nrn_state at t + dt
-    for (int gi = 0; gi < d.ngroups; ++gi) {
-      const GroupDev& G = d.g[gi];
+    for (int gi = 0; gi < ngr; ++gi) {
+      const GroupDev& G = sg[gi];
       if (G.mech != MIND_MECH_HH && G.mech != MIND_MECH_EXP2SYN && G.mech != MIND_MECH_EXPSYN) continue;
       const Cols col = cols(G);
-      const int64_t r0 = G.cell_rows[c0], r1 = G.cell_rows[c1];
+      const int64_t r0 = sr0[gi], r1 = sr1[gi];
       for (int64_t r = r0 + tid; r < r1; r += T)
         phase_advance(G.mech, col, r, v[G.node[r] - n0], dt, d.celsius);
     }
@@ -214,21 +297,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_batched(Dev d, BatchDev b, cons
       }
     }
     __syncthreads();
-    if (tid == 0) {  // end-of-step delivery
-      while (seg < seg_end && d.win[seg].call == 2 * k + 1) {
-        const int32_t e = d.win[seg].edge;
-        const int32_t target = d.edge_target[e];
-        const GroupDev& G = d.g[d.point_group[target]];
-        phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
-        ++seg;
-      }
-    }
+    if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k + 1);  // end-of-step delivery
   }
-  if (MODE == 0) {
+  if (M == 0) {
     __syncthreads();
     for (int i = tid; i < nn; i += T) d.v[n0 + i] = v[i];
   }
 }
+
 
 namespace {
 
@@ -250,120 +326,125 @@ void free_plan(BatchPlan* P) {
   delete P;
 }
 
-// Builds groups, chain units and level schedules for a node budget per workgroup.
+// Levels by height and by depth of nodes [a0, a1) (parents precede children).
+void node_levels(const Layout& L, int64_t a0, int64_t a1, std::vector<int32_t>& h, std::vector<int32_t>& dp,
+                 int64_t& nlh, int64_t& nld) {
+  h.assign(a1 - a0, 0);
+  dp.assign(a1 - a0, 0);
+  for (int64_t i = a1 - 1; i >= a0; --i) {
+    const int32_t p = L.parent[i];
+    if (p >= 0) h[p - a0] = std::max(h[p - a0], h[i - a0] + 1);
+  }
+  nlh = nld = 0;
+  for (int64_t i = a0; i < a1; ++i) {
+    const int32_t p = L.parent[i];
+    dp[i - a0] = p >= 0 ? dp[p - a0] + 1 : 0;
+    nlh = std::max<int64_t>(nlh, h[i - a0] + 1);
+    nld = std::max<int64_t>(nld, dp[i - a0] + 1);
+  }
+}
+
+// Groups consecutive cells within a node budget and the LDS capacity, and builds
+// each group's link and schedule block.
 int build_plan(State& S, int64_t budget, int threads, BatchPlan*& out) {
   const Layout& L = S.L;
   BatchPlan* P = new BatchPlan();
   P->budget = budget;
   P->threads = threads;
-  std::vector<int32_t> cell0, cell1, gmode;
-  const int64_t lds_nodes0 = kLdsLimit / 24, lds_nodes1 = kLdsLimit / 16;
-  int64_t cur_nodes = 0;
+  std::vector<int32_t> cell0, cell1, glist;
+  std::vector<int32_t> hh, dd;
+  int64_t cur_nodes = 0, cur_lh = 0, cur_ld = 0;
   int32_t start = 0;
+  int rc = 0;
   for (int32_t c = 0; c <= L.ncells; ++c) {
-    const int64_t nc = c < L.ncells ? L.cell_off[c + 1] - L.cell_off[c] : 0;
-    const bool flush = c == L.ncells || cur_nodes + nc > budget || nc > lds_nodes0;
+    int64_t nc = 0, lh = 0, ld = 0;
+    if (c < L.ncells) {
+      nc = L.cell_off[c + 1] - L.cell_off[c];
+      node_levels(L, L.cell_off[c], L.cell_off[c + 1], hh, dd, lh, ld);
+    }
+    const bool alone_fits = window_bytes(0, nc, lh, ld) <= kLdsResident;
+    const bool fits = window_bytes(0, cur_nodes + nc, std::max(cur_lh, lh), std::max(cur_ld, ld)) <= kLdsResident;
+    const bool flush = c == L.ncells || cur_nodes + nc > budget || !fits || !alone_fits;
     if (flush && c > start) {
       cell0.push_back(start);
       cell1.push_back(c);
-      gmode.push_back(0);
+      glist.push_back(c - start == 1 && cur_nodes > budget ? 1 : 0);
       start = c;
-      cur_nodes = 0;
+      cur_nodes = cur_lh = cur_ld = 0;
     }
     if (c == L.ncells) break;
-    if (nc > lds_nodes0) {  // oversized cell: its own workgroup, v (or all) in HBM
+    if (!alone_fits) {  // a cell beyond the LDS capacity: its own workgroup, part in HBM
+      if (nc > 32767) {
+        free_plan(P);
+        return fail(MIND_EUNSUPPORTED, "a cell has more than 32767 nodes");
+      }
+      int m = 1;
+      while (m < 3 && window_bytes(m, nc, lh, ld) > kLdsResident) ++m;
       cell0.push_back(c);
       cell1.push_back(c + 1);
-      gmode.push_back(nc <= lds_nodes1 ? 1 : 2);
+      glist.push_back(1 + m);
       start = c + 1;
-      cur_nodes = 0;
       continue;
     }
     cur_nodes += nc;
+    cur_lh = std::max(cur_lh, lh);
+    cur_ld = std::max(cur_ld, ld);
   }
   const int ng = (int)cell0.size();
   P->ngroups = ng;
-  std::vector<int32_t> unit_start, unit_len, unit_nodes, tri_order, tri_lvl, grp_tri, grp_ntri, bk_order, bk_lvl,
-      grp_bk, grp_nbk;
-  std::vector<int32_t> unit_of, height, depth;
+  std::vector<int16_t> r_tab;
+  std::vector<int32_t> grp_rtab(ng), grp_nlh(ng), grp_nld(ng);
   for (int g = 0; g < ng; ++g) {
     const int64_t n0 = L.cell_off[cell0[g]], n1 = L.cell_off[cell1[g]];
     const int nn = (int)(n1 - n0);
-    const int list = gmode[g] == 0 && nn > budget ? 3 : gmode[g];
-    P->max_nodes[list] = std::max<int64_t>(P->max_nodes[list], nn);
-    P->groups_by_mode[list].push_back(g);
-    unit_of.assign(nn, -1);
-    const int ubase = (int)unit_start.size();
-    std::vector<int32_t> uh;  // unit heights (local unit index)
-    std::vector<std::vector<int32_t>> members;
-    for (int li = nn - 1; li >= 0; --li) {
+    int64_t nlh = 0, nld = 0;
+    node_levels(L, n0, n1, hh, dd, nlh, nld);
+    const int list = glist[g];
+    P->lists[list].push_back(g);
+    const int mode = list == 0 || list == 1 ? 0 : list - 1;
+    P->lds[list] = std::max(P->lds[list], window_bytes(mode, nn, nlh, nld));
+    grp_rtab[g] = (int32_t)r_tab.size();
+    grp_nlh[g] = (int32_t)nlh;
+    grp_nld[g] = (int32_t)nld;
+    const size_t base = r_tab.size();
+    r_tab.resize(base + (size_t)rtab_len(nn, nlh, nld), 0);
+    int16_t* t = r_tab.data() + base;
+    int16_t *t_par = t, *t_lo = t + nn, *t_hi = t + 2 * nn, *t_oh = t + 3 * nn, *t_od = t + 4 * nn,
+            *t_lh = t + 5 * nn, *t_ld = t_lh + nlh + 1;
+    for (int li = 0; li < nn; ++li) {
       const int64_t gi = n0 + li;
-      const int64_t nch = L.child_off[gi + 1] - L.child_off[gi];
-      if (nch == 1) {
-        const int u = unit_of[L.child[L.child_off[gi]] - n0];
-        unit_of[li] = u;
-        members[u].push_back(li);
-      } else {
-        int h = 0;
-        for (int64_t q = L.child_off[gi]; q < L.child_off[gi + 1]; ++q)
-          h = std::max(h, uh[unit_of[L.child[q] - n0]] + 1);
-        unit_of[li] = (int)members.size();
-        members.push_back({li});
-        uh.push_back(h);
-      }
+      const int32_t p = L.parent[gi];
+      t_par[li] = (int16_t)(p < 0 ? -1 : p - (int32_t)n0);
+      const int64_t o0 = L.child_off[gi], o1 = L.child_off[gi + 1];
+      t_lo[li] = (int16_t)(o1 > o0 ? L.child[o0] - (int32_t)n0 : -1);
+      t_hi[li] = (int16_t)(o1 - o0 == 2 ? L.child[o0 + 1] - (int32_t)n0 : (o1 - o0 > 2 ? -2 : -1));
     }
-    const int nu = (int)members.size();
-    std::vector<int32_t> ud(nu, 0);
-    // depth by increasing top index (a parent unit's top precedes its children's)
-    std::vector<int32_t> by_top(nu);
-    for (int u = 0; u < nu; ++u) by_top[u] = u;
-    std::sort(by_top.begin(), by_top.end(), [&](int a, int b2) { return members[a].back() < members[b2].back(); });
-    for (int u : by_top) {
-      const int top = members[u].back();
-      const int32_t p = L.parent[n0 + top];
-      ud[u] = p < 0 ? 0 : ud[unit_of[p - n0]] + 1;
-    }
-    for (int u = 0; u < nu; ++u) {
-      unit_start.push_back((int32_t)unit_nodes.size());
-      unit_len.push_back((int32_t)members[u].size());
-      for (int32_t x : members[u]) unit_nodes.push_back(x);
-    }
-    auto levels = [&](const std::vector<int32_t>& lv, std::vector<int32_t>& order, std::vector<int32_t>& off,
-                      std::vector<int32_t>& gbase, std::vector<int32_t>& gn) {
-      int nl = 0;
-      for (int u = 0; u < nu; ++u) nl = std::max(nl, lv[u] + 1);
+    auto order = [&](const std::vector<int32_t>& lv, int64_t nl, int16_t* ord, int16_t* off) {
       std::vector<int32_t> cnt(nl + 1, 0);
-      for (int u = 0; u < nu; ++u) cnt[lv[u] + 1] += 1;
-      for (int l = 0; l < nl; ++l) cnt[l + 1] += cnt[l];
-      const int obase = (int)order.size();
-      order.resize(obase + nu);
-      std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
-      for (int u = 0; u < nu; ++u) order[obase + pos[lv[u]]++] = ubase + u;
-      gbase.push_back((int32_t)off.size());
-      gn.push_back(nl);
-      for (int l = 0; l <= nl; ++l) off.push_back(obase + cnt[l]);
+      for (int li = 0; li < nn; ++li) cnt[lv[li] + 1] += 1;
+      for (int64_t l = 0; l < nl; ++l) cnt[l + 1] += cnt[l];
+      for (int64_t l = 0; l <= nl; ++l) off[l] = (int16_t)cnt[l];
+      for (int li = 0; li < nn; ++li) ord[cnt[lv[li]]++] = (int16_t)li;
     };
-    levels(uh, tri_order, tri_lvl, grp_tri, grp_ntri);
-    levels(ud, bk_order, bk_lvl, grp_bk, grp_nbk);
+    order(hh, nlh, t_oh, t_lh);
+    order(dd, nld, t_od, t_ld);
   }
-  int rc = 0;
   BatchDev& b = P->b;
-  int32_t* p;
+  int32_t* p = nullptr;
+  int16_t* p16 = nullptr;
   if ((rc = plan_upload(S, *P, &p, cell0))) goto error; b.grp_cell0 = p;
   if ((rc = plan_upload(S, *P, &p, cell1))) goto error; b.grp_cell1 = p;
-  if ((rc = plan_upload(S, *P, &p, unit_start))) goto error; b.unit_start = p;
-  if ((rc = plan_upload(S, *P, &p, unit_len))) goto error; b.unit_len = p;
-  if ((rc = plan_upload(S, *P, &p, unit_nodes))) goto error; b.unit_nodes = p;
-  if ((rc = plan_upload(S, *P, &p, tri_order))) goto error; b.tri_order = p;
-  if ((rc = plan_upload(S, *P, &p, tri_lvl))) goto error; b.tri_lvl = p;
-  if ((rc = plan_upload(S, *P, &p, grp_tri))) goto error; b.grp_tri = p;
-  if ((rc = plan_upload(S, *P, &p, grp_ntri))) goto error; b.grp_ntri = p;
-  if ((rc = plan_upload(S, *P, &p, bk_order))) goto error; b.bk_order = p;
-  if ((rc = plan_upload(S, *P, &p, bk_lvl))) goto error; b.bk_lvl = p;
-  if ((rc = plan_upload(S, *P, &p, grp_bk))) goto error; b.grp_bk = p;
-  if ((rc = plan_upload(S, *P, &p, grp_nbk))) goto error; b.grp_nbk = p;
-  for (int m = 0; m < 4; ++m)
-    if ((rc = plan_upload(S, *P, &P->dev_groups[m], P->groups_by_mode[m]))) goto error;
+  if ((rc = plan_upload(S, *P, &p16, r_tab))) goto error; b.r_tab = p16;
+  if ((rc = plan_upload(S, *P, &p, grp_rtab))) goto error; b.grp_rtab = p;
+  if ((rc = plan_upload(S, *P, &p, grp_nlh))) goto error; b.grp_nlh = p;
+  if ((rc = plan_upload(S, *P, &p, grp_nld))) goto error; b.grp_nld = p;
+  for (int m = 0; m < kLists; ++m)
+    if ((rc = plan_upload(S, *P, &P->dev_lists[m], P->lists[m]))) goto error;
+  for (int m = 0; m < kLists; ++m)
+    if (P->lds[m] > kLdsResident) {
+      rc = fail(MIND_EUNSUPPORTED, "batched plan exceeds the LDS capacity");
+      goto error;
+    }
   // events are bucketed per group
   S.cell_bucket.assign(L.ncells, 0);
   for (int g = 0; g < ng; ++g)
@@ -375,37 +456,32 @@ error:
   return rc;
 }
 
-size_t lds_bytes(const BatchPlan& P, int mode) {
-  const int64_t nn = P.max_nodes[mode];
-  return (mode == 0 || mode == 3) ? (size_t)nn * 24 : mode == 1 ? (size_t)nn * 16 : 0;
-}
-
-template <int MODE>
+template <int M>
 void launch_list(State& S, const BatchPlan& P, int list, int32_t n, hipStream_t st) {
-  const unsigned ng = (unsigned)P.groups_by_mode[list].size();
+  const unsigned ng = (unsigned)P.lists[list].size();
   if (ng)
-    hipLaunchKernelGGL(k_batched<MODE>, dim3(ng), dim3(P.threads), lds_bytes(P, list), st, S.d, P.b,
-                       P.dev_groups[list], n);
+    hipLaunchKernelGGL(k_window<M>, dim3(ng), dim3(P.threads), P.lds[list], st, S.d, P.b, P.dev_lists[list], n);
 }
 
-// The workgroups within the node budget launch on the main stream; oversized
-// cells (each a long critical path on one workgroup) run concurrently on a
-// second stream, joined before the exchange.
+// The workgroups within the node budget launch on the main stream; larger cells
+// (each a long critical path on one workgroup) run concurrently on a second
+// stream, joined before the exchange.
 int launch_cells(State& S, const BatchPlan& P, int32_t n) {
-  const bool side = !P.groups_by_mode[1].empty() || !P.groups_by_mode[2].empty() ||
-                    !P.groups_by_mode[3].empty();
+  bool side = false;
+  for (int m = 1; m < kLists; ++m) side |= !P.lists[m].empty();
   if (side) {
     HIP_TRY(hipEventRecord(S.fork, S.stream));
     HIP_TRY(hipStreamWaitEvent(S.aux, S.fork, 0));
-    launch_list<2>(S, P, 2, n, S.aux);
-    launch_list<1>(S, P, 1, n, S.aux);
-    launch_list<0>(S, P, 3, n, S.aux);
+    launch_list<3>(S, P, 4, n, S.aux);
+    launch_list<2>(S, P, 3, n, S.aux);
+    launch_list<1>(S, P, 2, n, S.aux);
+    launch_list<0>(S, P, 1, n, S.aux);
     HIP_TRY(hipEventRecord(S.join, S.aux));
   }
   int rc = 0;
-  if (S.profiling && !P.groups_by_mode[0].empty() && (rc = timer_start(S))) return rc;
+  if (S.profiling && !P.lists[0].empty() && (rc = timer_start(S))) return rc;
   launch_list<0>(S, P, 0, n, S.stream);
-  if (S.profiling && !P.groups_by_mode[0].empty()) {
+  if (S.profiling && !P.lists[0].empty()) {
     if ((rc = timer_stop(S))) return rc;
     S.kernel_steps += n;
   }
@@ -432,8 +508,12 @@ int batched_setup(State& S, int32_t cells_per_group) {
   HIP_TRY(hipStreamCreateWithFlags(&S.aux, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_batched<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_batched<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_window<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsResident));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_window<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsResident));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_window<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsResident));
   int64_t max_cell = 1;
   for (int32_t c = 0; c < S.L.ncells; ++c) max_cell = std::max<int64_t>(max_cell, S.L.cell_off[c + 1] - S.L.cell_off[c]);
   struct Cand { int64_t budget; int threads; };
@@ -441,7 +521,8 @@ int batched_setup(State& S, int32_t cells_per_group) {
   if (cells_per_group > 0) {
     cands.push_back({std::max<int64_t>(1, (int64_t)cells_per_group * max_cell), 256});
   } else {
-    for (int64_t bud : {512, 1024, 2048, 3072, 4096, 6144}) cands.push_back({bud, 256});
+    for (int threads : {256, 512})
+      for (int64_t bud : {768, 1024, 1400, 1700, 2200, 2750}) cands.push_back({bud, threads});
   }
   int rc;
   double best = 1e30;
@@ -476,15 +557,6 @@ int batched_setup(State& S, int32_t cells_per_group) {
     }
   }
   if (tune) {
-    for (int threads : {128, 512}) {
-      double secs = 0;
-      const Cand c{pick.budget, threads};
-      if ((rc = trial(c, secs))) return rc;
-      if (secs < best) {
-        best = secs;
-        pick = c;
-      }
-    }
     double secs = 0;
     if ((rc = trial(pick, secs))) return rc;
   }
@@ -493,8 +565,8 @@ int batched_setup(State& S, int32_t cells_per_group) {
   return 0;
 }
 
-// Nodes, pas/hh/Exp2Syn rows and detectors of the cells in LDS workgroups within
-// the node budget: the work of the dominant (mode 0) launch.
+// Nodes, pas/hh/Exp2Syn rows and detectors of the cells in the workgroups within
+// the node budget: the work of the dominant launch (list 0).
 int batched_dominant_work(const State& S, int64_t* counts) {
   if (!S.batch) return fail(MIND_EARG, "state has no batched plan");
   for (int i = 0; i < 6; ++i) counts[i] = 0;
@@ -503,7 +575,7 @@ int batched_dominant_work(const State& S, int64_t* counts) {
   std::vector<int32_t> c0(S.batch->ngroups), c1(S.batch->ngroups);
   hipMemcpy(c0.data(), S.batch->b.grp_cell0, sizeof(int32_t) * c0.size(), hipMemcpyDeviceToHost);
   hipMemcpy(c1.data(), S.batch->b.grp_cell1, sizeof(int32_t) * c1.size(), hipMemcpyDeviceToHost);
-  for (int32_t g : S.batch->groups_by_mode[0])
+  for (int32_t g : S.batch->lists[0])
     for (int32_t c = c0[g]; c < c1[g]; ++c) in0[c] = 1;
   for (int32_t c = 0; c < L.ncells; ++c) {
     if (!in0[c]) continue;

@@ -1115,6 +1115,47 @@ int mind_device_count(int32_t* count) {
   return 0;
 }
 
+// Plain streaming copy (32 B per lane per iteration): the known-size reference
+// for HBM counters and the achievable copy bandwidth of the device.
+__global__ void k_stream_copy(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int mind_copy_bandwidth(int32_t device, int64_t bytes, int32_t repeats, double* gbps) {
+  if (bytes <= 0 || bytes % 32 != 0 || repeats <= 0 || !gbps) return fail(MIND_EARG, "bytes must be a positive multiple of 32");
+  HIP_TRY(hipSetDevice(device));
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return fail(MIND_EOOM, "device allocation failed");
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(MIND_EOOM, "device allocation failed");
+  }
+  hipEvent_t e0, e1;
+  hipError_t err = hipMemset(a, 0, bytes);
+  if (err == hipSuccess) err = hipEventCreate(&e0);
+  if (err == hipSuccess) err = hipEventCreate(&e1);
+  const int64_t n = bytes / 32;
+  const unsigned grid = 256 * 8 * 4;  // 8 XCDs x 32 CUs x 4 workgroups
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, (const double4*)a, (double4*)b, n);
+    err = hipEventRecord(e0, 0);
+  }
+  for (int r = 0; err == hipSuccess && r < repeats; ++r)
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, (const double4*)a, (double4*)b, n);
+  if (err == hipSuccess) err = hipEventRecord(e1, 0);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return fail(MIND_EHIP, std::string("copy bandwidth: ") + hipGetErrorString(err));
+  *gbps = 2.0 * (double)bytes * repeats / (ms * 1e-3) / 1e9;
+  return 0;
+}
+
 int mind_synchronize(mind_state* state) {
   State& S = *reinterpret_cast<State*>(state);
   HIP_TRY(hipSetDevice(S.device));

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 PACKAGE_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PACKAGE_DIR, "csrc", "libmind.so")
+LIB_PATH = os.environ.get("MIND_LIBRARY") or os.path.join(PACKAGE_DIR, "csrc", "libmind.so")
 
 MIND_METHOD_PLAIN = 0
 MIND_METHOD_BATCHED = 1
@@ -171,6 +171,7 @@ def lib():
     L.mind_batch_work.argtypes = [C.c_void_p, _p(C.c_int64)]
     L.mind_last_error.restype = C.c_char_p
     L.mind_build_info.restype = C.c_char_p
+    L.mind_copy_bandwidth.argtypes = [C.c_int32, C.c_int64, C.c_int32, _p(C.c_double)]
     _lib = L
     return L
 
