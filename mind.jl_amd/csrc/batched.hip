@@ -50,6 +50,12 @@ struct BatchPlan {
   std::vector<int32_t> lists[kLists];
   int32_t* dev_lists[kLists] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t lds[kLists] = {0, 0, 0, 0, 0};  // dynamic LDS bytes of each list
+  // lists 1-4 as one launch: groups and their modes
+  std::vector<int32_t> large;
+  std::vector<int8_t> large_mode;
+  int32_t* dev_large = nullptr;
+  int8_t* dev_large_mode = nullptr;
+  size_t large_lds = 0;
   BatchDev b{};
   std::vector<void*> allocations;
 };
@@ -86,10 +92,18 @@ __device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_e
 // conductances, links and schedule), 1 v in HBM, 2 v and the axial conductances
 // in HBM, 3 everything in HBM (rhs and diag in the global scratch columns, the
 // links and schedule read from the plan).
+// Per-window constants of the step's critical path: mechanism groups, their row
+// ranges for this workgroup's cells, and its detectors.
+struct WinShared {
+  GroupDev sg[kMaxGroups];
+  int64_t sr0[kMaxGroups], sr1[kMaxGroups];
+  int32_t sdet_k[kMaxDetLds], sdet_n[kMaxDetLds];
+  double sdet_thr[kMaxDetLds];
+};
+
 template <int M>
-__global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
-  extern __shared__ double lds[];
-  const int grp = groups[blockIdx.x];
+__device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, const int grp, const int32_t n,
+                                            WinShared& sh, double* lds) {
   const int tid = threadIdx.x, T = blockDim.x;
   const int64_t c0 = b.grp_cell0[grp], c1 = b.grp_cell1[grp];
   const int64_t n0 = d.cell_off[c0], n1 = d.cell_off[c1];
@@ -112,12 +126,12 @@ __global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const
   const int16_t* od = oh + nn;       // nodes by depth
   const int16_t* lh = od + nn;       // nlh + 1 offsets into oh
   const int16_t* ld = lh + nlh + 1;  // nld + 1 offsets into od
-  // per-window constants of the step's critical path: mechanism groups, their
-  // row ranges for this group's cells, and the detectors
-  __shared__ GroupDev sg[kMaxGroups];
-  __shared__ int64_t sr0[kMaxGroups], sr1[kMaxGroups];
-  __shared__ int32_t sdet_k[kMaxDetLds], sdet_n[kMaxDetLds];
-  __shared__ double sdet_thr[kMaxDetLds];
+  GroupDev* sg = sh.sg;
+  int64_t* sr0 = sh.sr0;
+  int64_t* sr1 = sh.sr1;
+  int32_t* sdet_k = sh.sdet_k;
+  int32_t* sdet_n = sh.sdet_n;
+  double* sdet_thr = sh.sdet_thr;
   const int ngr = d.ngroups < kMaxGroups ? d.ngroups : kMaxGroups;
   if (tid < ngr) {
     sg[tid] = d.g[tid];
@@ -305,6 +319,28 @@ __global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const
   }
 }
 
+// Groups within the node budget: all state in LDS.
+__global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  window_body<0>(d, b, groups[blockIdx.x], n, sh, lds);
+}
+
+// Cells above the node budget, one per workgroup, each with the layout that
+// fits (modes[k] for groups[k]): one launch, so that they run side by side.
+__global__ void __launch_bounds__(kMaxThreads) k_window_large(Dev d, BatchDev b, const int32_t* groups,
+                                                              const int8_t* modes, int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  const int grp = groups[blockIdx.x];
+  switch (modes[blockIdx.x]) {
+    case 0: window_body<0>(d, b, grp, n, sh, lds); break;
+    case 1: window_body<1>(d, b, grp, n, sh, lds); break;
+    case 2: window_body<2>(d, b, grp, n, sh, lds); break;
+    default: window_body<3>(d, b, grp, n, sh, lds); break;
+  }
+}
+
 
 namespace {
 
@@ -440,6 +476,20 @@ int build_plan(State& S, int64_t budget, int threads, BatchPlan*& out) {
   if ((rc = plan_upload(S, *P, &p, grp_nld))) goto error; b.grp_nld = p;
   for (int m = 0; m < kLists; ++m)
     if ((rc = plan_upload(S, *P, &P->dev_lists[m], P->lists[m]))) goto error;
+  for (int m = 1; m < kLists; ++m) {  // largest layouts first: the longest workgroups start early
+    const int list = kLists - m;
+    for (int32_t g : P->lists[list]) {
+      P->large.push_back(g);
+      P->large_mode.push_back((int8_t)(list == 1 ? 0 : list - 1));
+    }
+    P->large_lds = std::max(P->large_lds, P->lds[list]);
+  }
+  if ((rc = plan_upload(S, *P, &P->dev_large, P->large))) goto error;
+  {
+    int8_t* p8 = nullptr;
+    if ((rc = plan_upload(S, *P, &p8, P->large_mode))) goto error;
+    P->dev_large_mode = p8;
+  }
   for (int m = 0; m < kLists; ++m)
     if (P->lds[m] > kLdsResident) {
       rc = fail(MIND_EUNSUPPORTED, "batched plan exceeds the LDS capacity");
@@ -456,31 +506,25 @@ error:
   return rc;
 }
 
-template <int M>
-void launch_list(State& S, const BatchPlan& P, int list, int32_t n, hipStream_t st) {
-  const unsigned ng = (unsigned)P.lists[list].size();
-  if (ng)
-    hipLaunchKernelGGL(k_window<M>, dim3(ng), dim3(P.threads), P.lds[list], st, S.d, P.b, P.dev_lists[list], n);
-}
+
 
 // The workgroups within the node budget launch on the main stream; larger cells
 // (each a long critical path on one workgroup) run concurrently on a second
 // stream, joined before the exchange.
 int launch_cells(State& S, const BatchPlan& P, int32_t n) {
-  bool side = false;
-  for (int m = 1; m < kLists; ++m) side |= !P.lists[m].empty();
+  const bool side = !P.large.empty();
   if (side) {
     HIP_TRY(hipEventRecord(S.fork, S.stream));
     HIP_TRY(hipStreamWaitEvent(S.aux, S.fork, 0));
-    launch_list<3>(S, P, 4, n, S.aux);
-    launch_list<2>(S, P, 3, n, S.aux);
-    launch_list<1>(S, P, 2, n, S.aux);
-    launch_list<0>(S, P, 1, n, S.aux);
+    hipLaunchKernelGGL(k_window_large, dim3((unsigned)P.large.size()), dim3(P.threads), P.large_lds, S.aux, S.d,
+                       P.b, P.dev_large, P.dev_large_mode, n);
     HIP_TRY(hipEventRecord(S.join, S.aux));
   }
   int rc = 0;
   if (S.profiling && !P.lists[0].empty() && (rc = timer_start(S))) return rc;
-  launch_list<0>(S, P, 0, n, S.stream);
+  if (!P.lists[0].empty())
+    hipLaunchKernelGGL(k_window, dim3((unsigned)P.lists[0].size()), dim3(P.threads), P.lds[0], S.stream, S.d, P.b,
+                       P.dev_lists[0], n);
   if (S.profiling && !P.lists[0].empty()) {
     if ((rc = timer_stop(S))) return rc;
     S.kernel_steps += n;
@@ -508,11 +552,9 @@ int batched_setup(State& S, int32_t cells_per_group) {
   HIP_TRY(hipStreamCreateWithFlags(&S.aux, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_window<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIP_TRY(hipFuncSetAttribute((const void*)k_window, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kLdsResident));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_window<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kLdsResident));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_window<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIP_TRY(hipFuncSetAttribute((const void*)k_window_large, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kLdsResident));
   int64_t max_cell = 1;
   for (int32_t c = 0; c < S.L.ncells; ++c) max_cell = std::max<int64_t>(max_cell, S.L.cell_off[c + 1] - S.L.cell_off[c]);
