@@ -8,6 +8,7 @@ from .mechanisms import Mechanism, builtin  # noqa: F401
 from .model import (  # noqa: F401
     Cell, MicroModel, Morphology, PointProcess, Section, SimulationResult, SpikeSource,
     add_pointprocess, add_population, batch_plan, build, connect, destroy, initialize, insert,
+    set_batch_plan,
     mechanism_record, read_field, read_voltage, register_spike_source, run, step,
     voltage_record, window_steps,
 )

@@ -537,11 +537,17 @@ def pack_nodes(model, dt_micro=0.025):
 
 
 def batch_plan(model):
-    """(node budget per workgroup, threads per workgroup, workgroups) of the batched kernel."""
+    """The batched kernel's plan: node budget per workgroup, threads per workgroup,
+    layout of those workgroups (0 all in LDS, 4 lean) and workgroup count."""
     L = _abi.lib()
-    b, t, g = C.c_int64(), C.c_int32(), C.c_int64()
-    _abi.check(L.mind_batch_plan(model.state, C.byref(b), C.byref(t), C.byref(g)))
-    return b.value, t.value, g.value
+    b, t, lay, g = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64()
+    _abi.check(L.mind_batch_plan(model.state, C.byref(b), C.byref(t), C.byref(lay), C.byref(g)))
+    return dict(budget=b.value, threads=t.value, layout=lay.value, groups=g.value)
+
+
+def set_batch_plan(model, budget, threads=256, layout=0):
+    """Replaces the autotuned plan; initialize(model) must follow."""
+    _abi.check(_abi.lib().mind_set_batch_plan(model.state, int(budget), int(threads), int(layout)))
 
 
 def synchronize(model):

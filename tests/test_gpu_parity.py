@@ -3,15 +3,18 @@ import numpy as np
 import pytest
 
 import oracle
-from mind_amd import (build, configs, destroy, initialize, mechanism_record, read_field,
-                      read_voltage, run, voltage_record)
+from mind_amd import (batch_plan, build, configs, destroy, initialize, mechanism_record,
+                      read_field, read_voltage, run, set_batch_plan, voltage_record)
 
 pytestmark = pytest.mark.gpu
 
 
-def _compare(model, stop, records, method="plain", **kw):
-    print(f"case: {method} stop={stop} records={len(records)} {kw}", flush=True)
+def _compare(model, stop, records, method="plain", plan=None, **kw):
+    print(f"case: {method} stop={stop} records={len(records)} plan={plan} {kw}", flush=True)
     build(model, dt_micro=0.025, seed=1, method=method, **kw)
+    if plan is not None:
+        set_batch_plan(model, *plan)
+        assert batch_plan(model)["layout"] == plan[2]
     initialize(model)
     got = run(model, stop, records)
     ref_sim = oracle.OracleSim(model).initialize()
@@ -92,12 +95,29 @@ def test_trees_batched_and_plain(gpu):
     destroy(model)
 
 
-def test_trees_oversized_cells_batched(gpu):
-    """Cells above the LDS capacity run with v (and rhs/diag) in HBM; results unchanged."""
-    model, info = configs.tree_network(ncells=4, median=4500, sigma=0.3, lo=3500, hi=6000,
+@pytest.mark.parametrize("plan", [(300, 128, 0), (600, 64, 4), (2400, 512, 4)])
+def test_trees_fixed_plans_batched(gpu, plan):
+    """Both layouts of the window kernel (all in LDS; lean, with the solve's node
+    records streamed from HBM) and block sizes below and above a level's width:
+    bit-identical voltages and raster."""
+    model, info = configs.tree_network(ncells=30, median=120, lo=20, hi=600, indegree=10,
+                                       rate_hz=50.0, netstim_weight=3e-2)
+    recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"][:3]]
+    _, _, vg, vr, _ = _compare(model, 40.0, recs, method="batched", plan=plan)
+    assert np.array_equal(vg, vr)
+    destroy(model)
+
+
+@pytest.mark.parametrize("plan", [None, (4000, 256, 4)])
+def test_trees_oversized_cells_batched(gpu, plan):
+    """Cells above the LDS capacity of the all-in-LDS layout run lean, cells above
+    that of the lean layout with everything in HBM; results unchanged."""
+    model, info = configs.tree_network(ncells=4, median=6000, sigma=0.8, lo=2000, hi=11500,
                                        indegree=3, synapses=4, rate_hz=50.0, netstim_weight=3e-2)
-    assert info["sizes"].max() > 5200 and info["sizes"].min() < 5100  # both HBM modes
-    recs = [voltage_record(model, info["cells"][0], "soma", 0.5)]
-    _, _, vg, vr, _ = _compare(model, 5.0, recs, method="batched", cells_per_group=1)
+    sizes = np.sort(info["sizes"])
+    # one cell for each layout: all in LDS (< 3700 nodes), lean (< 9700), all in HBM
+    assert sizes[0] < 3700 and 3700 < sizes[2] < 9700 < sizes[3]
+    recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"]]
+    _, _, vg, vr, _ = _compare(model, 5.0, recs, method="batched", cells_per_group=1, plan=plan)
     assert np.array_equal(vg, vr)
     destroy(model)
