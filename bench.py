@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--macro-ms", type=float, default=2000.0)
     ap.add_argument("--no-macro", action="store_true")
+    ap.add_argument("--plan", default="", metavar="BUDGET,THREADS,LAYOUT",
+                    help="batched plan instead of the autotuner's (e.g. to profile a given plan)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain streaming copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -69,8 +71,8 @@ def main():
         sys.exit(2)
 
     from mind_amd import build, configs, initialize, run, destroy
-    from mind_amd.model import (batch_plan, batch_work, kernel_time, set_profiling, synchronize,
-                                window_steps)
+    from mind_amd.model import (batch_plan, batch_work, kernel_time, set_batch_plan, set_profiling,
+                                synchronize, window_steps)
 
     dt = 0.025
     t0 = time.time()
@@ -80,6 +82,8 @@ def main():
     def timed(method, steps, warmup, profile):
         t1 = time.time()
         build(model, dt_micro=dt, method=method)
+        if method == "batched" and args.plan:
+            set_batch_plan(model, *(int(x) for x in args.plan.split(",")))
         initialize(model)
         log(f"{method}: build+autotune+initialize {time.time() - t1:.1f}s")
         if warmup:
@@ -114,6 +118,9 @@ def main():
     bytes_per_launch = bytes_per_step * (ksteps / launches)
     avg_launch_s = kms / launches / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    kernel = "k_window_lean" if extra["plan"]["layout"] == 4 else "k_window"
+    traffic, traffic_note = measured_traffic(kernel, extra["plan"], extra["window"], ksteps / launches,
+                                             bytes_per_launch)
     destroy(model)
 
     plain = None
@@ -159,8 +166,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_window<0>", "bytes_per_launch": bytes_per_launch,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": traffic_note,
+            "kernel": kernel, "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": kms / launches, "launches": launches,
         },
         "plain": plain,
@@ -171,6 +179,37 @@ def main():
         "macro": macro,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(kernel, plan, window, steps_per_launch, algorithmic_per_launch):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summaries (profiles/*.md, 'bench-traffic:' lines),
+    scaled from a full window to this run's average launch. A summary of the same
+    plan is used as measured; one of the same kernel with another plan (the
+    autotuner's pick can change between runs) through its ratio of measured to
+    algorithmic bytes, and the returned note says so. (None, None) without one."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    near = None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, f)):
+            if not line.startswith("bench-traffic: "):
+                continue
+            rec = json.loads(line[len("bench-traffic: "):])
+            if rec["kernel"] != kernel or rec["window_steps"] != window:
+                continue
+            if rec["plan"] == plan:
+                return rec["hbm_bytes_per_window"] * steps_per_launch / window, f"profiles/{f}"
+            if near is None and rec.get("algorithmic_bytes_per_window"):
+                near = (rec, f)
+    if near is None:
+        return None, None
+    rec, f = near
+    ratio = rec["hbm_bytes_per_window"] / rec["algorithmic_bytes_per_window"]
+    return ratio * algorithmic_per_launch, f"profiles/{f}, plan {rec['plan']}, scaled by measured/algorithmic bytes"
 
 
 def macro_leg(stop):

@@ -8,9 +8,13 @@
 // level by level: elimination visits nodes by height (a node's children are all
 // lower), each node *pulling* its children's contributions in decreasing child
 // index, which reproduces the CPU restatement's elimination order exactly and
-// needs no atomics; back substitution visits nodes by depth. One node per lane,
-// one barrier per level. The int16 parent / child links and the two node orders
-// are built on the host per group. Two layouts of a workgroup's node state:
+// needs no atomics; back substitution visits nodes by depth. Each cell of a
+// group belongs to one wavefront of the workgroup for the solve, so a level
+// needs no workgroup barrier: the lanes of a wave run in lockstep and LDS
+// executes a wave's accesses in order, and the waves' independent level chains
+// hide each other's latency. The host assigns cells to waves (balancing passes
+// of 64 lanes) and builds the int16 parent / child links and each wave's two
+// node orders per group. Two layouts of a workgroup's node state:
 // all in LDS (mode 0: v, rhs, diag, axial conductances, links and schedule stay
 // there for the whole window, so the solve reads no global memory), or lean
 // (mode 4: only rhs and diag in LDS, 16 bytes per node, and the solve's per-node
@@ -29,13 +33,18 @@
 
 namespace mind {
 
-// Lean layout (mode 4): the solve's per-level node records, read from HBM one
-// level ahead. Elimination, in height order: the node, its higher and lower
-// child (-1 if none, higher -2 if more than two) and their axial conductances.
-// Back substitution, in depth order: the node, its parent and its own axial
-// conductance. Indexed by group node base + position.
+// Lean layout (mode 4): the solve's node records, read from HBM three passes
+// ahead. A group of nn nodes has LDS slots nn (the dummy: pivot 1, rhs 1) and
+// nn + 1 (the sink), and nn + 1 records per order, the last the sink record.
+// Elimination, by position in the elimination order: the node, its higher and
+// lower child (the dummy if missing) and their axial conductances (-0 for the
+// dummy); csr = 1 for a node with more than two children, whose children are
+// then pulled from the CSR lists. Back substitution, by position in depth order:
+// the node, its parent (the dummy at a root) and its own axial conductance (-0
+// at a root). With the dummy's values every term of a missing neighbour is an
+// exact identity (x - (+0) = x, x + (-0) = x), so no lane branches.
 struct HRec {
-  int16_t li, h, l, pad;
+  int16_t li, h, l, csr;
   double gh, gl;
 };
 struct DRec {
@@ -46,14 +55,16 @@ struct DRec {
 struct BatchDev {
   const int32_t* grp_cell0;
   const int32_t* grp_cell1;
-  const HRec* r_hrec;  // mode 4 only
+  const HRec* r_hrec;  // mode 4 only, group g's from grp_rbase[g]
   const DRec* r_drec;
-  // per group one int16 block (local node ids): parent, lower and higher child,
-  // nodes by height, nodes by depth, and the level offsets of both orders
+  const int32_t* grp_rbase;
+  // per group one int16 block (local node ids, 8-byte aligned): per node its
+  // links {higher child, lower child, parent, 0}, then the nodes by height and
+  // by depth (each the concatenation of the waves' orders), then the schedule
+  // block of sched_at()
   const int16_t* r_tab;
   const int32_t* grp_rtab;
-  const int32_t* grp_nlh;  // levels by height
-  const int32_t* grp_nld;  // levels by depth
+  const int32_t* grp_slen;  // schedule block length
 };
 
 // Launch lists: 0 groups within the node budget (main stream); 1 single cells
@@ -86,14 +97,23 @@ constexpr int kMaxDetLds = 256;  // detectors of a workgroup cached in LDS
 // dynamic LDS of a workgroup, leaving room for its static tables
 constexpr size_t kLdsResident = 152 * 1024;
 
+// Schedule block of a group with W waves: entries 0..W give where wave w's
+// elimination pass offsets start in the block (entry W: their end), entries
+// W+1..2W+1 the same for its back-substitution passes, each entry two int16 words
+// (low, high 16 bits); then per wave its pass offsets, positions in oh (od) where
+// each pass starts, plus the end. A pass is at most 64 nodes of one level (a
+// wider level takes several passes), and a wave's passes run in level order.
+__host__ __device__ inline int sched_at(const int16_t* s, int k) {
+  return (int)(uint16_t)s[2 * k] | ((int)s[2 * k + 1] << 16);
+}
 // int16 entries of a group's table block.
-__host__ __device__ inline int64_t rtab_len(int64_t nn, int64_t nlh, int64_t nld) { return 5 * nn + nlh + 1 + nld + 1; }
+__host__ __device__ inline int64_t rtab_len(int64_t nn, int64_t slen) { return 6 * nn + slen; }
 // Dynamic LDS bytes of a workgroup in mode M: mode 0 keeps 4 doubles per node and
-// the table block, mode 4 rhs, diag and the level offsets, mode 3 nothing.
-inline size_t window_bytes(int m, int64_t nn, int64_t nlh, int64_t nld) {
+// the table block, mode 4 rhs, diag and the schedule block, mode 3 nothing.
+inline size_t window_bytes(int m, int64_t nn, int64_t slen) {
   if (m == 3) return 0;
-  if (m == 4) return (size_t)(16 * nn + (2 * (nlh + 1 + nld + 1) + 7) / 8 * 8);
-  return (size_t)(32 * nn + (2 * rtab_len(nn, nlh, nld) + 7) / 8 * 8);
+  if (m == 4) return (size_t)(16 * (nn + 2) + (2 * slen + 7) / 8 * 8);
+  return (size_t)(32 * nn + (2 * rtab_len(nn, slen) + 7) / 8 * 8);
 }
 
 // Delivery of the window-list events due at delivery call `call` (thread 0).
@@ -112,85 +132,325 @@ __device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_e
 // conductances, links and schedule), 4 rhs, diag and the level offsets in LDS, 3
 // everything in HBM (rhs and diag in the global scratch columns, the links and
 // schedule read from the plan).
-__device__ inline HRec hrec_at(const HRec* hr, const int16_t* lh, int nlh, int L, int tid) {
-  if (L < nlh) {
-    const int i = lh[L] + tid;
-    if (i < lh[L + 1]) return hr[i];
+// Orders one wavefront's solve accesses between levels. Its lanes run in
+// lockstep, so a fence is all a level needs from the one before: at wavefront
+// scope for rhs and diag in LDS (executed in order, no wait), at workgroup scope
+// for them in global memory (mode 3: waits for the stores).
+template <int M>
+__device__ __forceinline__ void wave_sync() {
+  if (M == 3) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  return HRec{-1, -1, -1, 0, 0.0, 0.0};
 }
 
-__device__ inline DRec drec_at(const DRec* dr, const int16_t* ld, int nld, int L, int tid) {
-  if (L < nld) {
-    const int i = ld[L] + tid;
-    if (i < ld[L + 1]) return dr[i];
-  }
-  return DRec{-1, -1, 0, 0, 0.0};
+// A node's links: higher child (-1 if fewer than two children, -2 if more than
+// two), lower child (-1 if none), parent (-1 at a root).
+struct Links {
+  int h, l, p;
+};
+__device__ __forceinline__ Links links_of(const uint2 w) {
+  return Links{(int16_t)(w.x & 0xffffu), (int16_t)(w.x >> 16), (int16_t)(w.y & 0xffffu)};
 }
 
-// Tree solve of a lean workgroup: rhs and diag in LDS, each lane's node record
-// of a level loaded from HBM two levels ahead. Same operations in the same
-// order as the LDS-resident solve.
-__device__ void solve_lean(const Dev& d, const BatchDev& b, int64_t n0, int nlh, int nld, const int16_t* lh,
-                           const int16_t* ld, double* rhs, double* diag, double* v) {
-  const int tid = threadIdx.x, T = blockDim.x;
-  const HRec* hr = b.r_hrec + n0;
-  bool bad = false;
-  HRec cur = hrec_at(hr, lh, nlh, 0, tid), nxt = hrec_at(hr, lh, nlh, 1, tid);
-  for (int L = 0; L < nlh; ++L) {
-    const HRec ahead = hrec_at(hr, lh, nlh, L + 2, tid);
-    const int a = lh[L], z = lh[L + 1];
-    for (int i = a + tid; i < z; i += T) {
-      const HRec r = i == a + tid ? cur : hr[i];
-      const int li = r.li;
-      double dd = diag[li], bb = rhs[li];
-      if (r.h == -2) {
-        const int64_t gi = n0 + li;
-        for (int64_t cq = d.child_off[gi + 1] - 1; cq >= d.child_off[gi]; --cq) {
-          const int lc = d.child[cq] - (int)n0;
-          const double g = d.axial[n0 + lc];
-          const double f = g / diag[lc];
-          dd -= f * g;
-          bb += f * rhs[lc];
-        }
-      } else {
-        if (r.h >= 0) {
-          const double f = r.gh / diag[r.h];
-          dd -= f * r.gh;
-          bb += f * rhs[r.h];
-        }
-        if (r.l >= 0) {
-          const double f = r.gl / diag[r.l];
-          dd -= f * r.gl;
-          bb += f * rhs[r.l];
-        }
-      }
-      diag[li] = dd;
-      rhs[li] = bb;
-      bad |= (dd == 0.0) || !mind_isfinite(dd);
+// Elimination of node li: pulls its higher child h, then its lower child l (gh,
+// gl their axial conductances), or, with more than two children (h == -2), all
+// of them from the CSR lists in decreasing index. Without branches for one or
+// two children: a missing child enters as conductance -0, pivot 1 and rhs 1,
+// whose terms are exact identities (x - (+0) = x, x + (-0) = x), so both reads
+// and both divisions issue at once. Stores only if `store`; returns true if a
+// stored pivot is zero or not finite.
+// rhs and pivot of a node: interleaved (pivot, rhs) pairs in LDS, one 16-byte
+// access per node, or separate global columns (mode 3). S is the stride.
+template <int S>
+struct Col {
+  double* p;
+  __device__ __forceinline__ double& operator[](int i) const { return p[(int64_t)i * S]; }
+};
+
+// The reads of a one- or two-child elimination, issued before the next level's
+// prefetch so that the prefetch does not wait for them.
+struct ElimIn {
+  double dd, bb, dh, rh, dl, rl, gh, gl;
+  bool hv, lv;
+};
+template <typename C>
+__device__ __forceinline__ ElimIn elim_reads(int li, int h, int l, const double* ax, C rhs, C diag) {
+  ElimIn e;
+  e.hv = h >= 0;
+  e.lv = l >= 0;
+  const int hs = e.hv ? h : li, ls = e.lv ? l : li;
+  e.dd = diag[li];
+  e.bb = rhs[li];
+  e.dh = diag[hs];
+  e.rh = rhs[hs];
+  e.dl = diag[ls];
+  e.rl = rhs[ls];
+  e.gh = ax[hs];
+  e.gl = ax[ls];
+  return e;
+}
+// Nothing is scheduled across this point: every read above it is issued before
+// the first wait for one of them.
+__device__ __forceinline__ void reads_issued() { __builtin_amdgcn_sched_barrier(0); }
+template <typename C>
+__device__ __forceinline__ bool elim_finish(const ElimIn& e, int li, C rhs, C diag, bool store) {
+  const double dh = e.hv ? e.dh : 1.0, rh = e.hv ? e.rh : 1.0, dl = e.lv ? e.dl : 1.0, rl = e.lv ? e.rl : 1.0;
+  const double gh = e.hv ? e.gh : -0.0, gl = e.lv ? e.gl : -0.0;
+  const double fh = gh / dh, fl = gl / dl;
+  const double dd = (e.dd - fh * gh) - fl * gl;
+  const double bb = (e.bb + fh * rh) + fl * rl;
+  if (!store) return false;
+  diag[li] = dd;
+  rhs[li] = bb;
+  return (dd == 0.0) || !mind_isfinite(dd);
+}
+
+template <typename C>
+__device__ __forceinline__ bool eliminate(const Dev& d, int64_t n0, int li, int h, int l, double gh, double gl,
+                                          const double* ax, C rhs, C diag, bool store) {
+  double dd = diag[li], bb = rhs[li];
+  if (h == -2) {
+    const int64_t gi = n0 + li;
+    for (int64_t cq = d.child_off[gi + 1] - 1; cq >= d.child_off[gi]; --cq) {
+      const int lc = d.child[cq] - (int)n0;
+      const double g = ax[lc];
+      const double f = g / diag[lc];
+      dd -= f * g;
+      bb += f * rhs[lc];
     }
-    __syncthreads();
-    cur = nxt;
-    nxt = ahead;
+  } else {
+    const bool hv = h >= 0, lv = l >= 0;
+    const int hs = hv ? h : li, ls = lv ? l : li;
+    const double dh0 = diag[hs], rh0 = rhs[hs], dl0 = diag[ls], rl0 = rhs[ls];
+    const double dh = hv ? dh0 : 1.0, rh = hv ? rh0 : 1.0, dl = lv ? dl0 : 1.0, rl = lv ? rl0 : 1.0;
+    const double ghs = hv ? gh : -0.0, gls = lv ? gl : -0.0;
+    const double fh = ghs / dh, fl = gls / dl;
+    dd = (dd - fh * ghs) - fl * gls;
+    bb = (bb + fh * rh) + fl * rl;
+  }
+  if (!store) return false;
+  diag[li] = dd;
+  rhs[li] = bb;
+  return (dd == 0.0) || !mind_isfinite(dd);
+}
+
+// Back substitution of node li with parent p (-1 at a root) and its own axial
+// conductance g; v += dv. A root's parent term enters as -0 x 1. Stores only if
+// `store`.
+template <typename C>
+__device__ __forceinline__ void substitute(int li, int p, double g, C rhs, C diag, double* v,
+                                           bool store) {
+  const bool pv = p >= 0;
+  const double rp0 = rhs[pv ? p : li];
+  const double x0 = rhs[li], dg = diag[li];
+  const double x = (x0 + (pv ? g : -0.0) * (pv ? rp0 : 1.0)) / dg;
+  if (!store) return;
+  rhs[li] = x;
+  v[li] += x;
+}
+
+// A wave's pass offsets (nl + 1 of them at lv) held 64 at a time in one
+// register, lane j holding offset base + j: reading one is a cross-lane move, not
+// a memory access. at() takes wave-uniform indices j <= nl that grow by at most
+// one per call; every call also reads the following 64 offsets, so that moving
+// on to them needs no branch and no wait.
+struct LevelOffsets {
+  const int16_t* lv;
+  int nl, base, reg, nxt;
+  __device__ LevelOffsets(const int16_t* lv_, int nl_) : lv(lv_), nl(nl_), base(0) {
+    const int lane = threadIdx.x & 63;
+    reg = lv[min(lane, nl)];
+    nxt = lv[min(64 + lane, nl)];
+  }
+  __device__ __forceinline__ int at(int j) {
+    const bool cross = j >= base + 64;
+    reg = cross ? nxt : reg;
+    base = cross ? base + 64 : base;
+    nxt = lv[min(base + 64 + (int)(threadIdx.x & 63), nl)];
+    return __builtin_amdgcn_readlane(reg, j - base);
+  }
+};
+
+// Tree solve of one wavefront's cells (modes 0 and 3): elimination pass by pass
+// in its order, back substitution pass by pass in order of depth, a pass being
+// at most 64 nodes of one level, one per lane. A lane's node two passes ahead
+// and the links of its node one pass ahead are read at the top of a pass, before
+// the pass's own reads, so that a pass waits for one round of reads: its
+// children's (parent's) values. Lanes past a pass's end compute on a node they
+// do not store.
+template <int M, typename C>
+__device__ __forceinline__ void solve_waves(const Dev& d, const BatchDev& b, int64_t n0, int nn,
+                                            const int16_t* sched, const uint2* hn, const int16_t* oh,
+                                            const int16_t* od, const double* ax, C rhs, C diag, double* v) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int h0 = sched_at(sched, w), nlh = sched_at(sched, w + 1) - h0 - 1;
+  const int d0 = sched_at(sched, W + 1 + w), nld = sched_at(sched, W + 2 + w) - d0 - 1;
+  const int last = nn - 1;
+  bool bad = false;
+  if (nlh > 0) {
+    LevelOffsets lo(sched + h0, nlh);
+    int a = lo.at(0), z = lo.at(1);
+    {
+      int li = oh[min(a + lane, last)];
+      Links k = links_of(hn[li]);
+      const int o2 = lo.at(min(2, nlh));
+      int z1 = nlh >= 2 ? o2 : z;
+      int li1 = oh[min(z + lane, last)];
+      for (int L = 0; L < nlh; ++L) {
+        const int o3 = lo.at(min(L + 3, nlh));
+        const int z2 = L + 2 < nlh ? o3 : z1;
+        const int li2 = oh[min(z1 + lane, last)];
+        const Links k1 = links_of(hn[li1]);
+        if (k.h == -2) {
+          bad |= eliminate(d, n0, li, k.h, k.l, 0.0, 0.0, ax, rhs, diag, a + lane < z);
+        } else {
+          const ElimIn e = elim_reads(li, k.h, k.l, ax, rhs, diag);
+          reads_issued();
+          bad |= elim_finish(e, li, rhs, diag, a + lane < z);
+        }
+        wave_sync<M>();
+        li = li1;
+        li1 = li2;
+        k = k1;
+        a = z;
+        z = z1;
+        z1 = z2;
+      }
+    }
   }
   if (bad) raise_error(d.error, MIND_ESINGULAR);
-  const DRec* dr = b.r_drec + n0;
-  DRec dc = drec_at(dr, ld, nld, 0, tid), dn = drec_at(dr, ld, nld, 1, tid);
-  for (int L = 0; L < nld; ++L) {
-    const DRec ahead = drec_at(dr, ld, nld, L + 2, tid);
-    const int a = ld[L], z = ld[L + 1];
-    for (int i = a + tid; i < z; i += T) {
-      const DRec r = i == a + tid ? dc : dr[i];
-      const int li = r.li;
-      double x = rhs[li];
-      if (r.p >= 0) x += r.g * rhs[r.p];
-      x = x / diag[li];
-      rhs[li] = x;
-      v[li] += x;
+  if (nld > 0) {
+    LevelOffsets lo(sched + d0, nld);
+    int a = lo.at(0), z = lo.at(1);
+    {
+      int li = od[min(a + lane, last)];
+      int p = links_of(hn[li]).p;
+      const int o2 = lo.at(min(2, nld));
+      int z1 = nld >= 2 ? o2 : z;
+      int li1 = od[min(z + lane, last)];
+      for (int L = 0; L < nld; ++L) {
+        const int o3 = lo.at(min(L + 3, nld));
+        const int z2 = L + 2 < nld ? o3 : z1;
+        const int li2 = od[min(z1 + lane, last)];
+        const int p1 = links_of(hn[li1]).p;
+        const bool pv = p >= 0;
+        const double g = ax[li], rp0 = rhs[pv ? p : li], x0 = rhs[li], dg = diag[li];
+        const double vi = v[li];
+        reads_issued();
+        if (a + lane < z) {
+          const double x = (x0 + (pv ? g : -0.0) * (pv ? rp0 : 1.0)) / dg;
+          rhs[li] = x;
+          v[li] = vi + x;
+        }
+        wave_sync<M>();
+        li = li1;
+        li1 = li2;
+        p = p1;
+        a = z;
+        z = z1;
+        z1 = z2;
+      }
     }
-    __syncthreads();
-    dc = dn;
-    dn = ahead;
+  }
+}
+
+// Elimination of node li with more than two children, from its own pivot and
+// rhs (dd, bb): children from the CSR lists in decreasing index. Returns true if
+// the pivot is zero or not finite.
+template <typename C>
+__device__ __forceinline__ bool elim_csr(const Dev& d, int64_t n0, int li, double dd, double bb, C rhs, C diag) {
+  const int64_t gi = n0 + li;
+  for (int64_t cq = d.child_off[gi + 1] - 1; cq >= d.child_off[gi]; --cq) {
+    const int lc = d.child[cq] - (int)n0;
+    const double g = d.axial[n0 + lc];
+    const double f = g / diag[lc];
+    dd -= f * g;
+    bb += f * rhs[lc];
+  }
+  diag[li] = dd;
+  rhs[li] = bb;
+  return (dd == 0.0) || !mind_isfinite(dd);
+}
+
+// True if a pivot is zero or not finite (no branches).
+__device__ __forceinline__ bool bad_pivot(double dd) {
+  return !((dd != 0.0) & (__builtin_fabs(dd) < __builtin_huge_val()));
+}
+
+// Tree solve of one wave's cells in the lean layout (rhs and pivots in LDS
+// pairs, records from HRec / DRec). The loop runs passes in threes with three
+// records in flight: a pass reads its node's pair and both children's (three
+// LDS reads), divides twice, stores the pair, then loads the record of the pass
+// three ahead into the registers it has just freed, so that no register moves
+// and no wait stand between a load and its use three passes later. Passes past
+// the wave's last only move the sink. The pass offsets are read one pass ahead
+// (a broadcast LDS read alongside the pass's own). Back substitution leaves the
+// node's change x in rhs; v += x follows for all nodes after the solve.
+template <typename C>
+__device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int grp, int64_t n0, int nn,
+                                           const int16_t* sched, C rhs, C diag) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const int h0 = sched_at(sched, w), nph = sched_at(sched, w + 1) - h0 - 1;
+  const int d0 = sched_at(sched, W + 1 + w), npd = sched_at(sched, W + 2 + w) - d0 - 1;
+  const int64_t rb = b.grp_rbase[grp];
+  bool bad = false;
+  if (nph > 0) {
+    const HRec* hr = b.r_hrec + rb;
+    const int16_t* lv = sched + h0;
+    auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
+    const int o0 = lv[0], o1 = lv[min(1, nph)], o2 = lv[min(2, nph)], o3 = lv[min(3, nph)];
+    int next = o3;  // start of the pass three ahead
+    HRec x0 = hr[slot(o0, o1)], x1 = hr[slot(o1, o2)], x2 = hr[slot(o2, o3)];
+    auto pass = [&](HRec& r, int q) {
+      const int e = lv[min(q + 4, nph)];  // end of the pass three ahead
+      const double od = diag[r.li], orr = rhs[r.li];
+      const double dh = diag[r.h], rh = rhs[r.h], dl = diag[r.l], rl = rhs[r.l];
+      reads_issued();
+      const double fh = r.gh / dh, fl = r.gl / dl;
+      const double dd = (od - fh * r.gh) - fl * r.gl;
+      const double bb = (orr + fh * rh) + fl * rl;
+      diag[r.li] = dd;
+      rhs[r.li] = bb;
+      bad |= (r.li < nn) & bad_pivot(dd);
+      if (r.csr) bad |= elim_csr(d, n0, r.li, od, orr, rhs, diag);
+      wave_sync<4>();
+      r = hr[slot(next, e)];
+      next = e;
+    };
+    for (int q = 0; q < nph; q += 3) {
+      pass(x0, q);
+      pass(x1, q + 1);
+      pass(x2, q + 2);
+    }
+  }
+  if (bad) raise_error(d.error, MIND_ESINGULAR);
+  if (npd > 0) {
+    const DRec* dr = b.r_drec + rb;
+    const int16_t* lv = sched + d0;
+    auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
+    const int o0 = lv[0], o1 = lv[min(1, npd)], o2 = lv[min(2, npd)], o3 = lv[min(3, npd)];
+    int next = o3;
+    DRec x0 = dr[slot(o0, o1)], x1 = dr[slot(o1, o2)], x2 = dr[slot(o2, o3)];
+    auto pass = [&](DRec& r, int q) {
+      const int e = lv[min(q + 4, npd)];
+      const double od = diag[r.li], orr = rhs[r.li], rp = rhs[r.p];
+      reads_issued();
+      rhs[r.li] = (orr + r.g * rp) / od;
+      wave_sync<4>();
+      r = dr[slot(next, e)];
+      next = e;
+    };
+    for (int q = 0; q < npd; q += 3) {
+      pass(x0, q);
+      pass(x1, q + 1);
+      pass(x2, q + 2);
+    }
   }
 }
 
@@ -210,24 +470,24 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
   const int64_t c0 = b.grp_cell0[grp], c1 = b.grp_cell1[grp];
   const int64_t n0 = d.cell_off[c0], n1 = d.cell_off[c1];
   const int nn = (int)(n1 - n0);
-  const int nlh = b.grp_nlh[grp], nld = b.grp_nld[grp];
+  const int slen = b.grp_slen[grp];
   const int ldsd = M == 0 ? 4 : M == 4 ? 2 : 0;  // doubles per node in LDS
-  double* rhs = M == 3 ? d.rhs + n0 : lds;
-  double* diag = M == 3 ? d.diag + n0 : lds + nn;
+  constexpr int RS = M == 3 ? 1 : 2;  // (pivot, rhs) pairs in LDS
+  const Col<RS> rhs{M == 3 ? d.rhs + n0 : lds + 1};
+  const Col<RS> diag{M == 3 ? d.diag + n0 : lds};
   double* v = M == 0 ? lds + 2 * nn : d.v + n0;
   double* axl = lds + 3 * nn;  // LDS copy (M = 0)
   const double* ax = M == 0 ? axl : d.axial + n0;
-  const int tlen = (int)rtab_len(nn, nlh, nld);
+  const int tlen = (int)rtab_len(nn, slen);
   const int16_t* rt = b.r_tab + b.grp_rtab[grp];
-  int16_t* tabl = reinterpret_cast<int16_t*>(lds + ldsd * nn);  // LDS copy (M = 0; level offsets M = 4)
+  // LDS copy of the table (M = 0) or of the schedule block (M = 4, after the
+  // pairs of the nodes, the dummy and the sink)
+  int16_t* tabl = reinterpret_cast<int16_t*>(lds + (M == 4 ? 2 * (nn + 2) : ldsd * nn));
   const int16_t* tab = M == 0 ? tabl : rt;
-  const int16_t* par = tab;          // parent, -1 at a root
-  const int16_t* clo = par + nn;     // lower-index child, -1 if none
-  const int16_t* chi = clo + nn;     // higher-index child; -1 if fewer than two, -2 if more
-  const int16_t* oh = chi + nn;      // nodes by height
-  const int16_t* od = oh + nn;       // nodes by depth
-  const int16_t* lh = M == 4 ? tabl : od + nn;  // nlh + 1 offsets into oh (LDS copy in mode 4)
-  const int16_t* ld = lh + nlh + 1;             // nld + 1 offsets into od
+  const uint2* hn = reinterpret_cast<const uint2*>(tab);  // links of each node
+  const int16_t* oh = tab + 4 * nn;                        // nodes by height
+  const int16_t* od = oh + nn;                             // nodes by depth
+  const int16_t* sched = M == 4 ? tabl : od + nn;          // schedule block (LDS copy in mode 4)
   GroupDev* sg = sh.sg;
   int64_t* sr0 = sh.sr0;
   int64_t* sr1 = sh.sr1;
@@ -247,8 +507,15 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     }
     for (int i = tid; i < tlen; i += T) tabl[i] = rt[i];
   }
-  if (M == 4)
-    for (int i = tid; i < nlh + 1 + nld + 1; i += T) tabl[i] = rt[5 * nn + i];
+  if (M == 4) {
+    for (int i = tid; i < slen; i += T) tabl[i] = rt[6 * nn + i];
+    if (tid == 0) {
+      diag[nn] = 1.0;  // the dummy
+      rhs[nn] = 1.0;
+      diag[nn + 1] = 1.0;  // the sink
+      rhs[nn + 1] = 0.0;
+    }
+  }
   const int64_t k0 = d.clock->step;
   const double dt = d.dt;
   const int32_t bucket = d.cell_bucket[c0];
@@ -320,19 +587,20 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     // cable assembly: own axial current first, children in increasing index
     for (int li = tid; li < nn; li += T) {
       const int64_t gi = n0 + li;
+      const Links k = links_of(hn[li]);
       const double dg = d.cdt[gi] + diag[li] + d.adiag[gi];
       double bb = rhs[li];
-      const int p = par[li];
+      const int p = k.p;
       const double vi = v[li];
       if (p >= 0) bb += ax[li] * (v[p] - vi);
-      const int h = chi[li];
+      const int h = k.h;
       if (h == -2) {
         for (int64_t q = d.child_off[gi]; q < d.child_off[gi + 1]; ++q) {
           const int c = d.child[q] - (int)n0;
           bb -= ax[c] * (vi - v[c]);
         }
       } else {
-        const int l = clo[li];
+        const int l = k.l;
         if (l >= 0) bb -= ax[l] * (vi - v[l]);
         if (h >= 0) bb -= ax[h] * (vi - v[h]);
       }
@@ -341,63 +609,13 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     }
     __syncthreads();
     if (M == 4) {
-      solve_lean(d, b, n0, nlh, nld, lh, ld, rhs, diag, v);
+      solve_lean(d, b, grp, n0, nn, sched, rhs, diag);
+      __syncthreads();
+      for (int i = tid; i < nn; i += T) v[i] += rhs[i];
     } else {
-      // elimination by height: a node pulls its children (lower levels) in
-      // decreasing index
-      bool bad = false;
-      for (int L = 0; L < nlh; ++L) {
-        const int z = lh[L + 1];
-        for (int i = lh[L] + tid; i < z; i += T) {
-          const int li = oh[i];
-          double dd = diag[li], bb = rhs[li];
-          const int h = chi[li];
-          if (h == -2) {
-            const int64_t gi = n0 + li;
-            for (int64_t cq = d.child_off[gi + 1] - 1; cq >= d.child_off[gi]; --cq) {
-              const int lc = d.child[cq] - (int)n0;
-              const double g = ax[lc];
-              const double f = g / diag[lc];
-              dd -= f * g;
-              bb += f * rhs[lc];
-            }
-          } else {
-            if (h >= 0) {
-              const double g = ax[h];
-              const double f = g / diag[h];
-              dd -= f * g;
-              bb += f * rhs[h];
-            }
-            const int l = clo[li];
-            if (l >= 0) {
-              const double g = ax[l];
-              const double f = g / diag[l];
-              dd -= f * g;
-              bb += f * rhs[l];
-            }
-          }
-          diag[li] = dd;
-          rhs[li] = bb;
-          bad |= (dd == 0.0) || !mind_isfinite(dd);
-        }
-        __syncthreads();
-      }
-      if (bad) raise_error(d.error, MIND_ESINGULAR);
-      // back substitution by depth, root to leaves; v += dv
-      for (int L = 0; L < nld; ++L) {
-        const int z = ld[L + 1];
-        for (int i = ld[L] + tid; i < z; i += T) {
-          const int li = od[i];
-          const int p = par[li];
-          double x = rhs[li];
-          if (p >= 0) x += ax[li] * rhs[p];
-          x = x / diag[li];
-          rhs[li] = x;
-          v[li] += x;
-        }
-        __syncthreads();
-      }
+      solve_waves<M>(d, b, n0, nn, sched, hn, oh, od, ax, rhs, diag, v);
     }
+    __syncthreads();
     // nrn_state at t + dt
     for (int gi = 0; gi < ngr; ++gi) {
       const GroupDev& G = sg[gi];
@@ -497,15 +715,72 @@ void node_levels(const Layout& L, int64_t a0, int64_t a1, std::vector<int32_t>& 
 
 // Groups consecutive cells within a node budget and the LDS capacity, and builds
 // each group's link and schedule block.
+// Assigns a group's cells to the W wavefronts of its workgroup for the tree
+// solve: cells in decreasing order of their own cost, each to the wave whose
+// cost grows least (ties: the lower wave). A wave's cost is its passes of 64
+// lanes summed over its height and depth levels (the cells' levels merge).
+std::vector<int> assign_waves(const std::vector<std::vector<int32_t>>& hist_h,
+                              const std::vector<std::vector<int32_t>>& hist_d, int W) {
+  const int nc = (int)hist_h.size();
+  auto passes = [](int32_t n) { return (n + 63) / 64; };
+  std::vector<int64_t> own(nc, 0);
+  for (int c = 0; c < nc; ++c) {
+    for (int32_t n : hist_h[c]) own[c] += passes(n);
+    for (int32_t n : hist_d[c]) own[c] += passes(n);
+  }
+  std::vector<int> ord(nc);
+  for (int c = 0; c < nc; ++c) ord[c] = c;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return own[a] > own[b]; });
+  std::vector<std::vector<int32_t>> wh(W), wd(W);
+  std::vector<int64_t> cost(W, 0);
+  std::vector<int> wave(nc, 0);
+  for (int c : ord) {
+    int best = 0;
+    int64_t best_cost = -1;
+    for (int w = 0; w < W; ++w) {
+      int64_t nw = cost[w];
+      for (size_t l = 0; l < hist_h[c].size(); ++l) {
+        const int32_t have = l < wh[w].size() ? wh[w][l] : 0;
+        nw += passes(have + hist_h[c][l]) - passes(have);
+      }
+      for (size_t l = 0; l < hist_d[c].size(); ++l) {
+        const int32_t have = l < wd[w].size() ? wd[w][l] : 0;
+        nw += passes(have + hist_d[c][l]) - passes(have);
+      }
+      if (best_cost < 0 || nw < best_cost) {
+        best = w;
+        best_cost = nw;
+      }
+    }
+    wave[c] = best;
+    cost[best] = best_cost;
+    if (wh[best].size() < hist_h[c].size()) wh[best].resize(hist_h[c].size(), 0);
+    if (wd[best].size() < hist_d[c].size()) wd[best].resize(hist_d[c].size(), 0);
+    for (size_t l = 0; l < hist_h[c].size(); ++l) wh[best][l] += hist_h[c][l];
+    for (size_t l = 0; l < hist_d[c].size(); ++l) wd[best][l] += hist_d[c][l];
+  }
+  return wave;
+}
+
+// Upper bound of a schedule block's length: the headers plus, per wave, one more
+// offset than its passes; a wave's levels are at most the sum of its cells', and
+// splitting levels into passes of 64 adds at most nodes / 64 per order.
+inline int64_t sched_bound(int W, int64_t sum_levels, int64_t nodes) {
+  return 4 * (W + 1) + 2 * W + sum_levels + 2 * (nodes / 64);
+}
+
+// Groups consecutive cells within a node budget and the LDS capacity, and builds
+// each group's link and schedule block.
 int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& out) {
   const Layout& L = S.L;
   BatchPlan* P = new BatchPlan();
   P->budget = budget;
   P->threads = threads;
   P->layout = layout;
+  const int W = threads / 64;
   std::vector<int32_t> cell0, cell1, glist;
   std::vector<int32_t> hh, dd;
-  int64_t cur_nodes = 0, cur_lh = 0, cur_ld = 0;
+  int64_t cur_nodes = 0, cur_levels = 0;
   int32_t start = 0;
   int rc = 0;
   for (int32_t c = 0; c <= L.ncells; ++c) {
@@ -514,16 +789,16 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
       nc = L.cell_off[c + 1] - L.cell_off[c];
       node_levels(L, L.cell_off[c], L.cell_off[c + 1], hh, dd, lh, ld);
     }
-    const bool alone_fits = window_bytes(layout, nc, lh, ld) <= kLdsResident;
+    const bool alone_fits = window_bytes(layout, nc, sched_bound(W, lh + ld, nc)) <= kLdsResident;
     const bool fits =
-        window_bytes(layout, cur_nodes + nc, std::max(cur_lh, lh), std::max(cur_ld, ld)) <= kLdsResident;
+        window_bytes(layout, cur_nodes + nc, sched_bound(W, cur_levels + lh + ld, cur_nodes + nc)) <= kLdsResident;
     const bool flush = c == L.ncells || cur_nodes + nc > budget || !fits || !alone_fits;
     if (flush && c > start) {
       cell0.push_back(start);
       cell1.push_back(c);
       glist.push_back(c - start == 1 && cur_nodes > budget ? 1 : 0);
       start = c;
-      cur_nodes = cur_lh = cur_ld = 0;
+      cur_nodes = cur_levels = 0;
     }
     if (c == L.ncells) break;
     if (!alone_fits) {  // a cell beyond the LDS capacity: its own workgroup, part in HBM
@@ -532,7 +807,7 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
         return fail(MIND_EUNSUPPORTED, "a cell has more than 32767 nodes");
       }
       // the lean layout if it fits, else everything in HBM
-      const int m = window_bytes(4, nc, lh, ld) <= kLdsResident ? 4 : 3;
+      const int m = window_bytes(4, nc, sched_bound(W, lh + ld, nc)) <= kLdsResident ? 4 : 3;
       cell0.push_back(c);
       cell1.push_back(c + 1);
       glist.push_back(m == 4 ? 2 : 3);
@@ -540,68 +815,175 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
       continue;
     }
     cur_nodes += nc;
-    cur_lh = std::max(cur_lh, lh);
-    cur_ld = std::max(cur_ld, ld);
+    cur_levels += lh + ld;
   }
   const int ng = (int)cell0.size();
   P->ngroups = ng;
   std::vector<int16_t> r_tab;
-  std::vector<int32_t> grp_rtab(ng), grp_nlh(ng), grp_nld(ng);
+  std::vector<int32_t> grp_rtab(ng), grp_slen(ng);
   bool any_lean = false;
   for (int g = 0; g < ng; ++g) {
     const int64_t n0 = L.cell_off[cell0[g]], n1 = L.cell_off[cell1[g]];
     const int nn = (int)(n1 - n0);
     int64_t nlh = 0, nld = 0;
     node_levels(L, n0, n1, hh, dd, nlh, nld);
+    // Both sweeps run by depth: back substitution from the roots, elimination
+    // in reverse (a node's children are one level deeper, so any order by
+    // decreasing depth eliminates them first; depth levels are narrower than
+    // height levels). Per cell histograms of nodes by depth, then the waves.
+    const int ncg = cell1[g] - cell0[g];
+    std::vector<std::vector<int32_t>> hist_d(ncg);
+    std::vector<int32_t> cell_of(nn);
+    for (int k = 0; k < ncg; ++k) {
+      const int64_t a = L.cell_off[cell0[g] + k] - n0, z = L.cell_off[cell0[g] + k + 1] - n0;
+      for (int64_t li = a; li < z; ++li) {
+        cell_of[li] = k;
+        if ((int64_t)hist_d[k].size() <= dd[li]) hist_d[k].resize(dd[li] + 1, 0);
+        hist_d[k][dd[li]] += 1;
+      }
+    }
+    const std::vector<int> wave = assign_waves(hist_d, hist_d, W);
+    // nodes per level of each wave (elimination levels: reversed depth levels)
+    std::vector<std::vector<int32_t>> wnh(W), wnd(W);
+    for (int k = 0; k < ncg; ++k) {
+      auto& b = wnd[wave[k]];
+      if (b.size() < hist_d[k].size()) b.resize(hist_d[k].size(), 0);
+      for (size_t l = 0; l < hist_d[k].size(); ++l) b[l] += hist_d[k][l];
+    }
+    for (int w = 0; w < W; ++w) wnh[w].assign(wnd[w].rbegin(), wnd[w].rend());
+    for (int li = 0; li < nn; ++li) hh[li] = (int32_t)wnd[wave[cell_of[li]]].size() - 1 - dd[li];
+    auto npasses = [](const std::vector<int32_t>& n) {
+      int64_t p = 0;
+      for (int32_t x : n) p += (x + 63) / 64;
+      return p;
+    };
+    int64_t slen = 4 * (W + 1);
+    for (int w = 0; w < W; ++w) slen += npasses(wnh[w]) + 1 + npasses(wnd[w]) + 1;
+    if (slen > (1 << 30)) {
+      free_plan(P);
+      return fail(MIND_EUNSUPPORTED, "schedule block too long");
+    }
     const int list = glist[g];
     P->lists[list].push_back(g);
     const int mode = list == 0 || list == 1 ? layout : list == 2 ? 4 : 3;
     if (mode == 4) any_lean = true;
-    P->lds[list] = std::max(P->lds[list], window_bytes(mode, nn, nlh, nld));
+    P->lds[list] = std::max(P->lds[list], window_bytes(mode, nn, slen));
+    r_tab.resize((r_tab.size() + 3) / 4 * 4, 0);  // 8-byte aligned blocks
     grp_rtab[g] = (int32_t)r_tab.size();
-    grp_nlh[g] = (int32_t)nlh;
-    grp_nld[g] = (int32_t)nld;
+    grp_slen[g] = (int32_t)slen;
     const size_t base = r_tab.size();
-    r_tab.resize(base + (size_t)rtab_len(nn, nlh, nld), 0);
+    r_tab.resize(base + (size_t)rtab_len(nn, slen), 0);
     int16_t* t = r_tab.data() + base;
-    int16_t *t_par = t, *t_lo = t + nn, *t_hi = t + 2 * nn, *t_oh = t + 3 * nn, *t_od = t + 4 * nn,
-            *t_lh = t + 5 * nn, *t_ld = t_lh + nlh + 1;
+    int16_t *t_hn = t, *t_oh = t + 4 * nn, *t_od = t + 5 * nn, *t_s = t + 6 * nn;
     for (int li = 0; li < nn; ++li) {
       const int64_t gi = n0 + li;
       const int32_t p = L.parent[gi];
-      t_par[li] = (int16_t)(p < 0 ? -1 : p - (int32_t)n0);
       const int64_t o0 = L.child_off[gi], o1 = L.child_off[gi + 1];
-      t_lo[li] = (int16_t)(o1 > o0 ? L.child[o0] - (int32_t)n0 : -1);
-      t_hi[li] = (int16_t)(o1 - o0 == 2 ? L.child[o0 + 1] - (int32_t)n0 : (o1 - o0 > 2 ? -2 : -1));
+      t_hn[4 * li + 0] = (int16_t)(o1 - o0 == 2 ? L.child[o0 + 1] - (int32_t)n0 : (o1 - o0 > 2 ? -2 : -1));
+      t_hn[4 * li + 1] = (int16_t)(o1 > o0 ? L.child[o0] - (int32_t)n0 : -1);
+      t_hn[4 * li + 2] = (int16_t)(p < 0 ? -1 : p - (int32_t)n0);
+      t_hn[4 * li + 3] = 0;
     }
-    auto order = [&](const std::vector<int32_t>& lv, int64_t nl, int16_t* ord, int16_t* off) {
-      std::vector<int32_t> cnt(nl + 1, 0);
-      for (int li = 0; li < nn; ++li) cnt[lv[li] + 1] += 1;
-      for (int64_t l = 0; l < nl; ++l) cnt[l + 1] += cnt[l];
-      for (int64_t l = 0; l <= nl; ++l) off[l] = (int16_t)cnt[l];
-      for (int li = 0; li < nn; ++li) ord[cnt[lv[li]]++] = (int16_t)li;
+    auto put = [&](int k, int64_t value) {
+      t_s[2 * k] = (int16_t)(uint16_t)(value & 0xffff);
+      t_s[2 * k + 1] = (int16_t)(value >> 16);
     };
-    order(hh, nlh, t_oh, t_lh);
-    order(dd, nld, t_od, t_ld);
+    // per wave: its nodes by level (increasing node index within a level), the
+    // level offsets into the concatenated order, and the header entries
+    // per wave: its nodes by level (increasing node index within a level), and
+    // the offsets of its passes into the concatenated order
+    auto order = [&](const std::vector<int32_t>& lv, const std::vector<std::vector<int32_t>>& wn, int16_t* ord,
+                     int hdr0, int64_t& cursor) {
+      int32_t pos = 0;
+      for (int w = 0; w < W; ++w) {
+        put(hdr0 + w, cursor);
+        const int64_t nl = (int64_t)wn[w].size();
+        std::vector<int32_t> cnt(nl + 1, 0);
+        for (int64_t l = 0; l < nl; ++l) cnt[l + 1] = cnt[l] + wn[w][l];
+        for (int64_t l = 0; l < nl; ++l)
+          for (int32_t q = cnt[l]; q < cnt[l + 1]; q += 64) t_s[cursor++] = (int16_t)(pos + q);
+        t_s[cursor++] = (int16_t)(pos + cnt[nl]);
+        for (int li = 0; li < nn; ++li)
+          if (wave[cell_of[li]] == w) ord[pos + cnt[lv[li]]++] = (int16_t)li;
+        pos += cnt[nl];  // the fill leaves cnt[nl], the wave's node count, unchanged
+      }
+      put(hdr0 + W, cursor);
+    };
+    int64_t cursor = 4 * (W + 1);
+    order(hh, wnh, t_oh, 0, cursor);
+    order(dd, wnd, t_od, W + 1, cursor);
+    // the kernels index by these tables without bounds checks: every node once
+    // per order, in its own cell's wave; passes of at most 64 nodes of one
+    // level, in level order
+    auto valid = [&](const std::vector<int32_t>& lv, const int16_t* ord, int hdr0) {
+      std::vector<uint8_t> seen(nn, 0);
+      int32_t pos = 0;
+      for (int w = 0; w < W; ++w) {
+        const int64_t a = sched_at(t_s, hdr0 + w), z = sched_at(t_s, hdr0 + w + 1);
+        if (a < 4 * (W + 1) || z <= a || z > slen || t_s[a] != pos) return false;
+        int32_t level = 0;
+        for (int64_t q = 0; q + 1 < z - a; ++q) {
+          const int32_t i0 = t_s[a + q], i1 = t_s[a + q + 1];
+          if (i1 <= i0 || i1 - i0 > 64 || i1 > nn) return false;
+          const int32_t l0 = lv[ord[i0] >= 0 && ord[i0] < nn ? ord[i0] : 0];
+          if (l0 < level) return false;
+          level = l0;
+          for (int32_t i = i0; i < i1; ++i) {
+            const int li = ord[i];
+            if (li < 0 || li >= nn || seen[li] || wave[cell_of[li]] != w || lv[li] != level) return false;
+            seen[li] = 1;
+          }
+          pos = i1;
+        }
+      }
+      return pos == nn;
+    };
+    if (!valid(hh, t_oh, 0) || !valid(dd, t_od, W + 1)) {
+      free_plan(P);
+      return fail(MIND_EUNSUPPORTED, "batched plan: inconsistent solve schedule");
+    }
   }
-  std::vector<HRec> hrec;
-  std::vector<DRec> drec;
-  if (any_lean) {  // per-node level records of the lean layout, from the int16 block
-    hrec.resize(L.nodes);
-    drec.resize(L.nodes);
-    for (int g = 0; g < ng; ++g) {
-      const int64_t n0 = L.cell_off[cell0[g]];
-      const int nn = (int)(L.cell_off[cell1[g]] - n0);
-      const int16_t* t = r_tab.data() + grp_rtab[g];
-      const int16_t *t_par = t, *t_lo = t + nn, *t_hi = t + 2 * nn, *t_oh = t + 3 * nn, *t_od = t + 4 * nn;
-      for (int i = 0; i < nn; ++i) {
-        const int li = t_oh[i], h = t_hi[li], l = t_lo[li];
-        hrec[n0 + i] = HRec{(int16_t)li, (int16_t)h, (int16_t)l, 0, h >= 0 ? L.axial[n0 + h] : 0.0,
-                            l >= 0 ? L.axial[n0 + l] : 0.0};
-        const int lj = t_od[i];
-        drec[n0 + i] = DRec{(int16_t)lj, t_par[lj], 0, 0, L.axial[n0 + lj]};
+  {
+    std::vector<HRec> hrec;
+    std::vector<DRec> drec;
+    std::vector<int32_t> rbase(ng);
+    if (any_lean) {  // records of the lean layout, from the int16 block; group g's from n0 + g
+      hrec.resize(L.nodes + ng);
+      drec.resize(L.nodes + ng);
+      for (int g = 0; g < ng; ++g) {
+        const int64_t n0 = L.cell_off[cell0[g]];
+        const int nn = (int)(L.cell_off[cell1[g]] - n0);
+        const int64_t rb = n0 + g;
+        rbase[g] = (int32_t)rb;
+        const int16_t dummy = (int16_t)nn, sink = (int16_t)(nn + 1);
+        const int16_t* t = r_tab.data() + grp_rtab[g];
+        const int16_t *t_hn = t, *t_oh = t + 4 * nn, *t_od = t + 5 * nn;
+        for (int i = 0; i < nn; ++i) {
+          const int li = t_oh[i], h = t_hn[4 * li], l = t_hn[4 * li + 1];
+          if (h == -2) {
+            hrec[rb + i] = HRec{(int16_t)li, dummy, dummy, 1, -0.0, -0.0};
+          } else {
+            hrec[rb + i] = HRec{(int16_t)li, h >= 0 ? (int16_t)h : dummy, l >= 0 ? (int16_t)l : dummy, 0,
+                                h >= 0 ? L.axial[n0 + h] : -0.0, l >= 0 ? L.axial[n0 + l] : -0.0};
+          }
+          const int lj = t_od[i], pj = t_hn[4 * lj + 2];
+          drec[rb + i] = DRec{(int16_t)lj, pj >= 0 ? (int16_t)pj : dummy, 0, 0, pj >= 0 ? L.axial[n0 + lj] : -0.0};
+        }
+        hrec[rb + nn] = HRec{sink, dummy, dummy, 0, -0.0, -0.0};
+        drec[rb + nn] = DRec{sink, dummy, 0, 0, -0.0};
       }
     }
+    HRec* ph = nullptr;
+    DRec* pd = nullptr;
+    int32_t* pr = nullptr;
+    if ((rc = plan_upload(S, *P, &ph, hrec)) || (rc = plan_upload(S, *P, &pd, drec)) ||
+        (rc = plan_upload(S, *P, &pr, rbase))) {
+      free_plan(P);
+      return rc;
+    }
+    P->b.r_hrec = ph;
+    P->b.r_drec = pd;
+    P->b.grp_rbase = pr;
   }
   BatchDev& b = P->b;
   int32_t* p = nullptr;
@@ -609,15 +991,8 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
   if ((rc = plan_upload(S, *P, &p, cell0))) goto error; b.grp_cell0 = p;
   if ((rc = plan_upload(S, *P, &p, cell1))) goto error; b.grp_cell1 = p;
   if ((rc = plan_upload(S, *P, &p16, r_tab))) goto error; b.r_tab = p16;
-  {
-    HRec* ph = nullptr;
-    DRec* pd = nullptr;
-    if ((rc = plan_upload(S, *P, &ph, hrec))) goto error; b.r_hrec = ph;
-    if ((rc = plan_upload(S, *P, &pd, drec))) goto error; b.r_drec = pd;
-  }
   if ((rc = plan_upload(S, *P, &p, grp_rtab))) goto error; b.grp_rtab = p;
-  if ((rc = plan_upload(S, *P, &p, grp_nlh))) goto error; b.grp_nlh = p;
-  if ((rc = plan_upload(S, *P, &p, grp_nld))) goto error; b.grp_nld = p;
+  if ((rc = plan_upload(S, *P, &p, grp_slen))) goto error; b.grp_slen = p;
   for (int m = 0; m < kLists; ++m)
     if ((rc = plan_upload(S, *P, &P->dev_lists[m], P->lists[m]))) goto error;
   for (int m = 1; m < kLists; ++m) {  // largest layouts first: the longest workgroups start early
