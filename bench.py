@@ -118,7 +118,7 @@ def main():
     bytes_per_launch = bytes_per_step * (ksteps / launches)
     avg_launch_s = kms / launches / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    kernel = "k_window_lean" if extra["plan"]["layout"] == 4 else "k_window"
+    kernel = {0: "k_window", 4: "k_window_lean_hbm", 5: "k_window_lean"}[extra["plan"]["layout"]]
     traffic, traffic_note = measured_traffic(kernel, extra["plan"], extra["window"], ksteps / launches,
                                              bytes_per_launch)
     destroy(model)

@@ -248,11 +248,12 @@ int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows);
 int mind_time(const mind_state* state, double* t, int64_t* step);
 int mind_window_steps(const mind_state* state, int32_t* steps);
 /* Batched method: node budget per workgroup, block size, node-state layout of those
- * workgroups (0 all in LDS, 4 lean: rhs and diag in LDS, the solve's node records
- * streamed from HBM) and workgroup count chosen by the autotuner. */
+ * workgroups (0 all in LDS; 5 lean: rhs, pivots and v in LDS, the solve's node
+ * records streamed from HBM; 4 lean with v in HBM) and workgroup count chosen by
+ * the autotuner. */
 int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int32_t* layout,
                     int64_t* groups);
-/* Replaces that plan (threads a multiple of 64 up to 512, layout 0 or 4);
+/* Replaces that plan (threads a multiple of 64 up to 512, layout 0, 4 or 5);
  * mind_initialize must follow. */
 int mind_set_batch_plan(mind_state* state, int64_t budget, int32_t threads, int32_t layout);
 /* ---- Surrogate MLP (no counterpart in the reference) ----------------------------

@@ -52,10 +52,20 @@ struct DRec {
   double g;
 };
 
+// Cable assembly record of a node (modes 3-5, whose links and conductances are
+// not in LDS): higher and lower child, parent (-1 if none, higher -2 with more
+// than two children), and the axial conductances of the node and of those two
+// children. Indexed by global node.
+struct ARec {
+  int16_t h, l, p, pad;
+  double g, gh, gl;
+};
+
 struct BatchDev {
   const int32_t* grp_cell0;
   const int32_t* grp_cell1;
-  const HRec* r_hrec;  // mode 4 only, group g's from grp_rbase[g]
+  const ARec* r_arec;  // modes 3-5
+  const HRec* r_hrec;  // modes 4 and 5, group g's from grp_rbase[g]
   const DRec* r_drec;
   const int32_t* grp_rbase;
   // per group one int16 block (local node ids, 8-byte aligned): per node its
@@ -68,10 +78,11 @@ struct BatchDev {
 };
 
 // Launch lists: 0 groups within the node budget (main stream); 1 single cells
-// above the budget, in the plan's layout; 2 cells too large for that, lean; 3
-// cells too large for the lean layout, all in HBM (lists 1-3 on the second
-// stream).
-constexpr int kLists = 4;
+// above the budget, in the plan's layout; cells too large for that: 2 lean with
+// v in LDS, 3 lean with v in HBM, 4 all in HBM (lists 1-4 on the second stream).
+constexpr int kLists = 5;
+// Mode of the workgroups of a list under a plan layout.
+inline int list_mode(int list, int layout) { return list <= 1 ? layout : list == 2 ? 5 : list == 3 ? 4 : 3; }
 
 struct BatchPlan {
   int64_t budget = 0;  // node budget per workgroup
@@ -92,6 +103,7 @@ struct BatchPlan {
 };
 
 constexpr int kMaxThreads = 512;
+constexpr int kLargeThreads = 512;  // block size cap of the launch of cells above the budget
 constexpr int kMaxGroups = 8;    // mechanism groups cached in LDS (the six of the step)
 constexpr int kMaxDetLds = 256;  // detectors of a workgroup cached in LDS
 // dynamic LDS of a workgroup, leaving room for its static tables
@@ -109,10 +121,12 @@ __host__ __device__ inline int sched_at(const int16_t* s, int k) {
 // int16 entries of a group's table block.
 __host__ __device__ inline int64_t rtab_len(int64_t nn, int64_t slen) { return 6 * nn + slen; }
 // Dynamic LDS bytes of a workgroup in mode M: mode 0 keeps 4 doubles per node and
-// the table block, mode 4 rhs, diag and the schedule block, mode 3 nothing.
+// the table block, mode 4 the (pivot, rhs) pairs (plus dummy and sink) and the
+// schedule block, mode 5 also v, mode 3 nothing.
 inline size_t window_bytes(int m, int64_t nn, int64_t slen) {
   if (m == 3) return 0;
   if (m == 4) return (size_t)(16 * (nn + 2) + (2 * slen + 7) / 8 * 8);
+  if (m == 5) return (size_t)(16 * (nn + 2) + 8 * nn + (2 * slen + 7) / 8 * 8);
   return (size_t)(32 * nn + (2 * rtab_len(nn, slen) + 7) / 8 * 8);
 }
 
@@ -129,9 +143,10 @@ __device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_e
 }
 
 // M says where a workgroup's node state lives: 0 all in LDS (v, rhs, diag, axial
-// conductances, links and schedule), 4 rhs, diag and the level offsets in LDS, 3
-// everything in HBM (rhs and diag in the global scratch columns, the links and
-// schedule read from the plan).
+// conductances, links and schedule), 5 (pivot, rhs) pairs, v and the schedule
+// in LDS, 4 the pairs and the schedule in LDS, 3 everything in HBM (rhs and diag
+// in the global scratch columns, the links and schedule read from the plan).
+// Modes 4 and 5 solve with the lean records.
 // Orders one wavefront's solve accesses between levels. Its lanes run in
 // lockstep, so a fence is all a level needs from the one before: at wavefront
 // scope for rhs and diag in LDS (executed in order, no wait), at workgroup scope
@@ -454,6 +469,11 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
   }
 }
 
+// Mechanisms with a state update (nrn_state).
+__device__ __forceinline__ bool stateful(int32_t mech) {
+  return mech == MIND_MECH_HH || mech == MIND_MECH_EXP2SYN || mech == MIND_MECH_EXPSYN;
+}
+
 // Per-window constants of the step's critical path: mechanism groups, their row
 // ranges for this workgroup's cells, and its detectors.
 struct WinShared {
@@ -475,19 +495,20 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
   constexpr int RS = M == 3 ? 1 : 2;  // (pivot, rhs) pairs in LDS
   const Col<RS> rhs{M == 3 ? d.rhs + n0 : lds + 1};
   const Col<RS> diag{M == 3 ? d.diag + n0 : lds};
-  double* v = M == 0 ? lds + 2 * nn : d.v + n0;
+  constexpr bool lean = M == 4 || M == 5;
+  double* v = M == 0 ? lds + 2 * nn : M == 5 ? lds + 2 * (nn + 2) : d.v + n0;
   double* axl = lds + 3 * nn;  // LDS copy (M = 0)
   const double* ax = M == 0 ? axl : d.axial + n0;
   const int tlen = (int)rtab_len(nn, slen);
   const int16_t* rt = b.r_tab + b.grp_rtab[grp];
   // LDS copy of the table (M = 0) or of the schedule block (M = 4, after the
   // pairs of the nodes, the dummy and the sink)
-  int16_t* tabl = reinterpret_cast<int16_t*>(lds + (M == 4 ? 2 * (nn + 2) : ldsd * nn));
+  int16_t* tabl = reinterpret_cast<int16_t*>(lds + (M == 4 ? 2 * (nn + 2) : M == 5 ? 2 * (nn + 2) + nn : ldsd * nn));
   const int16_t* tab = M == 0 ? tabl : rt;
   const uint2* hn = reinterpret_cast<const uint2*>(tab);  // links of each node
   const int16_t* oh = tab + 4 * nn;                        // nodes by height
   const int16_t* od = oh + nn;                             // nodes by depth
-  const int16_t* sched = M == 4 ? tabl : od + nn;          // schedule block (LDS copy in mode 4)
+  const int16_t* sched = lean ? tabl : od + nn;            // schedule block (LDS copy in modes 4, 5)
   GroupDev* sg = sh.sg;
   int64_t* sr0 = sh.sr0;
   int64_t* sr1 = sh.sr1;
@@ -507,7 +528,9 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     }
     for (int i = tid; i < tlen; i += T) tabl[i] = rt[i];
   }
-  if (M == 4) {
+  if (M == 5)
+    for (int i = tid; i < nn; i += T) v[i] = d.v[n0 + i];
+  if (lean) {
     for (int i = tid; i < slen; i += T) tabl[i] = rt[6 * nn + i];
     if (tid == 0) {
       diag[nn] = 1.0;  // the dummy
@@ -585,30 +608,42 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       __syncthreads();
     }
     // cable assembly: own axial current first, children in increasing index
-    for (int li = tid; li < nn; li += T) {
+    auto assemble = [&](int li, int p, int l, int h, double g, double gl, double gh, double cdt, double adg) {
       const int64_t gi = n0 + li;
-      const Links k = links_of(hn[li]);
-      const double dg = d.cdt[gi] + diag[li] + d.adiag[gi];
+      const double dg = cdt + diag[li] + adg;
       double bb = rhs[li];
-      const int p = k.p;
       const double vi = v[li];
-      if (p >= 0) bb += ax[li] * (v[p] - vi);
-      const int h = k.h;
+      if (p >= 0) bb += g * (v[p] - vi);
       if (h == -2) {
         for (int64_t q = d.child_off[gi]; q < d.child_off[gi + 1]; ++q) {
           const int c = d.child[q] - (int)n0;
-          bb -= ax[c] * (vi - v[c]);
+          bb -= d.axial[n0 + c] * (vi - v[c]);
         }
       } else {
-        const int l = k.l;
-        if (l >= 0) bb -= ax[l] * (vi - v[l]);
-        if (h >= 0) bb -= ax[h] * (vi - v[h]);
+        if (l >= 0) bb -= gl * (vi - v[l]);
+        if (h >= 0) bb -= gh * (vi - v[h]);
       }
       diag[li] = dg;
       rhs[li] = bb;
+    };
+    if (M == 0) {
+      for (int li = tid; li < nn; li += T) {
+        const Links k = links_of(hn[li]);
+        assemble(li, k.p, k.l, k.h, ax[li], k.l >= 0 ? ax[k.l] : 0.0, k.h >= 0 ? ax[k.h] : 0.0, d.cdt[n0 + li],
+                 d.adiag[n0 + li]);
+      }
+    } else {  // two nodes per thread per round, their records read first
+      const ARec* ar = b.r_arec + n0;
+      for (int l0 = tid; l0 < nn; l0 += 2 * T) {
+        const int l1 = l0 + T < nn ? l0 + T : l0;
+        const ARec r0 = ar[l0], r1 = ar[l1];
+        const double c0 = d.cdt[n0 + l0], a0 = d.adiag[n0 + l0], c1 = d.cdt[n0 + l1], a1 = d.adiag[n0 + l1];
+        assemble(l0, r0.p, r0.l, r0.h, r0.g, r0.gl, r0.gh, c0, a0);
+        if (l0 + T < nn) assemble(l1, r1.p, r1.l, r1.h, r1.g, r1.gl, r1.gh, c1, a1);
+      }
     }
     __syncthreads();
-    if (M == 4) {
+    if (lean) {
       solve_lean(d, b, grp, n0, nn, sched, rhs, diag);
       __syncthreads();
       for (int i = tid; i < nn; i += T) v[i] += rhs[i];
@@ -616,14 +651,25 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       solve_waves<M>(d, b, n0, nn, sched, hn, oh, od, ax, rhs, diag, v);
     }
     __syncthreads();
-    // nrn_state at t + dt
-    for (int gi = 0; gi < ngr; ++gi) {
-      const GroupDev& G = sg[gi];
-      if (G.mech != MIND_MECH_HH && G.mech != MIND_MECH_EXP2SYN && G.mech != MIND_MECH_EXPSYN) continue;
-      const Cols col = cols(G);
-      const int64_t r0 = sr0[gi], r1 = sr1[gi];
-      for (int64_t r = r0 + tid; r < r1; r += T)
-        phase_advance(G.mech, col, r, v[G.node[r] - n0], dt, d.celsius);
+    // nrn_state at t + dt: the rows of all stateful groups as one index space
+    // (rows are independent), so that the groups' rows run side by side
+    {
+      int64_t total = 0;
+      for (int gi = 0; gi < ngr; ++gi)
+        if (stateful(sg[gi].mech)) total += sr1[gi] - sr0[gi];
+      for (int64_t f = tid; f < total; f += T) {
+        int gi = 0;
+        int64_t r = f;
+        for (;; ++gi) {
+          if (!stateful(sg[gi].mech)) continue;
+          const int64_t cnt = sr1[gi] - sr0[gi];
+          if (r < cnt) break;
+          r -= cnt;
+        }
+        const GroupDev& G = sg[gi];
+        r += sr0[gi];
+        phase_advance(G.mech, cols(G), r, v[G.node[r] - n0], dt, d.celsius);
+      }
     }
     if (rec1 > rec0) {
       __syncthreads();
@@ -640,7 +686,7 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     __syncthreads();
     if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k + 1);  // end-of-step delivery
   }
-  if (M == 0) {
+  if (M == 0 || M == 5) {
     __syncthreads();
     for (int i = tid; i < nn; i += T) d.v[n0 + i] = v[i];
   }
@@ -653,8 +699,16 @@ __global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const
   window_body<0>(d, b, groups[blockIdx.x], n, sh, lds);
 }
 
-// Groups within the node budget in the lean layout.
+// Groups within the node budget in the lean layout, v in LDS.
 __global__ void __launch_bounds__(kMaxThreads) k_window_lean(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  window_body<5>(d, b, groups[blockIdx.x], n, sh, lds);
+}
+
+// Groups within the node budget in the lean layout, v in HBM.
+__global__ void __launch_bounds__(kMaxThreads) k_window_lean_hbm(Dev d, BatchDev b, const int32_t* groups,
+                                                                 int32_t n) {
   extern __shared__ double lds[];
   __shared__ WinShared sh;
   window_body<4>(d, b, groups[blockIdx.x], n, sh, lds);
@@ -662,7 +716,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_window_lean(Dev d, BatchDev b, 
 
 // Cells above the node budget, one per workgroup, each with the layout that
 // fits (modes[k] for groups[k]): one launch, so that they run side by side.
-__global__ void __launch_bounds__(kMaxThreads) k_window_large(Dev d, BatchDev b, const int32_t* groups,
+__global__ void __launch_bounds__(kLargeThreads) k_window_large(Dev d, BatchDev b, const int32_t* groups,
                                                               const int8_t* modes, int32_t n) {
   extern __shared__ double lds[];
   __shared__ WinShared sh;
@@ -670,6 +724,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_window_large(Dev d, BatchDev b,
   switch (modes[blockIdx.x]) {
     case 0: window_body<0>(d, b, grp, n, sh, lds); break;
     case 4: window_body<4>(d, b, grp, n, sh, lds); break;
+    case 5: window_body<5>(d, b, grp, n, sh, lds); break;
     default: window_body<3>(d, b, grp, n, sh, lds); break;
   }
 }
@@ -777,7 +832,8 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
   P->budget = budget;
   P->threads = threads;
   P->layout = layout;
-  const int W = threads / 64;
+  const int W0 = threads / 64;                             // waves of a list-0 workgroup
+  const int Wl = std::min(threads, kLargeThreads) / 64;    // of the others
   std::vector<int32_t> cell0, cell1, glist;
   std::vector<int32_t> hh, dd;
   int64_t cur_nodes = 0, cur_levels = 0;
@@ -789,9 +845,9 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
       nc = L.cell_off[c + 1] - L.cell_off[c];
       node_levels(L, L.cell_off[c], L.cell_off[c + 1], hh, dd, lh, ld);
     }
-    const bool alone_fits = window_bytes(layout, nc, sched_bound(W, lh + ld, nc)) <= kLdsResident;
+    const bool alone_fits = window_bytes(layout, nc, sched_bound(W0, lh + ld, nc)) <= kLdsResident;
     const bool fits =
-        window_bytes(layout, cur_nodes + nc, sched_bound(W, cur_levels + lh + ld, cur_nodes + nc)) <= kLdsResident;
+        window_bytes(layout, cur_nodes + nc, sched_bound(W0, cur_levels + lh + ld, cur_nodes + nc)) <= kLdsResident;
     const bool flush = c == L.ncells || cur_nodes + nc > budget || !fits || !alone_fits;
     if (flush && c > start) {
       cell0.push_back(start);
@@ -806,11 +862,11 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
         free_plan(P);
         return fail(MIND_EUNSUPPORTED, "a cell has more than 32767 nodes");
       }
-      // the lean layout if it fits, else everything in HBM
-      const int m = window_bytes(4, nc, sched_bound(W, lh + ld, nc)) <= kLdsResident ? 4 : 3;
+      // the lean layout if it fits (v in LDS if that fits too), else everything in HBM
+      const int64_t sb = sched_bound(Wl, lh + ld, nc);
       cell0.push_back(c);
       cell1.push_back(c + 1);
-      glist.push_back(m == 4 ? 2 : 3);
+      glist.push_back(window_bytes(5, nc, sb) <= kLdsResident ? 2 : window_bytes(4, nc, sb) <= kLdsResident ? 3 : 4);
       start = c + 1;
       continue;
     }
@@ -821,7 +877,7 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
   P->ngroups = ng;
   std::vector<int16_t> r_tab;
   std::vector<int32_t> grp_rtab(ng), grp_slen(ng);
-  bool any_lean = false;
+  bool any_lean = false, any_hbm_links = false;
   for (int g = 0; g < ng; ++g) {
     const int64_t n0 = L.cell_off[cell0[g]], n1 = L.cell_off[cell1[g]];
     const int nn = (int)(n1 - n0);
@@ -842,6 +898,7 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
         hist_d[k][dd[li]] += 1;
       }
     }
+    const int W = glist[g] == 0 ? W0 : Wl;
     const std::vector<int> wave = assign_waves(hist_d, hist_d, W);
     // nodes per level of each wave (elimination levels: reversed depth levels)
     std::vector<std::vector<int32_t>> wnh(W), wnd(W);
@@ -865,8 +922,9 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
     }
     const int list = glist[g];
     P->lists[list].push_back(g);
-    const int mode = list == 0 || list == 1 ? layout : list == 2 ? 4 : 3;
-    if (mode == 4) any_lean = true;
+    const int mode = list_mode(list, layout);
+    if (mode == 4 || mode == 5) any_lean = true;
+    if (mode != 0) any_hbm_links = true;
     P->lds[list] = std::max(P->lds[list], window_bytes(mode, nn, slen));
     r_tab.resize((r_tab.size() + 3) / 4 * 4, 0);  // 8-byte aligned blocks
     grp_rtab[g] = (int32_t)r_tab.size();
@@ -973,14 +1031,30 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
         drec[rb + nn] = DRec{sink, dummy, 0, 0, -0.0};
       }
     }
+    std::vector<ARec> arec;
+    if (any_hbm_links) {  // assembly records, from the int16 block
+      arec.resize(L.nodes);
+      for (int g = 0; g < ng; ++g) {
+        const int64_t n0 = L.cell_off[cell0[g]];
+        const int nn = (int)(L.cell_off[cell1[g]] - n0);
+        const int16_t* t_hn = r_tab.data() + grp_rtab[g];
+        for (int li = 0; li < nn; ++li) {
+          const int h = t_hn[4 * li], l = t_hn[4 * li + 1], p = t_hn[4 * li + 2];
+          arec[n0 + li] = ARec{(int16_t)h, (int16_t)l, (int16_t)p, 0, L.axial[n0 + li],
+                               h >= 0 ? L.axial[n0 + h] : 0.0, l >= 0 ? L.axial[n0 + l] : 0.0};
+        }
+      }
+    }
     HRec* ph = nullptr;
     DRec* pd = nullptr;
+    ARec* pa = nullptr;
     int32_t* pr = nullptr;
     if ((rc = plan_upload(S, *P, &ph, hrec)) || (rc = plan_upload(S, *P, &pd, drec)) ||
-        (rc = plan_upload(S, *P, &pr, rbase))) {
+        (rc = plan_upload(S, *P, &pr, rbase)) || (rc = plan_upload(S, *P, &pa, arec))) {
       free_plan(P);
       return rc;
     }
+    P->b.r_arec = pa;
     P->b.r_hrec = ph;
     P->b.r_drec = pd;
     P->b.grp_rbase = pr;
@@ -999,7 +1073,7 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
     const int list = kLists - m;
     for (int32_t g : P->lists[list]) {
       P->large.push_back(g);
-      P->large_mode.push_back((int8_t)(list == 1 ? layout : list == 2 ? 4 : 3));
+      P->large_mode.push_back((int8_t)list_mode(list, layout));
     }
     P->large_lds = std::max(P->large_lds, P->lds[list]);
   }
@@ -1035,16 +1109,20 @@ int launch_cells(State& S, const BatchPlan& P, int32_t n) {
   if (side) {
     HIP_TRY(hipEventRecord(S.fork, S.stream));
     HIP_TRY(hipStreamWaitEvent(S.aux, S.fork, 0));
-    hipLaunchKernelGGL(k_window_large, dim3((unsigned)P.large.size()), dim3(P.threads), P.large_lds, S.aux, S.d,
+    hipLaunchKernelGGL(k_window_large, dim3((unsigned)P.large.size()), dim3(std::min(P.threads, kLargeThreads)),
+                       P.large_lds, S.aux, S.d,
                        P.b, P.dev_large, P.dev_large_mode, n);
     HIP_TRY(hipEventRecord(S.join, S.aux));
   }
   int rc = 0;
   if (S.profiling && !P.lists[0].empty() && (rc = timer_start(S))) return rc;
   if (!P.lists[0].empty()) {
-    if (P.layout == 4)
+    if (P.layout == 5)
       hipLaunchKernelGGL(k_window_lean, dim3((unsigned)P.lists[0].size()), dim3(P.threads), P.lds[0], S.stream, S.d,
                          P.b, P.dev_lists[0], n);
+    else if (P.layout == 4)
+      hipLaunchKernelGGL(k_window_lean_hbm, dim3((unsigned)P.lists[0].size()), dim3(P.threads), P.lds[0], S.stream,
+                         S.d, P.b, P.dev_lists[0], n);
     else
       hipLaunchKernelGGL(k_window, dim3((unsigned)P.lists[0].size()), dim3(P.threads), P.lds[0], S.stream, S.d,
                          P.b, P.dev_lists[0], n);
@@ -1092,6 +1170,8 @@ int batched_setup(State& S, int32_t cells_per_group) {
                               (int)kLdsResident));
   HIP_TRY(hipFuncSetAttribute((const void*)k_window_lean, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kLdsResident));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_window_lean_hbm, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsResident));
   int64_t max_cell = 1;
   for (int32_t c = 0; c < S.L.ncells; ++c) max_cell = std::max<int64_t>(max_cell, S.L.cell_off[c + 1] - S.L.cell_off[c]);
   struct Cand { int64_t budget; int threads; int layout; };
@@ -1100,8 +1180,9 @@ int batched_setup(State& S, int32_t cells_per_group) {
     cands.push_back({std::max<int64_t>(1, (int64_t)cells_per_group * max_cell), 256, 0});
   } else {
     for (int threads : {256, 512}) {
-      for (int64_t bud : {768, 1024, 1400, 1700, 2200, 2750}) cands.push_back({bud, threads, 0});
-      for (int64_t bud : {1200, 1800, 2400, 3600, 4800, 7200}) cands.push_back({bud, threads, 4});
+      for (int64_t bud : {1024, 1700, 2750}) cands.push_back({bud, threads, 0});
+      for (int64_t bud : {1800, 2400, 3600, 4800, 6000}) cands.push_back({bud, threads, 5});
+      cands.push_back({7200, threads, 4});
     }
   }
   int rc;
@@ -1180,8 +1261,10 @@ int batched_plan_info(const State& S, int64_t* budget, int32_t* threads, int32_t
 int batched_replan(State& S, int64_t budget, int32_t threads, int32_t layout) {
   if (!S.batch) return fail(MIND_EARG, "state has no batched plan");
   if (budget < 1) return fail(MIND_EARG, "node budget must be positive");
-  if (threads < 64 || threads > kMaxThreads || threads % 64) return fail(MIND_EARG, "threads must be 64, 128, ..., 512");
-  if (layout != 0 && layout != 4) return fail(MIND_EARG, "layout must be 0 (all in LDS) or 4 (lean)");
+  if (threads < 64 || threads > kMaxThreads || threads % 64)
+    return fail(MIND_EARG, "threads must be 64, 128, ..., 512");
+  if (layout != 0 && layout != 4 && layout != 5)
+    return fail(MIND_EARG, "layout must be 0 (all in LDS), 4 (lean) or 5 (lean, v in LDS)");
   HIP_TRY(hipStreamSynchronize(S.stream));
   if (S.graph) {  // the captured window holds the old plan
     HIP_TRY(hipGraphExecDestroy(S.graph));

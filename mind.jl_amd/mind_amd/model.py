@@ -538,7 +538,8 @@ def pack_nodes(model, dt_micro=0.025):
 
 def batch_plan(model):
     """The batched kernel's plan: node budget per workgroup, threads per workgroup,
-    layout of those workgroups (0 all in LDS, 4 lean) and workgroup count."""
+    layout of those workgroups (0 all in LDS, 5 lean with v in LDS, 4 lean with v
+    in HBM) and workgroup count."""
     L = _abi.lib()
     b, t, lay, g = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64()
     _abi.check(L.mind_batch_plan(model.state, C.byref(b), C.byref(t), C.byref(lay), C.byref(g)))

@@ -95,7 +95,13 @@ def test_trees_batched_and_plain(gpu):
     destroy(model)
 
 
-@pytest.mark.parametrize("plan", [(300, 128, 0), (600, 64, 4), (2400, 512, 4)])
+_NO_1024 = pytest.mark.skip(reason="1024-thread workgroups are disabled: a run with them faulted the GPU "
+                                   "during the autotune of the 5000-cell model and the cause is not found")
+
+
+@pytest.mark.parametrize("plan", [(300, 128, 0), (600, 64, 4), (2400, 512, 4), (500, 64, 5), (3000, 256, 5),
+                                  pytest.param((1200, 1024, 0), marks=_NO_1024),
+                                  pytest.param((4000, 1024, 5), marks=_NO_1024)])
 def test_trees_fixed_plans_batched(gpu, plan):
     """Both layouts of the window kernel (all in LDS; lean, with the solve's node
     records streamed from HBM) and block sizes below and above a level's width:
@@ -108,15 +114,17 @@ def test_trees_fixed_plans_batched(gpu, plan):
     destroy(model)
 
 
-@pytest.mark.parametrize("plan", [None, (4000, 256, 4)])
+@pytest.mark.parametrize("plan", [None, (4000, 256, 4), (2000, 512, 5)])
 def test_trees_oversized_cells_batched(gpu, plan):
-    """Cells above the LDS capacity of the all-in-LDS layout run lean, cells above
-    that of the lean layout with everything in HBM; results unchanged."""
+    """Cells above the LDS capacity of the all-in-LDS layout run lean with v in
+    LDS, larger ones lean with v in HBM, the largest with everything in HBM;
+    results unchanged."""
     model, info = configs.tree_network(ncells=4, median=6000, sigma=0.8, lo=2000, hi=11500,
                                        indegree=3, synapses=4, rate_hz=50.0, netstim_weight=3e-2)
     sizes = np.sort(info["sizes"])
-    # one cell for each layout: all in LDS (< 3700 nodes), lean (< 9700), all in HBM
-    assert sizes[0] < 3700 and 3700 < sizes[2] < 9700 < sizes[3]
+    # about two nodes per compartment: one cell lean with v in LDS (< 6300 nodes),
+    # one lean (< 9700), two all in HBM
+    assert 1800 < sizes[0] < 3100 and 3200 < sizes[1] < 4800 and sizes[2] > 4900
     recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"]]
     _, _, vg, vr, _ = _compare(model, 5.0, recs, method="batched", cells_per_group=1, plan=plan)
     assert np.array_equal(vg, vr)
