@@ -256,6 +256,11 @@ int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, 
 /* Replaces that plan (threads a multiple of 64 up to 512, layout 0, 4 or 5);
  * mind_initialize must follow. */
 int mind_set_batch_plan(mind_state* state, int64_t budget, int32_t threads, int32_t layout);
+/* Host only (no device): builds the batched plan of a model for (budget, threads,
+ * layout) and checks every index its kernels derive from the plan's tables
+ * against the bounds of what it indexes. No counterpart in the reference. */
+int mind_check_batch_plan(const mind_model_desc* model, double dt, int64_t budget, int32_t threads,
+                          int32_t layout, int64_t* groups);
 /* ---- Surrogate MLP (no counterpart in the reference) ----------------------------
  * n_in -> hidden (x layers, GELU) -> n_out on BF16 MFMA with FP32 accumulation;
  * FP32 master weights and Adam moments. Flat parameter order: per linear layer

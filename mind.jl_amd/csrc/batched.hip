@@ -770,6 +770,24 @@ void node_levels(const Layout& L, int64_t a0, int64_t a1, std::vector<int32_t>& 
 
 // Groups consecutive cells within a node budget and the LDS capacity, and builds
 // each group's link and schedule block.
+// A batched plan on the host: the groups, their launch lists and LDS sizes, the
+// int16 table blocks and the records of the lean and HBM layouts.
+struct PlanHost {
+  int64_t budget = 0;
+  int threads = 256, layout = 0;
+  std::vector<int32_t> cell0, cell1, glist;
+  std::vector<int16_t> r_tab;
+  std::vector<int32_t> grp_rtab, grp_slen, rbase;
+  std::vector<HRec> hrec;
+  std::vector<DRec> drec;
+  std::vector<ARec> arec;
+  std::vector<int32_t> lists[kLists];
+  size_t lds[kLists] = {};
+  std::vector<int32_t> large;
+  std::vector<int8_t> large_mode;
+  size_t large_lds = 0;
+};
+
 // Assigns a group's cells to the W wavefronts of its workgroup for the tree
 // solve: cells in decreasing order of their own cost, each to the wave whose
 // cost grows least (ties: the lower wave). A wave's cost is its passes of 64
@@ -826,15 +844,16 @@ inline int64_t sched_bound(int W, int64_t sum_levels, int64_t nodes) {
 
 // Groups consecutive cells within a node budget and the LDS capacity, and builds
 // each group's link and schedule block.
-int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& out) {
-  const Layout& L = S.L;
-  BatchPlan* P = new BatchPlan();
-  P->budget = budget;
-  P->threads = threads;
-  P->layout = layout;
+int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost& H) {
+  H = PlanHost();
+  H.budget = budget;
+  H.threads = threads;
+  H.layout = layout;
   const int W0 = threads / 64;                             // waves of a list-0 workgroup
   const int Wl = std::min(threads, kLargeThreads) / 64;    // of the others
-  std::vector<int32_t> cell0, cell1, glist;
+  std::vector<int32_t>& cell0 = H.cell0;
+  std::vector<int32_t>& cell1 = H.cell1;
+  std::vector<int32_t>& glist = H.glist;
   std::vector<int32_t> hh, dd;
   int64_t cur_nodes = 0, cur_levels = 0;
   int32_t start = 0;
@@ -858,10 +877,7 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
     }
     if (c == L.ncells) break;
     if (!alone_fits) {  // a cell beyond the LDS capacity: its own workgroup, part in HBM
-      if (nc > 32767) {
-        free_plan(P);
-        return fail(MIND_EUNSUPPORTED, "a cell has more than 32767 nodes");
-      }
+      if (nc > 32765) return fail(MIND_EUNSUPPORTED, "a cell has more than 32765 nodes");
       // the lean layout if it fits (v in LDS if that fits too), else everything in HBM
       const int64_t sb = sched_bound(Wl, lh + ld, nc);
       cell0.push_back(c);
@@ -874,9 +890,11 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
     cur_levels += lh + ld;
   }
   const int ng = (int)cell0.size();
-  P->ngroups = ng;
-  std::vector<int16_t> r_tab;
-  std::vector<int32_t> grp_rtab(ng), grp_slen(ng);
+  std::vector<int16_t>& r_tab = H.r_tab;
+  std::vector<int32_t>& grp_rtab = H.grp_rtab;
+  std::vector<int32_t>& grp_slen = H.grp_slen;
+  grp_rtab.assign(ng, 0);
+  grp_slen.assign(ng, 0);
   bool any_lean = false, any_hbm_links = false;
   for (int g = 0; g < ng; ++g) {
     const int64_t n0 = L.cell_off[cell0[g]], n1 = L.cell_off[cell1[g]];
@@ -916,16 +934,13 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
     };
     int64_t slen = 4 * (W + 1);
     for (int w = 0; w < W; ++w) slen += npasses(wnh[w]) + 1 + npasses(wnd[w]) + 1;
-    if (slen > (1 << 30)) {
-      free_plan(P);
-      return fail(MIND_EUNSUPPORTED, "schedule block too long");
-    }
+    if (slen > (1 << 30)) return fail(MIND_EUNSUPPORTED, "schedule block too long");
     const int list = glist[g];
-    P->lists[list].push_back(g);
+    H.lists[list].push_back(g);
     const int mode = list_mode(list, layout);
     if (mode == 4 || mode == 5) any_lean = true;
     if (mode != 0) any_hbm_links = true;
-    P->lds[list] = std::max(P->lds[list], window_bytes(mode, nn, slen));
+    H.lds[list] = std::max(H.lds[list], window_bytes(mode, nn, slen));
     r_tab.resize((r_tab.size() + 3) / 4 * 4, 0);  // 8-byte aligned blocks
     grp_rtab[g] = (int32_t)r_tab.size();
     grp_slen[g] = (int32_t)slen;
@@ -996,15 +1011,14 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
       }
       return pos == nn;
     };
-    if (!valid(hh, t_oh, 0) || !valid(dd, t_od, W + 1)) {
-      free_plan(P);
+    if (!valid(hh, t_oh, 0) || !valid(dd, t_od, W + 1))
       return fail(MIND_EUNSUPPORTED, "batched plan: inconsistent solve schedule");
-    }
   }
   {
-    std::vector<HRec> hrec;
-    std::vector<DRec> drec;
-    std::vector<int32_t> rbase(ng);
+    std::vector<HRec>& hrec = H.hrec;
+    std::vector<DRec>& drec = H.drec;
+    std::vector<int32_t>& rbase = H.rbase;
+    rbase.assign(ng, 0);
     if (any_lean) {  // records of the lean layout, from the int16 block; group g's from n0 + g
       hrec.resize(L.nodes + ng);
       drec.resize(L.nodes + ng);
@@ -1031,7 +1045,7 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
         drec[rb + nn] = DRec{sink, dummy, 0, 0, -0.0};
       }
     }
-    std::vector<ARec> arec;
+    std::vector<ARec>& arec = H.arec;
     if (any_hbm_links) {  // assembly records, from the int16 block
       arec.resize(L.nodes);
       for (int g = 0; g < ng; ++g) {
@@ -1045,53 +1059,141 @@ int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& ou
         }
       }
     }
-    HRec* ph = nullptr;
-    DRec* pd = nullptr;
-    ARec* pa = nullptr;
-    int32_t* pr = nullptr;
-    if ((rc = plan_upload(S, *P, &ph, hrec)) || (rc = plan_upload(S, *P, &pd, drec)) ||
-        (rc = plan_upload(S, *P, &pr, rbase)) || (rc = plan_upload(S, *P, &pa, arec))) {
-      free_plan(P);
-      return rc;
-    }
-    P->b.r_arec = pa;
-    P->b.r_hrec = ph;
-    P->b.r_drec = pd;
-    P->b.grp_rbase = pr;
   }
+  for (int m = 1; m < kLists; ++m) {  // largest layouts first: the longest workgroups start early
+    const int list = kLists - m;
+    for (int32_t g : H.lists[list]) {
+      H.large.push_back(g);
+      H.large_mode.push_back((int8_t)list_mode(list, layout));
+    }
+    H.large_lds = std::max(H.large_lds, H.lds[list]);
+  }
+  for (int m = 0; m < kLists; ++m)
+    if (H.lds[m] > kLdsResident) return fail(MIND_EUNSUPPORTED, "batched plan exceeds the LDS capacity");
+  return 0;
+}
+
+// Checks on the host every index the window kernels derive from a plan's tables
+// (node links and orders, schedule headers and pass offsets, the records of the
+// lean and HBM layouts, launch lists and LDS sizes) against the bounds of what it
+// indexes; positions the kernels compute are clamped to these ranges or chosen
+// between two of them. Empty string if the plan is consistent.
+std::string check_plan(const Layout& L, const PlanHost& H) {
+  const int ng = (int)H.cell0.size();
+  const int W0 = H.threads / 64, Wl = std::min(H.threads, kLargeThreads) / 64;
+  auto bad = [](int g, const char* what) { return "group " + std::to_string(g) + ": " + what; };
+  if ((int)H.glist.size() != ng || (int)H.cell1.size() != ng || (int)H.grp_rtab.size() != ng ||
+      (int)H.grp_slen.size() != ng || (int)H.rbase.size() != ng)
+    return "per-group arrays differ in length";
+  std::vector<int> in_list(ng, -1);
+  for (int m = 0; m < kLists; ++m)
+    for (int32_t g : H.lists[m]) {
+      if (g < 0 || g >= ng || in_list[g] >= 0) return "launch list entry out of range or repeated";
+      in_list[g] = m;
+    }
+  int32_t next_cell = 0;
+  for (int g = 0; g < ng; ++g) {
+    if (in_list[g] != H.glist[g]) return bad(g, "not in its launch list");
+    if (H.cell0[g] != next_cell || H.cell1[g] <= H.cell0[g] || H.cell1[g] > L.ncells) return bad(g, "cell range");
+    next_cell = H.cell1[g];
+    const int list = H.glist[g], mode = list_mode(list, H.layout);
+    const int W = list == 0 ? W0 : Wl;
+    const int64_t n0 = L.cell_off[H.cell0[g]], nn = L.cell_off[H.cell1[g]] - n0;
+    if (nn <= 0 || nn > 32765) return bad(g, "node count");
+    const int64_t slen = H.grp_slen[g], base = H.grp_rtab[g];
+    if (base < 0 || base % 4 || base + rtab_len(nn, slen) > (int64_t)H.r_tab.size()) return bad(g, "table block");
+    if (window_bytes(mode, nn, slen) > H.lds[list] || H.lds[list] > kLdsResident) return bad(g, "LDS size");
+    const int16_t* t = H.r_tab.data() + base;
+    const int16_t *hn = t, *oh = t + 4 * nn, *od = t + 5 * nn, *sc = t + 6 * nn;
+    for (int64_t li = 0; li < nn; ++li) {
+      const int h = hn[4 * li], l = hn[4 * li + 1], p = hn[4 * li + 2];
+      if (h < -2 || h >= nn || l < -1 || l >= nn || p < -1 || p >= nn) return bad(g, "node links");
+      if (oh[li] < 0 || oh[li] >= nn || od[li] < 0 || od[li] >= nn) return bad(g, "node orders");
+    }
+    if (4 * (W + 1) > slen) return bad(g, "schedule header");
+    for (int sweep = 0; sweep < 2; ++sweep)
+      for (int w = 0; w < W; ++w) {
+        const int64_t a = sched_at(sc, sweep * (W + 1) + w), z = sched_at(sc, sweep * (W + 1) + w + 1);
+        if (a < 4 * (W + 1) || z <= a || z > slen) return bad(g, "schedule offsets block");
+        for (int64_t q = a; q < z; ++q) {
+          if (sc[q] < 0 || sc[q] > nn) return bad(g, "pass offset");
+          if (q + 1 < z && (sc[q + 1] < sc[q] || sc[q + 1] - sc[q] > 64)) return bad(g, "pass size");
+        }
+      }
+    if (mode == 4 || mode == 5) {
+      const int64_t rb = H.rbase[g];
+      if (rb < 0 || rb + nn >= (int64_t)H.hrec.size() || rb + nn >= (int64_t)H.drec.size()) return bad(g, "records");
+      for (int64_t i = 0; i <= nn; ++i) {
+        const HRec& r = H.hrec[rb + i];
+        const DRec& q = H.drec[rb + i];
+        if (r.li < 0 || r.li > nn + 1 || r.h < 0 || r.h > nn + 1 || r.l < 0 || r.l > nn + 1 || r.csr < 0 ||
+            r.csr > 1 || q.li < 0 || q.li > nn + 1 || q.p < 0 || q.p > nn + 1)
+          return bad(g, "lean record");
+        if (i < nn && (r.li >= nn || q.li >= nn)) return bad(g, "lean record node");
+        if (i == nn && (r.li != nn + 1 || r.h != nn || r.l != nn || r.csr || q.li != nn + 1 || q.p != nn))
+          return bad(g, "sink record");
+      }
+    }
+    if (mode != 0) {
+      if (n0 + nn > (int64_t)H.arec.size()) return bad(g, "assembly records");
+      for (int64_t li = 0; li < nn; ++li) {
+        const ARec& r = H.arec[n0 + li];
+        if (r.h < -2 || r.h >= nn || r.l < -1 || r.l >= nn || r.p < -1 || r.p >= nn) return bad(g, "assembly record");
+      }
+    }
+  }
+  if (next_cell != L.ncells) return "groups do not cover the cells";
+  for (size_t k = 0; k < H.large.size(); ++k) {
+    const int g = H.large[k];
+    if (g < 0 || g >= ng || H.glist[g] == 0 || H.large_mode[k] != list_mode(H.glist[g], H.layout))
+      return "large launch entry";
+    if (H.lds[H.glist[g]] > H.large_lds) return "large launch LDS size";
+  }
+  return "";
+}
+
+// Builds a plan on the host and uploads it.
+int build_plan(State& S, int64_t budget, int threads, int layout, BatchPlan*& out) {
+  PlanHost H;
+  int rc = plan_host(S.L, budget, threads, layout, H);
+  if (rc) return rc;
+  BatchPlan* P = new BatchPlan();
+  P->budget = budget;
+  P->threads = threads;
+  P->layout = layout;
+  P->ngroups = (int64_t)H.cell0.size();
+  for (int m = 0; m < kLists; ++m) {
+    P->lists[m] = H.lists[m];
+    P->lds[m] = H.lds[m];
+  }
+  P->large = H.large;
+  P->large_mode = H.large_mode;
+  P->large_lds = H.large_lds;
   BatchDev& b = P->b;
   int32_t* p = nullptr;
   int16_t* p16 = nullptr;
-  if ((rc = plan_upload(S, *P, &p, cell0))) goto error; b.grp_cell0 = p;
-  if ((rc = plan_upload(S, *P, &p, cell1))) goto error; b.grp_cell1 = p;
-  if ((rc = plan_upload(S, *P, &p16, r_tab))) goto error; b.r_tab = p16;
-  if ((rc = plan_upload(S, *P, &p, grp_rtab))) goto error; b.grp_rtab = p;
-  if ((rc = plan_upload(S, *P, &p, grp_slen))) goto error; b.grp_slen = p;
+  HRec* ph = nullptr;
+  DRec* pd = nullptr;
+  ARec* pa = nullptr;
+  int8_t* p8 = nullptr;
+  if ((rc = plan_upload(S, *P, &p, H.cell0))) goto error; b.grp_cell0 = p;
+  if ((rc = plan_upload(S, *P, &p, H.cell1))) goto error; b.grp_cell1 = p;
+  if ((rc = plan_upload(S, *P, &p16, H.r_tab))) goto error; b.r_tab = p16;
+  if ((rc = plan_upload(S, *P, &p, H.grp_rtab))) goto error; b.grp_rtab = p;
+  if ((rc = plan_upload(S, *P, &p, H.grp_slen))) goto error; b.grp_slen = p;
+  if ((rc = plan_upload(S, *P, &ph, H.hrec))) goto error; b.r_hrec = ph;
+  if ((rc = plan_upload(S, *P, &pd, H.drec))) goto error; b.r_drec = pd;
+  if ((rc = plan_upload(S, *P, &p, H.rbase))) goto error; b.grp_rbase = p;
+  if ((rc = plan_upload(S, *P, &pa, H.arec))) goto error; b.r_arec = pa;
   for (int m = 0; m < kLists; ++m)
     if ((rc = plan_upload(S, *P, &P->dev_lists[m], P->lists[m]))) goto error;
-  for (int m = 1; m < kLists; ++m) {  // largest layouts first: the longest workgroups start early
-    const int list = kLists - m;
-    for (int32_t g : P->lists[list]) {
-      P->large.push_back(g);
-      P->large_mode.push_back((int8_t)list_mode(list, layout));
-    }
-    P->large_lds = std::max(P->large_lds, P->lds[list]);
-  }
   if ((rc = plan_upload(S, *P, &P->dev_large, P->large))) goto error;
-  {
-    int8_t* p8 = nullptr;
-    if ((rc = plan_upload(S, *P, &p8, P->large_mode))) goto error;
-    P->dev_large_mode = p8;
-  }
-  for (int m = 0; m < kLists; ++m)
-    if (P->lds[m] > kLdsResident) {
-      rc = fail(MIND_EUNSUPPORTED, "batched plan exceeds the LDS capacity");
-      goto error;
-    }
+  if ((rc = plan_upload(S, *P, &p8, P->large_mode))) goto error;
+  P->dev_large_mode = p8;
   // events are bucketed per group
-  S.cell_bucket.assign(L.ncells, 0);
-  for (int g = 0; g < ng; ++g)
-    for (int32_t c = cell0[g]; c < cell1[g]; ++c) S.cell_bucket[c] = g;
+  S.cell_bucket.assign(S.L.ncells, 0);
+  for (size_t g = 0; g < H.cell0.size(); ++g)
+    for (int32_t c = H.cell0[g]; c < H.cell1[g]; ++c) S.cell_bucket[c] = (int32_t)g;
   out = P;
   return 0;
 error:
@@ -1289,4 +1391,24 @@ extern "C" int mind_set_batch_plan(mind_state* state, int64_t budget, int32_t th
 
 extern "C" int mind_batch_work(const mind_state* state, int64_t* counts) {
   return mind::batched_dominant_work(*reinterpret_cast<const mind::State*>(state), counts);
+}
+
+// Host-only: builds the batched plan of a model as the autotuner would for
+// (budget, threads, layout) and checks every index its kernels would derive from
+// it. No device is used.
+extern "C" int mind_check_batch_plan(const mind_model_desc* model, double dt, int64_t budget, int32_t threads,
+                                     int32_t layout, int64_t* groups) {
+  if (!model || budget < 1 || threads < 64 || threads > mind::kMaxThreads || threads % 64 ||
+      (layout != 0 && layout != 4 && layout != 5))
+    return mind::fail(MIND_EARG, "invalid plan");
+  mind::Layout L;
+  const std::string err = mind::pack(*model, dt, 1, L);
+  if (!err.empty()) return mind::fail(MIND_EARG, err);
+  mind::PlanHost H;
+  int rc = mind::plan_host(L, budget, threads, layout, H);
+  if (rc) return rc;
+  const std::string bad = mind::check_plan(L, H);
+  if (!bad.empty()) return mind::fail(MIND_EUNSUPPORTED, "batched plan check: " + bad);
+  if (groups) *groups = (int64_t)H.cell0.size();
+  return 0;
 }

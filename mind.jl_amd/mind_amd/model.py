@@ -536,6 +536,19 @@ def pack_nodes(model, dt_micro=0.025):
                 axial=axial, adiag=adiag)
 
 
+def check_batch_plan(model, budget, threads, layout, dt_micro=0.025):
+    """Builds the batched plan of a model for (budget, threads, layout) on the host
+    and checks every index its kernels derive from it (no device needed); returns
+    the workgroup count, raises MindError on an inconsistent plan."""
+    L = _abi.lib()
+    desc, keep = model.descriptor()
+    g = C.c_int64()
+    _abi.check(L.mind_check_batch_plan(C.byref(desc), float(dt_micro), int(budget), int(threads), int(layout),
+                                       C.byref(g)))
+    del keep
+    return g.value
+
+
 def batch_plan(model):
     """The batched kernel's plan: node budget per workgroup, threads per workgroup,
     layout of those workgroups (0 all in LDS, 5 lean with v in LDS, 4 lean with v
