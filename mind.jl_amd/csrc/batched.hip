@@ -54,11 +54,11 @@ struct DRec {
 
 // Cable assembly record of a node (modes 3-5, whose links and conductances are
 // not in LDS): higher and lower child, parent (-1 if none, higher -2 with more
-// than two children), and the axial conductances of the node and of those two
-// children. Indexed by global node.
+// than two children), the axial conductances of the node and of those two
+// children, and the node's C/dt and axial diagonal. Indexed by global node.
 struct ARec {
   int16_t h, l, p, pad;
-  double g, gh, gl;
+  double g, gh, gl, cdt, adiag;
 };
 
 struct BatchDev {
@@ -469,6 +469,48 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
   }
 }
 
+// nrn_cur of a density mechanism's rows [r0, r1) of one workgroup: U rows per
+// thread per round, the column reads of all U issued before any row's evaluation
+// and stores (each row is its own node, so the rows of a round are independent).
+// Same operations per row as the generic loop.
+template <int MECH, int U, typename C>
+__device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, const int32_t* node_of, int64_t r0,
+                                                 int64_t r1, int64_t n0, double tc, const double* v, C rhs, C diag,
+                                                 int tid, int T) {
+  using Ops = CurOps<MECH>;
+  for (int64_t r = r0 + tid; r < r1; r += (int64_t)U * T) {
+    int64_t rr[U];
+    int32_t nd[U];
+    typename Ops::In in[U];
+    double area[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = r + (int64_t)u * T;
+      rr[u] = q < r1 ? q : r;
+      nd[u] = node_of[rr[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      in[u] = Ops::load(col, rr[u]);
+      area[u] = d.area[nd[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r + (int64_t)u * T < r1) {
+        const int li = nd[u] - (int)n0;
+        double value, shifted;
+        Ops::eval(col, rr[u], in[u], v[li], tc, value, shifted, true);
+        const double scale = area[u] * 0.01;
+        double acc_r = rhs[li], acc_d = diag[li];
+        acc_r -= scale * value;
+        acc_d += scale * (shifted - value) / 0.001;
+        rhs[li] = acc_r;
+        diag[li] = acc_d;
+      }
+    }
+  }
+}
+
 // Mechanisms with a state update (nrn_state).
 __device__ __forceinline__ bool stateful(int32_t mech) {
   return mech == MIND_MECH_HH || mech == MIND_MECH_EXP2SYN || mech == MIND_MECH_EXPSYN;
@@ -588,6 +630,16 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       const int64_t r0 = sr0[gi], r1 = sr1[gi];
       if (r0 == r1) continue;
       const Cols col = cols(G);
+      if (G.mech == MIND_MECH_PAS && !G.point) {
+        density_currents<MIND_MECH_PAS, 4>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T);
+        __syncthreads();
+        continue;
+      }
+      if (G.mech == MIND_MECH_HH && !G.point) {
+        density_currents<MIND_MECH_HH, 1>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T);
+        __syncthreads();
+        continue;
+      }
       for (int64_t r = r0 + tid; r < r1; r += T) {
         if (G.point && !G.run_start[r]) continue;
         const int32_t node = G.node[r];
@@ -632,14 +684,17 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         assemble(li, k.p, k.l, k.h, ax[li], k.l >= 0 ? ax[k.l] : 0.0, k.h >= 0 ? ax[k.h] : 0.0, d.cdt[n0 + li],
                  d.adiag[n0 + li]);
       }
-    } else {  // two nodes per thread per round, their records read first
+    } else {  // four nodes per thread per round, their records read first
+      constexpr int U = 4;
       const ARec* ar = b.r_arec + n0;
-      for (int l0 = tid; l0 < nn; l0 += 2 * T) {
-        const int l1 = l0 + T < nn ? l0 + T : l0;
-        const ARec r0 = ar[l0], r1 = ar[l1];
-        const double c0 = d.cdt[n0 + l0], a0 = d.adiag[n0 + l0], c1 = d.cdt[n0 + l1], a1 = d.adiag[n0 + l1];
-        assemble(l0, r0.p, r0.l, r0.h, r0.g, r0.gl, r0.gh, c0, a0);
-        if (l0 + T < nn) assemble(l1, r1.p, r1.l, r1.h, r1.g, r1.gl, r1.gh, c1, a1);
+      for (int l0 = tid; l0 < nn; l0 += U * T) {
+        ARec r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u] = ar[l0 + u * T < nn ? l0 + u * T : l0];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (l0 + u * T < nn)
+            assemble(l0 + u * T, r[u].p, r[u].l, r[u].h, r[u].g, r[u].gl, r[u].gh, r[u].cdt, r[u].adiag);
       }
     }
     __syncthreads();
@@ -1055,7 +1110,8 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
         for (int li = 0; li < nn; ++li) {
           const int h = t_hn[4 * li], l = t_hn[4 * li + 1], p = t_hn[4 * li + 2];
           arec[n0 + li] = ARec{(int16_t)h, (int16_t)l, (int16_t)p, 0, L.axial[n0 + li],
-                               h >= 0 ? L.axial[n0 + h] : 0.0, l >= 0 ? L.axial[n0 + l] : 0.0};
+                               h >= 0 ? L.axial[n0 + h] : 0.0, l >= 0 ? L.axial[n0 + l] : 0.0,
+                               L.cap[n0 + li] / L.dt, L.adiag[n0 + li]};  // C/dt as in the cdt column
         }
       }
     }

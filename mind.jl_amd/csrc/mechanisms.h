@@ -20,6 +20,10 @@ MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double 
                            bool store) {
   gen::current(mech, c, r, v, t, value, shifted, store);
 }
+// nrn_cur of mechanism slot MECH in two parts: the column reads of a row (load),
+// then the evaluation and stores (eval).
+template <int MECH>
+using CurOps = gen::CurOf<MECH>;
 MIND_HD void phase_advance(int mech, const Cols& c, int64_t r, double v, double dt, double celsius) {
   gen::advance(mech, c, r, v, dt, celsius);
 }
@@ -43,6 +47,17 @@ MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double 
                            bool store) {
   mech_current(mech, c, r, v, t, value, shifted, store);
 }
+// nrn_cur of mechanism slot MECH in two parts; the hand lowering reads its
+// columns in the evaluation.
+template <int MECH>
+struct CurOps {
+  struct In {};
+  static MIND_HD In load(const Cols&, int64_t) { return In{}; }
+  static MIND_HD void eval(const Cols& c, int64_t r, const In&, double v, double t, double& value, double& shifted,
+                           bool store) {
+    mech_current(MECH, c, r, v, t, value, shifted, store);
+  }
+};
 MIND_HD void phase_advance(int mech, const Cols& c, int64_t r, double v, double dt, double celsius) {
   mech_advance(mech, c, r, v, dt, celsius);
 }

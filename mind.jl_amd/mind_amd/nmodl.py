@@ -682,9 +682,13 @@ class Gen:
         return self.m.blocks.get(("breakpoint", "breakpoint"))
 
     def current(self):
+        """nrn_cur in two parts, so that a caller can issue the column reads of
+        several rows before any row's stores: <M>_current_load reads every column
+        the evaluation reads (parameters, states, and the initial values of the
+        assigned variables it writes) into <M>_cur_in; <M>_current_eval evaluates
+        BREAKPOINT at v + 0.001 then v from those values and stores the assigned
+        variables. <M>_current is the two in sequence."""
         p, m = self.p, self.m
-        out = [f"MIND_HD void {p}_current(const Cols& c, int64_t r, double v, double t, double& value, "
-               "double& shifted, bool store) {"]
         bp = self.breakpoint()
         body = [s for s in (bp.body if bp else []) if not isinstance(s, Solve)]
         written = []
@@ -693,30 +697,62 @@ class Gen:
         for n in currents + list(m.electrode):
             if n not in self.findex:
                 self.err(f"current {n} is not declared")
-        out.append("  for (int probe = 0; probe < 2; ++probe) {")
-        out.append("    GenCtx x = gen_ctx(probe == 0 ? v + 0.001 : v, t);")
-        out.append("    (void)x;")
+        lines = []
+        lines.append("  for (int probe = 0; probe < 2; ++probe) {")
+        lines.append("    GenCtx x = gen_ctx(probe == 0 ? v + 0.001 : v, t);")
+        lines.append("    (void)x;")
         scope = self.base_scope()
         for n in written:  # assigned values of this evaluation, stored after the probe at v
-            out.append(f"    double w_{n} = c({p}_{n}, r);")
+            lines.append(f"    double w_{n} = c({p}_{n}, r);")
             scope[n] = f"w_{n}"
-        self.stmts(body, scope, out, "    ")
-        out.append("    double sum = 0.0;")
+        self.stmts(body, scope, lines, "    ")
+        lines.append("    double sum = 0.0;")
         for n in currents:
-            out.append(f"    sum = (sum + {scope[n]});")
+            lines.append(f"    sum = (sum + {scope[n]});")
         for n in m.electrode:
-            out.append(f"    sum = (sum - {scope[n]});")
-        out.append("    if (probe == 0) {")
-        out.append("      shifted = sum;")
-        out.append("    } else {")
-        out.append("      value = sum;")
+            lines.append(f"    sum = (sum - {scope[n]});")
+        lines.append("    if (probe == 0) {")
+        lines.append("      shifted = sum;")
+        lines.append("    } else {")
+        lines.append("      value = sum;")
+        stores = []
         if written:
-            out.append("      if (store) {")
+            lines.append("      if (store) {")
             for n in written:
-                out.append(f"        c({p}_{n}, r) = w_{n};")
-            out.append("      }")
-        out.append("    }")
-        out.append("  }")
+                stores.append(f"        c({p}_{n}, r) = w_{n};")
+            lines.extend(stores)
+            lines.append("      }")
+        lines.append("    }")
+        lines.append("  }")
+        # every column read of the evaluation (outside the store lines), in field order
+        text = "\n".join(l for l in lines if l not in stores)
+        reads = [n for n in self.fields if f"c({p}_{n}, r)" in text]
+        out = [f"struct {p}_cur_in {{"]
+        out.extend(f"  double {n};" for n in reads)
+        if not reads:
+            out.append("  double unused;")
+        out.append("};")
+        out.append(f"MIND_HD {p}_cur_in {p}_current_load(const Cols& c, int64_t r) {{")
+        out.append(f"  {p}_cur_in in;")
+        out.extend(f"  in.{n} = c({p}_{n}, r);" for n in reads)
+        if not reads:
+            out.append("  in.unused = 0.0;")
+            out.append("  (void)c;")
+            out.append("  (void)r;")
+        out.append("  return in;")
+        out.append("}")
+        out.append(f"MIND_HD void {p}_current_eval(const Cols& c, int64_t r, const {p}_cur_in& in, double v, "
+                   "double t, double& value, double& shifted, bool store) {")
+        out.append("  (void)in;")
+        for l in lines:
+            if l not in stores:
+                for n in reads:
+                    l = l.replace(f"c({p}_{n}, r)", f"in.{n}")
+            out.append(l)
+        out.append("}")
+        out.append(f"MIND_HD void {p}_current(const Cols& c, int64_t r, double v, double t, double& value, "
+                   "double& shifted, bool store) {")
+        out.append(f"  {p}_current_eval(c, r, {p}_current_load(c, r), v, t, value, shifted, store);")
         out.append("}")
         return out
 
@@ -874,6 +910,14 @@ def dispatch(mechs):
                "double& shifted, bool store) {\n  switch (mech) {\n"
                + cases(lambda n: f"{n}_current(c, r, v, t, value, shifted, store)")
                + "\n    default: value = shifted = 0.0;\n  }\n}")
+    # the two parts of nrn_cur per mechanism slot, for callers specialised on the mechanism
+    out.append("template <int MECH>\nstruct CurOf;")
+    for n in BUILTIN_IDS:
+        out.append(f"template <>\nstruct CurOf<{BUILTIN_IDS[n]}> {{\n  using In = {n}_cur_in;\n"
+                   f"  static MIND_HD In load(const Cols& c, int64_t r) {{ return {n}_current_load(c, r); }}\n"
+                   f"  static MIND_HD void eval(const Cols& c, int64_t r, const In& in, double v, double t, "
+                   f"double& value, double& shifted, bool store) {{\n"
+                   f"    {n}_current_eval(c, r, in, v, t, value, shifted, store);\n  }}\n}};")
     out.append("MIND_HD void advance(int mech, const Cols& c, int64_t r, double v, double dt, double celsius) {"
                "\n  switch (mech) {\n" + cases(lambda n: f"{n}_advance(c, r, v, dt, celsius)")
                + "\n    default: break;\n  }\n}")

@@ -255,9 +255,9 @@ def test_equivalence_check_sees_a_reassociated_product(tmp_path):
     product (same value in exact arithmetic) is reported."""
     paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in SIX]
     hdr = nmodl.compile_files(paths)
-    exact = "((((c(hh_gnabar, r) * c(hh_m, r)) * c(hh_m, r)) * c(hh_m, r)) * c(hh_h, r))"
+    exact = "((((in.gnabar * in.m) * in.m) * in.m) * in.h)"  # hh_current_eval, from the loaded columns
     assert exact in hdr
-    hdr = hdr.replace(exact, "(c(hh_gnabar, r) * ((c(hh_m, r) * c(hh_m, r)) * (c(hh_m, r) * c(hh_h, r))))")
+    hdr = hdr.replace(exact, "(in.gnabar * ((in.m * in.m) * (in.m * in.h)))")
     out = gxx_run(tmp_path, hdr, HARNESS, "planted")
     count = int(out.strip().split()[-1])
     assert count > 0 and "current" in out
