@@ -407,6 +407,9 @@ __device__ __forceinline__ bool bad_pivot(double dd) {
 // the wave's last only move the sink. The pass offsets are read one pass ahead
 // (a broadcast LDS read alongside the pass's own). Back substitution leaves the
 // node's change x in rhs; v += x follows for all nodes after the solve.
+// Passes of record loads in flight in the lean solve.
+constexpr int kAhead = 3;
+
 template <typename C>
 __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int grp, int64_t n0, int nn,
                                            const int16_t* sched, C rhs, C diag) {
@@ -419,11 +422,16 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
     const HRec* hr = b.r_hrec + rb;
     const int16_t* lv = sched + h0;
     auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
-    const int o0 = lv[0], o1 = lv[min(1, nph)], o2 = lv[min(2, nph)], o3 = lv[min(3, nph)];
-    int next = o3;  // start of the pass three ahead
-    HRec x0 = hr[slot(o0, o1)], x1 = hr[slot(o1, o2)], x2 = hr[slot(o2, o3)];
+    HRec x[kAhead];
+    int next = lv[0];  // start of the next pass to load
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int e = lv[min(k + 1, nph)];
+      x[k] = hr[slot(next, e)];
+      next = e;
+    }
     auto pass = [&](HRec& r, int q) {
-      const int e = lv[min(q + 4, nph)];  // end of the pass three ahead
+      const int e = lv[min(q + kAhead + 1, nph)];  // end of the pass kAhead ahead
       const double od = diag[r.li], orr = rhs[r.li];
       const double dh = diag[r.h], rh = rhs[r.h], dl = diag[r.l], rl = rhs[r.l];
       reads_issued();
@@ -438,10 +446,9 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
       r = hr[slot(next, e)];
       next = e;
     };
-    for (int q = 0; q < nph; q += 3) {
-      pass(x0, q);
-      pass(x1, q + 1);
-      pass(x2, q + 2);
+    for (int q = 0; q < nph; q += kAhead) {
+#pragma unroll
+      for (int k = 0; k < kAhead; ++k) pass(x[k], q + k);
     }
   }
   if (bad) raise_error(d.error, MIND_ESINGULAR);
@@ -449,11 +456,16 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
     const DRec* dr = b.r_drec + rb;
     const int16_t* lv = sched + d0;
     auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
-    const int o0 = lv[0], o1 = lv[min(1, npd)], o2 = lv[min(2, npd)], o3 = lv[min(3, npd)];
-    int next = o3;
-    DRec x0 = dr[slot(o0, o1)], x1 = dr[slot(o1, o2)], x2 = dr[slot(o2, o3)];
+    DRec x[kAhead];
+    int next = lv[0];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int e = lv[min(k + 1, npd)];
+      x[k] = dr[slot(next, e)];
+      next = e;
+    }
     auto pass = [&](DRec& r, int q) {
-      const int e = lv[min(q + 4, npd)];
+      const int e = lv[min(q + kAhead + 1, npd)];
       const double od = diag[r.li], orr = rhs[r.li], rp = rhs[r.p];
       reads_issued();
       rhs[r.li] = (orr + r.g * rp) / od;
@@ -461,10 +473,9 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
       r = dr[slot(next, e)];
       next = e;
     };
-    for (int q = 0; q < npd; q += 3) {
-      pass(x0, q);
-      pass(x1, q + 1);
-      pass(x2, q + 2);
+    for (int q = 0; q < npd; q += kAhead) {
+#pragma unroll
+      for (int k = 0; k < kAhead; ++k) pass(x[k], q + k);
     }
   }
 }
