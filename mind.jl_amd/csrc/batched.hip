@@ -1,26 +1,26 @@
-// Batched window kernel: a workgroup advances a group of consecutive cells
+// Batched window kernels: a workgroup advances a group of consecutive cells
 // through a whole window (W steps of the micro step) with its node state staged
 // in LDS as far as it fits.
 //
 // The Hines solve of the reference runs one warp per group of cells with
 // 32-lane "DHS" rounds and FP64 atomics into parents (src/runtime/cuda/micro.jl:688-749,
-// plan src/model/micro/model.jl:906-1077). Here a workgroup solves its cells
-// level by level: elimination visits nodes by height (a node's children are all
-// lower), each node *pulling* its children's contributions in decreasing child
-// index, which reproduces the CPU restatement's elimination order exactly and
-// needs no atomics; back substitution visits nodes by depth. Each cell of a
-// group belongs to one wavefront of the workgroup for the solve, so a level
-// needs no workgroup barrier: the lanes of a wave run in lockstep and LDS
-// executes a wave's accesses in order, and the waves' independent level chains
-// hide each other's latency. The host assigns cells to waves (balancing passes
-// of 64 lanes) and builds the int16 parent / child links and each wave's two
-// node orders per group. Two layouts of a workgroup's node state:
-// all in LDS (mode 0: v, rhs, diag, axial conductances, links and schedule stay
-// there for the whole window, so the solve reads no global memory), or lean
-// (mode 4: only rhs and diag in LDS, 16 bytes per node, and the solve's per-node
-// records streamed from HBM two levels ahead, so that more nodes fit a CU). The
-// autotuner chooses the layout, the node budget per workgroup and the block
-// size. Cells too large for either keep everything in HBM (mode 3).
+// plan src/model/micro/model.jl:906-1077). Here each cell of a group belongs to
+// one wavefront, which solves its cells pass by pass, a pass being at most 64
+// nodes of one depth level: elimination in decreasing depth (a node's children
+// are one level deeper), each node *pulling* its children's contributions in
+// decreasing child index, which reproduces the CPU restatement's elimination
+// order exactly and needs no atomics; back substitution in increasing depth. A
+// pass needs no workgroup barrier: the lanes of a wave run in lockstep and LDS
+// executes a wave's accesses in order, and the waves' independent chains hide
+// each other's latency. The host assigns cells to waves (balancing passes of 64
+// lanes) and builds the int16 node links, each wave's two node orders and pass
+// offsets per group. Layouts of a workgroup's node state: all in LDS (mode 0: v,
+// rhs, pivots, axial conductances, links and schedule, so the solve reads no
+// global memory); lean (mode 5: (pivot, rhs) pairs and v in LDS, 24 bytes per
+// node; mode 4: the pairs only, 16 bytes per node), where the solve's per-node
+// records stream from HBM three passes ahead so that more nodes fit a CU; and
+// everything in HBM for cells too large for the others (mode 3). The autotuner
+// chooses the layout, the node budget per workgroup and the block size.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1350,8 +1350,8 @@ int batched_setup(State& S, int32_t cells_per_group) {
   } else {
     for (int threads : {256, 512}) {
       for (int64_t bud : {1024, 1700, 2750}) cands.push_back({bud, threads, 0});
-      for (int64_t bud : {1800, 2400, 3600, 4800, 6000}) cands.push_back({bud, threads, 5});
-      cands.push_back({7200, threads, 4});
+      for (int64_t bud : {1800, 2400, 3000, 3600, 4800, 6000}) cands.push_back({bud, threads, 5});
+      for (int64_t bud : {4500, 7200}) cands.push_back({bud, threads, 4});
     }
   }
   int rc;
