@@ -962,6 +962,7 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
   grp_rtab.assign(ng, 0);
   grp_slen.assign(ng, 0);
   bool any_lean = false, any_hbm_links = false;
+  std::vector<int64_t> group_cost(ng, 0);
   for (int g = 0; g < ng; ++g) {
     const int64_t n0 = L.cell_off[cell0[g]], n1 = L.cell_off[cell1[g]];
     const int nn = (int)(n1 - n0);
@@ -1007,6 +1008,13 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
     if (mode == 4 || mode == 5) any_lean = true;
     if (mode != 0) any_hbm_links = true;
     H.lds[list] = std::max(H.lds[list], window_bytes(mode, nn, slen));
+    {  // estimated cycles of one step: the longest wave's solve passes, and the
+       // node-parallel phases (about 450 cycles per pass and 5400 per node per thread)
+      int64_t longest = 0;
+      for (int w = 0; w < W; ++w) longest = std::max(longest, npasses(wnh[w]) + npasses(wnd[w]));
+      const int Tg = W * 64;
+      group_cost[g] = 450 * longest + 5400 * ((nn + Tg - 1) / Tg);
+    }
     r_tab.resize((r_tab.size() + 3) / 4 * 4, 0);  // 8-byte aligned blocks
     grp_rtab[g] = (int32_t)r_tab.size();
     grp_slen[g] = (int32_t)slen;
@@ -1127,6 +1135,11 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
       }
     }
   }
+  // each launch list costliest group first, so that the last workgroups to start
+  // are short ones
+  for (int m = 0; m < kLists; ++m)
+    std::stable_sort(H.lists[m].begin(), H.lists[m].end(),
+                     [&](int32_t a, int32_t b) { return group_cost[a] > group_cost[b]; });
   for (int m = 1; m < kLists; ++m) {  // largest layouts first: the longest workgroups start early
     const int list = kLists - m;
     for (int32_t g : H.lists[list]) {
