@@ -253,7 +253,7 @@ int mind_window_steps(const mind_state* state, int32_t* steps);
  * the autotuner. */
 int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int32_t* layout,
                     int64_t* groups);
-/* Replaces that plan (threads a multiple of 64 up to 512, layout 0, 4 or 5);
+/* Replaces that plan (threads a multiple of 64 up to 1024, layout 0, 4 or 5);
  * mind_initialize must follow. */
 int mind_set_batch_plan(mind_state* state, int64_t budget, int32_t threads, int32_t layout);
 /* Host only (no device): builds the batched plan of a model for (budget, threads,

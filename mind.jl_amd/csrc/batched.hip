@@ -102,7 +102,7 @@ struct BatchPlan {
   std::vector<void*> allocations;
 };
 
-constexpr int kMaxThreads = 512;
+constexpr int kMaxThreads = 1024;
 constexpr int kLargeThreads = 512;  // block size cap of the launch of cells above the budget
 constexpr int kMaxGroups = 8;    // mechanism groups cached in LDS (the six of the step)
 constexpr int kMaxDetLds = 256;  // detectors of a workgroup cached in LDS
@@ -1366,6 +1366,7 @@ int batched_setup(State& S, int32_t cells_per_group) {
       for (int64_t bud : {1800, 2400, 3000, 3600, 4800, 6000}) cands.push_back({bud, threads, 5});
       for (int64_t bud : {4500, 7200}) cands.push_back({bud, threads, 4});
     }
+    for (int64_t bud : {4800, 6000}) cands.push_back({bud, 1024, 5});
   }
   int rc;
   double best = 1e30;
@@ -1444,7 +1445,7 @@ int batched_replan(State& S, int64_t budget, int32_t threads, int32_t layout) {
   if (!S.batch) return fail(MIND_EARG, "state has no batched plan");
   if (budget < 1) return fail(MIND_EARG, "node budget must be positive");
   if (threads < 64 || threads > kMaxThreads || threads % 64)
-    return fail(MIND_EARG, "threads must be 64, 128, ..., 512");
+    return fail(MIND_EARG, "threads must be 64, 128, ..., 1024");
   if (layout != 0 && layout != 4 && layout != 5)
     return fail(MIND_EARG, "layout must be 0 (all in LDS), 4 (lean) or 5 (lean, v in LDS)");
   HIP_TRY(hipStreamSynchronize(S.stream));

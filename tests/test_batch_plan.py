@@ -11,7 +11,8 @@ from mind_amd.model import check_batch_plan
 CANDIDATES = ([(b, t, 0) for t in (256, 512) for b in (1024, 1700, 2750)]
               + [(b, t, 5) for t in (256, 512) for b in (1800, 2400, 3000, 3600, 4800, 6000)]
               + [(b, t, 4) for t in (256, 512) for b in (4500, 7200)]
-              + [(300, 64, 0), (600, 64, 5), (900, 128, 4)])
+              + [(b, 1024, 5) for b in (4800, 6000)]
+              + [(300, 64, 0), (600, 64, 5), (900, 128, 4), (2750, 1024, 0), (7200, 1024, 4)])
 
 
 @pytest.fixture(scope="module")
@@ -37,6 +38,6 @@ def test_invalid_plans_are_rejected():
     from mind_amd._abi import MindError
 
     model, _ = configs.single_cell()
-    for plan in [(0, 256, 0), (100, 1024, 0), (100, 96, 0), (100, 256, 3)]:
+    for plan in [(0, 256, 0), (100, 2048, 0), (100, 96, 0), (100, 256, 3)]:
         with pytest.raises(MindError):
             check_batch_plan(model, *plan)

@@ -95,13 +95,8 @@ def test_trees_batched_and_plain(gpu):
     destroy(model)
 
 
-_NO_1024 = pytest.mark.skip(reason="1024-thread workgroups are disabled: a run with them faulted the GPU "
-                                   "during the autotune of the 5000-cell model and the cause is not found")
-
-
 @pytest.mark.parametrize("plan", [(300, 128, 0), (600, 64, 4), (2400, 512, 4), (500, 64, 5), (3000, 256, 5),
-                                  pytest.param((1200, 1024, 0), marks=_NO_1024),
-                                  pytest.param((4000, 1024, 5), marks=_NO_1024)])
+                                  (1200, 1024, 0), (4000, 1024, 5)])
 def test_trees_fixed_plans_batched(gpu, plan):
     """Both layouts of the window kernel (all in LDS; lean, with the solve's node
     records streamed from HBM) and block sizes below and above a level's width:
