@@ -89,13 +89,14 @@ def ball_stick_network(ncells=1000, nseg=50, indegree=100, seed=2, rate_hz=10.0,
     return model, dict(cells=cells, syns=syns, stims=stims, tstop=1000.0)
 
 
-def tree_morphology(g: np.random.Generator, compartments: int) -> Morphology:
-    """Soma plus (compartments - 1) one-segment sections by random binary branching."""
+def tree_morphology(g: np.random.Generator, compartments: int, fanout: int = 2) -> Morphology:
+    """Soma plus (compartments - 1) one-segment sections by random branching, each
+    section accepting up to `fanout` children at x = 1 (binary by default)."""
     nd = compartments - 1
     parent = np.empty(nd + 1, np.int32)
     parent[0] = -1
-    # open attachment slots: each section accepts two children at x = 1
-    slots = [0, 0]
+    # open attachment slots: each section accepts `fanout` children at x = 1
+    slots = [0] * fanout
     picks = g.random(nd)
     for i in range(1, nd + 1):
         j = int(picks[i - 1] * len(slots))
@@ -103,8 +104,7 @@ def tree_morphology(g: np.random.Generator, compartments: int) -> Morphology:
         slots[j] = slots[-1]
         slots.pop()
         parent[i] = p
-        slots.append(i)
-        slots.append(i)
+        slots.extend([i] * fanout)
     length = np.concatenate([[20.0], g.uniform(10.0, 200.0, nd)])
     diam = np.concatenate([[20.0], g.uniform(0.5, 4.0, nd)])
     names = ["soma"] + [f"dend[{i}]" for i in range(nd)]
@@ -115,14 +115,14 @@ def tree_morphology(g: np.random.Generator, compartments: int) -> Morphology:
 
 
 def tree_network(ncells=5000, median=400, sigma=0.8, lo=50, hi=4000, synapses=20, indegree=50,
-                 seed=1, rate_hz=5.0, threshold=-20.0, netstim_weight=5e-4):
+                 seed=1, rate_hz=5.0, threshold=-20.0, netstim_weight=5e-4, fanout=2):
     model = MicroModel()
     gm = rng(seed, "morphology")
     sizes = np.clip(np.round(np.exp(gm.normal(np.log(median), sigma, ncells))), lo, hi).astype(int)
     gs = rng(seed, "synapses")
     cells, syns, sources = [], [], []
     for c in range(ncells):
-        morph = tree_morphology(gm, int(sizes[c]))
+        morph = tree_morphology(gm, int(sizes[c]), fanout)
         (cell,) = model.add_population(morph, 1)
         cells.append(cell)
         model.insert(cell, HH, sections="soma")

@@ -34,6 +34,12 @@ def test_oversized_cells_plans_are_consistent(plan):
     assert check_batch_plan(model, *plan) >= 4
 
 
+@pytest.mark.parametrize("plan", [(300, 128, 0), (600, 64, 4), (500, 64, 5), (3000, 1024, 5)])
+def test_many_children_plans_are_consistent(plan):
+    model, _ = configs.tree_network(ncells=30, median=120, lo=20, hi=600, indegree=10, fanout=4)
+    assert check_batch_plan(model, *plan) > 0
+
+
 def test_invalid_plans_are_rejected():
     from mind_amd._abi import MindError
 

@@ -109,6 +109,24 @@ def test_trees_fixed_plans_batched(gpu, plan):
     destroy(model)
 
 
+@pytest.mark.parametrize("plan", [None, (300, 128, 0), (600, 64, 4), (500, 64, 5), (3000, 1024, 5)])
+def test_trees_with_many_children_batched(gpu, plan):
+    """Sections with up to four children at one point (nodes with three and four
+    children take the CSR path of assembly and elimination) in every layout:
+    bit-identical to the plain kernel and within tolerance of the oracle."""
+    model, info = configs.tree_network(ncells=30, median=120, lo=20, hi=600, indegree=10,
+                                       rate_hz=50.0, netstim_weight=3e-2, fanout=4)
+    recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"][:3]]
+    got_b, _, vg_b, vr, _ = _compare(model, 40.0, recs, method="batched", plan=plan)
+    assert np.array_equal(vg_b, vr)
+    destroy(model)
+    if plan is None:
+        got_p, _, vg_p, _, _ = _compare(model, 40.0, recs, method="plain")
+        assert np.array_equal(vg_p, vg_b)
+        assert np.array_equal(got_p.spike_times, got_b.spike_times)
+        destroy(model)
+
+
 @pytest.mark.parametrize("plan", [None, (4000, 256, 4), (2000, 512, 5)])
 def test_trees_oversized_cells_batched(gpu, plan):
     """Cells above the LDS capacity of the all-in-LDS layout run lean with v in
