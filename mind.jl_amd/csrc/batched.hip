@@ -26,6 +26,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "device.h"
@@ -33,23 +35,37 @@
 
 namespace mind {
 
-// Lean layout (mode 4): the solve's node records, read from HBM three passes
-// ahead. A group of nn nodes has LDS slots nn (the dummy: pivot 1, rhs 1) and
-// nn + 1 (the sink), and nn + 1 records per order, the last the sink record.
-// Elimination, by position in the elimination order: the node, its higher and
-// lower child (the dummy if missing) and their axial conductances (-0 for the
-// dummy); csr = 1 for a node with more than two children, whose children are
-// then pulled from the CSR lists. Back substitution, by position in depth order:
-// the node, its parent (the dummy at a root) and its own axial conductance (-0
-// at a root). With the dummy's values every term of a missing neighbour is an
-// exact identity (x - (+0) = x, x + (-0) = x), so no lane branches.
+// Lean layouts (modes 4 and 5): the solve works on units, a unit being a node
+// whose parent has no other child paired with that parent (the bottom and the
+// top of the unit), or a node alone. The host pairs nodes top-down: a node that
+// is not itself a bottom and has exactly one child takes that child as its
+// bottom. A unit's depth is its top's depth in the tree of units, so one lane
+// eliminates a unit's two nodes in one pass (bottom, then top pulling the
+// bottom from registers) and substitutes them in one pass (top, then bottom):
+// on unbranched stretches of cable half the passes, each one division longer.
+// Each node keeps its own operation sequence, so the results do not change.
+// The solve's records are read from HBM kAhead passes ahead. A group of nn
+// nodes has LDS slots nn (the dummy: pivot 1, rhs 1) and nn + 1 (the sink), and
+// nn + 1 records per order: one per unit by position in the order, then sink
+// records up to position nn, the sink record of the padding lanes.
+// Elimination record (by position in the elimination order): x the unit's
+// bottom (its only node if alone), h and l x's higher and lower child (the dummy
+// if missing) with their axial conductances gh, gl (-0 for the dummy); y the
+// unit's top (the sink if alone) and gy x's own axial conductance (-0 if alone).
+// x is stored as ~x (negative) when it has more than two children, which it
+// then pulls from the CSR lists. Back-substitution record (by position in the
+// order of increasing depth): y the unit's top, p its parent (the dummy at a
+// root), gy y's axial conductance (-0 at a root), x the bottom (the sink if
+// alone) and gx its axial conductance (-0 if alone). With the dummy's values
+// every term of a missing neighbour is an exact identity (x - (+0) = x,
+// x + (-0) = x), so no lane branches.
 struct HRec {
-  int16_t li, h, l, csr;
-  double gh, gl;
+  int16_t x, h, l, y;
+  double gh, gl, gy;
 };
 struct DRec {
-  int16_t li, p, pad0, pad1;
-  double g;
+  int16_t y, p, x, pad;
+  double gy, gx;
 };
 
 // Cable assembly record of a node (modes 3-5, whose links and conductances are
@@ -379,7 +395,7 @@ __device__ __forceinline__ void solve_waves(const Dev& d, const BatchDev& b, int
 // rhs (dd, bb): children from the CSR lists in decreasing index. Returns true if
 // the pivot is zero or not finite.
 template <typename C>
-__device__ __forceinline__ bool elim_csr(const Dev& d, int64_t n0, int li, double dd, double bb, C rhs, C diag) {
+__device__ __forceinline__ bool elim_csr(const Dev& d, int64_t n0, int li, double& dd, double& bb, C rhs, C diag) {
   const int64_t gi = n0 + li;
   for (int64_t cq = d.child_off[gi + 1] - 1; cq >= d.child_off[gi]; --cq) {
     const int lc = d.child[cq] - (int)n0;
@@ -398,15 +414,18 @@ __device__ __forceinline__ bool bad_pivot(double dd) {
   return !((dd != 0.0) & (__builtin_fabs(dd) < __builtin_huge_val()));
 }
 
-// Tree solve of one wave's cells in the lean layout (rhs and pivots in LDS
-// pairs, records from HRec / DRec). The loop runs passes in threes with three
-// records in flight: a pass reads its node's pair and both children's (three
-// LDS reads), divides twice, stores the pair, then loads the record of the pass
-// three ahead into the registers it has just freed, so that no register moves
-// and no wait stand between a load and its use three passes later. Passes past
-// the wave's last only move the sink. The pass offsets are read one pass ahead
-// (a broadcast LDS read alongside the pass's own). Back substitution leaves the
-// node's change x in rhs; v += x follows for all nodes after the solve.
+// Tree solve of one wave's cells in the lean layouts, by units (rhs and pivots
+// in LDS pairs, records from HRec / DRec). The loop runs passes in threes with
+// three records in flight. An elimination pass reads the pairs of its bottom x,
+// x's children and its top y (four LDS reads), eliminates x (two divisions
+// issued together), then y pulling x (a third), and stores both pairs; it then
+// loads the record of the pass three ahead into the registers it has just
+// freed, so that no register moves and no wait stand between a load and its use
+// three passes later. Passes past the wave's last only move the sink. A
+// back-substitution pass solves the top from its parent, then the bottom from
+// the top. The pass offsets are read one pass ahead (a broadcast LDS read
+// alongside the pass's own). Back substitution leaves the node's change in rhs;
+// v += change follows for all nodes after the solve.
 // Passes of record loads in flight in the lean solve.
 constexpr int kAhead = 3;
 
@@ -432,16 +451,27 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
     }
     auto pass = [&](HRec& r, int q) {
       const int e = lv[min(q + kAhead + 1, nph)];  // end of the pass kAhead ahead
-      const double od = diag[r.li], orr = rhs[r.li];
+      const int xi = r.x < 0 ? ~r.x : r.x;
+      const double od = diag[xi], orr = rhs[xi];
       const double dh = diag[r.h], rh = rhs[r.h], dl = diag[r.l], rl = rhs[r.l];
+      const double oy = diag[r.y], ry = rhs[r.y];
       reads_issued();
       const double fh = r.gh / dh, fl = r.gl / dl;
-      const double dd = (od - fh * r.gh) - fl * r.gl;
-      const double bb = (orr + fh * rh) + fl * rl;
-      diag[r.li] = dd;
-      rhs[r.li] = bb;
-      bad |= (r.li < nn) & bad_pivot(dd);
-      if (r.csr) bad |= elim_csr(d, n0, r.li, od, orr, rhs, diag);
+      double dd = (od - fh * r.gh) - fl * r.gl;
+      double bb = (orr + fh * rh) + fl * rl;
+      if (r.x < 0) {  // more than two children: the CSR lists, from x's own pivot and rhs
+        dd = od;
+        bb = orr;
+        bad |= elim_csr(d, n0, xi, dd, bb, rhs, diag);
+      }
+      const double fy = r.gy / dd;
+      const double dy = oy - fy * r.gy;
+      const double by = ry + fy * bb;
+      diag[xi] = dd;
+      rhs[xi] = bb;
+      diag[r.y] = dy;
+      rhs[r.y] = by;
+      bad |= ((xi < nn) & bad_pivot(dd)) | ((r.y < nn) & bad_pivot(dy));
       wave_sync<4>();
       r = hr[slot(next, e)];
       next = e;
@@ -466,9 +496,12 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
     }
     auto pass = [&](DRec& r, int q) {
       const int e = lv[min(q + kAhead + 1, npd)];
-      const double od = diag[r.li], orr = rhs[r.li], rp = rhs[r.p];
+      const double oy = diag[r.y], ry = rhs[r.y], rp = rhs[r.p];
+      const double ox = diag[r.x], rx = rhs[r.x];
       reads_issued();
-      rhs[r.li] = (orr + r.g * rp) / od;
+      const double xy = (ry + r.gy * rp) / oy;
+      rhs[r.y] = xy;
+      rhs[r.x] = (rx + r.gx * xy) / ox;
       wave_sync<4>();
       r = dr[slot(next, e)];
       next = e;
@@ -901,6 +934,25 @@ std::vector<int> assign_waves(const std::vector<std::vector<int32_t>>& hist_h,
   return wave;
 }
 
+// Units of the lean layouts' solve over nodes [n0, n0 + nn) of one group: pairs
+// nodes top-down, a node that is not itself a bottom and has exactly one child
+// taking that child as its bottom (with pair = false every node is alone).
+// bottom[li] marks bottoms; below[li] is a top's bottom, -1 if it is alone.
+void pair_units(const Layout& L, int64_t n0, int nn, bool pair, std::vector<uint8_t>& bottom,
+                std::vector<int32_t>& below) {
+  bottom.assign(nn, 0);
+  below.assign(nn, -1);
+  if (!pair) return;
+  for (int li = 0; li < nn; ++li) {
+    const int32_t pg = L.parent[n0 + li];
+    const int p = pg < 0 ? -1 : pg - (int32_t)n0;
+    if (p >= 0 && !bottom[p] && L.child_off[n0 + p + 1] - L.child_off[n0 + p] == 1) {
+      bottom[li] = 1;
+      below[p] = li;
+    }
+  }
+}
+
 // Upper bound of a schedule block's length: the headers plus, per wave, one more
 // offset than its passes; a wave's levels are at most the sum of its cells', and
 // splitting levels into passes of 64 adds at most nodes / 64 per order.
@@ -971,7 +1023,23 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
     // Both sweeps run by depth: back substitution from the roots, elimination
     // in reverse (a node's children are one level deeper, so any order by
     // decreasing depth eliminates them first; depth levels are narrower than
-    // height levels). Per cell histograms of nodes by depth, then the waves.
+    // height levels). The lean layouts schedule units (a node and, if it has
+    // exactly one child and is not itself a bottom, that child as its bottom) by
+    // their depth in the tree of units; the others schedule nodes. Per cell
+    // histograms of the scheduled entries by level, then the waves.
+    const int list = glist[g];
+    const int mode = list_mode(list, layout);
+    const bool units = mode == 4 || mode == 5;
+    std::vector<uint8_t> bottom;
+    std::vector<int32_t> below;
+    pair_units(L, n0, nn, units, bottom, below);
+    for (int li = 0; li < nn; ++li) {  // depth in the tree of units
+      const int32_t pg = L.parent[n0 + li];
+      const int p = pg < 0 ? -1 : pg - (int32_t)n0;
+      dd[li] = p < 0 ? 0 : bottom[li] ? dd[p] : dd[p] + 1;
+    }
+    int32_t nsched = 0;  // scheduled entries: nodes, or the tops of units
+    for (int li = 0; li < nn; ++li) nsched += bottom[li] ? 0 : 1;
     const int ncg = cell1[g] - cell0[g];
     std::vector<std::vector<int32_t>> hist_d(ncg);
     std::vector<int32_t> cell_of(nn);
@@ -979,6 +1047,7 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
       const int64_t a = L.cell_off[cell0[g] + k] - n0, z = L.cell_off[cell0[g] + k + 1] - n0;
       for (int64_t li = a; li < z; ++li) {
         cell_of[li] = k;
+        if (bottom[li]) continue;
         if ((int64_t)hist_d[k].size() <= dd[li]) hist_d[k].resize(dd[li] + 1, 0);
         hist_d[k][dd[li]] += 1;
       }
@@ -1002,18 +1071,17 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
     int64_t slen = 4 * (W + 1);
     for (int w = 0; w < W; ++w) slen += npasses(wnh[w]) + 1 + npasses(wnd[w]) + 1;
     if (slen > (1 << 30)) return fail(MIND_EUNSUPPORTED, "schedule block too long");
-    const int list = glist[g];
     H.lists[list].push_back(g);
-    const int mode = list_mode(list, layout);
-    if (mode == 4 || mode == 5) any_lean = true;
+    if (units) any_lean = true;
     if (mode != 0) any_hbm_links = true;
     H.lds[list] = std::max(H.lds[list], window_bytes(mode, nn, slen));
     {  // estimated cycles of one step: the longest wave's solve passes, and the
-       // node-parallel phases (about 450 cycles per pass and 5400 per node per thread)
+       // node-parallel phases (about 450 cycles per pass of nodes, 600 per pass
+       // of units, and 5400 per node per thread)
       int64_t longest = 0;
       for (int w = 0; w < W; ++w) longest = std::max(longest, npasses(wnh[w]) + npasses(wnd[w]));
       const int Tg = W * 64;
-      group_cost[g] = 450 * longest + 5400 * ((nn + Tg - 1) / Tg);
+      group_cost[g] = (units ? 600 : 450) * longest + 5400 * ((nn + Tg - 1) / Tg);
     }
     r_tab.resize((r_tab.size() + 3) / 4 * 4, 0);  // 8-byte aligned blocks
     grp_rtab[g] = (int32_t)r_tab.size();
@@ -1051,7 +1119,7 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
           for (int32_t q = cnt[l]; q < cnt[l + 1]; q += 64) t_s[cursor++] = (int16_t)(pos + q);
         t_s[cursor++] = (int16_t)(pos + cnt[nl]);
         for (int li = 0; li < nn; ++li)
-          if (wave[cell_of[li]] == w) ord[pos + cnt[lv[li]]++] = (int16_t)li;
+          if (wave[cell_of[li]] == w && !bottom[li]) ord[pos + cnt[lv[li]]++] = (int16_t)li;
         pos += cnt[nl];  // the fill leaves cnt[nl], the wave's node count, unchanged
       }
       put(hdr0 + W, cursor);
@@ -1059,9 +1127,9 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
     int64_t cursor = 4 * (W + 1);
     order(hh, wnh, t_oh, 0, cursor);
     order(dd, wnd, t_od, W + 1, cursor);
-    // the kernels index by these tables without bounds checks: every node once
-    // per order, in its own cell's wave; passes of at most 64 nodes of one
-    // level, in level order
+    // the kernels index by these tables without bounds checks: every scheduled
+    // node once per order, in its own cell's wave; passes of at most 64 entries
+    // of one level, in level order
     auto valid = [&](const std::vector<int32_t>& lv, const int16_t* ord, int hdr0) {
       std::vector<uint8_t> seen(nn, 0);
       int32_t pos = 0;
@@ -1071,19 +1139,19 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
         int32_t level = 0;
         for (int64_t q = 0; q + 1 < z - a; ++q) {
           const int32_t i0 = t_s[a + q], i1 = t_s[a + q + 1];
-          if (i1 <= i0 || i1 - i0 > 64 || i1 > nn) return false;
+          if (i1 <= i0 || i1 - i0 > 64 || i1 > nsched) return false;
           const int32_t l0 = lv[ord[i0] >= 0 && ord[i0] < nn ? ord[i0] : 0];
           if (l0 < level) return false;
           level = l0;
           for (int32_t i = i0; i < i1; ++i) {
             const int li = ord[i];
-            if (li < 0 || li >= nn || seen[li] || wave[cell_of[li]] != w || lv[li] != level) return false;
+            if (li < 0 || li >= nn || seen[li] || bottom[li] || wave[cell_of[li]] != w || lv[li] != level) return false;
             seen[li] = 1;
           }
           pos = i1;
         }
       }
-      return pos == nn;
+      return pos == nsched;
     };
     if (!valid(hh, t_oh, 0) || !valid(dd, t_od, W + 1))
       return fail(MIND_EUNSUPPORTED, "batched plan: inconsistent solve schedule");
@@ -1104,19 +1172,33 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
         const int16_t dummy = (int16_t)nn, sink = (int16_t)(nn + 1);
         const int16_t* t = r_tab.data() + grp_rtab[g];
         const int16_t *t_hn = t, *t_oh = t + 4 * nn, *t_od = t + 5 * nn;
-        for (int i = 0; i < nn; ++i) {
-          const int li = t_oh[i], h = t_hn[4 * li], l = t_hn[4 * li + 1];
-          if (h == -2) {
-            hrec[rb + i] = HRec{(int16_t)li, dummy, dummy, 1, -0.0, -0.0};
-          } else {
-            hrec[rb + i] = HRec{(int16_t)li, h >= 0 ? (int16_t)h : dummy, l >= 0 ? (int16_t)l : dummy, 0,
-                                h >= 0 ? L.axial[n0 + h] : -0.0, l >= 0 ? L.axial[n0 + l] : -0.0};
-          }
-          const int lj = t_od[i], pj = t_hn[4 * lj + 2];
-          drec[rb + i] = DRec{(int16_t)lj, pj >= 0 ? (int16_t)pj : dummy, 0, 0, pj >= 0 ? L.axial[n0 + lj] : -0.0};
+        const int mode = list_mode(glist[g], layout);
+        int nu = 0;  // units (groups of the other layouts keep sink records only)
+        std::vector<uint8_t> bottom;
+        std::vector<int32_t> below;
+        if (mode == 4 || mode == 5) {
+          pair_units(L, n0, nn, true, bottom, below);
+          for (int li = 0; li < nn; ++li) nu += bottom[li] ? 0 : 1;
         }
-        hrec[rb + nn] = HRec{sink, dummy, dummy, 0, -0.0, -0.0};
-        drec[rb + nn] = DRec{sink, dummy, 0, 0, -0.0};
+        for (int i = 0; i < nu; ++i) {
+          const int u = t_oh[i], c = below[u], x = c >= 0 ? c : u;
+          const int h = t_hn[4 * x], l = t_hn[4 * x + 1];
+          const int16_t y = c >= 0 ? (int16_t)u : sink;
+          const double gy = c >= 0 ? L.axial[n0 + c] : -0.0;
+          if (h == -2) {  // more than two children: pulled from the CSR lists
+            hrec[rb + i] = HRec{(int16_t)~x, dummy, dummy, y, -0.0, -0.0, gy};
+          } else {
+            hrec[rb + i] = HRec{(int16_t)x, h >= 0 ? (int16_t)h : dummy, l >= 0 ? (int16_t)l : dummy, y,
+                                h >= 0 ? L.axial[n0 + h] : -0.0, l >= 0 ? L.axial[n0 + l] : -0.0, gy};
+          }
+          const int v = t_od[i], pv = t_hn[4 * v + 2], cv = below[v];
+          drec[rb + i] = DRec{(int16_t)v, pv >= 0 ? (int16_t)pv : dummy, cv >= 0 ? (int16_t)cv : sink, 0,
+                              pv >= 0 ? L.axial[n0 + v] : -0.0, cv >= 0 ? L.axial[n0 + cv] : -0.0};
+        }
+        for (int i = nu; i <= nn; ++i) {
+          hrec[rb + i] = HRec{sink, dummy, dummy, sink, -0.0, -0.0, -0.0};
+          drec[rb + i] = DRec{sink, dummy, sink, 0, -0.0, -0.0};
+        }
       }
     }
     std::vector<ARec>& arec = H.arec;
@@ -1151,6 +1233,105 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
   for (int m = 0; m < kLists; ++m)
     if (H.lds[m] > kLdsResident) return fail(MIND_EUNSUPPORTED, "batched plan exceeds the LDS capacity");
   return 0;
+}
+
+// Runs the lean solve of group g (mode 4 or 5) on the host as the kernel does,
+// pass by pass per wave with all lanes of a pass reading before any writes,
+// on a seeded system with the group's axial conductances, and compares it bit
+// for bit with the canonical elimination (decreasing node index, each parent
+// taking its children's terms in that order; substitution in increasing index).
+// Empty string if they agree.
+std::string emulate_lean(const Layout& L, const PlanHost& H, int g, int W) {
+  const int64_t n0 = L.cell_off[H.cell0[g]];
+  const int nn = (int)(L.cell_off[H.cell1[g]] - n0);
+  const int16_t* sc = H.r_tab.data() + H.grp_rtab[g] + 6 * nn;
+  const int64_t rb = H.rbase[g];
+  std::vector<double> d0(nn), b0(nn);
+  uint64_t z = 0x9e3779b97f4a7c15ull * (uint64_t)(g + 1);
+  auto uni = [&]() {  // splitmix64 -> [0, 1)
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return (double)((x ^ (x >> 31)) >> 11) * 0x1.0p-53;
+  };
+  for (int i = 0; i < nn; ++i) {
+    double sum = 0.0;
+    for (int64_t q = L.child_off[n0 + i]; q < L.child_off[n0 + i + 1]; ++q) sum += L.axial[L.child[q]];
+    d0[i] = 1.0 + uni() + sum + (L.parent[n0 + i] >= 0 ? L.axial[n0 + i] : 0.0);
+    b0[i] = uni() - 0.5;
+  }
+  std::vector<double> dr = d0, br = b0;  // canonical
+  for (int i = nn - 1; i >= 0; --i) {
+    const int32_t p = L.parent[n0 + i];
+    if (p >= 0) {
+      const double f = L.axial[n0 + i] / dr[i];
+      dr[p - n0] -= f * L.axial[n0 + i];
+      br[p - n0] += f * br[i];
+    }
+  }
+  for (int i = 0; i < nn; ++i) {
+    const int32_t p = L.parent[n0 + i];
+    double value = br[i];
+    if (p >= 0) value += L.axial[n0 + i] * br[p - n0];
+    br[i] = value / dr[i];
+  }
+  std::vector<double> dg(nn + 2), bg(nn + 2);  // the LDS pairs, dummy and sink
+  for (int i = 0; i < nn; ++i) {
+    dg[i] = d0[i];
+    bg[i] = b0[i];
+  }
+  dg[nn] = bg[nn] = 1.0;
+  dg[nn + 1] = 1.0;
+  bg[nn + 1] = 0.0;
+  struct Wr {
+    int k;
+    double dv, bv;
+  };
+  std::vector<Wr> out;
+  for (int sweep = 0; sweep < 2; ++sweep)
+    for (int w = 0; w < W; ++w) {
+      const int64_t a = sched_at(sc, sweep * (W + 1) + w), e = sched_at(sc, sweep * (W + 1) + w + 1);
+      for (int64_t q = a; q + 1 < e; ++q) {
+        out.clear();
+        for (int i = sc[q]; i < sc[q + 1]; ++i) {
+          if (sweep == 0) {
+            const HRec& r = H.hrec[rb + i];
+            const int x = r.x < 0 ? ~r.x : r.x;
+            const double fh = r.gh / dg[r.h], fl = r.gl / dg[r.l];
+            double dd = (dg[x] - fh * r.gh) - fl * r.gl;
+            double bb = (bg[x] + fh * bg[r.h]) + fl * bg[r.l];
+            if (r.x < 0) {
+              dd = dg[x];
+              bb = bg[x];
+              for (int64_t cq = L.child_off[n0 + x + 1] - 1; cq >= L.child_off[n0 + x]; --cq) {
+                const int lc = L.child[cq] - (int32_t)n0;
+                const double f = L.axial[n0 + lc] / dg[lc];
+                dd -= f * L.axial[n0 + lc];
+                bb += f * bg[lc];
+              }
+            }
+            const double fy = r.gy / dd;
+            out.push_back({x, dd, bb});
+            out.push_back({r.y, dg[r.y] - fy * r.gy, bg[r.y] + fy * bb});
+          } else {
+            const DRec& r = H.drec[rb + i];
+            const double xy = (bg[r.y] + r.gy * bg[r.p]) / dg[r.y];
+            out.push_back({r.y, dg[r.y], xy});
+            out.push_back({r.x, dg[r.x], (bg[r.x] + r.gx * xy) / dg[r.x]});
+          }
+        }
+        for (const Wr& o : out) {
+          if (o.k >= nn && o.k != nn + 1) return "lean solve writes the dummy";
+          dg[o.k] = o.dv;
+          bg[o.k] = o.bv;
+        }
+      }
+    }
+  for (int i = 0; i < nn; ++i)
+    if (std::memcmp(&dg[i], &dr[i], sizeof(double)) || std::memcmp(&bg[i], &br[i], sizeof(double)))
+      return "lean solve differs from the canonical elimination at node " + std::to_string(i);
+  return "";
 }
 
 // Checks on the host every index the window kernels derive from a plan's tables
@@ -1201,18 +1382,40 @@ std::string check_plan(const Layout& L, const PlanHost& H) {
         }
       }
     if (mode == 4 || mode == 5) {
+      // records: every slot index within the nodes, the dummy (read only) and
+      // the sink; each node written once per order (as x or y), the dummy
+      // never; positions past the units hold sink records up to position nn
       const int64_t rb = H.rbase[g];
       if (rb < 0 || rb + nn >= (int64_t)H.hrec.size() || rb + nn >= (int64_t)H.drec.size()) return bad(g, "records");
+      std::vector<uint8_t> done_h(nn, 0), done_d(nn, 0);
+      int64_t nu = 0;
       for (int64_t i = 0; i <= nn; ++i) {
         const HRec& r = H.hrec[rb + i];
         const DRec& q = H.drec[rb + i];
-        if (r.li < 0 || r.li > nn + 1 || r.h < 0 || r.h > nn + 1 || r.l < 0 || r.l > nn + 1 || r.csr < 0 ||
-            r.csr > 1 || q.li < 0 || q.li > nn + 1 || q.p < 0 || q.p > nn + 1)
+        const int x = r.x < 0 ? ~r.x : r.x;
+        if (x < 0 || x > nn + 1 || x == nn || r.h < 0 || r.h > nn + 1 || r.l < 0 || r.l > nn + 1 || r.y < 0 ||
+            r.y > nn + 1 || r.y == nn || q.y < 0 || q.y > nn + 1 || q.y == nn || q.p < 0 || q.p > nn + 1 ||
+            q.x < 0 || q.x > nn + 1 || q.x == nn)
           return bad(g, "lean record");
-        if (i < nn && (r.li >= nn || q.li >= nn)) return bad(g, "lean record node");
-        if (i == nn && (r.li != nn + 1 || r.h != nn || r.l != nn || r.csr || q.li != nn + 1 || q.p != nn))
-          return bad(g, "sink record");
+        const bool sink_h = x == nn + 1, sink_d = q.y == nn + 1;
+        if (sink_h != sink_d || (sink_h && (r.y != nn + 1 || q.x != nn + 1))) return bad(g, "lean record pair");
+        if (sink_h) continue;
+        if (i != nu++) return bad(g, "record after a sink record");
+        for (int node : {x, (int)r.y})
+          if (node < nn && done_h[node]++) return bad(g, "node eliminated twice");
+        for (int node : {(int)q.y, (int)q.x})
+          if (node < nn && done_d[node]++) return bad(g, "node substituted twice");
       }
+      if (H.hrec[rb + nn].x != nn + 1 || H.drec[rb + nn].y != nn + 1) return bad(g, "sink record");
+      for (int64_t li = 0; li < nn; ++li)
+        if (!done_h[li] || !done_d[li]) return bad(g, "node missing from the lean records");
+      for (int sweep = 0; sweep < 2; ++sweep)
+        for (int w = 0; w < W; ++w) {
+          const int64_t z = sched_at(sc, sweep * (W + 1) + w + 1);
+          if (sc[z - 1] > nu) return bad(g, "pass offset beyond the units");
+        }
+      const std::string e = emulate_lean(L, H, g, W);
+      if (!e.empty()) return bad(g, e.c_str());
     }
     if (mode != 0) {
       if (n0 + nn > (int64_t)H.arec.size()) return bad(g, "assembly records");
