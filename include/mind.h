@@ -105,7 +105,8 @@ typedef struct mind_model_desc {
 
   /* Spike sources (reference register_spike_source!, model.jl:286-365): kind 0 is a
    * membrane-voltage threshold detector at (cell, section, x); kind 1 is the
-   * net_event output of point process `point`. */
+   * net_event output of point process `point`; kind 2 an external spike input
+   * that only mind_inject_spike drives. */
   int64_t n_sources;
   const int32_t* src_kind;
   const int32_t* src_cell;
@@ -138,6 +139,12 @@ typedef struct mind_build_options {
   int32_t window_steps;   /* steps per device window, 0 = longest the minimum delay allows */
   int64_t event_capacity; /* events per delivery-step ring slot, 0 = automatic */
   int32_t device;         /* HIP device ordinal */
+  int32_t secondorder;    /* reference build! keyword (build.jl:120-131): 0 backward Euler;
+                             1 and 2 Crank-Nicolson, the cable assembled with C/(dt/2) and
+                             v += 2 dv (step.jl:112, cuda/micro.jl:660-679). 2 also corrects
+                             the per-node ion current columns, which none of the six
+                             mechanisms reads (fixed ena/ek, no concentration dynamics),
+                             so 2 behaves as 1. */
 } mind_build_options;
 
 typedef struct mind_state mind_state;
@@ -246,6 +253,24 @@ int mind_read_voltage(mind_state* state, double* out, int64_t n); /* canonical n
 int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out, int64_t n);
 int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows);
 int mind_time(const mind_state* state, double* t, int64_t* step);
+/* read_value (src/runtime/state.jl:586-600): the current value of one record
+ * selector (voltage, point-process field, density field); NetStim fields too.
+ * Needs an initialized state. */
+int mind_read_value(mind_state* state, const mind_record* record, double* out);
+/* connection_weights (src/runtime/state.jl:603-616): the NET_RECEIVE arguments of
+ * connection `connection` (0-based creation order) into out[0..capacity); *count
+ * is their number (1 for the six mechanisms: the weight). */
+int mind_connection_weights(const mind_state* state, int64_t connection, double* out, int32_t capacity,
+                            int32_t* count);
+/* inject_event! (src/runtime/state.jl:618-636): one event of connection
+ * `connection` (0-based creation order) arriving at absolute time `time` >= now,
+ * without its delay, ticketed at the call like any enqueued event. */
+int mind_inject_event(mind_state* state, int64_t connection, double time);
+/* inject_spike! (src/runtime/state.jl:638-659): a spike of external source
+ * `source` (kind 2) at absolute time `time` >= now, routed over its connections
+ * and recorded at its delivery call. At most 256 injected spikes may fall in one
+ * window. Pending injections are cleared by mind_initialize. */
+int mind_inject_spike(mind_state* state, int64_t source, double time);
 int mind_window_steps(const mind_state* state, int32_t* steps);
 /* Batched method: node budget per workgroup, block size, node-state layout of those
  * workgroups (0 all in LDS; 5 lean: rhs, pivots and v in LDS, the solve's node

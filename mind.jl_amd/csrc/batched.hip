@@ -271,21 +271,6 @@ __device__ __forceinline__ bool eliminate(const Dev& d, int64_t n0, int li, int 
   return (dd == 0.0) || !mind_isfinite(dd);
 }
 
-// Back substitution of node li with parent p (-1 at a root) and its own axial
-// conductance g; v += dv. A root's parent term enters as -0 x 1. Stores only if
-// `store`.
-template <typename C>
-__device__ __forceinline__ void substitute(int li, int p, double g, C rhs, C diag, double* v,
-                                           bool store) {
-  const bool pv = p >= 0;
-  const double rp0 = rhs[pv ? p : li];
-  const double x0 = rhs[li], dg = diag[li];
-  const double x = (x0 + (pv ? g : -0.0) * (pv ? rp0 : 1.0)) / dg;
-  if (!store) return;
-  rhs[li] = x;
-  v[li] += x;
-}
-
 // A wave's pass offsets (nl + 1 of them at lv) held 64 at a time in one
 // register, lane j holding offset base + j: reading one is a cross-lane move, not
 // a memory access. at() takes wave-uniform indices j <= nl that grow by at most
@@ -377,7 +362,7 @@ __device__ __forceinline__ void solve_waves(const Dev& d, const BatchDev& b, int
         if (a + lane < z) {
           const double x = (x0 + (pv ? g : -0.0) * (pv ? rp0 : 1.0)) / dg;
           rhs[li] = x;
-          v[li] = vi + x;
+          v[li] = vi + d.vscale * x;
         }
         wave_sync<M>();
         li = li1;
@@ -745,7 +730,7 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     if (lean) {
       solve_lean(d, b, grp, n0, nn, sched, rhs, diag);
       __syncthreads();
-      for (int i = tid; i < nn; i += T) v[i] += rhs[i];
+      for (int i = tid; i < nn; i += T) v[i] += d.vscale * rhs[i];
     } else {
       solve_waves<M>(d, b, n0, nn, sched, hn, oh, od, ax, rhs, diag, v);
     }
@@ -1212,7 +1197,7 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
           const int h = t_hn[4 * li], l = t_hn[4 * li + 1], p = t_hn[4 * li + 2];
           arec[n0 + li] = ARec{(int16_t)h, (int16_t)l, (int16_t)p, 0, L.axial[n0 + li],
                                h >= 0 ? L.axial[n0 + h] : 0.0, l >= 0 ? L.axial[n0 + l] : 0.0,
-                               L.cap[n0 + li] / L.dt, L.adiag[n0 + li]};  // C/dt as in the cdt column
+                               L.cap[n0 + li] / L.assembly_dt(), L.adiag[n0 + li]};  // as the cdt column
         }
       }
     }

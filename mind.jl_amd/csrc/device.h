@@ -17,6 +17,15 @@ struct QEvent {
   int32_t pad;
 };
 
+// An injected source spike (inject_spike!): time, heap insertion order, source.
+struct InjSpike {
+  double t;
+  uint64_t seq;
+  int32_t source;
+  int32_t pad;
+};
+constexpr int kInjCap = 256;  // injected spikes handed to one window, events per staging copy
+
 struct Clock {
   int64_t step;
   uint64_t ticket;   // routed events so far
@@ -50,7 +59,8 @@ struct Dev {
   const int64_t* cell_off;
   const int32_t* parent;
   const double* area;
-  const double* cdt;  // capacitance / dt
+  const double* cdt;  // capacitance / assembly dt (dt, or dt/2 with secondorder)
+  double vscale;      // v += vscale * dv: 1, or 2 with secondorder (update_voltage_kernel!)
   const double* axial;
   const double* adiag;
   double* v;
@@ -120,6 +130,10 @@ struct Dev {
   double* rec_buf;
   int64_t rec_stride;
   int64_t rec_start;
+  // injected source spikes due in the next window, and their count (the window's
+  // exchange consumes them and clears the count)
+  InjSpike* inj;
+  int32_t* inj_count;
   Clock* clock;
   int32_t* error;
 };
@@ -136,7 +150,7 @@ __host__ __device__ inline double call_cutoff(int64_t j, double dt) {
 }
 
 // First delivery call j >= jmin whose cutoff reaches t.
-__device__ inline int64_t call_of(double t, int64_t jmin, double dt) {
+__host__ __device__ inline int64_t call_of(double t, int64_t jmin, double dt) {
   double est = floor(t / dt);
   int64_t j = jmin;
   if (est * 2.0 - 2.0 > (double)jmin) j = (int64_t)est * 2 - 2;

@@ -372,6 +372,7 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
   L.nsources = d.n_sources;
   L.source_sid.assign(d.src_sid, d.src_sid + d.n_sources);
   L.source_point.assign(d.n_sources, -1);
+  L.source_external.assign(d.n_sources, 0);
   std::vector<int32_t> det_of_cell_count(L.ncells + 1, 0);
   for (int64_t s = 0; s < d.n_sources; ++s) {
     if (d.src_kind[s] == 0) {
@@ -383,10 +384,14 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
       L.det_cell.push_back(c);
       L.det_threshold.push_back(d.src_threshold[s]);
       det_of_cell_count[c + 1] += 1;
-    } else {
+    } else if (d.src_kind[s] == 1) {
       const int p = d.src_point[s];
       if (p < 0 || p >= d.n_points) return "event source on an unknown point process";
       L.source_point[s] = p;
+    } else if (d.src_kind[s] == 2) {
+      L.source_external[s] = 1;
+    } else {
+      return "unknown spike-source kind";
     }
   }
   L.cell_det_off.assign(L.ncells + 1, 0);
@@ -411,10 +416,12 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
   L.edge_target.resize(d.n_connections);
   L.edge_delay.resize(d.n_connections);
   L.edge_weight.resize(d.n_connections);
+  L.connection_edge.resize(d.n_connections);
   {
     std::vector<int64_t> cur(L.fanout.begin(), L.fanout.end() - 1);
     for (int64_t e = 0; e < d.n_connections; ++e) {
       const int64_t k = cur[d.con_source[e]]++;
+      L.connection_edge[e] = k;
       L.edge_target[k] = d.con_target[e];
       L.edge_delay[k] = d.con_delay[e];
       L.edge_weight[k] = d.con_weight[e];

@@ -38,6 +38,9 @@ struct Template {  // one discretized morphology
 
 struct Layout {
   double dt = 0.025, celsius = 6.3;
+  int32_t secondorder = 0;  // 0 backward Euler; 1, 2 Crank-Nicolson (assembly at dt/2, v += 2 dv)
+  // dt of the cable assembly's C/dt term (assemble_cable_kernel!'s argument, step.jl:112)
+  double assembly_dt() const { return secondorder ? 0.5 * dt : dt; }
   uint64_t seed = 1;
   int32_t ncells = 0;
   int64_t nodes = 0, compartments = 0;
@@ -56,6 +59,7 @@ struct Layout {
   int64_t nsources = 0;
   std::vector<int64_t> source_sid;
   std::vector<int32_t> source_point;  // point index of an event source, -1 for detectors
+  std::vector<uint8_t> source_external;  // 1 for external spike inputs (inject_spike!)
   std::vector<int32_t> det_node, det_source, det_cell;
   std::vector<double> det_threshold;
   std::vector<int64_t> cell_det_off;  // detectors of cell c (indices into det_*)
@@ -64,6 +68,7 @@ struct Layout {
   std::vector<int64_t> fanout;  // nsources + 1
   std::vector<int32_t> edge_target;
   std::vector<double> edge_delay, edge_weight;
+  std::vector<int64_t> connection_edge;  // edge of each connection, in creation order
   double min_delay = 0.0, max_delay = 0.0;
 
   std::vector<int32_t> netstim_source;  // per NetStim row: its source index or -1

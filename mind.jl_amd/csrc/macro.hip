@@ -362,6 +362,8 @@ extern "C" {
 int mind_build_macro(const mind_model_desc* micro, const mind_macro_desc* macro,
                      const mind_build_options* options, mind_state** out) {
   if (!micro || !macro || !options || !out) return mind::fail(MIND_EARG, "null argument");
+  if (options->secondorder != 0)
+    return mind::fail(MIND_EARG, "secondorder is not supported with micro-macro coupling");
   State* S = new State();
   std::string err = mind::pack(*micro, options->dt, options->seed, S->L);
   if (err.empty()) err = mind::pack_inputs(*micro, *macro, S->L);
@@ -423,6 +425,7 @@ int mind_run_macro(mind_state* state, double stop, int32_t record_spikes, mind_m
   const auto t0 = std::chrono::steady_clock::now();
   int rc = 0;
   for (int64_t j = 0; j < count; ++j) {
+    if ((rc = mind::stage_injections(S, H.m.ratio))) break;
     if (!S.profiling) {
       if (!H.graph) {
         hipGraph_t g;

@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
       if (p >= 0) x += d.axial[i] * rhs[p];
       x = x / diag[i];
       rhs[i] = x;
-      v[i] += x;
+      v[i] += d.vscale * x;
     }
     // nrn_state at t + dt
     for (int gi = 0; gi < d.ngroups; ++gi) {
@@ -291,7 +291,9 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
   const int64_t call_hi = n == 0 ? -1 : 2 * (k0 + n) - 1;
   const double dt = d.dt;
   const double cut_hi = call_cutoff(call_hi, dt);
-  // A. NetStim events of the window, in heap order (time, insertion order).
+  // A. NetStim events and injected source spikes of the window, in heap order
+  //    (time, insertion order); an injected spike k enters as row -(k + 1).
+  const int32_t ninj = *d.inj_count;
   if (tid == 0) { heap_n = 0; n_ns = 0; }
   __syncthreads();
   for (int64_t r = tid; r < d.ns_rows; r += kBlock) {
@@ -300,8 +302,12 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
       if (slot < kHeapCap) heap[slot] = HeapItem{d.ns_t[r], d.ns_seq[r], (int32_t)r};
     }
   }
+  for (int k = tid; k < ninj && k < kInjCap; k += kBlock) {
+    const int slot = atomicAdd(&heap_n, 1);
+    if (slot < kHeapCap) heap[slot] = HeapItem{d.inj[k].t, d.inj[k].seq, -(k + 1)};
+  }
   __syncthreads();
-  if (tid == 0 && d.ns_rows > 0) {
+  if (tid == 0 && (d.ns_rows > 0 || ninj > 0)) {
     int hn = heap_n;
     if (hn > kHeapCap) {
       raise_error(d.error, MIND_EUNSUPPORTED);
@@ -310,14 +316,20 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     // heapify
     int m = 0;
     for (int i = 0; i < hn; ++i) heap_push(heap, m, heap[i]);
-    const GroupDev& G = d.g[d.ns_group];
-    const Cols col = cols(G);
+    const Cols col = d.ns_group >= 0 ? cols(d.g[d.ns_group]) : Cols{nullptr, 0};
     int64_t count = 0;
     uint64_t seq = d.clock->ns_seq;
     while (m > 0) {
       const HeapItem it = heap_pop(heap, m);
       const int32_t r = it.row;
       const double t = it.t;
+      if (r < 0) {  // injected source spike: routed at its delivery call, like a net_event
+        const int64_t j = call_of(t, call_lo, dt);
+        if (count < d.emit_cap) d.emit_ns[count] = Emit{emit_key(j), t, d.inj[-r - 1].source, 0};
+        else raise_error(d.error, MIND_EQUEUE);
+        ++count;
+        continue;
+      }
       const double flag = d.ns_flag[r];
       const int32_t edge = d.ns_edge[r];
       d.ns_valid[r] = 0;
@@ -454,6 +466,7 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     d.clock->ticket = ticket0 + (uint64_t)running;
     d.clock->step = k0 + n;
     d.clock->emitted = ne;
+    *d.inj_count = 0;
   }
 }
 
@@ -577,7 +590,8 @@ int setup_state(State& S, const mind_build_options& o) {
   d.nodes = L.nodes;
   int rc = 0;
   std::vector<double> cdt(L.nodes);
-  for (int64_t i = 0; i < L.nodes; ++i) cdt[i] = L.cap[i] / L.dt;
+  for (int64_t i = 0; i < L.nodes; ++i) cdt[i] = L.cap[i] / L.assembly_dt();
+  d.vscale = L.secondorder ? 2.0 : 1.0;
   {
     int64_t* p64; int32_t* p32; double* pd;
     if ((rc = S.upload(&p64, L.cell_off))) return rc; d.cell_off = p64;
@@ -696,6 +710,10 @@ int setup_state(State& S, const mind_build_options& o) {
   if ((rc = S.alloc(&d.clock, 1))) return rc;
   if ((rc = S.alloc(&d.error, 1))) return rc;
   HIP_TRY(hipMemset(d.error, 0, sizeof(int32_t)));
+  if ((rc = S.alloc(&d.inj, kInjCap))) return rc;
+  if ((rc = S.alloc(&d.inj_count, 1))) return rc;
+  HIP_TRY(hipMemset(d.inj_count, 0, sizeof(int32_t)));
+  if ((rc = S.alloc(&S.inj_ev, kInjCap))) return rc;
   return 0;
 }
 
@@ -767,6 +785,51 @@ int timer_stop(State& S) {
   return 0;
 }
 
+// Injected connection events (already ticketed) into their delivery-ring buckets.
+__global__ void k_inject_events(Dev d, const QEvent* ev, int32_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const QEvent e = ev[i];
+  const int64_t b = (e.call / 2) % d.ring_slots;
+  const unsigned long long pos = atomicAdd(&d.ring_count[b], 1ull);
+  if ((int64_t)pos >= d.ring_cap) {
+    raise_error(d.error, MIND_EQUEUE);
+    return;
+  }
+  d.ring[b * d.ring_cap + (int64_t)pos] = e;
+}
+
+// Before a window of n steps from S.step: injected connection events whose step
+// is inside the delivery ring's horizon go into the ring; injected source spikes
+// due in the window go to the window's exchange. Nothing to do (and no
+// synchronisation) without injected input.
+int stage_injections(State& S, int32_t n) {
+  if (S.pending_events.empty() && S.pending_spikes.empty()) return 0;
+  Dev& d = S.d;
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  std::vector<QEvent> now, later;
+  for (const QEvent& e : S.pending_events) (e.call / 2 - S.step < d.ring_slots - 1 ? now : later).push_back(e);
+  for (size_t a = 0; a < now.size(); a += kInjCap) {
+    const int32_t m = (int32_t)std::min<size_t>(kInjCap, now.size() - a);
+    HIP_TRY(hipMemcpy(S.inj_ev, now.data() + a, sizeof(QEvent) * m, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_inject_events, dim3(blocks_for(m, 256)), dim3(256), 0, S.stream, d, S.inj_ev, m);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(S.stream));  // the staging buffer is reused
+  }
+  S.pending_events.swap(later);
+  const double cut_hi = call_cutoff(2 * (S.step + n) - 1, S.L.dt);
+  std::vector<InjSpike> due, rest;
+  for (const InjSpike& q : S.pending_spikes) (q.t <= cut_hi ? due : rest).push_back(q);
+  if ((int64_t)due.size() > kInjCap) return fail(MIND_EQUEUE, "more injected spikes due in one window than it takes");
+  if (!due.empty()) {
+    const int32_t m = (int32_t)due.size();
+    HIP_TRY(hipMemcpy(d.inj, due.data(), sizeof(InjSpike) * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.inj_count, &m, sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  S.pending_spikes.swap(rest);
+  return 0;
+}
+
 int launch_window(State& S, int32_t n) {
   Dev& d = S.d;
   hipLaunchKernelGGL(k_gather, dim3(1), dim3(kBlock), 0, S.stream, d, n);
@@ -798,12 +861,14 @@ const char* mind_build_info(void) { return mind::kMechanismSource; }
 
 int mind_build(const mind_model_desc* model, const mind_build_options* options, mind_state** out) {
   if (!model || !options || !out) return fail(MIND_EARG, "null argument");
+  if (options->secondorder < 0 || options->secondorder > 2) return fail(MIND_EARG, "secondorder must be 0, 1 or 2");
   State* S = new State();
   std::string err = mind::pack(*model, options->dt, options->seed, S->L);
   if (!err.empty()) {
     delete S;
     return fail(MIND_EARG, err);
   }
+  S->L.secondorder = options->secondorder;
   int rc = mind::setup_state(*S, *options);
   if (!rc && options->method == MIND_METHOD_BATCHED) rc = mind::batched_setup(*S, options->cells_per_group);
   if (rc) {
@@ -822,6 +887,9 @@ int mind_initialize(mind_state* state) {
   HIP_TRY(hipSetDevice(S.device));
   S.initialized = false;
   S.step = 0;
+  S.pending_events.clear();  // injected input does not survive initialize!
+  S.pending_spikes.clear();
+  HIP_TRY(hipMemsetAsync(d.inj_count, 0, sizeof(int32_t), S.stream));
   HIP_TRY(hipMemsetAsync(d.error, 0, sizeof(int32_t), S.stream));
   mind::Clock clock{0, 0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(d.clock, &clock, sizeof(clock), hipMemcpyHostToDevice, S.stream));
@@ -977,6 +1045,7 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
   int64_t left = count;
   while (left > 0) {
     const int32_t n = (int32_t)std::min<int64_t>(left, S.wmax);
+    if ((rc = mind::stage_injections(S, n))) return rc;
     if (n == S.wmax && !S.profiling) {
       if (!S.graph) {
         hipGraph_t g;
@@ -1035,7 +1104,8 @@ int mind_step(mind_state* state) {
     hipGraphExecDestroy(S.graph);
     S.graph = nullptr;
   }
-  int rc = mind::launch_window(S, 1);
+  int rc = mind::stage_injections(S, 1);
+  if (!rc) rc = mind::launch_window(S, 1);
   if (rc) return rc;
   S.step += 1;
   return mind::check_device_error(S);
@@ -1080,6 +1150,101 @@ int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out,
   if (n != rows || field < 0 || field >= mind::field_count(mech)) return fail(MIND_EARG, "size or field mismatch");
   HIP_TRY(hipStreamSynchronize(S.stream));
   HIP_TRY(hipMemcpy(out, S.groups_host[g].data + (size_t)field * rows, sizeof(double) * rows, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// read_value (state.jl:586-600): one resolved scalar, read from the device after
+// the state's stream has drained. Point processes of artificial cells (NetStim)
+// are read from their row as well.
+int mind_read_value(mind_state* state, const mind_record* record, double* out) {
+  if (!state || !record || !out) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  const mind::Layout& L = S.L;
+  int32_t kind = 0, g = -1, field = -1;
+  int64_t index = -1;
+  if (record->kind == 1 && record->point >= 0 && record->point < (int32_t)L.point_group.size() &&
+      L.point_cell[record->point] < 0) {
+    g = L.point_group[record->point];
+    if (record->field < 0 || record->field >= mind::field_count(L.groups[g].mech)) return fail(MIND_EARG, "unknown field");
+    kind = 1;
+    field = record->field;
+    index = L.point_row[record->point];
+  } else {
+    std::vector<int32_t> kv, gv, fv, cv;
+    std::vector<int64_t> iv;
+    const int rc = resolve_records(S, record, 1, kv, gv, fv, iv, cv);
+    if (rc) return rc;
+    kind = kv[0];
+    g = gv[0];
+    field = fv[0];
+    index = iv[0];
+  }
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  const double* src = kind == 0 ? S.d.v + index : S.groups_host[g].data + (size_t)field * L.groups[g].rows + index;
+  HIP_TRY(hipMemcpy(out, src, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// connection_weights (state.jl:603-616): the NET_RECEIVE arguments of a
+// connection (0-based creation index). The six mechanisms take one argument,
+// the weight, and none of them writes it, so it is the built weight.
+int mind_connection_weights(const mind_state* state, int64_t connection, double* out, int32_t capacity,
+                            int32_t* count) {
+  if (!state || !count) return fail(MIND_EARG, "null argument");
+  const State& S = *reinterpret_cast<const State*>(state);
+  const mind::Layout& L = S.L;
+  if (connection < 0 || connection >= (int64_t)L.connection_edge.size()) return fail(MIND_EARG, "unknown connection");
+  *count = 1;
+  if (capacity < 1 || !out) return fail(MIND_EARG, "weight buffer too small");
+  out[0] = L.edge_weight[L.connection_edge[connection]];
+  return 0;
+}
+
+// inject_event! (state.jl:618-636): one event of connection `connection` (0-based
+// creation order) at absolute time `time`, without its delay, ticketed now like
+// any enqueued event; it enters the delivery ring before the window that is
+// the first to reach the ring's horizon at its step.
+int mind_inject_event(mind_state* state, int64_t connection, double time) {
+  if (!state) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  const mind::Layout& L = S.L;
+  if (connection < 0 || connection >= (int64_t)L.connection_edge.size()) return fail(MIND_EARG, "unknown connection");
+  const double now = (double)S.step * L.dt;
+  if (!(std::isfinite(time) && time >= now)) return fail(MIND_EARG, "event arrival precedes the current time");
+  const int64_t edge = L.connection_edge[connection];
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  mind::Clock c;
+  HIP_TRY(hipMemcpy(&c, S.d.clock, sizeof(c), hipMemcpyDeviceToHost));
+  c.ticket += 1;
+  HIP_TRY(hipMemcpy(S.d.clock, &c, sizeof(c), hipMemcpyHostToDevice));
+  S.pending_events.push_back(mind::QEvent{time, c.ticket, mind::call_of(time, 2 * S.step, L.dt), (int32_t)edge, 0});
+  return 0;
+}
+
+// inject_spike! (state.jl:638-659): a spike of external source `source` at
+// absolute time `time`; it is ordered with NetStim events by (time, insertion
+// order), routed over the source's connections at its delivery call and
+// recorded like any spike.
+int mind_inject_spike(mind_state* state, int64_t source, double time) {
+  if (!state) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  const mind::Layout& L = S.L;
+  if (source < 0 || source >= (int64_t)L.source_external.size() || !L.source_external[source])
+    return fail(MIND_EARG, "source is not a built external spike input");
+  const double now = (double)S.step * L.dt;
+  if (!(std::isfinite(time) && time >= now)) return fail(MIND_EARG, "spike time precedes the current time");
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  mind::Clock c;
+  HIP_TRY(hipMemcpy(&c, S.d.clock, sizeof(c), hipMemcpyDeviceToHost));
+  c.ns_seq += 1;
+  HIP_TRY(hipMemcpy(S.d.clock, &c, sizeof(c), hipMemcpyHostToDevice));
+  S.pending_spikes.push_back(mind::InjSpike{time, c.ns_seq, (int32_t)source, 0});
   return 0;
 }
 

@@ -46,6 +46,11 @@ struct State {
   BatchPlan* batch = nullptr;
   MacroHost* macro = nullptr;
   std::vector<int32_t> cell_bucket;  // event bucket of each cell: the cell, or its group
+  // injected input not yet on the device: connection events beyond the delivery
+  // ring's horizon, source spikes due in a later window (inject_event!, inject_spike!)
+  std::vector<QEvent> pending_events;
+  std::vector<InjSpike> pending_spikes;
+  QEvent* inj_ev = nullptr;  // device staging of injected events (kInjCap)
   std::vector<void*> run_allocations;
   // optional timing of the dominant cell-kernel launch (HIP events on `stream`)
   bool profiling = false;
@@ -91,6 +96,7 @@ struct State {
 
 
 int launch_window(State& S, int32_t n);
+int stage_injections(State& S, int32_t n);
 int setup_state(State& S, const mind_build_options& o);
 int check_device_error(State& S);
 int collect_spikes(State& S, int64_t* n, double** times, int64_t** sids);

@@ -90,6 +90,7 @@ struct BuildOptions
     window_steps::Int32    # 0 = longest the minimum delay allows
     event_capacity::Int64  # 0 = automatic
     device::Int32
+    secondorder::Int32     # 0 backward Euler, 1 or 2 Crank-Nicolson
 end
 
 struct RecordSpec
@@ -176,16 +177,18 @@ end
 # ---- calls -----------------------------------------------------------------------
 """
     build_state(arrays; dt_micro=0.025, seed=1, method=:batched, cells_per_group=0,
-                window_steps=0, event_capacity=0, device=0) -> Ptr{Cvoid}
+                window_steps=0, event_capacity=0, device=0, secondorder=0) -> Ptr{Cvoid}
 
 `build!` (src/model/build.jl:120-131): packs and uploads the model; the batched
 method autotunes its cells-per-workgroup plan.
 """
 function build_state(a::ModelArrays; dt_micro = 0.025, seed = 1, method = :batched,
-                     cells_per_group = 0, window_steps = 0, event_capacity = 0, device = 0)
+                     cells_per_group = 0, window_steps = 0, event_capacity = 0, device = 0,
+                     secondorder = 0)
+    secondorder in (0, 1, 2) || throw(ArgumentError("secondorder must be 0, 1 or 2"))
     opts = Ref(BuildOptions(dt_micro, UInt64(seed), method === :batched ? Int32(1) : Int32(0),
                             Int32(cells_per_group), Int32(window_steps), Int64(event_capacity),
-                            Int32(device)))
+                            Int32(device), Int32(secondorder)))
     out = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve a begin
         desc = Ref(descriptor(a))

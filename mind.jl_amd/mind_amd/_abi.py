@@ -87,6 +87,7 @@ class BuildOptions(C.Structure):
         ("window_steps", C.c_int32),
         ("event_capacity", C.c_int64),
         ("device", C.c_int32),
+        ("secondorder", C.c_int32),
     ]
 
 
@@ -159,6 +160,10 @@ def lib():
     L.mind_read_field.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _p(C.c_double), C.c_int64]
     L.mind_row_count.argtypes = [C.c_void_p, C.c_int32, _p(C.c_int64)]
     L.mind_time.argtypes = [C.c_void_p, _p(C.c_double), _p(C.c_int64)]
+    L.mind_read_value.argtypes = [C.c_void_p, _p(Record), _p(C.c_double)]
+    L.mind_inject_event.argtypes = [C.c_void_p, C.c_int64, C.c_double]
+    L.mind_inject_spike.argtypes = [C.c_void_p, C.c_int64, C.c_double]
+    L.mind_connection_weights.argtypes = [C.c_void_p, C.c_int64, _p(C.c_double), C.c_int32, _p(C.c_int32)]
     L.mind_window_steps.argtypes = [C.c_void_p, _p(C.c_int32)]
     L.mind_batch_plan.argtypes = [C.c_void_p, _p(C.c_int64), _p(C.c_int32), _p(C.c_int32), _p(C.c_int64)]
     L.mind_set_batch_plan.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32]

@@ -95,6 +95,7 @@ class MacroResult:
     spike_times: np.ndarray
     spike_sources: np.ndarray
     run_seconds: float
+    traces: dict = field(default_factory=dict)
 
 
 class MacroModel:
@@ -272,6 +273,7 @@ def build(model: MacroModel, dt_micro=0.025, dt_macro=0.1, seed=1, method="batch
     opts.window_steps = 0
     opts.event_capacity = 0
     opts.device = int(device)
+    opts.secondorder = 0
     h = C.c_void_p()
     _abi.check(L.mind_build_macro(C.byref(desc), C.byref(mdesc), C.byref(opts), C.byref(h)))
     del keep, mkeep
@@ -285,11 +287,57 @@ def initialize(model: MacroModel):
     return model
 
 
-def run(model: MacroModel, stop, record_spikes=True) -> MacroResult:
+# ROI quantities of the macro result, per macro step boundary: the REGION's E and I,
+# the population rate its MICRO2MACRO transform publishes, the MACRO2MACRO coupling
+# input and the mean soma voltage of its sources.
+_ROI_TRACES = {"E": 0, "I": 1, "rate": 2, "coupling": 3, "mean_v": 4}
+
+
+def _roi_index(model: MacroModel, region) -> int:
+    if isinstance(region, ROI):
+        if region.model is not model:
+            raise ValueError("ROI belongs to another model")
+        return region.index
+    return model.roi(region).index
+
+
+def roi_record(model: MacroModel, region, exposure, name=None):
+    """roi_record(model, region, exposure) (recording.jl:47-48, roi_ref references.jl:194-200):
+    an ROI exposure (rate, E, I, coupling, mean_v) at every macro step boundary."""
+    from .model import RecordSpec
+    key = str(exposure).lstrip(":")
+    if key not in _ROI_TRACES:
+        raise ValueError(f"unknown exposure {key}")
+    return RecordSpec(name or key, ("roi", _roi_index(model, region), _ROI_TRACES[key]))
+
+
+def mechanism_record(model: MacroModel, target, fname, name=None):
+    """mechanism_record(model, region, field) for the REGION mechanism (WilsonCowan: E, I)
+    and mechanism_record(model, output, field) for a MICRO2MACRO transform (PopRate:
+    rate), recording.jl:24-30, 71-77. Micro selectors: mind_amd.mechanism_record."""
+    from .model import RecordSpec
+    key = str(fname).lstrip(":")
+    if isinstance(target, Micro2Macro):
+        if key != "rate":
+            raise ValueError(f"PopRate has no field {key}")
+        return RecordSpec(name or key, ("roi", target.index, _ROI_TRACES["rate"]))
+    if key not in ("E", "I"):
+        raise ValueError(f"WilsonCowan has no field {key}")
+    return RecordSpec(name or key, ("roi", _roi_index(model, target), _ROI_TRACES[key]))
+
+
+def run(model: MacroModel, stop, records=(), record_spikes=True) -> MacroResult:
+    """run!(model, stop; records) for a macro model: traces of the records per macro step."""
+    for r in records:
+        if not (isinstance(r.spec, tuple) and r.spec and r.spec[0] == "roi"):
+            raise ValueError(f"record {r.name} is not an ROI selector")
     L = _lib()
     res = MacroResultC()
     _abi.check(L.mind_run_macro(model.state, float(stop), int(bool(record_spikes)), C.byref(res)))
     try:
-        return result_from_c(res, len(model.connectivity.labels))
+        out = result_from_c(res, len(model.connectivity.labels))
     finally:
         L.mind_macro_result_free(C.byref(res))
+    arrays = (out.E, out.I, out.rate, out.coupling, out.mean_v)
+    out.traces = {r.name: arrays[r.spec[2]][r.spec[1]].copy() for r in records}
+    return out

@@ -130,6 +130,30 @@ class Cell:
 
 
 @dataclass
+class CellSection:
+    """One section of one cell (reference CellSection, model.jl:73-76)."""
+    cell: Cell
+    name: str
+
+
+def section(cell: Cell, name) -> CellSection:
+    """Select one section by its exact name (model.jl:128-137)."""
+    key = str(name).lstrip(":")
+    cell.model._morph(cell).section_index(key)
+    return CellSection(cell, key)
+
+
+def section_group(cell: Cell, label) -> list:
+    """Sections of one cell with a label (or all), in morphology order (model.jl:115-126)."""
+    key = str(label).lstrip(":")
+    morph = cell.model._morph(cell)
+    names = [n for n, lab in zip(morph.names, morph.labels) if key == "all" or lab == key]
+    if not names:
+        raise ValueError(f"unknown section group {key}")
+    return [CellSection(cell, n) for n in names]
+
+
+@dataclass
 class PointProcess:
     model: "MicroModel"
     index: int
@@ -261,7 +285,10 @@ class MicroModel:
         return SpikeSource(self, index, identity)
 
     def register_spike_source(self, target, sid=None, section="soma", x=0.5, threshold=10.0):
-        """Voltage detector on a cell, or the net_event output of a point process."""
+        """Voltage detector on a cell, the net_event output of a point process, or (target
+        the model itself) an external spike input driven by inject_spike (model.jl:286-365)."""
+        if target is self:
+            return self._new_source((2, -1, -1, 0.0, 0.0, -1), sid, None)
         if isinstance(target, PointProcess):
             existing = self.point_sources.get(target.index)
             if existing is not None:
@@ -414,7 +441,8 @@ def add_pointprocess(target, mechanism, section="soma", x=0.5, params=None):
 
 
 def register_spike_source(target, **kw):
-    return target.model.register_spike_source(target, **kw)
+    model = target if isinstance(target, MicroModel) else target.model
+    return model.register_spike_source(target, **kw)
 
 
 def connect(source, target, weight=1.0, delay=0.0):
@@ -428,13 +456,27 @@ class RecordSpec:
 
 
 def voltage_record(model, cell, section="soma", x=0.5, name=None):
+    """voltage_record(model, cell; section, x) or voltage_record(model, cell_section; x)
+    (recording.jl:4-9, 50-55)."""
+    if isinstance(cell, CellSection):
+        sec = cell
+        s = model._morph(sec.cell).section_index(sec.name)
+        return RecordSpec(name or f"{sec.cell.gid}.{sec.name}.v", (0, sec.cell.index, s, float(x), -1, -1, -1))
     s = model._morph(cell).section_index(section)
     return RecordSpec(name or f"v_{cell.gid}", (0, cell.index, s, float(x), -1, -1, -1))
 
 
 def mechanism_record(model, target, *args, name=None, section="soma", x=0.5):
-    """mechanism_record(model, point, field) or mechanism_record(model, cell, mech, field)."""
+    """mechanism_record(model, point, field), mechanism_record(model, cell, mech, field)
+    or mechanism_record(model, cell_section, mech, field) (recording.jl:10-30, 56-63).
+    ROI and MICRO2MACRO fields: mind_amd.macro.mechanism_record."""
     from .mechanisms import BY_ID
+    if isinstance(target, CellSection):
+        mechanism, fname = args
+        mech = resolve(mechanism)
+        s = model._morph(target.cell).section_index(target.name)
+        return RecordSpec(name or f"{target.cell.gid}.{target.name}.{fname}",
+                          (2, target.cell.index, s, float(x), -1, mech.id, mech.field_index(str(fname))))
     if isinstance(target, PointProcess):
         (fname,) = args
         mech = BY_ID[model.points[target.index][0]]
@@ -447,11 +489,67 @@ def mechanism_record(model, target, *args, name=None, section="soma", x=0.5):
                       (2, target.index, s, float(x), -1, mech.id, mech.field_index(str(fname))))
 
 
+# ion columns of the six mechanisms: hh reads ena/ek and writes ina/ik
+# (USEION na READ ena WRITE ina, USEION k READ ek WRITE ik)
+_ION_FIELDS = {"ena": "ena", "ek": "ek", "ina": "ina", "ik": "ik"}
+
+
+def ion_record(model, target, fname, name=None, section="soma", x=0.5):
+    """ion_record(model, cell, field; section, x) or ion_record(model, cell_section, field; x)
+    (recording.jl:40-46, 64-70; ion_ref, references.jl:157-178). The runtime keeps the
+    ion state of the six mechanisms in hh's rows: the reversal potentials ena/ek it
+    reads and the currents ina/ik it writes (the only writer of those currents).
+    Concentrations and di/dv columns are not stored (no mechanism here reads them)."""
+    key = str(fname).lstrip(":")
+    if key not in _ION_FIELDS:
+        raise ValueError(f"ion field {key} is not stored by this runtime (ena, ek, ina, ik are)")
+    if isinstance(target, CellSection):
+        return mechanism_record(model, target, "hh", _ION_FIELDS[key], name=name or key, x=x)
+    return mechanism_record(model, target, "hh", _ION_FIELDS[key], name=name or key, section=section, x=x)
+
+
+def read_value(model, spec):
+    """read_value(state, selection) (state.jl:586-600): the current value of one record
+    selector, read from the device."""
+    rec = model.record_struct(spec.spec if isinstance(spec, RecordSpec) else spec)
+    out = C.c_double()
+    _abi.check(_abi.lib().mind_read_value(model.state, C.byref(rec), C.byref(out)))
+    return out.value
+
+
+def inject_event(model, connection, time):
+    """inject_event!(state, connection, time) (state.jl:618-636): the connection's event
+    (numbered as connect() returned it) arriving at absolute time `time`, without its delay."""
+    _abi.check(_abi.lib().mind_inject_event(model.state, int(connection) - 1, float(time)))
+    return model
+
+
+def inject_spike(model, source, time):
+    """inject_spike!(state, source, time) (state.jl:638-659): a spike of an external input
+    (register_spike_source(model; sid)) at absolute time `time`, through all its connections."""
+    if source.model is not model:
+        raise ValueError("spike source belongs to another model")
+    _abi.check(_abi.lib().mind_inject_spike(model.state, int(source.index), float(time)))
+    return model
+
+
+def connection_weights(model, connection):
+    """connection_weights(state, connection) (state.jl:603-616): the NET_RECEIVE arguments of
+    the connection numbered as connect() returned it (1-based)."""
+    out = (C.c_double * 8)()
+    n = C.c_int32()
+    _abi.check(_abi.lib().mind_connection_weights(model.state, int(connection) - 1, out, 8, C.byref(n)))
+    return np.array(out[:n.value])
+
+
 def build(model, dt_micro=0.025, seed=1, method="plain", cells_per_group=0, window_steps=0,
-          event_capacity=0, device=0):
-    """Pack the model and allocate device state (build.jl:120-220 + allocate_simulation)."""
+          event_capacity=0, device=0, secondorder=0):
+    """Pack the model and allocate device state (build.jl:120-220 + allocate_simulation).
+    secondorder 1 or 2 selects Crank-Nicolson cable steps as the reference's keyword does."""
     if not (np.isfinite(dt_micro) and dt_micro > 0):
         raise ValueError("dt_micro must be positive")
+    if secondorder not in (0, 1, 2):
+        raise ValueError("secondorder must be 0, 1 or 2")
     L = _abi.lib()
     if model.state is not None:
         L.mind_state_destroy(model.state)
@@ -465,6 +563,7 @@ def build(model, dt_micro=0.025, seed=1, method="plain", cells_per_group=0, wind
     opts.window_steps = int(window_steps)
     opts.event_capacity = int(event_capacity)
     opts.device = int(device)
+    opts.secondorder = int(secondorder)
     handle = C.c_void_p()
     _abi.check(L.mind_build(C.byref(desc), C.byref(opts), C.byref(handle)))
     del keep

@@ -332,6 +332,7 @@ struct Sim {
   // configuration
   double dt = 0.025, celsius = 6.3;
   uint64_t seed = 1;
+  int32_t secondorder = 0;  // build! keyword: 1, 2 Crank-Nicolson (step.jl:112, cuda/micro.jl:660-679)
   // nodes
   int64_t total = 0, compartments = 0;
   std::vector<int32_t> parent;
@@ -364,6 +365,8 @@ struct Sim {
   std::vector<int64_t> target_rank, source_rank;
   std::vector<uint8_t> feedback_group;
   int64_t next_order = 0;
+  std::vector<int64_t> connection_edge;  // edge of each connection in creation order
+  std::vector<int32_t> source_external;  // external spike inputs
   int64_t step = 0;
   int error = 0;
   bool initialized = false;
@@ -675,7 +678,8 @@ void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
 }
 
 void cable(Sim& s) {  // assemble_cable_kernel!, solve_dhs_kernel!, update_voltage_kernel!
-  const double dt = s.dt;
+  // assembly at dt, or dt/2 with secondorder (step.jl:112)
+  const double dt = s.secondorder ? 0.5 * s.dt : s.dt;
   const int64_t n = s.total;
   for (int64_t i = 0; i < n; ++i) {  // each node pulls its own and its children's currents
     s.diag[i] = s.cap[i] / dt + s.diag[i] + s.axial_diag[i];
@@ -702,7 +706,14 @@ void cable(Sim& s) {  // assemble_cable_kernel!, solve_dhs_kernel!, update_volta
     if (p >= 0) value += s.axial[i] * s.rhs[p];
     s.rhs[i] = value / s.diag[i];
   }
-  for (int64_t i = 0; i < n; ++i) s.v[i] += s.rhs[i];
+  // v += dv, or 2 dv with secondorder (update_voltage_kernel!, cuda/micro.jl:660-679; the
+  // secondorder 2 ion-current correction writes per-node ion current columns that none of
+  // the six mechanisms reads)
+  if (s.secondorder) {
+    for (int64_t i = 0; i < n; ++i) s.v[i] += 2.0 * s.rhs[i];
+  } else {
+    for (int64_t i = 0; i < n; ++i) s.v[i] += s.rhs[i];
+  }
 }
 
 void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
@@ -800,6 +811,36 @@ extern "C" {
 const char* oracle_last_error(void) { return g_error.c_str(); }
 
 void oracle_free(void* p) { delete static_cast<Sim*>(p); }
+
+// inject_event! (state.jl:618-636): the connection's event at `time` without its
+// delay, enqueued (ticketed) now.
+int oracle_inject_event(void* p, int64_t connection, double time) {
+  Sim& s = *static_cast<Sim*>(p);
+  if (connection < 0 || connection >= (int64_t)s.connection_edge.size()) return fail(MIND_EARG, "unknown connection");
+  if (!(mind_isfinite(time) && time >= (double)s.step * s.dt))
+    return fail(MIND_EARG, "event arrival precedes the current time");
+  const int64_t e = s.connection_edge[connection];
+  enqueue(s, Event{time, 0.0, e + 1, e + 1, s.edge_target[e]});
+  return 0;
+}
+
+// inject_spike! (state.jl:638-659): a source event of an external input at `time`.
+int oracle_inject_spike(void* p, int64_t source, double time) {
+  Sim& s = *static_cast<Sim*>(p);
+  if (std::find(s.source_external.begin(), s.source_external.end(), (int32_t)source) == s.source_external.end())
+    return fail(MIND_EARG, "source is not a built external spike input");
+  if (!(mind_isfinite(time) && time >= (double)s.step * s.dt))
+    return fail(MIND_EARG, "spike time precedes the current time");
+  enqueue(s, Event{time, 0.0, 0, 0, -(int32_t)(source + 1)});
+  return 0;
+}
+
+// build!(...; secondorder) (build.jl:120-131); before oracle_initialize.
+int oracle_set_secondorder(void* p, int32_t secondorder) {
+  if (secondorder < 0 || secondorder > 2) return fail(MIND_EARG, "secondorder must be 0, 1 or 2");
+  static_cast<Sim*>(p)->secondorder = secondorder;
+  return 0;
+}
 
 // Build (pack_micro, model.jl:420-902, in canonical per-cell contiguous order).
 void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
@@ -949,6 +990,8 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
       s.det_node.push_back((int32_t)cell_node(d->src_cell[k], d->src_section[k], d->src_x[k], false));
       s.det_source.push_back((int32_t)k);
       s.det_threshold.push_back(d->src_threshold[k]);
+    } else if (d->src_kind[k] == 2) {  // external input: driven by inject_spike! only
+      s.source_external.push_back((int32_t)k);
     } else {
       s.source_target[k] = d->src_point[k] + 1;
       s.target_sources[d->src_point[k]].push_back((int32_t)k);
@@ -962,8 +1005,10 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
   s.edge_target.resize(d->n_connections);
   s.edge_delay.resize(d->n_connections);
   s.edge_weight.resize(d->n_connections);
+  s.connection_edge.resize(d->n_connections);
   for (int64_t c = 0; c < d->n_connections; ++c) {
     const int64_t e = cursor[d->con_source[c]]++;
+    s.connection_edge[c] = e;
     s.edge_target[e] = d->con_target[c] + 1;
     s.edge_delay[e] = d->con_delay[c];
     s.edge_weight[e] = d->con_weight[c];

@@ -29,6 +29,9 @@ def lib():
         L.oracle_build.restype = C.c_void_p
         L.oracle_build.argtypes = [P(_abi.ModelDesc), C.c_double, C.c_uint64]
         L.oracle_free.argtypes = [C.c_void_p]
+        L.oracle_set_secondorder.argtypes = [C.c_void_p, C.c_int32]
+        L.oracle_inject_event.argtypes = [C.c_void_p, C.c_int64, C.c_double]
+        L.oracle_inject_spike.argtypes = [C.c_void_p, C.c_int64, C.c_double]
         L.oracle_initialize.argtypes = [C.c_void_p]
         L.oracle_run.argtypes = [C.c_void_p, C.c_double, P(_abi.Record), C.c_int64, C.c_int32,
                                  P(_abi.Result)]
@@ -62,7 +65,7 @@ def lib():
 class OracleSim:
     """The oracle's view of one MicroModel (same descriptor as libmind)."""
 
-    def __init__(self, model, dt=0.025, seed=1):
+    def __init__(self, model, dt=0.025, seed=1, secondorder=0):
         desc, keep = model.descriptor()
         self.model = model
         self.dt = dt
@@ -70,6 +73,8 @@ class OracleSim:
         del keep
         if not self.h:
             raise RuntimeError("oracle build failed")
+        if secondorder and lib().oracle_set_secondorder(self.h, int(secondorder)):
+            raise ValueError(lib().oracle_last_error().decode())
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -99,10 +104,28 @@ class OracleSim:
         return SimulationResult(time, {r.name: traces[i] for i, r in enumerate(records)}, st, ss,
                                 secs)
 
+    def inject_event(self, connection, time):
+        """inject_event! with the 1-based connection number connect() returned."""
+        if lib().oracle_inject_event(self.h, int(connection) - 1, float(time)):
+            raise ValueError(lib().oracle_last_error().decode())
+        return self
+
+    def inject_spike(self, source, time):
+        if lib().oracle_inject_spike(self.h, int(source.index), float(time)):
+            raise ValueError(lib().oracle_last_error().decode())
+        return self
+
     def counts(self):
         n, s = C.c_int64(), C.c_int64()
         lib().oracle_node_count(self.h, C.byref(n), C.byref(s))
         return n.value, s.value
+
+    def point_row(self, point):
+        """Row of a point process in its mechanism's data."""
+        out = C.c_int32()
+        if lib().oracle_point_row(self.h, int(point.index), C.byref(out)):
+            raise ValueError(lib().oracle_last_error().decode())
+        return out.value
 
     def voltage(self):
         n, _ = self.counts()

@@ -1,0 +1,100 @@
+"""The state API beyond run!/step! through libmind on cuda:0, against the oracle:
+read_value, connection_weights, section/ion selectors, and injected events and
+spikes (inject_event!, inject_spike!) across split runs, in both kernels."""
+import numpy as np
+import pytest
+
+import oracle
+from mind_amd import (build, configs, connection_weights, destroy, initialize, inject_event, inject_spike,
+                      ion_record, mechanism_record, read_value, run, section, voltage_record)
+from mind_amd.mechanisms import builtin
+
+pytestmark = pytest.mark.gpu
+
+
+def _network():
+    """A small ball-and-stick network with NetStim drive, recurrent connections and an
+    external spike input (SID 900) connected to four synapses with delays 1-4 ms."""
+    model, info = configs.ball_stick_network(ncells=20, indegree=5, rate_hz=40.0, netstim_weight=2e-2)
+    ext = model.register_spike_source(model, sid=900)
+    conns = [model.connect(ext, info["syns"][k], weight=3e-2, delay=1.0 + k) for k in range(4)]
+    return model, info, ext, conns
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_injected_events_and_spikes_match_oracle(gpu, method):
+    model, info, ext, conns = _network()
+    cells = info["cells"]
+    recs = [voltage_record(model, c, "soma", 0.5) for c in cells[:4]]
+    recs.append(mechanism_record(model, info["syns"][1], "B"))
+    print(f"case: injection {method}", flush=True)
+    build(model, dt_micro=0.025, seed=1, method=method)
+    initialize(model)
+    sim = oracle.OracleSim(model).initialize()
+    got, ref = [], []
+    # two spikes in one window, one on a step boundary, one beyond the delivery
+    # ring's horizon; events near and far, one at the current time after a run
+    plan = [(10.0, [("s", 12.3), ("s", 12.6), ("s", 15.0), ("s", 150.0), ("e", conns[0], 11.0),
+                    ("e", conns[2], 100.0), ("e", conns[3], 10.0)]),
+            (60.0, [("s", 60.0), ("e", conns[1], 60.0)]),
+            (200.0, [])]
+    for stop, injections in plan:
+        got.append(run(model, stop, recs))
+        ref.append(sim.run(stop, recs))
+        for inj in injections:
+            if inj[0] == "s":
+                inject_spike(model, ext, inj[1])
+                sim.inject_spike(ext, inj[1])
+            else:
+                inject_event(model, inj[1], inj[2])
+                sim.inject_event(inj[1], inj[2])
+    for g, r in zip(got, ref):
+        for rec in recs:
+            assert np.array_equal(g.traces[rec.name], r.traces[rec.name]), rec.name
+        assert np.array_equal(g.spike_times, r.spike_times)
+        assert np.array_equal(g.spike_sources, r.spike_sources)
+    raster = np.concatenate([g.spike_times[g.spike_sources == 900] for g in got])
+    assert raster.tolist() == [12.3, 12.6, 15.0, 60.0, 150.0]
+    with pytest.raises(Exception):
+        inject_spike(model, ext, 100.0)  # before the current time
+    # initialize clears pending input
+    inject_spike(model, ext, 250.0)
+    initialize(model)
+    again = run(model, 10.0, recs)
+    assert np.array_equal(again.traces[recs[0].name], got[0].traces[recs[0].name])
+    destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_read_value_and_selectors_match_oracle(gpu, method):
+    model, info, ext, conns = _network()
+    cell = info["cells"][3]
+    recs = [voltage_record(model, section(cell, "dend"), x=0.3),
+            mechanism_record(model, section(cell, "soma"), "hh", "n"),
+            ion_record(model, cell, "ina"), ion_record(model, section(cell, "soma"), "ek", name="ek"),
+            mechanism_record(model, info["syns"][3], "A")]
+    assert recs[0].name == f"{cell.gid}.dend.v" and recs[1].name == f"{cell.gid}.soma.n"
+    print(f"case: selectors {method}", flush=True)
+    build(model, dt_micro=0.025, seed=1, method=method)
+    initialize(model)
+    got = run(model, 30.0, recs)
+    sim = oracle.OracleSim(model).initialize()
+    ref = sim.run(30.0, recs)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    assert np.all(got.traces["ek"] == -77.0)
+    # read_value: the current state, voltage and fields
+    v = sim.voltage()
+    node = sim.location_node(cell.index, 1, 0.3)
+    assert read_value(model, recs[0]) == v[node]
+    syn = info["syns"][3]
+    a = sim.field(builtin("Exp2Syn").id, builtin("Exp2Syn").field_index("A"), len(info["syns"]))
+    assert read_value(model, recs[4]) == a[sim.point_row(syn)]
+    ns = info["stims"][5]
+    ev = mechanism_record(model, ns, "event")
+    nsf = sim.field(builtin("NetStim").id, builtin("NetStim").field_index("event"), len(info["stims"]))
+    assert read_value(model, ev) == nsf[sim.point_row(ns)]
+    # connection weights: the built weight of each connection
+    assert connection_weights(model, conns[2]).tolist() == [3e-2]
+    assert connection_weights(model, 1).tolist() == [model.connections[0][2]]
+    destroy(model)

@@ -17,7 +17,7 @@ def _compare(model, stop, records, method="plain", plan=None, **kw):
         assert batch_plan(model)["layout"] == plan[2]
     initialize(model)
     got = run(model, stop, records)
-    ref_sim = oracle.OracleSim(model).initialize()
+    ref_sim = oracle.OracleSim(model, secondorder=kw.get("secondorder", 0)).initialize()
     ref = ref_sim.run(stop, records)
     for r in records:
         a, b = got.traces[r.name], ref.traces[r.name]
@@ -141,4 +141,20 @@ def test_trees_oversized_cells_batched(gpu, plan):
     recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"]]
     _, _, vg, vr, _ = _compare(model, 5.0, recs, method="batched", cells_per_group=1, plan=plan)
     assert np.array_equal(vg, vr)
+    destroy(model)
+
+
+@pytest.mark.parametrize("secondorder", [1, 2])
+@pytest.mark.parametrize("method,plan", [("plain", None), ("batched", (300, 128, 0)), ("batched", (600, 64, 4)),
+                                         ("batched", (3000, 256, 5))])
+def test_secondorder_matches_oracle(gpu, secondorder, method, plan):
+    """Crank-Nicolson steps (build! secondorder 1 and 2: assembly at dt/2, v += 2 dv)
+    in the plain kernel and every batched layout: bit-identical to the oracle."""
+    model, info = configs.tree_network(ncells=30, median=120, lo=20, hi=600, indegree=10,
+                                       rate_hz=50.0, netstim_weight=3e-2)
+    recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"][:3]]
+    got, ref, vg, vr, _ = _compare(model, 40.0, recs, method=method, plan=plan, secondorder=secondorder)
+    assert np.array_equal(vg, vr)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
     destroy(model)

@@ -160,6 +160,21 @@ def test_isopotential_decay_is_backward_euler():
     np.testing.assert_allclose(r.traces["v_0"], expect, atol=1e-9)
 
 
+def test_isopotential_decay_secondorder_is_crank_nicolson():
+    """secondorder 1 and 2 (assembly at dt/2, v += 2 dv): each step contracts v - e
+    by (1 - a)/(1 + a) with a = dt/(2 tau), the trapezoidal rule's factor."""
+    for so in (1, 2):
+        m, cell = passive_cell()
+        rec = [voltage_record(m, cell)]
+        r = oracle.OracleSim(m, secondorder=so).initialize().run(50.0, rec)
+        n = np.arange(len(r.time))
+        a = 0.025 / (2.0 * 10.0)
+        expect = -65.0 + 5.0 * ((1.0 - a) / (1.0 + a)) ** n
+        np.testing.assert_allclose(r.traces["v_0"], expect, atol=1e-9)
+    with pytest.raises(ValueError):
+        oracle.OracleSim(passive_cell()[0], secondorder=3)
+
+
 def test_rest_is_a_fixed_point():
     m, cell = passive_cell(v0=-65.0, nseg=5, length=200.0, diameter=2.0)
     sim = oracle.OracleSim(m).initialize()
@@ -231,6 +246,60 @@ def test_exp2syn_peak_is_weight_and_decay_is_exact():
     k = int(np.argmax(B))
     ratio = B[k + 10:k + 200] / B[k + 9:k + 199]
     np.testing.assert_allclose(ratio, math.exp(-0.025 / 5.0), rtol=1e-13)
+
+
+def _exp2syn_closed_form(t, t0, w, tau1=0.5, tau2=5.0):
+    """Exp2Syn state difference B - A = w * factor * (exp(-(t-t0)/tau2) - exp(-(t-t0)/tau1))
+    after an event of weight w applied at t0 (Exp2Syn.mod:56-85), zero before. (The
+    recorded `g` is this value from the last current evaluation, a step behind.)"""
+    tp = tau1 * tau2 / (tau2 - tau1) * math.log(tau2 / tau1)
+    factor = 1.0 / (-math.exp(-tp / tau1) + math.exp(-tp / tau2))
+    dt_ = np.maximum(t - t0, 0.0)
+    return np.where(t > t0, w * factor * (np.exp(-dt_ / tau2) - np.exp(-dt_ / tau1)), 0.0)
+
+
+def test_injected_event_is_applied_at_its_time():
+    """inject_event!: the connection's event arrives at the given time without its delay,
+    applied at the first delivery whose cutoff reaches it (here the end of the step
+    ending at 5.0 ms), after which the conductance follows the closed form."""
+    m, cell = passive_cell(v0=-65.0)
+    syn = m.add_pointprocess(cell, "Exp2Syn", params={"tau1": 0.5, "tau2": 5.0, "e": 0.0})
+    src = m.register_spike_source(m, sid=7)
+    w = 2e-3
+    conn = m.connect(src, syn, weight=w, delay=3.0)
+    recs = [mechanism_record(m, syn, "A"), mechanism_record(m, syn, "B")]
+    sim = oracle.OracleSim(m).initialize()
+    with pytest.raises(ValueError):
+        sim.inject_event(conn + 1, 1.0)  # unknown connection
+    sim.inject_event(conn, 5.0)
+    r = sim.run(30.0, recs)
+    expect = _exp2syn_closed_form(r.time, 5.0, w)
+    np.testing.assert_allclose(r.traces["B"] - r.traces["A"], expect, rtol=1e-9, atol=1e-15)
+    assert len(r.spike_times) == 0
+    with pytest.raises(ValueError):
+        sim.inject_event(conn, 29.0)  # before the current time
+
+
+def test_injected_spike_routes_through_every_connection():
+    """inject_spike!: an external input's spike at 3 ms is recorded with its SID and
+    reaches each target after that connection's delay."""
+    m, cell = passive_cell(v0=-65.0)
+    s1 = m.add_pointprocess(cell, "Exp2Syn", params={"tau1": 0.5, "tau2": 5.0, "e": 0.0})
+    s2 = m.add_pointprocess(cell, "Exp2Syn", params={"tau1": 0.5, "tau2": 5.0, "e": 0.0})
+    src = m.register_spike_source(m, sid=42)
+    m.connect(src, s1, weight=1e-3, delay=1.0)
+    m.connect(src, s2, weight=4e-3, delay=2.5)
+    recs = [mechanism_record(m, p, f, name=f"{f}{k}") for k, p in ((1, s1), (2, s2)) for f in ("A", "B")]
+    sim = oracle.OracleSim(m).initialize()
+    sim.inject_spike(src, 3.0)
+    r = sim.run(30.0, recs)
+    assert r.spike_times.tolist() == [3.0] and r.spike_sources.tolist() == [42]
+    g1, g2 = (r.traces[f"B{k}"] - r.traces[f"A{k}"] for k in (1, 2))
+    np.testing.assert_allclose(g1, _exp2syn_closed_form(r.time, 4.0, 1e-3), rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(g2, _exp2syn_closed_form(r.time, 5.5, 4e-3), rtol=1e-9, atol=1e-15)
+    det = m.register_spike_source(cell)
+    with pytest.raises(ValueError):
+        oracle.OracleSim(m).initialize().inject_spike(det, 1.0)  # not an external input
 
 
 def test_netstim_regular_train():
