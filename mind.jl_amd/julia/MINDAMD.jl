@@ -19,7 +19,8 @@ been executed; the Python layer (`mind_amd`) makes the same calls and is tested.
 module MINDAMD
 
 export ModelArrays, MindError, build_state, initialize_state!, run_state!, step_state!,
-       destroy_state!, read_voltage, voltage_record_spec, field_record_spec
+       destroy_state!, read_voltage, voltage_record_spec, field_record_spec, read_value,
+       connection_weights, inject_event!, inject_spike!
 
 const libmind = get(ENV, "MIND_LIBRARY", joinpath(@__DIR__, "..", "csrc", "libmind.so"))
 
@@ -236,6 +237,32 @@ function run_state!(s::Ptr{Cvoid}, stop::Real; records::Vector{RecordSpec} = Rec
         ccall((:mind_result_free, libmind), Cvoid, (Ptr{ResultC},), res)
     end
 end
+
+"`read_value` (src/runtime/state.jl:586-600): the current value of one record selector."
+function read_value(s::Ptr{Cvoid}, spec::RecordSpec)
+    out = Ref{Float64}(0.0)
+    check(ccall((:mind_read_value, libmind), Cint, (Ptr{Cvoid}, Ref{RecordSpec}, Ptr{Float64}), s, spec, out))
+    return out[]
+end
+
+"`connection_weights` (src/runtime/state.jl:603-616) of the 1-based connection number."
+function connection_weights(s::Ptr{Cvoid}, connection::Integer)
+    out = zeros(8)
+    n = Ref{Int32}(0)
+    check(ccall((:mind_connection_weights, libmind), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Int32, Ptr{Int32}),
+                s, Int64(connection) - 1, out, Int32(length(out)), n))
+    return out[1:n[]]
+end
+
+"`inject_event!` (src/runtime/state.jl:618-636): 1-based connection, absolute arrival time."
+inject_event!(s::Ptr{Cvoid}, connection::Integer, time::Real) =
+    check(ccall((:mind_inject_event, libmind), Cint, (Ptr{Cvoid}, Int64, Float64), s, Int64(connection) - 1,
+                Float64(time)))
+
+"`inject_spike!` (src/runtime/state.jl:638-659): 1-based external source index, absolute time."
+inject_spike!(s::Ptr{Cvoid}, source::Integer, time::Real) =
+    check(ccall((:mind_inject_spike, libmind), Cint, (Ptr{Cvoid}, Int64, Float64), s, Int64(source) - 1,
+                Float64(time)))
 
 "Membrane voltage of every node in canonical node order."
 function read_voltage(s::Ptr{Cvoid})

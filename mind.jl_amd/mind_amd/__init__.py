@@ -13,3 +13,4 @@ from .model import (  # noqa: F401
     mechanism_record, read_field, read_voltage, register_spike_source, run, step,
     voltage_record, window_steps,
 )
+from .swc import load_swc  # noqa: F401,E402
