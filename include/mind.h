@@ -271,6 +271,16 @@ int mind_inject_event(mind_state* state, int64_t connection, double time);
  * and recorded at its delivery call. At most 256 injected spikes may fall in one
  * window. Pending injections are cleared by mind_initialize. */
 int mind_inject_spike(mind_state* state, int64_t source, double time);
+
+/* Checkpoint and restore of a micro state between runs. No counterpart in the
+ * reference; they serve split runs across processes. mind_checkpoint writes the
+ * state a run continues from (voltages, mechanism data, detector flags, NetStim
+ * streams and self-events, clock, queued events, pending injections) into
+ * buffer[0..capacity) and sets *size to its length (buffer NULL: only the size).
+ * mind_restore loads one into a state built from the same model and options;
+ * the next run continues bit for bit. Micro-macro states are rejected. */
+int mind_checkpoint(mind_state* state, void* buffer, size_t capacity, size_t* size);
+int mind_restore(mind_state* state, const void* buffer, size_t size);
 int mind_window_steps(const mind_state* state, int32_t* steps);
 /* Batched method: node budget per workgroup, block size, node-state layout of those
  * workgroups (0 all in LDS; 5 lean: rhs, pivots and v in LDS, the solve's node

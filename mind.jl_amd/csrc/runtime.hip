@@ -1248,6 +1248,188 @@ int mind_inject_spike(mind_state* state, int64_t source, double time) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Checkpoint blob: a header, then the state a run continues from, in this order:
+// v; each mechanism group's data; detector flags; the NetStim rows' streams,
+// pending self-events and flags; the clock; the non-empty delivery-ring buckets
+// (index, count, events); injected input not yet on the device. Scratch that
+// every step rebuilds (rhs, diag, window lists, emissions) is not saved.
+constexpr char kCkptMagic[8] = {'M', 'I', 'N', 'D', 'C', 'K', 'P', '1'};
+
+struct CkptHeader {
+  char magic[8];
+  uint64_t fingerprint;  // of the packed model and the ring's shape
+  int64_t step;
+  int64_t bytes;         // the whole blob
+};
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ull;
+  return h;
+}
+
+uint64_t fingerprint(const State& S) {
+  const mind::Layout& L = S.L;
+  uint64_t h = 0xcbf29ce484222325ull;
+  h = fnv(h, &L.nodes, sizeof L.nodes);
+  h = fnv(h, &L.dt, sizeof L.dt);
+  h = fnv(h, &L.secondorder, sizeof L.secondorder);
+  h = fnv(h, L.parent.data(), L.parent.size() * sizeof(int32_t));
+  h = fnv(h, L.axial.data(), L.axial.size() * sizeof(double));
+  for (const auto& g : L.groups) {
+    h = fnv(h, &g.mech, sizeof g.mech);
+    h = fnv(h, &g.rows, sizeof g.rows);
+  }
+  h = fnv(h, L.edge_target.data(), L.edge_target.size() * sizeof(int32_t));
+  h = fnv(h, L.edge_delay.data(), L.edge_delay.size() * sizeof(double));
+  h = fnv(h, &S.d.ndet, sizeof S.d.ndet);
+  h = fnv(h, &S.d.ring_slots, sizeof S.d.ring_slots);
+  h = fnv(h, &S.d.ring_cap, sizeof S.d.ring_cap);
+  return h;
+}
+
+// The device arrays of a checkpoint as (pointer, bytes) pairs, in blob order.
+std::vector<std::pair<void*, size_t>> ckpt_arrays(State& S) {
+  mind::Dev& d = S.d;
+  std::vector<std::pair<void*, size_t>> a;
+  a.push_back({d.v, sizeof(double) * S.L.nodes});
+  for (size_t gi = 0; gi < S.groups_host.size(); ++gi)
+    a.push_back({S.groups_host[gi].data, sizeof(double) * S.L.groups[gi].data.size()});
+  a.push_back({d.det_above, sizeof(uint8_t) * d.ndet});
+  a.push_back({d.ns_stream, sizeof(mind::Stream) * d.ns_rows});
+  a.push_back({d.ns_t, sizeof(double) * d.ns_rows});
+  a.push_back({d.ns_flag, sizeof(double) * d.ns_rows});
+  a.push_back({d.ns_seq, sizeof(uint64_t) * d.ns_rows});
+  a.push_back({d.ns_edge, sizeof(int32_t) * d.ns_rows});
+  a.push_back({d.ns_valid, sizeof(int32_t) * d.ns_rows});
+  a.push_back({d.clock, sizeof(mind::Clock)});
+  return a;
+}
+
+template <typename T>
+void put(std::vector<uint8_t>& b, const T* p, size_t n) {
+  const uint8_t* c = reinterpret_cast<const uint8_t*>(p);
+  b.insert(b.end(), c, c + sizeof(T) * n);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Checkpoint of a micro state between runs (no counterpart in the reference).
+int mind_checkpoint(mind_state* state, void* buffer, size_t capacity, size_t* size) {
+  if (!state || !size) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
+  if (S.macro) return fail(MIND_EUNSUPPORTED, "checkpoints of micro-macro states are not supported");
+  mind::Dev& d = S.d;
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  std::vector<uint8_t> blob(sizeof(CkptHeader));
+  for (const auto& [p, n] : ckpt_arrays(S)) {
+    const size_t at = blob.size();
+    blob.resize(at + n);
+    if (n) HIP_TRY(hipMemcpy(blob.data() + at, p, n, hipMemcpyDeviceToHost));
+  }
+  std::vector<unsigned long long> counts(d.ring_slots);
+  HIP_TRY(hipMemcpy(counts.data(), d.ring_count, sizeof(unsigned long long) * d.ring_slots, hipMemcpyDeviceToHost));
+  std::vector<mind::QEvent> ev;
+  for (int64_t b = 0; b < d.ring_slots; ++b) {
+    const int64_t n = (int64_t)std::min<unsigned long long>(counts[b], (unsigned long long)d.ring_cap);
+    if (!n) continue;
+    ev.resize(n);
+    HIP_TRY(hipMemcpy(ev.data(), d.ring + b * d.ring_cap, sizeof(mind::QEvent) * n, hipMemcpyDeviceToHost));
+    put(blob, &b, 1);
+    put(blob, &n, 1);
+    put(blob, ev.data(), n);
+  }
+  const int64_t end = -1, npe = (int64_t)S.pending_events.size(), nps = (int64_t)S.pending_spikes.size();
+  put(blob, &end, 1);
+  put(blob, &npe, 1);
+  put(blob, S.pending_events.data(), npe);
+  put(blob, &nps, 1);
+  put(blob, S.pending_spikes.data(), nps);
+  CkptHeader h;
+  std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
+  h.fingerprint = fingerprint(S);
+  h.step = S.step;
+  h.bytes = (int64_t)blob.size();
+  std::memcpy(blob.data(), &h, sizeof h);
+  *size = blob.size();
+  if (!buffer) return 0;
+  if (capacity < blob.size()) return fail(MIND_EARG, "checkpoint buffer too small");
+  std::memcpy(buffer, blob.data(), blob.size());
+  return 0;
+}
+
+// Restore into a state built from the same model with the same options.
+int mind_restore(mind_state* state, const void* buffer, size_t size) {
+  if (!state || !buffer) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  if (S.macro) return fail(MIND_EUNSUPPORTED, "checkpoints of micro-macro states are not supported");
+  const uint8_t* b = static_cast<const uint8_t*>(buffer);
+  CkptHeader h;
+  if (size < sizeof h) return fail(MIND_EARG, "not a checkpoint");
+  std::memcpy(&h, b, sizeof h);
+  if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) || h.bytes != (int64_t)size)
+    return fail(MIND_EARG, "not a checkpoint, or a truncated one");
+  if (h.fingerprint != fingerprint(S)) return fail(MIND_EARG, "checkpoint of another model or build");
+  mind::Dev& d = S.d;
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  size_t at = sizeof h;
+  auto take = [&](void* dst, size_t n) -> bool {
+    if (at + n > size) return false;
+    std::memcpy(dst, b + at, n);
+    at += n;
+    return true;
+  };
+  for (const auto& [p, n] : ckpt_arrays(S)) {
+    if (at + n > size) return fail(MIND_EARG, "truncated checkpoint");
+    if (n) HIP_TRY(hipMemcpy(p, b + at, n, hipMemcpyHostToDevice));
+    at += n;
+  }
+  std::vector<unsigned long long> counts(d.ring_slots, 0);
+  std::vector<mind::QEvent> ev;
+  for (;;) {
+    int64_t bucket = 0, n = 0;
+    if (!take(&bucket, sizeof bucket)) return fail(MIND_EARG, "truncated checkpoint");
+    if (bucket < 0) break;
+    if (bucket >= d.ring_slots || !take(&n, sizeof n) || n <= 0 || n > d.ring_cap)
+      return fail(MIND_EARG, "corrupt checkpoint");
+    ev.resize(n);
+    if (!take(ev.data(), sizeof(mind::QEvent) * n)) return fail(MIND_EARG, "truncated checkpoint");
+    HIP_TRY(hipMemcpy(d.ring + bucket * d.ring_cap, ev.data(), sizeof(mind::QEvent) * n, hipMemcpyHostToDevice));
+    counts[bucket] = (unsigned long long)n;
+  }
+  int64_t npe = 0, nps = 0;
+  std::vector<mind::QEvent> pe;
+  std::vector<mind::InjSpike> ps;
+  if (!take(&npe, sizeof npe) || npe < 0) return fail(MIND_EARG, "corrupt checkpoint");
+  pe.resize(npe);
+  if (!take(pe.data(), sizeof(mind::QEvent) * npe) || !take(&nps, sizeof nps) || nps < 0)
+    return fail(MIND_EARG, "corrupt checkpoint");
+  ps.resize(nps);
+  if (!take(ps.data(), sizeof(mind::InjSpike) * nps) || at != size) return fail(MIND_EARG, "corrupt checkpoint");
+  HIP_TRY(hipMemcpy(d.ring_count, counts.data(), sizeof(unsigned long long) * d.ring_slots, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(d.inj_count, 0, sizeof(int32_t)));
+  HIP_TRY(hipMemset(d.error, 0, sizeof(int32_t)));
+  HIP_TRY(hipMemset(d.spike_count, 0, sizeof(unsigned long long)));
+  S.pending_events = std::move(pe);
+  S.pending_spikes = std::move(ps);
+  S.step = h.step;
+  S.initialized = true;
+  if (S.graph) {
+    (void)hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+  }
+  return 0;
+}
+
 int mind_time(const mind_state* state, double* t, int64_t* step) {
   const State& S = *reinterpret_cast<const State*>(state);
   *step = S.step;

@@ -7,7 +7,8 @@ build, initialize, run, step, *_record) over libmind's C ABI.
 from .mechanisms import Mechanism, builtin  # noqa: F401
 from .model import (  # noqa: F401
     Cell, CellSection, MicroModel, Morphology, PointProcess, Section, SimulationResult, SpikeSource,
-    add_pointprocess, add_population, batch_plan, build, connect, connection_weights, destroy,
+    add_pointprocess, add_population, batch_plan, build, checkpoint, connect, connection_weights, destroy,
+    restore,
     initialize, inject_event, inject_spike, insert, ion_record, read_value, section, section_group,
     set_batch_plan,
     mechanism_record, read_field, read_voltage, register_spike_source, run, step,

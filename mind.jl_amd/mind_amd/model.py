@@ -533,6 +533,23 @@ def inject_spike(model, source, time):
     return model
 
 
+def checkpoint(model) -> bytes:
+    """The state a run continues from, as bytes (restore() loads it into a state built
+    from the same model, in this or another process)."""
+    L = _abi.lib()
+    n = C.c_size_t()
+    _abi.check(L.mind_checkpoint(model.state, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    _abi.check(L.mind_checkpoint(model.state, buf, n.value, C.byref(n)))
+    return buf.raw[:n.value]
+
+
+def restore(model, data: bytes):
+    """Load a checkpoint into the built state of the same model; the next run continues it."""
+    _abi.check(_abi.lib().mind_restore(model.state, bytes(data), len(data)))
+    return model
+
+
 def connection_weights(model, connection):
     """connection_weights(state, connection) (state.jl:603-616): the NET_RECEIVE arguments of
     the connection numbered as connect() returned it (1-based)."""

@@ -5,8 +5,9 @@ import numpy as np
 import pytest
 
 import oracle
-from mind_amd import (build, configs, connection_weights, destroy, initialize, inject_event, inject_spike,
-                      ion_record, mechanism_record, read_value, run, section, voltage_record)
+from mind_amd import (build, checkpoint, configs, connection_weights, destroy, initialize, inject_event,
+                      inject_spike, ion_record, mechanism_record, read_value, read_voltage, restore, run,
+                      section, voltage_record)
 from mind_amd.mechanisms import builtin
 
 pytestmark = pytest.mark.gpu
@@ -98,3 +99,44 @@ def test_read_value_and_selectors_match_oracle(gpu, method):
     assert connection_weights(model, conns[2]).tolist() == [3e-2]
     assert connection_weights(model, 1).tolist() == [model.connections[0][2]]
     destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_checkpoint_resumes_bit_for_bit(gpu, method):
+    """A checkpoint taken between runs (with events in flight and injected input still
+    pending) restored into a second state of the same model continues exactly as the
+    original; other models and truncated blobs are rejected."""
+    def records(model, info):
+        return [voltage_record(model, c, "soma", 0.5) for c in info["cells"][:3]] + \
+               [mechanism_record(model, info["syns"][2], "A")]
+
+    model, info, ext, conns = _network()
+    recs = records(model, info)
+    print(f"case: checkpoint {method}", flush=True)
+    build(model, dt_micro=0.025, seed=1, method=method)
+    initialize(model)
+    run(model, 20.0, recs)
+    inject_spike(model, ext, 150.0)
+    inject_event(model, conns[2], 90.0)
+    blob = checkpoint(model)
+    a = run(model, 200.0, recs)
+    va = read_voltage(model)
+
+    model2, info2, ext2, _ = _network()
+    recs2 = records(model2, info2)
+    build(model2, dt_micro=0.025, seed=1, method=method)
+    restore(model2, blob)
+    b = run(model2, 200.0, recs2)
+    for r, r2 in zip(recs, recs2):
+        assert np.array_equal(a.traces[r.name], b.traces[r2.name]), r.name
+    assert np.array_equal(a.spike_times, b.spike_times) and np.array_equal(a.spike_sources, b.spike_sources)
+    assert 150.0 in b.spike_times.tolist()
+    assert np.array_equal(va, read_voltage(model2))
+    with pytest.raises(Exception):
+        restore(model2, blob[:-8])
+    other, _ = configs.single_cell()
+    build(other, dt_micro=0.025, seed=1, method=method)
+    with pytest.raises(Exception):
+        restore(other, blob)
+    for m in (model, model2, other):
+        destroy(m)
