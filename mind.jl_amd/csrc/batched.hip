@@ -129,8 +129,11 @@ constexpr size_t kLdsResident = 152 * 1024;
 // elimination pass offsets start in the block (entry W: their end), entries
 // W+1..2W+1 the same for its back-substitution passes, each entry two int16 words
 // (low, high 16 bits); then per wave its pass offsets, positions in oh (od) where
-// each pass starts, plus the end. A pass is at most 64 nodes of one level (a
+// each pass starts, plus the end. A pass is at most 64 entries of one level (a
 // wider level takes several passes), and a wave's passes run in level order.
+// Lean groups have two more sections, 2W+2..3W+2 and 3W+3..4W+3: the elimination
+// and back-substitution passes of stage 2 (the top of a cell split across waves,
+// entries after stage 1's in oh and od); the first two sections then hold stage 1.
 __host__ __device__ inline int sched_at(const int16_t* s, int k) {
   return (int)(uint16_t)s[2 * k] | ((int)s[2 * k + 1] << 16);
 }
@@ -414,88 +417,117 @@ __device__ __forceinline__ bool bad_pivot(double dd) {
 // Passes of record loads in flight in the lean solve.
 constexpr int kAhead = 3;
 
+// Elimination passes of one schedule section (header entries from hdr0) for this
+// wave, from the records hr. Returns true if a stored pivot is zero or not finite.
+template <typename C>
+__device__ __forceinline__ bool elim_section(const Dev& d, const HRec* hr, int64_t n0, int nn, const int16_t* sched,
+                                             int hdr0, C rhs, C diag) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int h0 = sched_at(sched, hdr0 + w), nph = sched_at(sched, hdr0 + w + 1) - h0 - 1;
+  bool bad = false;
+  if (nph <= 0) return false;
+  const int16_t* lv = sched + h0;
+  auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
+  HRec x[kAhead];
+  int next = lv[0];  // start of the next pass to load
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) {
+    const int e = lv[min(k + 1, nph)];
+    x[k] = hr[slot(next, e)];
+    next = e;
+  }
+  auto pass = [&](HRec& r, int q) {
+    const int e = lv[min(q + kAhead + 1, nph)];  // end of the pass kAhead ahead
+    const int xi = r.x < 0 ? ~r.x : r.x;
+    const double od = diag[xi], orr = rhs[xi];
+    const double dh = diag[r.h], rh = rhs[r.h], dl = diag[r.l], rl = rhs[r.l];
+    const double oy = diag[r.y], ry = rhs[r.y];
+    reads_issued();
+    const double fh = r.gh / dh, fl = r.gl / dl;
+    double dd = (od - fh * r.gh) - fl * r.gl;
+    double bb = (orr + fh * rh) + fl * rl;
+    if (r.x < 0) {  // more than two children: the CSR lists, from x's own pivot and rhs
+      dd = od;
+      bb = orr;
+      bad |= elim_csr(d, n0, xi, dd, bb, rhs, diag);
+    }
+    const double fy = r.gy / dd;
+    const double dy = oy - fy * r.gy;
+    const double by = ry + fy * bb;
+    diag[xi] = dd;
+    rhs[xi] = bb;
+    diag[r.y] = dy;
+    rhs[r.y] = by;
+    bad |= ((xi < nn) & bad_pivot(dd)) | ((r.y < nn) & bad_pivot(dy));
+    wave_sync<4>();
+    r = hr[slot(next, e)];
+    next = e;
+  };
+  for (int q = 0; q < nph; q += kAhead) {
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) pass(x[k], q + k);
+  }
+  return bad;
+}
+
+// Back-substitution passes of one schedule section for this wave.
+template <typename C>
+__device__ __forceinline__ void subst_section(const DRec* dr, int nn, const int16_t* sched, int hdr0, C rhs,
+                                              C diag) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int d0 = sched_at(sched, hdr0 + w), npd = sched_at(sched, hdr0 + w + 1) - d0 - 1;
+  if (npd <= 0) return;
+  const int16_t* lv = sched + d0;
+  auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
+  DRec x[kAhead];
+  int next = lv[0];
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) {
+    const int e = lv[min(k + 1, npd)];
+    x[k] = dr[slot(next, e)];
+    next = e;
+  }
+  auto pass = [&](DRec& r, int q) {
+    const int e = lv[min(q + kAhead + 1, npd)];
+    const double oy = diag[r.y], ry = rhs[r.y], rp = rhs[r.p];
+    const double ox = diag[r.x], rx = rhs[r.x];
+    reads_issued();
+    const double xy = (ry + r.gy * rp) / oy;
+    rhs[r.y] = xy;
+    rhs[r.x] = (rx + r.gx * xy) / ox;
+    wave_sync<4>();
+    r = dr[slot(next, e)];
+    next = e;
+  };
+  for (int q = 0; q < npd; q += kAhead) {
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) pass(x[k], q + k);
+  }
+}
+
+// Tree solve of a group in the lean layouts: stage 1 elimination; if a cell is
+// split across waves (stage-2 passes exist), a workgroup barrier, stage 2
+// elimination and substitution (the split cell's top), another barrier; then
+// stage 1 substitution. Every thread of the workgroup calls this (the barriers),
+// and whether a split exists is the same for all of them.
 template <typename C>
 __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int grp, int64_t n0, int nn,
                                            const int16_t* sched, C rhs, C diag) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, W = blockDim.x >> 6;
-  const int h0 = sched_at(sched, w), nph = sched_at(sched, w + 1) - h0 - 1;
-  const int d0 = sched_at(sched, W + 1 + w), npd = sched_at(sched, W + 2 + w) - d0 - 1;
+  const int W = blockDim.x >> 6;
   const int64_t rb = b.grp_rbase[grp];
-  bool bad = false;
-  if (nph > 0) {
-    const HRec* hr = b.r_hrec + rb;
-    const int16_t* lv = sched + h0;
-    auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
-    HRec x[kAhead];
-    int next = lv[0];  // start of the next pass to load
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      const int e = lv[min(k + 1, nph)];
-      x[k] = hr[slot(next, e)];
-      next = e;
-    }
-    auto pass = [&](HRec& r, int q) {
-      const int e = lv[min(q + kAhead + 1, nph)];  // end of the pass kAhead ahead
-      const int xi = r.x < 0 ? ~r.x : r.x;
-      const double od = diag[xi], orr = rhs[xi];
-      const double dh = diag[r.h], rh = rhs[r.h], dl = diag[r.l], rl = rhs[r.l];
-      const double oy = diag[r.y], ry = rhs[r.y];
-      reads_issued();
-      const double fh = r.gh / dh, fl = r.gl / dl;
-      double dd = (od - fh * r.gh) - fl * r.gl;
-      double bb = (orr + fh * rh) + fl * rl;
-      if (r.x < 0) {  // more than two children: the CSR lists, from x's own pivot and rhs
-        dd = od;
-        bb = orr;
-        bad |= elim_csr(d, n0, xi, dd, bb, rhs, diag);
-      }
-      const double fy = r.gy / dd;
-      const double dy = oy - fy * r.gy;
-      const double by = ry + fy * bb;
-      diag[xi] = dd;
-      rhs[xi] = bb;
-      diag[r.y] = dy;
-      rhs[r.y] = by;
-      bad |= ((xi < nn) & bad_pivot(dd)) | ((r.y < nn) & bad_pivot(dy));
-      wave_sync<4>();
-      r = hr[slot(next, e)];
-      next = e;
-    };
-    for (int q = 0; q < nph; q += kAhead) {
-#pragma unroll
-      for (int k = 0; k < kAhead; ++k) pass(x[k], q + k);
-    }
+  const HRec* hr = b.r_hrec + rb;
+  const DRec* dr = b.r_drec + rb;
+  const int e2 = 2 * (W + 1), s2 = 3 * (W + 1);
+  const bool split = sched_at(sched, e2 + W) - sched_at(sched, e2) > W;
+  bool bad = elim_section(d, hr, n0, nn, sched, 0, rhs, diag);
+  if (split) {
+    __syncthreads();
+    bad |= elim_section(d, hr, n0, nn, sched, e2, rhs, diag);
+    subst_section(dr, nn, sched, s2, rhs, diag);
+    __syncthreads();
   }
   if (bad) raise_error(d.error, MIND_ESINGULAR);
-  if (npd > 0) {
-    const DRec* dr = b.r_drec + rb;
-    const int16_t* lv = sched + d0;
-    auto slot = [&](int a, int z) { return a + lane < z ? a + lane : nn; };
-    DRec x[kAhead];
-    int next = lv[0];
-#pragma unroll
-    for (int k = 0; k < kAhead; ++k) {
-      const int e = lv[min(k + 1, npd)];
-      x[k] = dr[slot(next, e)];
-      next = e;
-    }
-    auto pass = [&](DRec& r, int q) {
-      const int e = lv[min(q + kAhead + 1, npd)];
-      const double oy = diag[r.y], ry = rhs[r.y], rp = rhs[r.p];
-      const double ox = diag[r.x], rx = rhs[r.x];
-      reads_issued();
-      const double xy = (ry + r.gy * rp) / oy;
-      rhs[r.y] = xy;
-      rhs[r.x] = (rx + r.gx * xy) / ox;
-      wave_sync<4>();
-      r = dr[slot(next, e)];
-      next = e;
-    };
-    for (int q = 0; q < npd; q += kAhead) {
-#pragma unroll
-      for (int k = 0; k < kAhead; ++k) pass(x[k], q + k);
-    }
-  }
+  subst_section(dr, nn, sched, W + 1, rhs, diag);
 }
 
 // nrn_cur of a density mechanism's rows [r0, r1) of one workgroup: U rows per
@@ -938,11 +970,112 @@ void pair_units(const Layout& L, int64_t n0, int nn, bool pair, std::vector<uint
   }
 }
 
+// Splits the costliest cell of a lean group across waves when that shortens the
+// solve. Its units at unit depth >= D form subtrees, one per unit at depth D,
+// that run as stage-1 items beside the other cells; its units above D form the
+// top (stage 2, wave 0). Picks the D (or no split) with the smallest estimate of
+// 2 x (longest stage-1 wave) + 2 x (the top's passes) + 2 passes for the two
+// barriers, a wave's passes being those of its items' entries merged by level,
+// items packed as assign_waves packs cells, among the splits whose schedule block
+// stays within max_slen int16 words. Updates node_wave and node_stage of the
+// scheduled nodes (a unit's bottom is scheduled with its top).
+void split_cell(const Layout& L, int64_t n0, int nn, int ncg, const std::vector<int32_t>& cell_of,
+                const std::vector<uint8_t>& bottom, const std::vector<int32_t>& dd,
+                const std::vector<std::vector<int32_t>>& hist_d, int W, int64_t max_slen,
+                std::vector<int32_t>& node_wave, std::vector<uint8_t>& node_stage) {
+  auto passes = [](const std::vector<int32_t>& h) {
+    int64_t p = 0;
+    for (int32_t x : h) p += (x + 63) / 64;
+    return p;
+  };
+  int64_t total = 0;  // passes of all waves, of the last call of longest()
+  auto longest = [&](const std::vector<std::vector<int32_t>>& items, const std::vector<int>& wave) {
+    std::vector<std::vector<int32_t>> m(W);
+    for (size_t k = 0; k < items.size(); ++k) {
+      std::vector<int32_t>& b = m[wave[k]];
+      if (b.size() < items[k].size()) b.resize(items[k].size(), 0);
+      for (size_t l = 0; l < items[k].size(); ++l) b[l] += items[k][l];
+    }
+    int64_t best = 0;
+    total = 0;
+    for (int w = 0; w < W; ++w) {
+      best = std::max(best, passes(m[w]));
+      total += passes(m[w]);
+    }
+    return best;
+  };
+  int c = 0;
+  for (int k = 1; k < ncg; ++k)
+    if (passes(hist_d[k]) > passes(hist_d[c])) c = k;
+  const int levels = (int)hist_d[c].size();
+  if (passes(hist_d[c]) <= levels + 1) return;  // no level wider than a pass: nothing to gain
+  int a0 = 0;
+  while (cell_of[a0] != c) ++a0;
+  int a1 = a0;
+  while (a1 < nn && cell_of[a1] == c) ++a1;
+  int64_t best_t = 2 * longest(hist_d, assign_waves(hist_d, hist_d, W));
+  int best_d = 0;
+  std::vector<int32_t> anchor(a1 - a0, -1);
+  auto build = [&](int D, std::vector<std::vector<int32_t>>& items, std::vector<int32_t>& top) {
+    items.clear();
+    for (int k = 0; k < ncg; ++k)
+      if (k != c) items.push_back(hist_d[k]);
+    top.assign(D, 0);
+    for (int li = a0; li < a1; ++li) {
+      const int32_t pg = L.parent[n0 + li];
+      const int p = pg < 0 ? -1 : pg - (int32_t)n0;
+      if (dd[li] < D) {
+        anchor[li - a0] = -1;
+        if (!bottom[li]) top[dd[li]] += 1;
+        continue;
+      }
+      if (dd[li] == D && !bottom[li]) {
+        anchor[li - a0] = (int32_t)items.size();
+        items.emplace_back(levels, 0);
+      } else {
+        anchor[li - a0] = anchor[p - a0];
+      }
+      if (!bottom[li]) items[anchor[li - a0]][dd[li]] += 1;
+    }
+  };
+  std::vector<std::vector<int32_t>> items;
+  std::vector<int32_t> top;
+  for (int D = 1; D < levels && D <= 24; ++D) {
+    build(D, items, top);
+    const int64_t t = 2 * longest(items, assign_waves(items, items, W)) + 2 * passes(top) + 2;
+    // the block: four section headers, each wave's four pass lists with their ends
+    const int64_t slen = 8 * (W + 1) + 2 * (total + W) + 2 * (passes(top) + W);
+    if (slen > max_slen) continue;
+    if (t < best_t) {
+      best_t = t;
+      best_d = D;
+    }
+  }
+  if (!best_d) return;
+  build(best_d, items, top);
+  const std::vector<int> wave = assign_waves(items, items, W);
+  std::vector<int> item_of_cell(ncg, -1);
+  for (int k = 0, q = 0; k < ncg; ++k)
+    if (k != c) item_of_cell[k] = q++;
+  for (int li = 0; li < nn; ++li) {
+    if (cell_of[li] != c) {
+      node_wave[li] = wave[item_of_cell[cell_of[li]]];
+      node_stage[li] = 1;
+    } else if (anchor[li - a0] < 0) {
+      node_wave[li] = 0;
+      node_stage[li] = 2;
+    } else {
+      node_wave[li] = wave[anchor[li - a0]];
+      node_stage[li] = 1;
+    }
+  }
+}
+
 // Upper bound of a schedule block's length: the headers plus, per wave, one more
 // offset than its passes; a wave's levels are at most the sum of its cells', and
 // splitting levels into passes of 64 adds at most nodes / 64 per order.
 inline int64_t sched_bound(int W, int64_t sum_levels, int64_t nodes) {
-  return 4 * (W + 1) + 2 * W + sum_levels + 2 * (nodes / 64);
+  return 8 * (W + 1) + 4 * W + sum_levels + 2 * (nodes / 64);
 }
 
 // Groups consecutive cells within a node budget and the LDS capacity, and builds
@@ -1038,35 +1171,63 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
       }
     }
     const int W = glist[g] == 0 ? W0 : Wl;
-    const std::vector<int> wave = assign_waves(hist_d, hist_d, W);
-    // nodes per level of each wave (elimination levels: reversed depth levels)
-    std::vector<std::vector<int32_t>> wnh(W), wnd(W);
-    for (int k = 0; k < ncg; ++k) {
-      auto& b = wnd[wave[k]];
-      if (b.size() < hist_d[k].size()) b.resize(hist_d[k].size(), 0);
-      for (size_t l = 0; l < hist_d[k].size(); ++l) b[l] += hist_d[k][l];
+    // Waves and stages of the scheduled entries. Stage 1: whole cells, and the
+    // subtrees below unit depth D of a split cell; stage 2: that cell's top (unit
+    // depth < D). A lean group with a split cell eliminates stage 1, then (after a
+    // workgroup barrier) eliminates and substitutes stage 2, then (after another)
+    // substitutes stage 1; without one, stage 1 is everything and no barrier runs.
+    std::vector<int32_t> node_wave(nn, 0);
+    std::vector<uint8_t> node_stage(nn, 1);
+    {
+      const std::vector<int> wave = assign_waves(hist_d, hist_d, W);
+      for (int li = 0; li < nn; ++li) node_wave[li] = wave[cell_of[li]];
+      if (units && W > 1) {
+        // the longest schedule block that keeps the workgroup within the LDS budget
+        const int64_t fixed = (int64_t)window_bytes(mode, nn, 0);
+        const int64_t max_slen = ((int64_t)kLdsResident - fixed) / 2 - 8;
+        split_cell(L, n0, nn, ncg, cell_of, bottom, dd, hist_d, W, max_slen, node_wave, node_stage);
+      }
     }
-    for (int w = 0; w < W; ++w) wnh[w].assign(wnd[w].rbegin(), wnd[w].rend());
-    for (int li = 0; li < nn; ++li) hh[li] = (int32_t)wnd[wave[cell_of[li]]].size() - 1 - dd[li];
+    const int nsec = units ? 4 : 2;  // schedule sections: E1, S1 (, E2, S2)
+    int32_t maxd = 0;
+    for (int li = 0; li < nn; ++li) maxd = std::max(maxd, dd[li]);
+    for (int li = 0; li < nn; ++li) hh[li] = maxd - dd[li];  // elimination level: reversed depth
+    // entries per level of each (stage, wave)
+    std::vector<std::vector<int32_t>> wnh[2], wnd[2];
+    for (int st = 0; st < 2; ++st) {
+      wnh[st].assign(W, std::vector<int32_t>(maxd + 1, 0));
+      wnd[st].assign(W, std::vector<int32_t>(maxd + 1, 0));
+    }
+    for (int li = 0; li < nn; ++li) {
+      if (bottom[li]) continue;
+      wnh[node_stage[li] - 1][node_wave[li]][hh[li]] += 1;
+      wnd[node_stage[li] - 1][node_wave[li]][dd[li]] += 1;
+    }
     auto npasses = [](const std::vector<int32_t>& n) {
       int64_t p = 0;
       for (int32_t x : n) p += (x + 63) / 64;
       return p;
     };
-    int64_t slen = 4 * (W + 1);
-    for (int w = 0; w < W; ++w) slen += npasses(wnh[w]) + 1 + npasses(wnd[w]) + 1;
+    const int64_t hdr = 2 * nsec * (W + 1);  // int16 words of the section headers
+    int64_t slen = hdr;
+    for (int st = 0; st < nsec / 2; ++st)
+      for (int w = 0; w < W; ++w) slen += npasses(wnh[st][w]) + 1 + npasses(wnd[st][w]) + 1;
     if (slen > (1 << 30)) return fail(MIND_EUNSUPPORTED, "schedule block too long");
     H.lists[list].push_back(g);
     if (units) any_lean = true;
     if (mode != 0) any_hbm_links = true;
     H.lds[list] = std::max(H.lds[list], window_bytes(mode, nn, slen));
-    {  // estimated cycles of one step: the longest wave's solve passes, and the
+    {  // estimated cycles of one step: the solve's critical passes, and the
        // node-parallel phases (about 450 cycles per pass of nodes, 600 per pass
        // of units, and 5400 per node per thread)
-      int64_t longest = 0;
-      for (int w = 0; w < W; ++w) longest = std::max(longest, npasses(wnh[w]) + npasses(wnd[w]));
+      int64_t crit = 0;
+      for (int st = 0; st < 2; ++st) {
+        int64_t longest = 0;
+        for (int w = 0; w < W; ++w) longest = std::max(longest, npasses(wnh[st][w]) + npasses(wnd[st][w]));
+        crit += longest;
+      }
       const int Tg = W * 64;
-      group_cost[g] = (units ? 600 : 450) * longest + 5400 * ((nn + Tg - 1) / Tg);
+      group_cost[g] = (units ? 600 : 450) * crit + 5400 * ((nn + Tg - 1) / Tg);
     }
     r_tab.resize((r_tab.size() + 3) / 4 * 4, 0);  // 8-byte aligned blocks
     grp_rtab[g] = (int32_t)r_tab.size();
@@ -1088,13 +1249,11 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
       t_s[2 * k] = (int16_t)(uint16_t)(value & 0xffff);
       t_s[2 * k + 1] = (int16_t)(value >> 16);
     };
-    // per wave: its nodes by level (increasing node index within a level), the
-    // level offsets into the concatenated order, and the header entries
-    // per wave: its nodes by level (increasing node index within a level), and
-    // the offsets of its passes into the concatenated order
+    // One section: per wave its entries of the stage by level (increasing node
+    // index within a level), appended to the order from position pos, and the
+    // offsets of its passes into the order; the header entries of the section.
     auto order = [&](const std::vector<int32_t>& lv, const std::vector<std::vector<int32_t>>& wn, int16_t* ord,
-                     int hdr0, int64_t& cursor) {
-      int32_t pos = 0;
+                     int hdr0, int stage, int64_t& cursor, int32_t& pos) {
       for (int w = 0; w < W; ++w) {
         put(hdr0 + w, cursor);
         const int64_t nl = (int64_t)wn[w].size();
@@ -1104,23 +1263,29 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
           for (int32_t q = cnt[l]; q < cnt[l + 1]; q += 64) t_s[cursor++] = (int16_t)(pos + q);
         t_s[cursor++] = (int16_t)(pos + cnt[nl]);
         for (int li = 0; li < nn; ++li)
-          if (wave[cell_of[li]] == w && !bottom[li]) ord[pos + cnt[lv[li]]++] = (int16_t)li;
-        pos += cnt[nl];  // the fill leaves cnt[nl], the wave's node count, unchanged
+          if (node_wave[li] == w && node_stage[li] == stage && !bottom[li]) ord[pos + cnt[lv[li]]++] = (int16_t)li;
+        pos += cnt[nl];  // the fill leaves cnt[nl], the wave's entry count, unchanged
       }
       put(hdr0 + W, cursor);
     };
-    int64_t cursor = 4 * (W + 1);
-    order(hh, wnh, t_oh, 0, cursor);
-    order(dd, wnd, t_od, W + 1, cursor);
+    {
+      int64_t cursor = hdr;
+      int32_t ph = 0, pd = 0;
+      order(hh, wnh[0], t_oh, 0, 1, cursor, ph);
+      order(dd, wnd[0], t_od, W + 1, 1, cursor, pd);
+      if (units) {
+        order(hh, wnh[1], t_oh, 2 * (W + 1), 2, cursor, ph);
+        order(dd, wnd[1], t_od, 3 * (W + 1), 2, cursor, pd);
+      }
+    }
     // the kernels index by these tables without bounds checks: every scheduled
-    // node once per order, in its own cell's wave; passes of at most 64 entries
-    // of one level, in level order
-    auto valid = [&](const std::vector<int32_t>& lv, const int16_t* ord, int hdr0) {
-      std::vector<uint8_t> seen(nn, 0);
-      int32_t pos = 0;
+    // node once per order, in its own wave and stage; passes of at most 64
+    // entries of one level, in level order
+    auto valid = [&](const std::vector<int32_t>& lv, const int16_t* ord, int hdr0, int stage, int32_t& pos,
+                     std::vector<uint8_t>& seen) {
       for (int w = 0; w < W; ++w) {
         const int64_t a = sched_at(t_s, hdr0 + w), z = sched_at(t_s, hdr0 + w + 1);
-        if (a < 4 * (W + 1) || z <= a || z > slen || t_s[a] != pos) return false;
+        if (a < hdr || z <= a || z > slen || t_s[a] != pos) return false;
         int32_t level = 0;
         for (int64_t q = 0; q + 1 < z - a; ++q) {
           const int32_t i0 = t_s[a + q], i1 = t_s[a + q + 1];
@@ -1130,16 +1295,24 @@ int plan_host(const Layout& L, int64_t budget, int threads, int layout, PlanHost
           level = l0;
           for (int32_t i = i0; i < i1; ++i) {
             const int li = ord[i];
-            if (li < 0 || li >= nn || seen[li] || bottom[li] || wave[cell_of[li]] != w || lv[li] != level) return false;
+            if (li < 0 || li >= nn || seen[li] || bottom[li] || node_wave[li] != w || node_stage[li] != stage ||
+                lv[li] != level)
+              return false;
             seen[li] = 1;
           }
           pos = i1;
         }
       }
-      return pos == nsched;
+      return true;
     };
-    if (!valid(hh, t_oh, 0) || !valid(dd, t_od, W + 1))
-      return fail(MIND_EUNSUPPORTED, "batched plan: inconsistent solve schedule");
+    {
+      std::vector<uint8_t> seen_h(nn, 0), seen_d(nn, 0);
+      int32_t ph = 0, pd = 0;
+      bool ok = valid(hh, t_oh, 0, 1, ph, seen_h) && valid(dd, t_od, W + 1, 1, pd, seen_d);
+      if (ok && units) ok = valid(hh, t_oh, 2 * (W + 1), 2, ph, seen_h) && valid(dd, t_od, 3 * (W + 1), 2, pd, seen_d);
+      if (!ok || ph != nsched || pd != nsched)
+        return fail(MIND_EUNSUPPORTED, "batched plan: inconsistent solve schedule");
+    }
   }
   {
     std::vector<HRec>& hrec = H.hrec;
@@ -1274,13 +1447,56 @@ std::string emulate_lean(const Layout& L, const PlanHost& H, int g, int W) {
     double dv, bv;
   };
   std::vector<Wr> out;
-  for (int sweep = 0; sweep < 2; ++sweep)
+  // the kernel's order: stage-1 elimination, stage-2 elimination and
+  // substitution, stage-1 substitution (sections 0, 2, 3, 1)
+  const int sections[4] = {0, 2, 3, 1};
+  // waves of one section run concurrently: a node one wave writes in a section
+  // must not be read or written by another wave in that section
+  std::vector<int32_t> writer(nn + 2), reader(nn + 2);  // -1 none; reader -2: several waves
+  auto write = [&](int k, int w) {
+    if (k >= nn) return true;  // dummy (read only) and sink (never read as a node)
+    if (writer[k] < 0) writer[k] = w;
+    return writer[k] == w;
+  };
+  auto read = [&](int k, int w) {
+    if (k < nn && reader[k] != w) reader[k] = reader[k] == -1 ? w : -2;
+    return true;
+  };
+  for (int sec : sections) {
+    std::fill(writer.begin(), writer.end(), -1);
+    std::fill(reader.begin(), reader.end(), -1);
     for (int w = 0; w < W; ++w) {
-      const int64_t a = sched_at(sc, sweep * (W + 1) + w), e = sched_at(sc, sweep * (W + 1) + w + 1);
+      const int64_t a = sched_at(sc, sec * (W + 1) + w), e = sched_at(sc, sec * (W + 1) + w + 1);
+      for (int64_t q = a; q + 1 < e; ++q)
+        for (int i = sc[q]; i < sc[q + 1]; ++i) {
+          bool ok = true;
+          if (sec == 0 || sec == 2) {
+            const HRec& r = H.hrec[rb + i];
+            const int x = r.x < 0 ? ~r.x : r.x;
+            ok = write(x, w) && write(r.y, w) && read(r.h, w) && read(r.l, w);
+            if (r.x < 0)
+              for (int64_t cq = L.child_off[n0 + x]; cq < L.child_off[n0 + x + 1]; ++cq)
+                ok = ok && read(L.child[cq] - (int32_t)n0, w);
+          } else {
+            const DRec& r = H.drec[rb + i];
+            ok = write(r.y, w) && write(r.x, w) && read(r.p, w);
+          }
+          if (!ok)
+            return "lean schedule: two waves of section " + std::to_string(sec) + " share a node (wave " +
+                   std::to_string(w) + ", position " + std::to_string(i) + ")";
+        }
+    }
+    for (int k = 0; k < nn; ++k)
+      if (writer[k] >= 0 && reader[k] != -1 && reader[k] != writer[k])
+        return "lean schedule: a wave reads a node another wave of its stage writes";
+  }
+  for (int sec : sections)
+    for (int w = 0; w < W; ++w) {
+      const int64_t a = sched_at(sc, sec * (W + 1) + w), e = sched_at(sc, sec * (W + 1) + w + 1);
       for (int64_t q = a; q + 1 < e; ++q) {
         out.clear();
         for (int i = sc[q]; i < sc[q + 1]; ++i) {
-          if (sweep == 0) {
+          if (sec == 0 || sec == 2) {
             const HRec& r = H.hrec[rb + i];
             const int x = r.x < 0 ? ~r.x : r.x;
             const double fh = r.gh / dg[r.h], fl = r.gl / dg[r.l];
@@ -1356,11 +1572,12 @@ std::string check_plan(const Layout& L, const PlanHost& H) {
       if (h < -2 || h >= nn || l < -1 || l >= nn || p < -1 || p >= nn) return bad(g, "node links");
       if (oh[li] < 0 || oh[li] >= nn || od[li] < 0 || od[li] >= nn) return bad(g, "node orders");
     }
-    if (4 * (W + 1) > slen) return bad(g, "schedule header");
-    for (int sweep = 0; sweep < 2; ++sweep)
+    const int nsec = mode == 4 || mode == 5 ? 4 : 2;
+    if (2 * nsec * (W + 1) > slen) return bad(g, "schedule header");
+    for (int sweep = 0; sweep < nsec; ++sweep)
       for (int w = 0; w < W; ++w) {
         const int64_t a = sched_at(sc, sweep * (W + 1) + w), z = sched_at(sc, sweep * (W + 1) + w + 1);
-        if (a < 4 * (W + 1) || z <= a || z > slen) return bad(g, "schedule offsets block");
+        if (a < 2 * nsec * (W + 1) || z <= a || z > slen) return bad(g, "schedule offsets block");
         for (int64_t q = a; q < z; ++q) {
           if (sc[q] < 0 || sc[q] > nn) return bad(g, "pass offset");
           if (q + 1 < z && (sc[q + 1] < sc[q] || sc[q + 1] - sc[q] > 64)) return bad(g, "pass size");
@@ -1394,7 +1611,7 @@ std::string check_plan(const Layout& L, const PlanHost& H) {
       if (H.hrec[rb + nn].x != nn + 1 || H.drec[rb + nn].y != nn + 1) return bad(g, "sink record");
       for (int64_t li = 0; li < nn; ++li)
         if (!done_h[li] || !done_d[li]) return bad(g, "node missing from the lean records");
-      for (int sweep = 0; sweep < 2; ++sweep)
+      for (int sweep = 0; sweep < 4; ++sweep)
         for (int w = 0; w < W; ++w) {
           const int64_t z = sched_at(sc, sweep * (W + 1) + w + 1);
           if (sc[z - 1] > nu) return bad(g, "pass offset beyond the units");
