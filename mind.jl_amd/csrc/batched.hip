@@ -1040,15 +1040,18 @@ void split_cell(const Layout& L, int64_t n0, int nn, int ncg, const std::vector<
   };
   std::vector<std::vector<int32_t>> items;
   std::vector<int32_t> top;
-  for (int D = 1; D < levels && D <= 24; ++D) {
+  int worse = 0;  // consecutive depths without a better estimate: stop at 4
+  for (int D = 1; D < levels && D <= 24 && worse < 4; ++D) {
     build(D, items, top);
     const int64_t t = 2 * longest(items, assign_waves(items, items, W)) + 2 * passes(top) + 2;
     // the block: four section headers, each wave's four pass lists with their ends
     const int64_t slen = 8 * (W + 1) + 2 * (total + W) + 2 * (passes(top) + W);
+    ++worse;
     if (slen > max_slen) continue;
     if (t < best_t) {
       best_t = t;
       best_d = D;
+      worse = 0;
     }
   }
   if (!best_d) return;
@@ -1769,9 +1772,10 @@ int batched_setup(State& S, int32_t cells_per_group) {
     for (int threads : {256, 512}) {
       for (int64_t bud : {1024, 1700, 2750}) cands.push_back({bud, threads, 0});
       for (int64_t bud : {1800, 2400, 3000, 3600, 4800, 6000}) cands.push_back({bud, threads, 5});
-      for (int64_t bud : {4500, 7200}) cands.push_back({bud, threads, 4});
+      for (int64_t bud : {3000, 3600, 4500, 5400, 7200}) cands.push_back({bud, threads, 4});
     }
     for (int64_t bud : {4800, 6000}) cands.push_back({bud, 1024, 5});
+    for (int64_t bud : {7200, 9000}) cands.push_back({bud, 1024, 4});
   }
   int rc;
   double best = 1e30;
@@ -1794,12 +1798,27 @@ int batched_setup(State& S, int32_t cells_per_group) {
     }
     return 0;
   };
-  for (const Cand& c : cands) {
+  // every candidate once, then the three fastest twice more: the pick has the
+  // lowest mean of its three timings
+  std::vector<std::pair<double, size_t>> timed;
+  for (size_t k = 0; k < cands.size(); ++k) {
     double secs = 0;
-    if ((rc = trial(c, secs))) return rc;
-    if (tune && secs < best) {
-      best = secs;
-      pick = c;
+    if ((rc = trial(cands[k], secs))) return rc;
+    timed.push_back({secs, k});
+  }
+  if (tune) {
+    std::sort(timed.begin(), timed.end());
+    for (size_t q = 0; q < std::min<size_t>(3, timed.size()); ++q) {
+      double total = timed[q].first;
+      for (int rep = 0; rep < 2; ++rep) {
+        double secs = 0;
+        if ((rc = trial(cands[timed[q].second], secs))) return rc;
+        total += secs;
+      }
+      if (total / 3 < best) {
+        best = total / 3;
+        pick = cands[timed[q].second];
+      }
     }
   }
   if (tune) {

@@ -10,8 +10,9 @@ from mind_amd.model import check_batch_plan
 # the autotuner's candidates (batched.hip, batched_setup) plus small block sizes
 CANDIDATES = ([(b, t, 0) for t in (256, 512) for b in (1024, 1700, 2750)]
               + [(b, t, 5) for t in (256, 512) for b in (1800, 2400, 3000, 3600, 4800, 6000)]
-              + [(b, t, 4) for t in (256, 512) for b in (4500, 7200)]
+              + [(b, t, 4) for t in (256, 512) for b in (3000, 3600, 4500, 5400, 7200)]
               + [(b, 1024, 5) for b in (4800, 6000)]
+              + [(b, 1024, 4) for b in (7200, 9000)]
               + [(300, 64, 0), (600, 64, 5), (900, 128, 4), (2750, 1024, 0), (7200, 1024, 4)])
 
 
