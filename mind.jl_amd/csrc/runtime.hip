@@ -364,38 +364,43 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     n_ns = count < d.emit_cap ? count : d.emit_cap;
   }
   __syncthreads();
-  // B. Detector spikes of each window step, compacted in source order by
-  //    wave ballot and a block prefix sum.
+  // B. Detector spikes of the window in (step, detector) order: the flags of all
+  //    steps, row-major [step][detector], are split into one contiguous chunk per
+  //    thread (a multiple of 8 bytes, read as 64-bit words of 0/1 bytes); each
+  //    thread counts its set flags, a block prefix sum gives its first position,
+  //    and a second pass over the chunk writes the emissions.
   int64_t ndet_total = 0;
-  for (int s = 0; s < n; ++s) {
-    const int64_t key = 3 * (k0 + s);
-    const double ts = (double)(k0 + s) * dt + 1.0e-10;
-    for (int64_t base = 0; base < d.ndet; base += kBlock) {
-      const int64_t q = base + tid;
-      const int flag = q < d.ndet ? d.det_flags[(int64_t)s * d.ndet + q] : 0;
-      const unsigned long long ballot = __ballot(flag);
-      const int lane = tid & 63;
-      const int wave_rank = __popcll(ballot & ((1ull << lane) - 1ull));
-      if (lane == 0) scan[tid >> 6] = __popcll(ballot);
-      __syncthreads();
-      if (tid == 0) {
-        int64_t acc = 0;
-        for (int w = 0; w < kBlock / 64; ++w) {
-          const int64_t x = scan[w];
-          scan[w] = acc;
-          acc += x;
-        }
-        scan[kBlock / 64] = acc;
+  {
+    const int64_t nf = (int64_t)n * d.ndet;
+    const int64_t chunk = ((nf + kBlock - 1) / kBlock + 7) / 8 * 8;
+    const int64_t f0 = std::min<int64_t>((int64_t)tid * chunk, nf), f1 = std::min<int64_t>(f0 + chunk, nf);
+    const uint64_t* words = reinterpret_cast<const uint64_t*>(d.det_flags);
+    const int64_t w1 = f1 / 8;  // whole words inside [f0, f1); f0 is a multiple of 8
+    int64_t mine = 0;
+    for (int64_t w = f0 / 8; w < w1; ++w) mine += __popcll(words[w]);
+    for (int64_t f = w1 * 8 > f0 ? w1 * 8 : f0; f < f1; ++f) mine += d.det_flags[f];
+    int64_t total;
+    const int64_t first = block_scan(mine, scan, &total);
+    int64_t pos = first;
+    auto emit = [&](int64_t f) {
+      const int64_t s = f / d.ndet, q = f - s * d.ndet;
+      if (pos < d.emit_cap)
+        d.emit_det[pos] = Emit{3 * (k0 + s), (double)(k0 + s) * dt + 1.0e-10, d.det_source[q], 0};
+      else
+        raise_error(d.error, MIND_EQUEUE);
+      ++pos;
+    };
+    for (int64_t w = f0 / 8; w < w1; ++w) {
+      uint64_t x = words[w];
+      while (x) {
+        const int byte = __builtin_ctzll(x) >> 3;
+        emit(w * 8 + byte);
+        x &= ~(0xffull << (8 * byte));
       }
-      __syncthreads();
-      if (flag) {
-        const int64_t pos = ndet_total + scan[tid >> 6] + wave_rank;
-        if (pos < d.emit_cap) d.emit_det[pos] = Emit{key, ts, d.det_source[q], 0};
-        else raise_error(d.error, MIND_EQUEUE);
-      }
-      ndet_total += scan[kBlock / 64];
-      __syncthreads();
     }
+    for (int64_t f = w1 * 8 > f0 ? w1 * 8 : f0; f < f1; ++f)
+      if (d.det_flags[f]) emit(f);
+    ndet_total = total;
   }
   if (tid == 0) n_det = ndet_total < d.emit_cap ? ndet_total : d.emit_cap;
   __syncthreads();
