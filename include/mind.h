@@ -37,7 +37,7 @@ enum {
   MIND_EUNSUPPORTED = 10 /* model feature outside the device path */
 };
 
-/* Builtin mechanisms (reference mod/{hh,pas,ExpSyn,Exp2Syn,IClamp,NetStim}.mod). */
+/* Builtin mechanisms (reference mod/{hh,pas,ExpSyn,Exp2Syn,IClamp,NetStim,AlphaSynapse}.mod). */
 enum {
   MIND_MECH_HH = 0,
   MIND_MECH_PAS = 1,
@@ -45,7 +45,8 @@ enum {
   MIND_MECH_EXP2SYN = 3,
   MIND_MECH_ICLAMP = 4,
   MIND_MECH_NETSTIM = 5,
-  MIND_MECH_COUNT = 6
+  MIND_MECH_ALPHASYNAPSE = 6,
+  MIND_MECH_COUNT = 7
 };
 
 /* Field columns of each builtin mechanism (parameters, states, assigned). */
@@ -58,6 +59,8 @@ enum { EXP2SYN_TAU1, EXP2SYN_TAU2, EXP2SYN_E, EXP2SYN_A, EXP2SYN_B, EXP2SYN_I, E
 enum { ICLAMP_DEL, ICLAMP_DUR, ICLAMP_AMP, ICLAMP_I, ICLAMP_NFIELDS };
 enum { NETSTIM_INTERVAL, NETSTIM_NUMBER, NETSTIM_START, NETSTIM_NOISE, NETSTIM_EVENT,
        NETSTIM_ON, NETSTIM_ISPIKE, NETSTIM_NFIELDS };
+enum { ALPHASYNAPSE_ONSET, ALPHASYNAPSE_TAU, ALPHASYNAPSE_GMAX, ALPHASYNAPSE_E, ALPHASYNAPSE_I,
+       ALPHASYNAPSE_G, ALPHASYNAPSE_NFIELDS };
 
 /* A flattened MicroModel (src/model/micro/model.jl:35-50) plus its sections
  * (src/model/micro/morphology.jl:17-29). Section indices are local to their

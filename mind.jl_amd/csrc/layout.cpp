@@ -18,8 +18,8 @@
 namespace mind {
 
 int field_count(int mech) {
-  static const int n[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS,
-                                         EXP2SYN_NFIELDS, ICLAMP_NFIELDS, NETSTIM_NFIELDS};
+  static const int n[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS,
+                                         ICLAMP_NFIELDS, NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS};
   return n[mech];
 }
 
@@ -38,6 +38,7 @@ void field_defaults(int mech, double* f) {
     case MIND_MECH_NETSTIM:
       f[NETSTIM_INTERVAL] = 10.0; f[NETSTIM_NUMBER] = 10.0; f[NETSTIM_START] = 50.0;
       break;
+    case MIND_MECH_ALPHASYNAPSE: f[ALPHASYNAPSE_TAU] = 0.1; break;  // AlphaSynapse.mod PARAMETER
     default: break;
   }
 }

@@ -1,5 +1,5 @@
-// Device (and host) lowering of the six builtin mechanisms of the simulation
-// step: hh, pas, ExpSyn, Exp2Syn, IClamp and NetStim.
+// Device (and host) lowering of the seven builtin mechanisms of the simulation
+// step: hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim and AlphaSynapse.
 //
 // Each function is the nrn_cur / nrn_state / NET_RECEIVE / INITIAL phase of one
 // MOD file as the reference lowers it (src/compiler/codegen.jl:362-485):
@@ -107,6 +107,15 @@ MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t
         const double i = (t < del + dur && t >= del) ? c(ICLAMP_AMP, r) : 0.0;
         sum -= i;
         if (probe == 1 && store) c(ICLAMP_I, r) = i;
+        break;
+      }
+      case MIND_MECH_ALPHASYNAPSE: {  // AlphaSynapse.mod BREAKPOINT; at_time only marks a discontinuity
+        const double x = (t - c(ALPHASYNAPSE_ONSET, r)) / c(ALPHASYNAPSE_TAU, r);
+        const double a = (x < 0.0 || x > 10.0) ? 0.0 : x * mind_exp(1.0 - x);
+        const double g = c(ALPHASYNAPSE_GMAX, r) * a;
+        const double i = g * (vv - c(ALPHASYNAPSE_E, r));
+        sum += i;
+        if (probe == 1 && store) { c(ALPHASYNAPSE_G, r) = g; c(ALPHASYNAPSE_I, r) = i; }
         break;
       }
       default: break;

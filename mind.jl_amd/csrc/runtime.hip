@@ -364,42 +364,40 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     n_ns = count < d.emit_cap ? count : d.emit_cap;
   }
   __syncthreads();
-  // B. Detector spikes of the window in (step, detector) order: the flags of all
-  //    steps, row-major [step][detector], are split into one contiguous chunk per
-  //    thread (a multiple of 8 bytes, read as 64-bit words of 0/1 bytes); each
-  //    thread counts its set flags, a block prefix sum gives its first position,
-  //    and a second pass over the chunk writes the emissions.
+  // B. Detector spikes of the window in (step, detector) order, compacted by
+  //    wavefront ballot and prefix sum in one pass over the flags of all steps
+  //    (row-major [step][detector]): each wave takes one contiguous slice (a
+  //    multiple of 64 flags) and counts it 64 flags per ballot; a block prefix
+  //    sum over the waves' counts gives each slice its first position; a second
+  //    walk writes every set flag at its position plus its rank in the ballot.
   int64_t ndet_total = 0;
   {
+    const int lane = tid & 63, wave = tid >> 6;
     const int64_t nf = (int64_t)n * d.ndet;
-    const int64_t chunk = ((nf + kBlock - 1) / kBlock + 7) / 8 * 8;
-    const int64_t f0 = std::min<int64_t>((int64_t)tid * chunk, nf), f1 = std::min<int64_t>(f0 + chunk, nf);
-    const uint64_t* words = reinterpret_cast<const uint64_t*>(d.det_flags);
-    const int64_t w1 = f1 / 8;  // whole words inside [f0, f1); f0 is a multiple of 8
-    int64_t mine = 0;
-    for (int64_t w = f0 / 8; w < w1; ++w) mine += __popcll(words[w]);
-    for (int64_t f = w1 * 8 > f0 ? w1 * 8 : f0; f < f1; ++f) mine += d.det_flags[f];
-    int64_t total;
-    const int64_t first = block_scan(mine, scan, &total);
-    int64_t pos = first;
-    auto emit = [&](int64_t f) {
-      const int64_t s = f / d.ndet, q = f - s * d.ndet;
-      if (pos < d.emit_cap)
-        d.emit_det[pos] = Emit{3 * (k0 + s), (double)(k0 + s) * dt + 1.0e-10, d.det_source[q], 0};
-      else
-        raise_error(d.error, MIND_EQUEUE);
-      ++pos;
-    };
-    for (int64_t w = f0 / 8; w < w1; ++w) {
-      uint64_t x = words[w];
-      while (x) {
-        const int byte = __builtin_ctzll(x) >> 3;
-        emit(w * 8 + byte);
-        x &= ~(0xffull << (8 * byte));
-      }
+    const int64_t slice = ((nf + kBlock / 64 - 1) / (kBlock / 64) + 63) / 64 * 64;
+    const int64_t f0 = std::min<int64_t>((int64_t)wave * slice, nf), f1 = std::min<int64_t>(f0 + slice, nf);
+    int64_t count = 0;
+    for (int64_t base = f0; base < f1; base += 64) {
+      const int64_t f = base + lane;
+      count += __popcll(__ballot(f < f1 && d.det_flags[f] != 0));
     }
-    for (int64_t f = w1 * 8 > f0 ? w1 * 8 : f0; f < f1; ++f)
-      if (d.det_flags[f]) emit(f);
+    int64_t total;
+    const int64_t first = block_scan(lane == 0 ? count : 0, scan, &total);
+    int64_t pos = __shfl(first, 0, 64);
+    for (int64_t base = f0; base < f1; base += 64) {
+      const int64_t f = base + lane;
+      const bool flag = f < f1 && d.det_flags[f] != 0;
+      const unsigned long long ballot = __ballot(flag);
+      if (flag) {
+        const int64_t at = pos + __popcll(ballot & ((1ull << lane) - 1ull));
+        const int64_t st = f / d.ndet, q = f - st * d.ndet;
+        if (at < d.emit_cap)
+          d.emit_det[at] = Emit{3 * (k0 + st), (double)(k0 + st) * dt + 1.0e-10, d.det_source[q], 0};
+        else
+          raise_error(d.error, MIND_EQUEUE);
+      }
+      pos += __popcll(ballot);
+    }
     ndet_total = total;
   }
   if (tid == 0) n_det = ndet_total < d.emit_cap ? ndet_total : d.emit_cap;

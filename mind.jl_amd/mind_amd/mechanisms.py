@@ -41,13 +41,15 @@ _TABLE = {
         "NetStim", 5, "artificial",
         ("interval", "number", "start", "noise", "event", "on", "ispike"), True,
     ),
+    "AlphaSynapse": Mechanism("AlphaSynapse", 6, "point", ("onset", "tau", "gmax", "e", "i", "g")),
 }
 
 BY_ID = {m.id: m for m in _TABLE.values()}
 
 
 def builtin(name) -> Mechanism:
-    """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim)."""
+    """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim,
+    AlphaSynapse)."""
     key = str(name).lstrip(":")
     if key not in _TABLE:
         raise ValueError(f"unknown mechanism {key}; load its MOD with load_mod")

@@ -41,7 +41,8 @@ from dataclasses import dataclass, field
 EXEC_INPUTS = ("v", "t", "dt", "celsius")
 FUNCS = {"exp": "mind_exp", "log": "mind_log", "fabs": "mind_fabs", "sqrt": "sqrt"}
 BUILTIN_IDS = {"hh": "MIND_MECH_HH", "pas": "MIND_MECH_PAS", "ExpSyn": "MIND_MECH_EXPSYN",
-               "Exp2Syn": "MIND_MECH_EXP2SYN", "IClamp": "MIND_MECH_ICLAMP", "NetStim": "MIND_MECH_NETSTIM"}
+               "Exp2Syn": "MIND_MECH_EXP2SYN", "IClamp": "MIND_MECH_ICLAMP", "NetStim": "MIND_MECH_NETSTIM",
+               "AlphaSynapse": "MIND_MECH_ALPHASYNAPSE"}
 
 
 class ModError(Exception):
@@ -541,6 +542,10 @@ class Gen:
         if c.name == "random_uniform":
             self.need_random(c)
             return "stream_uniform(*x.s, x.seed)"
+        if c.name == "at_time":  # 1 in the step whose interval (t - dt, t] holds the event (codegen.jl:102-110)
+            if len(args) != 1:
+                self.err("at_time takes one argument")
+            return f"((({args[0]} - 1.0e-11) <= x.t && ({args[0]} - 1.0e-11) > x.t - x.dt) ? 1.0 : 0.0)"
         if c.name in self.funcs:
             f = self.funcs[c.name]
             if len(args) != len(f.args):
@@ -601,6 +606,8 @@ class Gen:
         if c.name == "random_setseq":
             self.need_random(c)
             return f"stream_setseq(*x.s, x.seed, {args[1]})"
+        if c.name == "at_time":  # as a statement its value is unused: no effect on the state
+            return "(void)0"
         if c.name in self.procs:
             p = self.procs[c.name]
             if len(args) != len(p.args):

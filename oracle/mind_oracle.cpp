@@ -254,9 +254,9 @@ Template discretize(const std::vector<SectionIn>& secs) {
 }
 
 // ------------------------------------------------------------ Mechanisms ----
-const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS,
-                                      EXP2SYN_NFIELDS, ICLAMP_NFIELDS, NETSTIM_NFIELDS};
-const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true};
+const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS,
+                                      ICLAMP_NFIELDS, NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS};
+const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true};
 
 void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
   switch (mech) {
@@ -272,6 +272,7 @@ void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
       f[NETSTIM_INTERVAL] = 10.0; f[NETSTIM_NUMBER] = 10.0; f[NETSTIM_START] = 50.0;
       f[NETSTIM_NOISE] = 0.0;
       break;
+    case MIND_MECH_ALPHASYNAPSE: f[ALPHASYNAPSE_TAU] = 0.1; break;  // AlphaSynapse.mod PARAMETER
   }
 }
 
@@ -664,6 +665,15 @@ void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
               const double i = (t < del + dur && t >= del) ? g.at(ICLAMP_AMP, r) : 0.0;
               g.at(ICLAMP_I, r) = i;
               sum = 0.0; sum -= i;  // electrode current
+              break;
+            }
+            case MIND_MECH_ALPHASYNAPSE: {  // AlphaSynapse.mod: g = gmax alpha((t - onset)/tau)
+              const double x = (t - g.at(ALPHASYNAPSE_ONSET, r)) / g.at(ALPHASYNAPSE_TAU, r);
+              const double a = (x < 0.0 || x > 10.0) ? 0.0 : x * mind_exp(1.0 - x);  // FUNCTION alpha
+              const double gg = g.at(ALPHASYNAPSE_GMAX, r) * a;
+              const double i = gg * (vv - g.at(ALPHASYNAPSE_E, r));
+              g.at(ALPHASYNAPSE_G, r) = gg; g.at(ALPHASYNAPSE_I, r) = i;
+              sum = 0.0; sum += i;
               break;
             }
           }

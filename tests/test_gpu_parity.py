@@ -158,3 +158,20 @@ def test_secondorder_matches_oracle(gpu, secondorder, method, plan):
     for r in recs:
         assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
     destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_alpha_synapse_matches_oracle(gpu, method):
+    """AlphaSynapse on the single cell, in both kernels: bit-identical to the oracle."""
+    model, info = configs.single_cell()
+    cell = info["cells"][0]
+    syn = model.add_pointprocess(cell, "AlphaSynapse", section="dend", x=0.2,
+                                 params={"onset": 20.0, "tau": 1.5, "gmax": 5e-3, "e": 0.0})
+    recs = [voltage_record(model, cell, "soma", 0.5), mechanism_record(model, syn, "g"),
+            mechanism_record(model, syn, "i")]
+    got, ref, vg, vr, _ = _compare(model, 60.0, recs, method=method)
+    assert np.array_equal(vg, vr)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    assert got.traces["g"].max() > 4.9e-3
+    destroy(model)

@@ -18,7 +18,7 @@ from mind_amd import nmodl
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "mind.jl_amd", "csrc")
 MOD_DIR = "/root/reference/mod"
-SIX = ("hh", "pas", "ExpSyn", "Exp2Syn", "IClamp", "NetStim")
+BUILTINS = ("hh", "pas", "ExpSyn", "Exp2Syn", "IClamp", "NetStim", "AlphaSynapse")
 
 LEAKGATE = textwrap.dedent("""
     TITLE toy gated leak
@@ -138,10 +138,11 @@ HARNESS = textwrap.dedent("""
     int main() {
       std::mt19937_64 g(12345);
       auto U = [&](double a, double b) { return std::uniform_real_distribution<double>(a, b)(g); };
-      const int nf[6] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS, ICLAMP_NFIELDS,
-                         NETSTIM_NFIELDS};
+      const int nf[7] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS, ICLAMP_NFIELDS,
+                         NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS};
       for (int trial = 0; trial < 3000; ++trial) {
-        for (int mech = 0; mech < 5; ++mech) {
+        for (int mech = 0; mech < 7; ++mech) {
+          if (mech == MIND_MECH_NETSTIM) continue;  // through its stream, below
           double a[32], b[32];
           for (int k = 0; k < nf[mech]; ++k) a[k] = U(0.0, 1.0);
           if (mech == MIND_MECH_HH) { a[HH_GNABAR] = U(0, 0.3); a[HH_EL] = U(-80, -40);
@@ -152,6 +153,8 @@ HARNESS = textwrap.dedent("""
                                            a[EXP2SYN_E] = U(-80, 0); }
           if (mech == MIND_MECH_ICLAMP) { a[ICLAMP_DEL] = U(0, 100); a[ICLAMP_DUR] = U(0, 100);
                                           a[ICLAMP_AMP] = U(-1, 1); }
+          if (mech == MIND_MECH_ALPHASYNAPSE) { a[ALPHASYNAPSE_ONSET] = U(0, 250); a[ALPHASYNAPSE_TAU] = U(0.05, 20);
+                                                a[ALPHASYNAPSE_GMAX] = U(0, 1e-2); a[ALPHASYNAPSE_E] = U(-80, 0); }
           std::memcpy(b, a, sizeof a);
           Cols ca{a, 1}, cb{b, 1};
           const double v = U(-100, 60), t = U(0, 250), dt = trial % 2 ? 0.025 : U(0.001, 0.2),
@@ -206,8 +209,8 @@ HARNESS = textwrap.dedent("""
 
 
 @pytest.mark.skipif(not os.path.isdir(MOD_DIR), reason="the builtin MOD files are not present")
-def test_six_mod_files_lower_to_the_hand_lowering_bit_for_bit(tmp_path):
-    paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in SIX]
+def test_builtin_mod_files_lower_to_the_hand_lowering_bit_for_bit(tmp_path):
+    paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in BUILTINS]
     mechs = [nmodl.parse(open(p).read(), p) for p in paths]
     # NetStim's stream key comes from its source digest
     assert {m.name: m for m in mechs}["NetStim"].digest == 0xa186833b569ff3b2
@@ -218,7 +221,7 @@ def test_six_mod_files_lower_to_the_hand_lowering_bit_for_bit(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir(MOD_DIR), reason="the builtin MOD files are not present")
 def test_parameter_defaults_match_the_packing_defaults(tmp_path):
-    paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in SIX]
+    paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in BUILTINS]
     hdr = nmodl.compile_files(paths)
     (tmp_path / "mech_gen.h").write_text(hdr)
     src = tmp_path / "defaults.cpp"
@@ -234,6 +237,7 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
             for (int k = 0; k < gen::M##_NFIELDS; ++k) std::printf("%s %d %.17g %.17g\\n", #M, k, f[k], h[k]);
           SHOW(hh, MIND_MECH_HH) SHOW(pas, MIND_MECH_PAS) SHOW(ExpSyn, MIND_MECH_EXPSYN)
           SHOW(Exp2Syn, MIND_MECH_EXP2SYN) SHOW(IClamp, MIND_MECH_ICLAMP) SHOW(NetStim, MIND_MECH_NETSTIM)
+          SHOW(AlphaSynapse, MIND_MECH_ALPHASYNAPSE)
         }
     """))
     exe = tmp_path / "defaults"
@@ -253,7 +257,7 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
 def test_equivalence_check_sees_a_reassociated_product(tmp_path):
     """The comparison is sensitive to operation order: regrouping hh's m*m*m*h
     product (same value in exact arithmetic) is reported."""
-    paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in SIX]
+    paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in BUILTINS]
     hdr = nmodl.compile_files(paths)
     exact = "((((in.gnabar * in.m) * in.m) * in.m) * in.h)"  # hh_current_eval, from the loaded columns
     assert exact in hdr

@@ -302,6 +302,21 @@ def test_injected_spike_routes_through_every_connection():
         oracle.OracleSim(m).initialize().inject_spike(det, 1.0)  # not an external input
 
 
+def test_alpha_synapse_conductance_is_the_alpha_function():
+    """AlphaSynapse (AlphaSynapse.mod): g = gmax x e^(1-x), x = (t - onset)/tau, at the
+    current evaluation t + dt/2, zero outside 0 <= x <= 10; peak gmax at x = 1. The
+    recorded g is the last evaluation's, half a step before the sample."""
+    m, cell = passive_cell(v0=-65.0)
+    syn = m.add_pointprocess(cell, "AlphaSynapse", params={"onset": 5.0, "tau": 2.0, "gmax": 1e-3, "e": 0.0})
+    r = oracle.OracleSim(m).initialize().run(40.0, [mechanism_record(m, syn, "g"), voltage_record(m, cell)])
+    tc = r.time[1:] - 0.5 * 0.025
+    x = (tc - 5.0) / 2.0
+    expect = np.where((x < 0) | (x > 10), 0.0, 1e-3 * x * np.exp(1.0 - x))
+    np.testing.assert_allclose(r.traces["g"][1:], expect, rtol=1e-13, atol=1e-18)
+    assert abs(r.traces["g"].max() - 1e-3) < 1e-3 * 1e-4
+    assert r.traces["v_0"].max() > -65.0 + 1.0  # the synapse depolarises toward e = 0
+
+
 def test_netstim_regular_train():
     m = MicroModel()
     ns = m.add_pointprocess(m, "NetStim", params={"interval": 7.0, "number": 5, "start": 3.0,
