@@ -46,7 +46,8 @@ enum {
   MIND_MECH_ICLAMP = 4,
   MIND_MECH_NETSTIM = 5,
   MIND_MECH_ALPHASYNAPSE = 6,
-  MIND_MECH_COUNT = 7
+  MIND_MECH_PATTERNSTIM = 7, /* artificial cell that emits given spike times (no fields) */
+  MIND_MECH_COUNT = 8
 };
 
 /* Field columns of each builtin mechanism (parameters, states, assigned). */
@@ -61,6 +62,7 @@ enum { NETSTIM_INTERVAL, NETSTIM_NUMBER, NETSTIM_START, NETSTIM_NOISE, NETSTIM_E
        NETSTIM_ON, NETSTIM_ISPIKE, NETSTIM_NFIELDS };
 enum { ALPHASYNAPSE_ONSET, ALPHASYNAPSE_TAU, ALPHASYNAPSE_GMAX, ALPHASYNAPSE_E, ALPHASYNAPSE_I,
        ALPHASYNAPSE_G, ALPHASYNAPSE_NFIELDS };
+enum { PATTERNSTIM_NFIELDS = 0 };
 
 /* A flattened MicroModel (src/model/micro/model.jl:35-50) plus its sections
  * (src/model/micro/morphology.jl:17-29). Section indices are local to their
@@ -125,6 +127,14 @@ typedef struct mind_model_desc {
   const int32_t* con_target; /* point-process index */
   const double* con_weight;  /* uS */
   const double* con_delay;   /* ms */
+
+  /* Spike times of PatternStim points (add_pointprocess!(...; spike_times),
+   * model.jl:172-194): point p's sorted times are pt_spike_times[pt_spike_offsets[p]
+   * .. pt_spike_offsets[p + 1]); both NULL when no point has any. At initialize
+   * each time becomes a source event of every spike source on the point
+   * (reset_events!, src/runtime/utils/events.jl:92-110). */
+  const int64_t* pt_spike_offsets;
+  const double* pt_spike_times;
 } mind_model_desc;
 
 /* Kernel families for the micro step. */
@@ -271,7 +281,7 @@ int mind_connection_weights(const mind_state* state, int64_t connection, double*
 int mind_inject_event(mind_state* state, int64_t connection, double time);
 /* inject_spike! (src/runtime/state.jl:638-659): a spike of external source
  * `source` (kind 2) at absolute time `time` >= now, routed over its connections
- * and recorded at its delivery call. At most 256 injected spikes may fall in one
+ * and recorded at its delivery call. At most 1024 injected and PatternStim spikes may fall in one
  * window. Pending injections are cleared by mind_initialize. */
 int mind_inject_spike(mind_state* state, int64_t source, double time);
 

@@ -24,7 +24,7 @@ struct InjSpike {
   int32_t source;
   int32_t pad;
 };
-constexpr int kInjCap = 256;  // injected spikes handed to one window, events per staging copy
+constexpr int kInjCap = 1024;  // injected spikes handed to one window, events per staging copy
 
 struct Clock {
   int64_t step;

@@ -18,8 +18,9 @@
 namespace mind {
 
 int field_count(int mech) {
-  static const int n[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS,
-                                         ICLAMP_NFIELDS, NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS};
+  static const int n[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,     EXPSYN_NFIELDS,
+                                         EXP2SYN_NFIELDS, ICLAMP_NFIELDS,  NETSTIM_NFIELDS,
+                                         ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS};
   return n[mech];
 }
 
@@ -302,10 +303,24 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
   L.point_group.assign(d.n_points, -1);
   L.point_row.assign(d.n_points, -1);
   L.point_cell.assign(d.n_points, -1);
+  std::vector<std::pair<double, int32_t>> pattern;  // (time, point)
   for (int64_t p = 0; p < d.n_points; ++p) {
     const int mech = d.pt_mech[p];
     if (mech < 0 || mech >= MIND_MECH_COUNT || !mech_is_point(mech))
       return "point creation requires a point process";
+    const int64_t s0 = d.pt_spike_offsets ? d.pt_spike_offsets[p] : 0;
+    const int64_t s1 = d.pt_spike_offsets ? d.pt_spike_offsets[p + 1] : 0;
+    if (s1 > s0 && mech != MIND_MECH_PATTERNSTIM) return "spike_times applies to PatternStim only";
+    if (mech == MIND_MECH_PATTERNSTIM) {  // no mechanism data: its spikes enter at initialize
+      if (d.pt_cell[p] >= 0) return "PatternStim is an artificial cell without a location";
+      for (int64_t k = s0; k < s1; ++k) {
+        const double t = d.pt_spike_times[k];
+        if (!(isfinite(t) && t >= 0.0)) return "invalid spike times";
+        if (k > s0 && t < d.pt_spike_times[k - 1]) return "spike times must be sorted";
+        pattern.push_back({t, (int32_t)p});
+      }
+      continue;
+    }
     const int g = group_of(mech);
     int64_t n = -1;
     const int c = d.pt_cell[p];
@@ -436,6 +451,20 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
   }
   L.first_virtual = L.nsources;
   L.min_micro_delay = L.min_delay;
+  std::stable_sort(pattern.begin(), pattern.end(),
+                   [](const std::pair<double, int32_t>& a, const std::pair<double, int32_t>& b) {
+                     return a.first < b.first;
+                   });
+  {
+    std::vector<std::vector<int32_t>> sources_of(d.n_points);
+    for (int64_t s = 0; s < d.n_sources; ++s)
+      if (L.source_point[s] >= 0) sources_of[L.source_point[s]].push_back((int32_t)s);
+    for (const auto& [t, p] : pattern)
+      for (int32_t s : sources_of[p]) {
+        L.pattern_time.push_back(t);
+        L.pattern_source.push_back(s);
+      }
+  }
   const int gn = L.group_of_mech[MIND_MECH_NETSTIM];
   if (gn >= 0) {
     const MechGroup& G = L.groups[gn];

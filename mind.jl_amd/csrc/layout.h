@@ -60,6 +60,10 @@ struct Layout {
   std::vector<int64_t> source_sid;
   std::vector<int32_t> source_point;  // point index of an event source, -1 for detectors
   std::vector<uint8_t> source_external;  // 1 for external spike inputs (inject_spike!)
+  // PatternStim spikes as (time, source), in the order initialize enqueues them:
+  // by time (stable over point creation), then the point's sources in order
+  std::vector<double> pattern_time;
+  std::vector<int32_t> pattern_source;
   std::vector<int32_t> det_node, det_source, det_cell;
   std::vector<double> det_threshold;
   std::vector<int64_t> cell_det_off;  // detectors of cell c (indices into det_*)

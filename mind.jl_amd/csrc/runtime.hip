@@ -58,7 +58,7 @@ __global__ void k_init_group(Dev d, int gi) {
   phase_initial(G.mech, cols(G), r, d.v[G.node[r]], d.celsius);
 }
 
-__global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest) {
+__global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest, uint64_t seq0) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= d.ns_rows) return;
   const GroupDev& G = d.g[d.ns_group];
@@ -78,7 +78,7 @@ __global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest) 
     }
     d.ns_t[r] = 0.0 + o.send_delay;
     d.ns_flag[r] = o.send_flag;
-    d.ns_seq[r] = (uint64_t)r + 1;  // INITIAL flush orders NetStims by rank
+    d.ns_seq[r] = seq0 + (uint64_t)r + 1;  // INITIAL flush orders NetStims by rank, after the pattern spikes
     d.ns_edge[r] = 0;
     d.ns_valid[r] = 1;
   }
@@ -892,9 +892,14 @@ int mind_initialize(mind_state* state) {
   S.step = 0;
   S.pending_events.clear();  // injected input does not survive initialize!
   S.pending_spikes.clear();
+  // PatternStim spikes: source events enqueued before any INITIAL block runs
+  // (reset_events!), so they take the first insertion orders
+  const uint64_t npattern = S.L.pattern_time.size();
+  for (uint64_t k = 0; k < npattern; ++k)
+    S.pending_spikes.push_back(mind::InjSpike{S.L.pattern_time[k], k + 1, S.L.pattern_source[k], 0});
   HIP_TRY(hipMemsetAsync(d.inj_count, 0, sizeof(int32_t), S.stream));
   HIP_TRY(hipMemsetAsync(d.error, 0, sizeof(int32_t), S.stream));
-  mind::Clock clock{0, 0, 0, 0, 0};
+  mind::Clock clock{0, 0, npattern, 0, 0};
   HIP_TRY(hipMemcpyAsync(d.clock, &clock, sizeof(clock), hipMemcpyHostToDevice, S.stream));
   HIP_TRY(hipMemsetAsync(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots, S.stream));
   for (size_t gi = 0; gi < S.groups_host.size(); ++gi)
@@ -909,15 +914,16 @@ int mind_initialize(mind_state* state) {
   }
   if (d.ns_rows > 0) {
     hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
-                       S.ns_logical, mind::kNetStimDigest);
+                       S.ns_logical, mind::kNetStimDigest, npattern);
     // the netstim sequence counter continues after the INITIAL insertions
-    mind::Clock c2{0, 0, (uint64_t)d.ns_rows, 0, 0};
+    mind::Clock c2{0, 0, npattern + (uint64_t)d.ns_rows, 0, 0};
     HIP_TRY(hipMemcpyAsync(d.clock, &c2, sizeof(c2), hipMemcpyHostToDevice, S.stream));
   }
   // Spikes of the time-zero delivery are kept for the first run (step.jl:47-49).
   d.record_spikes = 1;
   HIP_TRY(hipMemsetAsync(d.spike_count, 0, sizeof(unsigned long long), S.stream));
-  // time-zero delivery (NetStim events at t <= 0), then currents at t = 0
+  // time-zero delivery (NetStim events and pattern spikes at t <= 0), then currents at t = 0
+  if (int rcs = mind::stage_injections(S, 0)) return rcs;
   hipLaunchKernelGGL(mind::k_exchange, dim3(1), dim3(mind::kBlock), 0, S.stream, d, 0);
   for (int gi = 0; gi < d.ngroups; ++gi) {
     if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
@@ -1169,6 +1175,7 @@ int mind_read_value(mind_state* state, const mind_record* record, double* out) {
   if (record->kind == 1 && record->point >= 0 && record->point < (int32_t)L.point_group.size() &&
       L.point_cell[record->point] < 0) {
     g = L.point_group[record->point];
+    if (g < 0) return fail(MIND_EARG, "PatternStim has no fields");
     if (record->field < 0 || record->field >= mind::field_count(L.groups[g].mech)) return fail(MIND_EARG, "unknown field");
     kind = 1;
     field = record->field;

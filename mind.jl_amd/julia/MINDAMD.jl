@@ -81,6 +81,8 @@ struct ModelDesc
     con_target::Ptr{Int32}
     con_weight::Ptr{Float64}
     con_delay::Ptr{Float64}
+    pt_spike_offsets::Ptr{Int64}   # PatternStim spike times (CSR over points), or NULL
+    pt_spike_times::Ptr{Float64}
 end
 
 struct BuildOptions
@@ -160,6 +162,8 @@ Base.@kwdef mutable struct ModelArrays
     con_target::Vector{Int32} = Int32[]
     con_weight::Vector{Float64} = Float64[]
     con_delay::Vector{Float64} = Float64[]
+    pt_spike_offsets::Vector{Int64} = Int64[]   # empty, or one offset per point + 1
+    pt_spike_times::Vector{Float64} = Float64[]
 end
 
 function descriptor(a::ModelArrays)
@@ -172,7 +176,8 @@ function descriptor(a::ModelArrays)
               p(a.pt_cell), p(a.pt_section), p(a.pt_x), p(a.pt_param_offsets), p(a.param_field),
               p(a.param_value), length(a.src_kind), p(a.src_kind), p(a.src_cell), p(a.src_section),
               p(a.src_x), p(a.src_threshold), p(a.src_point), p(a.src_sid), length(a.con_source),
-              p(a.con_source), p(a.con_target), p(a.con_weight), p(a.con_delay))
+              p(a.con_source), p(a.con_target), p(a.con_weight), p(a.con_delay),
+              p(a.pt_spike_offsets), p(a.pt_spike_times))
 end
 
 # ---- calls -----------------------------------------------------------------------

@@ -75,6 +75,8 @@ class ModelDesc(C.Structure):
         ("con_target", _p(C.c_int32)),
         ("con_weight", _p(C.c_double)),
         ("con_delay", _p(C.c_double)),
+        ("pt_spike_offsets", _p(C.c_int64)),
+        ("pt_spike_times", _p(C.c_double)),
     ]
 
 

@@ -1,7 +1,7 @@
 """Builtin mechanism table: names, ids and field columns.
 
 Mirrors the reference's `builtin(:name)` (src/compiler/frontend.jl:58-72) for
-the six mechanisms of the simulation step. Field order is the column order of
+the eight mechanisms of the simulation step. Field order is the column order of
 the packed mechanism data (include/mind.h).
 """
 from __future__ import annotations
@@ -42,6 +42,7 @@ _TABLE = {
         ("interval", "number", "start", "noise", "event", "on", "ispike"), True,
     ),
     "AlphaSynapse": Mechanism("AlphaSynapse", 6, "point", ("onset", "tau", "gmax", "e", "i", "g")),
+    "PatternStim": Mechanism("PatternStim", 7, "artificial", ()),
 }
 
 BY_ID = {m.id: m for m in _TABLE.values()}
@@ -49,7 +50,7 @@ BY_ID = {m.id: m for m in _TABLE.values()}
 
 def builtin(name) -> Mechanism:
     """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim,
-    AlphaSynapse)."""
+    AlphaSynapse, PatternStim)."""
     key = str(name).lstrip(":")
     if key not in _TABLE:
         raise ValueError(f"unknown mechanism {key}; load its MOD with load_mod")

@@ -252,9 +252,18 @@ class MicroModel:
             self.density.append((cell.index, s, mech.id, pv))
         return cell
 
-    def add_pointprocess(self, target, mechanism, section="soma", x=0.5, params=None):
-        """Create a point process on a cell, or an artificial cell on the model."""
+    def add_pointprocess(self, target, mechanism, section="soma", x=0.5, params=None, spike_times=()):
+        """Create a point process on a cell, or an artificial cell on the model
+        (create_pointprocess!, model/micro/utils/model.jl:72-105). `spike_times` are the
+        spikes a PatternStim emits: finite, nonnegative and sorted."""
         mech = resolve(mechanism)
+        times = [float(t) for t in spike_times]
+        if not all(np.isfinite(t) and t >= 0.0 for t in times):
+            raise ValueError("invalid spike times")
+        if any(b < a for a, b in zip(times, times[1:])):
+            raise ValueError("spike times must be sorted")
+        if times and mech.name != "PatternStim":
+            raise ValueError("spike_times applies to PatternStim only")
         if isinstance(target, MicroModel):
             if mech.kind != "artificial":
                 raise ValueError(f"{mech.name} needs a cell location")
@@ -266,7 +275,9 @@ class MicroModel:
             sec = self._morph(target).section_index(section)
             if not 0.0 <= x <= 1.0:
                 raise ValueError("section location must be in [0, 1]")
-        self.points.append((mech.id, cell, sec, float(x), self._params(mech, params or {})))
+            if mech.kind == "artificial":  # an artificial cell reads no voltage: location unused
+                cell, sec, x = -1, -1, 0.0
+        self.points.append((mech.id, cell, sec, float(x), self._params(mech, params or {}), times))
         return PointProcess(self, len(self.points) - 1)
 
     # -- sources and connections --------------------------------------------
@@ -400,6 +411,11 @@ class MicroModel:
         d.pt_x = _abi.as_ptr(arr("px", [e[3] for e in self.points], np.float64), C.c_double)
         d.pt_param_offsets = _abi.as_ptr(
             arr("ppo", params_csr(self.points, lambda e: e[4]), np.int64), C.c_int64)
+        if any(e[5] for e in self.points):
+            spike_off = np.cumsum([0] + [len(e[5]) for e in self.points])
+            d.pt_spike_offsets = _abi.as_ptr(arr("pso", spike_off, np.int64), C.c_int64)
+            d.pt_spike_times = _abi.as_ptr(
+                arr("pst", [t for e in self.points for t in e[5]], np.float64), C.c_double)
         d.param_field = _abi.as_ptr(arr("pf", pf, np.int32), C.c_int32)
         d.param_value = _abi.as_ptr(arr("pv", pv, np.float64), C.c_double)
         S = self.sources
@@ -435,9 +451,9 @@ def insert(cell, mechanism, sections="all", params=None):
     return cell.model.insert(cell, mechanism, sections, params)
 
 
-def add_pointprocess(target, mechanism, section="soma", x=0.5, params=None):
+def add_pointprocess(target, mechanism, section="soma", x=0.5, params=None, spike_times=()):
     model = target if isinstance(target, MicroModel) else target.model
-    return model.add_pointprocess(target, mechanism, section, x, params)
+    return model.add_pointprocess(target, mechanism, section, x, params, spike_times)
 
 
 def register_spike_source(target, **kw):

@@ -254,9 +254,10 @@ Template discretize(const std::vector<SectionIn>& secs) {
 }
 
 // ------------------------------------------------------------ Mechanisms ----
-const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS,
-                                      ICLAMP_NFIELDS, NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS};
-const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true};
+const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,    EXPSYN_NFIELDS,
+                                      EXP2SYN_NFIELDS, ICLAMP_NFIELDS, NETSTIM_NFIELDS,
+                                      ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS};
+const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true, true};
 
 void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
   switch (mech) {
@@ -367,6 +368,8 @@ struct Sim {
   std::vector<uint8_t> feedback_group;
   int64_t next_order = 0;
   std::vector<int64_t> connection_edge;  // edge of each connection in creation order
+  std::vector<double> pattern_time;      // PatternStim spikes (time, source) in enqueue order
+  std::vector<int32_t> pattern_source;
   std::vector<int32_t> source_external;  // external spike inputs
   int64_t step = 0;
   int error = 0;
@@ -944,7 +947,16 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
   }
   s.point_group.resize(d->n_points);
   s.point_row.resize(d->n_points);
+  std::vector<std::pair<double, int32_t>> pattern;  // PatternStim (time, point)
   for (int64_t p = 0; p < d->n_points; ++p) {
+    if (d->pt_mech[p] == MIND_MECH_PATTERNSTIM) {  // native: its spike times, no mechanism data
+      s.point_group[p] = -1;
+      s.point_row[p] = -1;
+      if (d->pt_spike_offsets)
+        for (int64_t k = d->pt_spike_offsets[p]; k < d->pt_spike_offsets[p + 1]; ++k)
+          pattern.push_back({d->pt_spike_times[k], (int32_t)p});
+      continue;
+    }
     const int gi = group_for(d->pt_mech[p]);
     Group& g = s.groups[gi];
     if ((int)row_params.size() <= gi) row_params.resize(gi + 1);
@@ -1008,6 +1020,18 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
     }
   }
   s.det_above.assign(s.det_node.size(), 0);
+  // PatternStim spikes: by time (stable over point creation), then the point's
+  // sources in order (pack_micro, model.jl:604-613; reset_events!, utils/events.jl:92-110)
+  std::stable_sort(pattern.begin(), pattern.end(),
+                   [](const std::pair<double, int32_t>& a, const std::pair<double, int32_t>& b) {
+                     return a.first < b.first;
+                   });
+  for (const auto& [t, p] : pattern)
+    for (int64_t k = 0; k < d->n_sources; ++k)
+      if (d->src_kind[k] == 1 && d->src_point[k] == p) {
+        s.pattern_time.push_back(t);
+        s.pattern_source.push_back((int32_t)k);
+      }
   s.fanout.assign(d->n_sources + 1, 0);
   for (int64_t c = 0; c < d->n_connections; ++c) s.fanout[d->con_source[c] + 1] += 1;
   for (int64_t k = 0; k < d->n_sources; ++k) s.fanout[k + 1] += s.fanout[k];
@@ -1057,6 +1081,8 @@ int oracle_initialize(void* p) {  // initialize! (initialize.jl:270-429)
   s.heap.clear();
   s.outbox.clear();
   s.next_order = 0;
+  for (size_t k = 0; k < s.pattern_time.size(); ++k)  // reset_events!: pattern spikes first
+    enqueue(s, Event{s.pattern_time[k], 0.0, 0, 0, -(s.pattern_source[k] + 1)});
   std::fill(s.sequences.begin(), s.sequences.end(), 0);
   std::fill(s.det_above.begin(), s.det_above.end(), 0);
   s.spike_t.clear();

@@ -302,6 +302,47 @@ def test_injected_spike_routes_through_every_connection():
         oracle.OracleSim(m).initialize().inject_spike(det, 1.0)  # not an external input
 
 
+def test_pattern_stim_raster_and_delivery():
+    """PatternStim (builtin ARTIFICIAL_CELL with spike_times): every given spike is
+    recorded with the SID of the point's source, sorted by (time, SID), including one at
+    t = 0 and a tie between two points; each reaches its synapse after the connection's
+    delay, so the conductance is the sum of the Exp2Syn closed forms. A point without a
+    source emits nothing."""
+    m, cell = passive_cell(v0=-65.0)
+    syn = m.add_pointprocess(cell, "Exp2Syn", params={"tau1": 0.5, "tau2": 5.0, "e": 0.0})
+    p1 = m.add_pointprocess(m, "PatternStim", spike_times=[0.0, 5.0, 12.3])
+    p2 = m.add_pointprocess(m, "PatternStim", spike_times=[5.0, 7.5])
+    m.add_pointprocess(m, "PatternStim", spike_times=[1.0])  # no source
+    s2 = m.register_spike_source(p2, sid=11)
+    s1 = m.register_spike_source(p1, sid=20)
+    m.connect(s1, syn, weight=1e-3, delay=1.0)
+    m.connect(s2, syn, weight=3e-3, delay=2.0)
+    recs = [mechanism_record(m, syn, "A"), mechanism_record(m, syn, "B")]
+    r = oracle.OracleSim(m).initialize().run(30.0, recs)
+    assert r.spike_times.tolist() == [0.0, 5.0, 5.0, 7.5, 12.3]
+    assert r.spike_sources.tolist() == [20, 11, 20, 11, 20]
+    arrivals = [(1.0, 1e-3), (6.0, 1e-3), (13.3, 1e-3), (7.0, 3e-3), (9.5, 3e-3)]
+    # an arrival at t is applied at the end of the step whose cutoff reaches it
+    expect = sum(_exp2syn_closed_form(r.time, 0.025 * math.ceil(t / 0.025 - 1e-9), w) for t, w in arrivals)
+    np.testing.assert_allclose(r.traces["B"] - r.traces["A"], expect, rtol=1e-9, atol=1e-15)
+
+
+def test_pattern_stim_validation():
+    m, cell = passive_cell(v0=-65.0)
+    with pytest.raises(ValueError, match="sorted"):
+        m.add_pointprocess(m, "PatternStim", spike_times=[2.0, 1.0])
+    with pytest.raises(ValueError, match="invalid"):
+        m.add_pointprocess(m, "PatternStim", spike_times=[-1.0])
+    with pytest.raises(ValueError, match="invalid"):
+        m.add_pointprocess(m, "PatternStim", spike_times=[float("nan")])
+    with pytest.raises(ValueError, match="PatternStim only"):
+        m.add_pointprocess(cell, "ExpSyn", spike_times=[1.0])
+    placed = m.add_pointprocess(cell, "PatternStim", spike_times=[1.0])  # location unused
+    assert m.points[placed.index][1] == -1
+    with pytest.raises(ValueError, match="NET_RECEIVE"):
+        m.connect(m.register_spike_source(cell), m.add_pointprocess(m, "PatternStim"))
+
+
 def test_alpha_synapse_conductance_is_the_alpha_function():
     """AlphaSynapse (AlphaSynapse.mod): g = gmax x e^(1-x), x = (t - onset)/tau, at the
     current evaluation t + dt/2, zero outside 0 <= x <= 10; peak gmax at x = 1. The
