@@ -37,7 +37,8 @@ enum {
   MIND_EUNSUPPORTED = 10 /* model feature outside the device path */
 };
 
-/* Builtin mechanisms (reference mod/{hh,pas,ExpSyn,Exp2Syn,IClamp,NetStim,AlphaSynapse}.mod). */
+/* Builtin mechanisms (reference mod/{hh,pas,ExpSyn,Exp2Syn,IClamp,NetStim,AlphaSynapse,
+ * PatternStim,SEClamp}.mod). */
 enum {
   MIND_MECH_HH = 0,
   MIND_MECH_PAS = 1,
@@ -47,7 +48,8 @@ enum {
   MIND_MECH_NETSTIM = 5,
   MIND_MECH_ALPHASYNAPSE = 6,
   MIND_MECH_PATTERNSTIM = 7, /* artificial cell that emits given spike times (no fields) */
-  MIND_MECH_COUNT = 8
+  MIND_MECH_SECLAMP = 8,     /* single-electrode three-level voltage clamp */
+  MIND_MECH_COUNT = 9
 };
 
 /* Field columns of each builtin mechanism (parameters, states, assigned). */
@@ -63,6 +65,8 @@ enum { NETSTIM_INTERVAL, NETSTIM_NUMBER, NETSTIM_START, NETSTIM_NOISE, NETSTIM_E
 enum { ALPHASYNAPSE_ONSET, ALPHASYNAPSE_TAU, ALPHASYNAPSE_GMAX, ALPHASYNAPSE_E, ALPHASYNAPSE_I,
        ALPHASYNAPSE_G, ALPHASYNAPSE_NFIELDS };
 enum { PATTERNSTIM_NFIELDS = 0 };
+enum { SECLAMP_RS, SECLAMP_DUR1, SECLAMP_AMP1, SECLAMP_DUR2, SECLAMP_AMP2, SECLAMP_DUR3, SECLAMP_AMP3,
+       SECLAMP_I, SECLAMP_VC, SECLAMP_TC2, SECLAMP_TC3, SECLAMP_ON, SECLAMP_NFIELDS };
 
 /* A flattened MicroModel (src/model/micro/model.jl:35-50) plus its sections
  * (src/model/micro/morphology.jl:17-29). Section indices are local to their

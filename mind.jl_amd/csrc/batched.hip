@@ -574,7 +574,7 @@ __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, 
 
 // Mechanisms with a state update (nrn_state).
 __device__ __forceinline__ bool stateful(int32_t mech) {
-  return mech == MIND_MECH_HH || mech == MIND_MECH_EXP2SYN || mech == MIND_MECH_EXPSYN;
+  return mech_has_state(mech);
 }
 
 // Per-window constants of the step's critical path: mechanism groups, their row

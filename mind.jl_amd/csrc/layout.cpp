@@ -20,7 +20,7 @@ namespace mind {
 int field_count(int mech) {
   static const int n[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,     EXPSYN_NFIELDS,
                                          EXP2SYN_NFIELDS, ICLAMP_NFIELDS,  NETSTIM_NFIELDS,
-                                         ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS};
+                                         ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS};
   return n[mech];
 }
 
@@ -40,6 +40,7 @@ void field_defaults(int mech, double* f) {
       f[NETSTIM_INTERVAL] = 10.0; f[NETSTIM_NUMBER] = 10.0; f[NETSTIM_START] = 50.0;
       break;
     case MIND_MECH_ALPHASYNAPSE: f[ALPHASYNAPSE_TAU] = 0.1; break;  // AlphaSynapse.mod PARAMETER
+    case MIND_MECH_SECLAMP: f[SECLAMP_RS] = 1.0; break;              // SEClamp.mod PARAMETER
     default: break;
   }
 }

@@ -1,5 +1,5 @@
 // Device (and host) lowering of the seven builtin mechanisms of the simulation
-// step: hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim and AlphaSynapse.
+// step: hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim, AlphaSynapse and SEClamp.
 //
 // Each function is the nrn_cur / nrn_state / NET_RECEIVE / INITIAL phase of one
 // MOD file as the reference lowers it (src/compiler/codegen.jl:362-485):
@@ -118,13 +118,31 @@ MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t
         if (probe == 1 && store) { c(ALPHASYNAPSE_G, r) = g; c(ALPHASYNAPSE_I, r) = i; }
         break;
       }
+      case MIND_MECH_SECLAMP: {  // SEClamp.mod BREAKPOINT without its SOLVE: vstim(), which calls icur()
+        double on = 1.0, vc = 0.0;
+        if (t < c(SECLAMP_DUR1, r)) vc = c(SECLAMP_AMP1, r);
+        else if (t < c(SECLAMP_TC2, r)) vc = c(SECLAMP_AMP2, r);
+        else if (t < c(SECLAMP_TC3, r)) vc = c(SECLAMP_AMP3, r);
+        else on = 0.0;
+        const double i = on != 0.0 ? (vc - vv) / c(SECLAMP_RS, r) : 0.0;
+        sum -= i;
+        if (probe == 1 && store) { c(SECLAMP_ON, r) = on; c(SECLAMP_VC, r) = vc; c(SECLAMP_I, r) = i; }
+        break;
+      }
       default: break;
     }
     if (probe == 0) shifted = sum; else value = sum;
   }
 }
 
-// State phase at t + dt: hoisted SOLVE of BREAKPOINT (pas and IClamp have none).
+// Mechanisms whose BREAKPOINT has a SOLVE, i.e. a state phase (nrn_state).
+MIND_HD bool mech_has_state(int mech) {
+  return mech == MIND_MECH_HH || mech == MIND_MECH_EXP2SYN || mech == MIND_MECH_EXPSYN ||
+         mech == MIND_MECH_SECLAMP;
+}
+
+// State phase at t + dt: hoisted SOLVE of BREAKPOINT (pas and IClamp have none;
+// SEClamp's is `SOLVE icur METHOD after_cvode`, the clamp current at the new v).
 MIND_HD void mech_advance(int mech, const Cols& c, int64_t r, double v, double dt,
                           double celsius) {
   switch (mech) {
@@ -143,6 +161,9 @@ MIND_HD void mech_advance(int mech, const Cols& c, int64_t r, double v, double d
       break;
     case MIND_MECH_EXPSYN:
       c(EXPSYN_G, r) = cnexp_relax(c(EXPSYN_G, r), 0.0, c(EXPSYN_TAU, r), dt);
+      break;
+    case MIND_MECH_SECLAMP:
+      c(SECLAMP_I, r) = c(SECLAMP_ON, r) != 0.0 ? (c(SECLAMP_VC, r) - v) / c(SECLAMP_RS, r) : 0.0;
       break;
     default: break;
   }
@@ -173,6 +194,11 @@ MIND_HD void mech_initial(int mech, const Cols& c, int64_t r, double v, double c
     }
     case MIND_MECH_EXPSYN: c(EXPSYN_G, r) = 0.0; break;
     case MIND_MECH_ICLAMP: c(ICLAMP_I, r) = 0.0; break;
+    case MIND_MECH_SECLAMP:  // SEClamp.mod INITIAL
+      c(SECLAMP_TC2, r) = c(SECLAMP_DUR1, r) + c(SECLAMP_DUR2, r);
+      c(SECLAMP_TC3, r) = c(SECLAMP_TC2, r) + c(SECLAMP_DUR3, r);
+      c(SECLAMP_ON, r) = 0.0;
+      break;
     default: break;
   }
 }

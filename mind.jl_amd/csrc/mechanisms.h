@@ -13,7 +13,8 @@
 namespace mind {
 
 #ifdef MIND_GENERATED_MECHANISMS
-static_assert(gen::kComplete, "the generated mechanisms must cover hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim and AlphaSynapse");
+static_assert(gen::kComplete,
+              "the generated mechanisms must cover hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim, AlphaSynapse and SEClamp");
 constexpr const char* kMechanismSource = gen::kSource;
 constexpr uint64_t kNetStimDigest = gen::kNetStimDigest;
 MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double t, double& value, double& shifted,

@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
     // nrn_state at t + dt
     for (int gi = 0; gi < d.ngroups; ++gi) {
       const GroupDev& G = d.g[gi];
-      if (G.mech != MIND_MECH_HH && G.mech != MIND_MECH_EXP2SYN && G.mech != MIND_MECH_EXPSYN) continue;
+      if (!mech_has_state(G.mech)) continue;
       const Cols col = cols(G);
       for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r)
         phase_advance(G.mech, col, r, v[G.node[r]], dt, d.celsius);

@@ -1,7 +1,7 @@
 """Builtin mechanism table: names, ids and field columns.
 
 Mirrors the reference's `builtin(:name)` (src/compiler/frontend.jl:58-72) for
-the eight mechanisms of the simulation step. Field order is the column order of
+the nine mechanisms of the simulation step. Field order is the column order of
 the packed mechanism data (include/mind.h).
 """
 from __future__ import annotations
@@ -43,6 +43,10 @@ _TABLE = {
     ),
     "AlphaSynapse": Mechanism("AlphaSynapse", 6, "point", ("onset", "tau", "gmax", "e", "i", "g")),
     "PatternStim": Mechanism("PatternStim", 7, "artificial", ()),
+    "SEClamp": Mechanism(
+        "SEClamp", 8, "point",
+        ("rs", "dur1", "amp1", "dur2", "amp2", "dur3", "amp3", "i", "vc", "tc2", "tc3", "on"),
+    ),
 }
 
 BY_ID = {m.id: m for m in _TABLE.values()}
@@ -50,7 +54,7 @@ BY_ID = {m.id: m for m in _TABLE.values()}
 
 def builtin(name) -> Mechanism:
     """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim,
-    AlphaSynapse, PatternStim)."""
+    AlphaSynapse, PatternStim, SEClamp)."""
     key = str(name).lstrip(":")
     if key not in _TABLE:
         raise ValueError(f"unknown mechanism {key}; load its MOD with load_mod")

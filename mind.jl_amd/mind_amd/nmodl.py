@@ -42,7 +42,7 @@ EXEC_INPUTS = ("v", "t", "dt", "celsius")
 FUNCS = {"exp": "mind_exp", "log": "mind_log", "fabs": "mind_fabs", "sqrt": "sqrt"}
 BUILTIN_IDS = {"hh": "MIND_MECH_HH", "pas": "MIND_MECH_PAS", "ExpSyn": "MIND_MECH_EXPSYN",
                "Exp2Syn": "MIND_MECH_EXP2SYN", "IClamp": "MIND_MECH_ICLAMP", "NetStim": "MIND_MECH_NETSTIM",
-               "AlphaSynapse": "MIND_MECH_ALPHASYNAPSE"}
+               "AlphaSynapse": "MIND_MECH_ALPHASYNAPSE", "SEClamp": "MIND_MECH_SECLAMP"}
 
 
 class ModError(Exception):
@@ -185,6 +185,7 @@ class Mechanism:
     randoms: list = field(default_factory=list)
     decls: list = field(default_factory=list)         # (name, storage, default)
     blocks: dict = field(default_factory=dict)
+    defines: dict = field(default_factory=dict)       # DEFINE name value (named constants)
 
     @property
     def fields(self):
@@ -269,6 +270,12 @@ class Parser:
                 continue
             elif k == "INDEPENDENT":
                 self.skip_block()
+            elif k == "DEFINE":
+                name = self.ident()
+                v = self.take()
+                if v.kind != "num":
+                    self.fail(v, "DEFINE takes a number")
+                self.m.defines[name] = float(v.text)
             elif k == "LOCAL":
                 self.names_list()
             elif k in ("INITIAL", "BREAKPOINT", "NET_RECEIVE", "DERIVATIVE", "PROCEDURE", "FUNCTION"):
@@ -384,7 +391,7 @@ class Parser:
             method = "cnexp"
             if self.accept("METHOD"):
                 method = self.ident()
-            if method not in ("cnexp", "euler"):
+            if method not in ("cnexp", "euler", "after_cvode"):
                 self.fail(t, f"SOLVE METHOD {method} is not supported")
             return Solve(blk, method, t.line)
         if self.accept("TABLE"):
@@ -558,13 +565,17 @@ class Gen:
             self.err(f"{c.name} needs a RANDOM variable")
 
     def base_scope(self, ctx_v="x.v"):
-        s = {n: f"c({self.p}_{n}, r)" for n in self.fields}
+        s = {n: lit(val) for n, val in self.m.defines.items()}
+        s.update({n: f"c({self.p}_{n}, r)" for n in self.fields})
         s.update({"v": ctx_v, "t": "x.t", "dt": "x.dt", "celsius": "x.celsius"})
         for rv in self.m.randoms:
             s[rv] = "0.0"
         return s
 
-    def stmts(self, body, scope, out, ind, deriv_method=None):
+    def stmts(self, body, scope, out, ind, deriv_method=None, inline=()):
+        """Statements of `body` into `out`. `inline` is the chain of PROCEDUREs being
+        expanded in place (the current phase works on local copies of the fields it
+        writes, so its procedure calls are expanded instead of called)."""
         scope = dict(scope)
         for st in body:
             if isinstance(st, Local):
@@ -585,16 +596,32 @@ class Gen:
                 if isinstance(st.cond, Binary) and st.cond.op != "^":
                     cond = cond[1:-1]  # the if's own parentheses
                 out.append(f"{ind}if ({cond}) {{")
-                self.stmts(st.then, scope, out, ind + "  ", deriv_method)
+                self.stmts(st.then, scope, out, ind + "  ", deriv_method, inline)
                 if st.other:
                     out.append(f"{ind}}} else {{")
-                    self.stmts(st.other, scope, out, ind + "  ", deriv_method)
+                    self.stmts(st.other, scope, out, ind + "  ", deriv_method, inline)
                 out.append(f"{ind}}}")
+            elif isinstance(st, CallStmt) and inline and st.call.name in self.procs:
+                self.inline_proc(st, scope, out, ind, inline)
             elif isinstance(st, CallStmt):
                 out.append(ind + self.call_stmt(st, scope) + ";")
             elif isinstance(st, Solve):
                 self.err("SOLVE outside BREAKPOINT", st.line)
         return scope
+
+    def inline_proc(self, st, scope, out, ind, chain):
+        p = self.procs[st.call.name]
+        if p.name in chain:
+            self.err(f"PROCEDURE {p.name} calls itself", st.line)
+        if len(st.call.args) != len(p.args):
+            self.err(f"{p.name} takes {len(p.args)} arguments", st.line)
+        out.append(f"{ind}{{  // PROCEDURE {p.name}")
+        inner = dict(scope)
+        for a, x in zip(p.args, st.call.args):
+            out.append(f"{ind}  const double p_{p.name}_{a} = {self.e(x, scope)};")
+            inner[a] = f"p_{p.name}_{a}"
+        self.stmts(p.body, inner, out, ind + "  ", None, tuple(chain) + (p.name,))
+        out.append(f"{ind}}}")
 
     def call_stmt(self, st, scope):
         c = st.call
@@ -712,7 +739,7 @@ class Gen:
         for n in written:  # assigned values of this evaluation, stored after the probe at v
             lines.append(f"    double w_{n} = c({p}_{n}, r);")
             scope[n] = f"w_{n}"
-        self.stmts(body, scope, lines, "    ")
+        self.stmts(body, scope, lines, "    ", None, ("breakpoint",))
         lines.append("    double sum = 0.0;")
         for n in currents:
             lines.append(f"    sum = (sum + {scope[n]});")
@@ -763,16 +790,23 @@ class Gen:
         out.append("}")
         return out
 
-    def collect_writes(self, body, acc):
+    def collect_writes(self, body, acc, chain=(), local=frozenset()):
+        """Fields the current phase assigns, in first-assignment order, through the
+        PROCEDUREs it calls (expanded in place)."""
         for st in body:
-            if isinstance(st, Assign) and st.target in self.findex and st.target not in acc:
+            if isinstance(st, Local):
+                local = local | set(st.names)
+            elif isinstance(st, Assign) and st.target in self.findex and st.target not in local \
+                    and st.target not in acc:
                 acc.append(st.target)
             elif isinstance(st, If):
-                self.collect_writes(st.then, acc)
-                self.collect_writes(st.other, acc)
+                self.collect_writes(st.then, acc, chain, local)
+                self.collect_writes(st.other, acc, chain, local)
             elif isinstance(st, CallStmt) and st.call.name in self.procs:
-                self.err(f"BREAKPOINT calls PROCEDURE {st.call.name}; the current phase supports "
-                         "direct assignments only", st.line)
+                p = self.procs[st.call.name]
+                if p.name in chain:
+                    self.err(f"PROCEDURE {p.name} calls itself", st.line)
+                self.collect_writes(p.body, acc, chain + (p.name,), frozenset(p.args))
 
     def advance(self):
         p = self.p
@@ -782,6 +816,8 @@ class Gen:
         for st in (bp.body if bp else []):
             if not isinstance(st, Solve):
                 continue
+            if st.method == "after_cvode" and st.block not in self.procs:
+                self.err("after_cvode in fixed-step execution requires a PROCEDURE", st.line)
             if st.block in self.derivs:
                 d = self.derivs[st.block]
                 out.append("  {")

@@ -256,8 +256,8 @@ Template discretize(const std::vector<SectionIn>& secs) {
 // ------------------------------------------------------------ Mechanisms ----
 const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,    EXPSYN_NFIELDS,
                                       EXP2SYN_NFIELDS, ICLAMP_NFIELDS, NETSTIM_NFIELDS,
-                                      ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS};
-const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true, true};
+                                      ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS};
+const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true, true, true};
 
 void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
   switch (mech) {
@@ -274,6 +274,7 @@ void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
       f[NETSTIM_NOISE] = 0.0;
       break;
     case MIND_MECH_ALPHASYNAPSE: f[ALPHASYNAPSE_TAU] = 0.1; break;  // AlphaSynapse.mod PARAMETER
+    case MIND_MECH_SECLAMP: f[SECLAMP_RS] = 1.0; break;              // SEClamp.mod PARAMETER
   }
 }
 
@@ -679,6 +680,18 @@ void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
               sum = 0.0; sum += i;
               break;
             }
+            case MIND_MECH_SECLAMP: {  // SEClamp.mod vstim() and icur(); at_time has no effect
+              double on = 1.0, vc;
+              if (t < g.at(SECLAMP_DUR1, r)) vc = g.at(SECLAMP_AMP1, r);
+              else if (t < g.at(SECLAMP_TC2, r)) vc = g.at(SECLAMP_AMP2, r);
+              else if (t < g.at(SECLAMP_TC3, r)) vc = g.at(SECLAMP_AMP3, r);
+              else { vc = 0.0; on = 0.0; }
+              double i = 0.0;
+              if (on != 0.0) i = (vc - vv) / g.at(SECLAMP_RS, r);
+              g.at(SECLAMP_ON, r) = on; g.at(SECLAMP_VC, r) = vc; g.at(SECLAMP_I, r) = i;
+              sum = 0.0; sum -= i;  // electrode current
+              break;
+            }
           }
           if (probe == 0) shifted = sum; else value = sum;
         }
@@ -751,6 +764,12 @@ void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
     } else if (g.mech == MIND_MECH_EXPSYN) {
       for (int r = 0; r < rows; ++r)
         g.at(EXPSYN_G, r) = relax(g.at(EXPSYN_G, r), 0.0, g.at(EXPSYN_TAU, r), dt);
+    } else if (g.mech == MIND_MECH_SECLAMP) {  // SOLVE icur METHOD after_cvode: i at the new v
+      for (int r = 0; r < rows; ++r) {
+        double i = 0.0;
+        if (g.at(SECLAMP_ON, r) != 0.0) i = (g.at(SECLAMP_VC, r) - s.v[g.node[r]]) / g.at(SECLAMP_RS, r);
+        g.at(SECLAMP_I, r) = i;
+      }
     }
   }
 }
@@ -808,6 +827,11 @@ void initial_blocks(Sim& s) {  // INITIAL per group (initialize.jl:342-374)
         }
         case MIND_MECH_EXPSYN: g.at(EXPSYN_G, r) = 0.0; break;
         case MIND_MECH_ICLAMP: g.at(ICLAMP_I, r) = 0.0; break;
+        case MIND_MECH_SECLAMP:  // SEClamp.mod INITIAL
+          g.at(SECLAMP_TC2, r) = g.at(SECLAMP_DUR1, r) + g.at(SECLAMP_DUR2, r);
+          g.at(SECLAMP_TC3, r) = g.at(SECLAMP_TC2, r) + g.at(SECLAMP_DUR3, r);
+          g.at(SECLAMP_ON, r) = 0.0;
+          break;
         case MIND_MECH_NETSTIM: netstim_initial(s, g, r); break;
         default: break;
       }

@@ -175,3 +175,23 @@ def test_alpha_synapse_matches_oracle(gpu, method):
         assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
     assert got.traces["g"].max() > 4.9e-3
     destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_seclamp_matches_oracle(gpu, method):
+    """SEClamp at the soma of the single cell (levels switching off the step grid, the
+    clamp releasing into hh spiking), in both kernels: bit-identical to the oracle."""
+    model, info = configs.single_cell()
+    cell = info["cells"][0]
+    clamp = model.add_pointprocess(cell, "SEClamp", section="soma", x=0.5,
+                                   params={"rs": 0.01, "dur1": 5.01, "amp1": -70.0, "dur2": 10.0,
+                                           "amp2": -10.0, "dur3": 4.99, "amp3": -80.0})
+    recs = [voltage_record(model, cell, "soma", 0.5), voltage_record(model, cell, "dend", 0.5, name="vd"),
+            mechanism_record(model, clamp, "i"), mechanism_record(model, clamp, "vc"),
+            mechanism_record(model, clamp, "on")]
+    got, ref, vg, vr, _ = _compare(model, 60.0, recs, method=method)
+    assert np.array_equal(vg, vr)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    assert got.traces["on"][-1] == 0.0 and got.traces["i"].min() < 0.0
+    destroy(model)

@@ -18,7 +18,7 @@ from mind_amd import nmodl
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "mind.jl_amd", "csrc")
 MOD_DIR = "/root/reference/mod"
-BUILTINS = ("hh", "pas", "ExpSyn", "Exp2Syn", "IClamp", "NetStim", "AlphaSynapse")
+BUILTINS = ("hh", "pas", "ExpSyn", "Exp2Syn", "IClamp", "NetStim", "AlphaSynapse", "SEClamp")
 
 LEAKGATE = textwrap.dedent("""
     TITLE toy gated leak
@@ -138,11 +138,13 @@ HARNESS = textwrap.dedent("""
     int main() {
       std::mt19937_64 g(12345);
       auto U = [&](double a, double b) { return std::uniform_real_distribution<double>(a, b)(g); };
-      const int nf[7] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS, ICLAMP_NFIELDS,
-                         NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS};
+      const int nf[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS,
+                                       ICLAMP_NFIELDS, NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS,
+                                       PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS};
       for (int trial = 0; trial < 3000; ++trial) {
-        for (int mech = 0; mech < 7; ++mech) {
+        for (int mech = 0; mech < MIND_MECH_COUNT; ++mech) {
           if (mech == MIND_MECH_NETSTIM) continue;  // through its stream, below
+          if (mech == MIND_MECH_PATTERNSTIM) continue;  // no code: its spike times enter the queue
           double a[32], b[32];
           for (int k = 0; k < nf[mech]; ++k) a[k] = U(0.0, 1.0);
           if (mech == MIND_MECH_HH) { a[HH_GNABAR] = U(0, 0.3); a[HH_EL] = U(-80, -40);
@@ -155,6 +157,10 @@ HARNESS = textwrap.dedent("""
                                           a[ICLAMP_AMP] = U(-1, 1); }
           if (mech == MIND_MECH_ALPHASYNAPSE) { a[ALPHASYNAPSE_ONSET] = U(0, 250); a[ALPHASYNAPSE_TAU] = U(0.05, 20);
                                                 a[ALPHASYNAPSE_GMAX] = U(0, 1e-2); a[ALPHASYNAPSE_E] = U(-80, 0); }
+          if (mech == MIND_MECH_SECLAMP) { a[SECLAMP_RS] = U(1e-3, 10); a[SECLAMP_DUR1] = U(0, 100);
+                                           a[SECLAMP_AMP1] = U(-100, 50); a[SECLAMP_DUR2] = trial % 3 ? U(0, 100) : 0.0;
+                                           a[SECLAMP_AMP2] = U(-100, 50); a[SECLAMP_DUR3] = U(0, 100);
+                                           a[SECLAMP_AMP3] = U(-100, 50); }
           std::memcpy(b, a, sizeof a);
           Cols ca{a, 1}, cb{b, 1};
           const double v = U(-100, 60), t = U(0, 250), dt = trial % 2 ? 0.025 : U(0.001, 0.2),
@@ -237,7 +243,7 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
             for (int k = 0; k < gen::M##_NFIELDS; ++k) std::printf("%s %d %.17g %.17g\\n", #M, k, f[k], h[k]);
           SHOW(hh, MIND_MECH_HH) SHOW(pas, MIND_MECH_PAS) SHOW(ExpSyn, MIND_MECH_EXPSYN)
           SHOW(Exp2Syn, MIND_MECH_EXP2SYN) SHOW(IClamp, MIND_MECH_ICLAMP) SHOW(NetStim, MIND_MECH_NETSTIM)
-          SHOW(AlphaSynapse, MIND_MECH_ALPHASYNAPSE)
+          SHOW(AlphaSynapse, MIND_MECH_ALPHASYNAPSE) SHOW(SEClamp, MIND_MECH_SECLAMP)
         }
     """))
     exe = tmp_path / "defaults"

@@ -343,6 +343,43 @@ def test_pattern_stim_validation():
         m.connect(m.register_spike_source(cell), m.add_pointprocess(m, "PatternStim"))
 
 
+def test_seclamp_on_a_passive_compartment_is_the_implicit_recurrence():
+    """SEClamp (SEClamp.mod) on an isopotential passive compartment (level changes off the
+    step grid, so that the evaluation time matters): with the clamp level
+    vc(t) and on(t) taken at each step's current evaluation time t + dt/2 (amp1 before
+    dur1, amp2 before dur1+dur2, amp3 before dur1+dur2+dur3, then off), a backward-Euler
+    step solves (C/dt + G + on/rs) v' = C/dt v + G e + on vc/rs exactly, since every current
+    is linear in v; the recorded i = on (vc - v')/rs is the after_cvode update at the new v,
+    and at t = 0 it is the initialization's (amp1 - v0)/rs."""
+    m, cell = passive_cell(v0=-65.0)
+    p = {"rs": 0.05, "dur1": 2.01, "amp1": -40.0, "dur2": 3.0, "amp2": -20.0, "dur3": 1.0, "amp3": -50.0}
+    clamp = m.add_pointprocess(cell, "SEClamp", params=p)
+    recs = [voltage_record(m, cell), mechanism_record(m, clamp, "i"), mechanism_record(m, clamp, "vc")]
+    r = oracle.OracleSim(m).initialize().run(10.0, recs)
+    dt, area = 0.025, math.pi * 20.0 * 20.0
+    C, G, e = 1.0 * area * 1e-5, 1e-4 * area * 0.01, -65.0
+    v, vs, iis, vcs = -65.0, [-65.0], [(p["amp1"] + 65.0) / p["rs"]], [p["amp1"]]
+    for k in range(len(r.time) - 1):
+        te = k * dt + 0.5 * dt
+        levels = [(p["dur1"], p["amp1"]), (p["dur1"] + p["dur2"], p["amp2"]),
+                  (p["dur1"] + p["dur2"] + p["dur3"], p["amp3"])]
+        on, vc = 0.0, 0.0
+        for end, amp in levels:
+            if te < end:
+                on, vc = 1.0, amp
+                break
+        gc = on / p["rs"]
+        v = (C / dt * v + G * e + gc * vc) / (C / dt + G + gc)
+        vs.append(v)
+        iis.append(gc * (vc - v))
+        vcs.append(vc)
+    np.testing.assert_allclose(r.traces["v_0"], vs, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(r.traces["i"], iis, rtol=0, atol=1e-6)
+    assert r.traces["vc"].tolist() == vcs
+    assert abs(r.traces["v_0"][int(round(4.0 / dt))] - p["amp2"]) < 0.1  # clamped
+    assert r.traces["i"][-1] == 0.0  # off after dur1 + dur2 + dur3
+
+
 def test_alpha_synapse_conductance_is_the_alpha_function():
     """AlphaSynapse (AlphaSynapse.mod): g = gmax x e^(1-x), x = (t - onset)/tau, at the
     current evaluation t + dt/2, zero outside 0 <= x <= 10; peak gmax at x = 1. The
