@@ -15,3 +15,4 @@ from .model import (  # noqa: F401
     voltage_record, window_steps,
 )
 from .swc import load_swc  # noqa: F401,E402
+from .asc import load_asc  # noqa: F401,E402

@@ -1,6 +1,7 @@
 """On-disk formats (no device): the connectome CSV reader (load_connectivity,
-src/model/macro/model.jl:22-54) on files written here, including quoted labels,
-and its rejections."""
+src/model/macro/model.jl:22-54), the SWC reader (load_swc, morphology.jl:130-338) and
+the Neurolucida ASC reader (load_asc, morphology.jl:344-398) on files written here,
+with closed-form geometry and their rejections."""
 import numpy as np
 import pytest
 
@@ -150,3 +151,124 @@ def test_swc_branches_and_zero_length_sections(tmp_path):
 def test_swc_rejections(tmp_path, text, message):
     with pytest.raises(ValueError, match=message):
         load_swc(_swc(tmp_path, text))
+
+
+# --- Neurolucida ASC (load_asc, src/model/micro/morphology.jl:344-398) --------------
+from mind_amd import MicroModel, load_asc  # noqa: E402
+from mind_amd import asc as A  # noqa: E402
+from mind_amd.model import pack_nodes  # noqa: E402
+
+
+def _ellipse(a, b, center, n=64):
+    cx, cy, cz = center
+    return [(cx + a * np.cos(2 * np.pi * k / n), cy + b * np.sin(2 * np.pi * k / n), cz, 1.0)
+            for k in range(n + 1)]
+
+
+def _asc_text(soma=None, trees=()):
+    parts = []
+    if soma is not None:
+        pts = " ".join(f"({float(x)!r} {float(y)!r} {float(z)!r} {float(d)!r})" for x, y, z, d in soma)
+        parts.append(f'("CellBody"\n  (Color Red)\n  (CellBody)\n  {pts}\n)  ;  End of contour')
+    parts.extend(trees)
+    return "\n".join(parts) + "\n"
+
+
+DEND = """( (Color Yellow)
+  (Dendrite)
+  (0 0 0 2) ; root sample
+  (10, 0, 0, 2)
+  (
+    (20 5 0 1) <(comment markup)> (30 10 0 1) Normal
+  |
+    (10 0 0 1.5) (20 -5 0 -1.5)
+    (
+      (25 -5 0 1)
+    |
+      (20 -15 0 0)
+    )
+  )
+)"""
+
+
+def test_asc_branches_terminals_and_labels(tmp_path):
+    """Branch structure without a soma: the root branch is a root section; a child that
+    does not start at its parent's last sample gets a copy of it with the child's first
+    diameter; one that does is not extended; single-sample branches become three points
+    one diameter long along x; |d| is the diameter and d = 0 becomes 1; markup, comments,
+    commas and words are ignored."""
+    f = tmp_path / "tree.asc"
+    f.write_text(_asc_text(trees=[DEND, DEND.replace("(Dendrite)", "(Apical)"),
+                                  "((Marker) (1 2 3 1))"]))
+    secs = {s.name: s for s in load_asc(f)}
+    assert sorted(secs) == ["apic_0", "apic_1", "apic_2", "apic_3", "apic_4",
+                            "dend_0", "dend_1", "dend_2", "dend_3", "dend_4"]
+    d = secs
+    assert d["dend_0"].parent is None and d["dend_0"].parent_x == 1.0
+    assert d["dend_0"].points == [(0.0, 0.0, 0.0, 2.0), (10.0, 0.0, 0.0, 2.0)]
+    assert d["dend_1"].parent == "dend_0" and d["dend_1"].parent_x == 1.0
+    assert d["dend_1"].points == [(10.0, 0.0, 0.0, 1.0), (20.0, 5.0, 0.0, 1.0), (30.0, 10.0, 0.0, 1.0)]
+    assert d["dend_2"].points == [(10.0, 0.0, 0.0, 1.5), (20.0, -5.0, 0.0, 1.5)]  # starts at the parent
+    assert d["dend_3"].parent == "dend_2"
+    assert d["dend_3"].points == [(20.0, -5.0, 0.0, 1.0), (25.0, -5.0, 0.0, 1.0)]
+    assert d["dend_4"].points == [(20.0, -5.0, 0.0, 1.0), (20.0, -15.0, 0.0, 1.0)]
+    one = A.branches(A.parse("((Axon) (4 5 6 2))")[0])
+    assert [b.points for b in one] == [[(4.0, 5.0, 6.0, 2.0)]]
+    f2 = tmp_path / "one.asc"
+    f2.write_text("((Axon) (4 5 6 2))\n")
+    (s,) = load_asc(f2)
+    assert s.name == "axon_0" and s.points == [(3.0, 5.0, 6.0, 2.0), (4.0, 5.0, 6.0, 2.0), (5.0, 5.0, 6.0, 2.0)]
+
+
+def test_asc_soma_contour_is_resampled_along_its_major_axis(tmp_path):
+    """An elliptic CellBody contour (semi-axes 12 along x, 5 along y, centre (1, 2, 3)):
+    21 soma points on the major axis through the centre, spanning it between the
+    second-extreme resampled positions, with diameters close to the chord 2b sqrt(1 -
+    (x/a)^2); dendrites attach to soma_0 at 0.5."""
+    a, b, c = 12.0, 5.0, (1.0, 2.0, 3.0)
+    f = tmp_path / "soma.asc"
+    f.write_text(_asc_text(soma=_ellipse(a, b, c), trees=[DEND]))
+    secs = load_asc(f, nseg=3)
+    soma = secs[0]
+    assert soma.name == "soma_0" and soma.label == "soma" and soma.nseg == 3
+    pts = np.array(soma.points)
+    assert pts.shape == (21, 4)
+    np.testing.assert_allclose(pts[:, 1], c[1], atol=0.05)  # on the major (x) axis
+    np.testing.assert_allclose(pts[:, 2], c[2], atol=1e-9)
+    x = pts[:, 0] - c[0]
+    assert np.all(np.diff(x) > 0) and x[0] < -0.9 * a and x[-1] > 0.9 * a
+    chord = 2 * b * np.sqrt(np.clip(1 - (x / a) ** 2, 0, None))
+    np.testing.assert_allclose(pts[1:-1, 3], chord[1:-1], rtol=0.02, atol=0.05)
+    assert all(s.parent == "soma_0" and s.parent_x == 0.5 for s in secs if s.name in ("dend_0",))
+    # a soma section in a model packs like the oracle's
+    m = MicroModel()
+    m.add_population(secs, 2)
+    import oracle
+    ref = oracle.OracleSim(m).geometry()
+    got = pack_nodes(m)
+    for k in ("parent", "area", "cap", "axial", "adiag"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
+def test_asc_soma_direction_and_degenerate_contours():
+    """The major axis is oriented with its largest component positive; a contour with
+    fewer than three samples, zero length, or a collinear outline gives no soma."""
+    tilted = [(y, x, 0.0, 1.0) for x, y, _, _ in _ellipse(9.0, 3.0, (0.0, 0.0, 0.0))]  # major along y
+    pts = np.array(A.soma_points(tilted))
+    assert np.all(np.diff(pts[:, 1]) > 0) and np.allclose(pts[:, 0], 0.0, atol=0.05)
+    assert A.soma_points([(0, 0, 0, 1), (1, 0, 0, 1)]) == []
+    assert A.soma_points([(1, 1, 1, 1)] * 5) == []
+    assert A.soma_points([(k, 0.0, 0.0, 1.0) for k in range(6)]) == []
+
+
+@pytest.mark.parametrize("text,message", [
+    ("((Dendrite) (0 0 0 1) (1 2 3 1)", "unclosed"),
+    ("((Dendrite) (0 0 inf 1))", "non-finite"),
+    ("((Dendrite) (0 0 1) (1 x 2 1))", "three coordinates"),
+    ('("CellBody" (0 0 0 1) (1 0 0 1) (0 1 0 1)) ("CellBody" (5 0 0 1) (6 0 0 1) (5 1 0 1))', "one CellBody"),
+])
+def test_asc_rejections(tmp_path, text, message):
+    f = tmp_path / "bad.asc"
+    f.write_text(text + "\n")
+    with pytest.raises(ValueError, match=message):
+        load_asc(f)
