@@ -38,7 +38,7 @@ enum {
 };
 
 /* Builtin mechanisms (reference mod/{hh,pas,ExpSyn,Exp2Syn,IClamp,NetStim,AlphaSynapse,
- * PatternStim,SEClamp}.mod). */
+ * PatternStim,SEClamp,VClamp}.mod). */
 enum {
   MIND_MECH_HH = 0,
   MIND_MECH_PAS = 1,
@@ -49,7 +49,8 @@ enum {
   MIND_MECH_ALPHASYNAPSE = 6,
   MIND_MECH_PATTERNSTIM = 7, /* artificial cell that emits given spike times (no fields) */
   MIND_MECH_SECLAMP = 8,     /* single-electrode three-level voltage clamp */
-  MIND_MECH_COUNT = 9
+  MIND_MECH_VCLAMP = 9,      /* two-electrode three-level voltage clamp (amplifier model) */
+  MIND_MECH_COUNT = 10
 };
 
 /* Field columns of each builtin mechanism (parameters, states, assigned). */
@@ -67,6 +68,10 @@ enum { ALPHASYNAPSE_ONSET, ALPHASYNAPSE_TAU, ALPHASYNAPSE_GMAX, ALPHASYNAPSE_E, 
 enum { PATTERNSTIM_NFIELDS = 0 };
 enum { SECLAMP_RS, SECLAMP_DUR1, SECLAMP_AMP1, SECLAMP_DUR2, SECLAMP_AMP2, SECLAMP_DUR3, SECLAMP_AMP3,
        SECLAMP_I, SECLAMP_VC, SECLAMP_TC2, SECLAMP_TC3, SECLAMP_ON, SECLAMP_NFIELDS };
+/* dur[3] and amp[3] are arrays: element k of dur is column VCLAMP_DUR + k. */
+enum { VCLAMP_DUR = 0, VCLAMP_AMP = 3, VCLAMP_GAIN = 6, VCLAMP_RSTIM, VCLAMP_TAU1, VCLAMP_TAU2, VCLAMP_E0,
+       VCLAMP_VO0, VCLAMP_VI0, VCLAMP_FAC, VCLAMP_E, VCLAMP_VO, VCLAMP_VI, VCLAMP_I, VCLAMP_STIM, VCLAMP_TC,
+       VCLAMP_NFIELDS };
 
 /* A flattened MicroModel (src/model/micro/model.jl:35-50) plus its sections
  * (src/model/micro/morphology.jl:17-29). Section indices are local to their

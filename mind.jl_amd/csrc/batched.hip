@@ -560,7 +560,7 @@ __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, 
       if (r + (int64_t)u * T < r1) {
         const int li = nd[u] - (int)n0;
         double value, shifted;
-        Ops::eval(col, rr[u], in[u], v[li], tc, value, shifted, true);
+        Ops::eval(col, rr[u], in[u], v[li], tc, d.dt, value, shifted, true);
         const double scale = area[u] * 0.01;
         double acc_r = rhs[li], acc_d = diag[li];
         acc_r -= scale * value;
@@ -709,7 +709,7 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         int64_t q = r;
         do {  // a run of rows on one node, in row order
           double value, shifted;
-          phase_current(G.mech, col, q, v[li], tc, value, shifted, true);
+          phase_current(G.mech, col, q, v[li], tc, dt, value, shifted, true);
           const double scale = G.point ? 1.0 : d.area[node] * 0.01;
           acc_r -= scale * value;
           acc_d += scale * (shifted - value) / 0.001;
@@ -784,7 +784,7 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         }
         const GroupDev& G = sg[gi];
         r += sr0[gi];
-        phase_advance(G.mech, cols(G), r, v[G.node[r] - n0], dt, d.celsius);
+        phase_advance(G.mech, cols(G), r, v[G.node[r] - n0], t + dt, dt, d.celsius);
       }
     }
     if (rec1 > rec0) {

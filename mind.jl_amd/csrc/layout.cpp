@@ -20,7 +20,8 @@ namespace mind {
 int field_count(int mech) {
   static const int n[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,     EXPSYN_NFIELDS,
                                          EXP2SYN_NFIELDS, ICLAMP_NFIELDS,  NETSTIM_NFIELDS,
-                                         ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS};
+                                         ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS,
+                                      VCLAMP_NFIELDS};
   return n[mech];
 }
 
@@ -41,6 +42,9 @@ void field_defaults(int mech, double* f) {
       break;
     case MIND_MECH_ALPHASYNAPSE: f[ALPHASYNAPSE_TAU] = 0.1; break;  // AlphaSynapse.mod PARAMETER
     case MIND_MECH_SECLAMP: f[SECLAMP_RS] = 1.0; break;              // SEClamp.mod PARAMETER
+    case MIND_MECH_VCLAMP:                                             // VClamp.mod PARAMETER
+      f[VCLAMP_GAIN] = 1e5; f[VCLAMP_RSTIM] = 1.0; f[VCLAMP_TAU1] = 0.001;
+      break;
     default: break;
   }
 }

@@ -1,5 +1,5 @@
 // Device (and host) lowering of the seven builtin mechanisms of the simulation
-// step: hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim, AlphaSynapse and SEClamp.
+// step: hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim, AlphaSynapse, SEClamp and VClamp.
 //
 // Each function is the nrn_cur / nrn_state / NET_RECEIVE / INITIAL phase of one
 // MOD file as the reference lowers it (src/compiler/codegen.jl:362-485):
@@ -56,13 +56,28 @@ struct Cols {
   MIND_HD double& operator()(int field, int64_t row) const { return data[field * rows + row]; }
 };
 
+// VClamp.mod clamp(): the three amplifier equations solved for e, vo and vi from
+// the previous step's e0, vo0, vi0 (VClamp.mod PROCEDURE clamp).
+MIND_HD void vclamp_amplifier(const Cols& c, int64_t r, double v, double dt, double stim, double e0, double vo0,
+                              double& e, double& vo, double& vi) {
+  const double t1 = c(VCLAMP_TAU1, r) / dt;
+  const double t2 = c(VCLAMP_TAU2, r) / dt;
+  const double fac = c(VCLAMP_FAC, r), gain = c(VCLAMP_GAIN, r);
+  e = (((((1.0 + t1) * (((1.0 - fac) * v) + (((fac * t2) * vo0) / (1.0 + t2)))) - stim) -
+        (t1 * (c(VCLAMP_VI0, r) - e0))) /
+       ((2.0 + t1) + ((((1.0 + t1) * fac) * gain) / (1.0 + t2))));
+  vo = (((t2 * vo0) - (gain * e)) / (1.0 + t2));
+  vi = ((v + (fac * vo)) - (fac * v));
+}
+
 // Current phase. Returns the membrane current sum I(v) in the units of the
 // mechanism (mA/cm2 for density, nA for point processes) and its probe at
 // v + 0.001; stores the assigned fields of the evaluation at v.
 // Current fields sum NONSPECIFIC currents first, then ion writes (hh: il, ina,
 // ik), and subtract ELECTRODE currents (src/runtime/initialize.jl:496-502).
-MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t,
+MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t, double dt,
                           double& value, double& shifted, bool store) {
+  (void)dt;
   for (int probe = 0; probe < 2; ++probe) {
     const double vv = probe == 0 ? v + 0.001 : v;
     double sum = 0.0;
@@ -129,6 +144,34 @@ MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t
         if (probe == 1 && store) { c(SECLAMP_ON, r) = on; c(SECLAMP_VC, r) = vc; c(SECLAMP_I, r) = i; }
         break;
       }
+      case MIND_MECH_VCLAMP: {  // VClamp.mod BREAKPOINT without its SOLVE: vstim(), then i = icur()
+        double tc = 0.0, stim = c(VCLAMP_STIM, r), j = 0.0, selected = 0.0;
+        while (j < 3.0 && !selected) {
+          stim = c(VCLAMP_AMP + (int)j, r);
+          tc = tc + c(VCLAMP_DUR + (int)j, r);
+          if (t < tc) {
+            tc = tc + 100.0;
+            selected = 1.0;
+          }
+          j = j + 1.0;
+        }
+        double e0 = c(VCLAMP_E0, r), vo0 = c(VCLAMP_VO0, r);
+        double e = c(VCLAMP_E, r), vo = c(VCLAMP_VO, r), vi = c(VCLAMP_VI, r), i;
+        if (t > tc) {
+          e0 = 0.0;
+          vo0 = 0.0;
+          i = 0.0;
+        } else {
+          vclamp_amplifier(c, r, vv, dt, stim, e0, vo0, e, vo, vi);
+          i = (vo - vv) / c(VCLAMP_RSTIM, r);
+        }
+        sum -= i;
+        if (probe == 1 && store) {
+          c(VCLAMP_TC, r) = tc; c(VCLAMP_STIM, r) = stim; c(VCLAMP_I, r) = i; c(VCLAMP_E0, r) = e0;
+          c(VCLAMP_VO0, r) = vo0; c(VCLAMP_E, r) = e; c(VCLAMP_VO, r) = vo; c(VCLAMP_VI, r) = vi;
+        }
+        break;
+      }
       default: break;
     }
     if (probe == 0) shifted = sum; else value = sum;
@@ -138,12 +181,13 @@ MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t
 // Mechanisms whose BREAKPOINT has a SOLVE, i.e. a state phase (nrn_state).
 MIND_HD bool mech_has_state(int mech) {
   return mech == MIND_MECH_HH || mech == MIND_MECH_EXP2SYN || mech == MIND_MECH_EXPSYN ||
-         mech == MIND_MECH_SECLAMP;
+         mech == MIND_MECH_SECLAMP || mech == MIND_MECH_VCLAMP;
 }
+
 
 // State phase at t + dt: hoisted SOLVE of BREAKPOINT (pas and IClamp have none;
 // SEClamp's is `SOLVE icur METHOD after_cvode`, the clamp current at the new v).
-MIND_HD void mech_advance(int mech, const Cols& c, int64_t r, double v, double dt,
+MIND_HD void mech_advance(int mech, const Cols& c, int64_t r, double v, double t, double dt,
                           double celsius) {
   switch (mech) {
     case MIND_MECH_HH: {
@@ -165,6 +209,24 @@ MIND_HD void mech_advance(int mech, const Cols& c, int64_t r, double v, double d
     case MIND_MECH_SECLAMP:
       c(SECLAMP_I, r) = c(SECLAMP_ON, r) != 0.0 ? (c(SECLAMP_VC, r) - v) / c(SECLAMP_RS, r) : 0.0;
       break;
+    case MIND_MECH_VCLAMP: {  // SOLVE update METHOD after_cvode: i = icur() at the new v, then e0, vo0, vi0
+      double i;
+      if (t > c(VCLAMP_TC, r)) {
+        c(VCLAMP_E0, r) = 0.0;
+        c(VCLAMP_VO0, r) = 0.0;
+        i = 0.0;
+      } else {
+        double e, vo, vi;
+        vclamp_amplifier(c, r, v, dt, c(VCLAMP_STIM, r), c(VCLAMP_E0, r), c(VCLAMP_VO0, r), e, vo, vi);
+        c(VCLAMP_E, r) = e; c(VCLAMP_VO, r) = vo; c(VCLAMP_VI, r) = vi;
+        i = (c(VCLAMP_VO, r) - v) / c(VCLAMP_RSTIM, r);
+      }
+      c(VCLAMP_I, r) = i;
+      c(VCLAMP_E0, r) = c(VCLAMP_E, r);
+      c(VCLAMP_VO0, r) = c(VCLAMP_VO, r);
+      c(VCLAMP_VI0, r) = c(VCLAMP_VI, r);
+      break;
+    }
     default: break;
   }
 }
@@ -194,6 +256,14 @@ MIND_HD void mech_initial(int mech, const Cols& c, int64_t r, double v, double c
     }
     case MIND_MECH_EXPSYN: c(EXPSYN_G, r) = 0.0; break;
     case MIND_MECH_ICLAMP: c(ICLAMP_I, r) = 0.0; break;
+    case MIND_MECH_VCLAMP:  // VClamp.mod INITIAL
+      c(VCLAMP_E0, r) = 0.0;
+      c(VCLAMP_VO, r) = v;
+      c(VCLAMP_VO0, r) = v;
+      c(VCLAMP_VI, r) = v;
+      c(VCLAMP_VI0, r) = v;
+      c(VCLAMP_E, r) = 0.0;
+      break;
     case MIND_MECH_SECLAMP:  // SEClamp.mod INITIAL
       c(SECLAMP_TC2, r) = c(SECLAMP_DUR1, r) + c(SECLAMP_DUR2, r);
       c(SECLAMP_TC3, r) = c(SECLAMP_TC2, r) + c(SECLAMP_DUR3, r);

@@ -90,7 +90,7 @@ __global__ void k_currents_fields(Dev d, int gi, double t) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= G.rows) return;
   double value, shifted;
-  phase_current(G.mech, cols(G), r, d.v[G.node[r]], t, value, shifted, true);
+  phase_current(G.mech, cols(G), r, d.v[G.node[r]], t, d.dt, value, shifted, true);
 }
 
 __global__ void k_record_sample(Dev d, int64_t sample) {
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
       for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r) {
         const int32_t node = G.node[r];
         double value, shifted;
-        phase_current(G.mech, col, r, v[node], tc, value, shifted, true);
+        phase_current(G.mech, col, r, v[node], tc, dt, value, shifted, true);
         const double scale = G.point ? 1.0 : d.area[node] * 0.01;
         rhs[node] -= scale * value;
         diag[node] += scale * (shifted - value) / 0.001;
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
       if (!mech_has_state(G.mech)) continue;
       const Cols col = cols(G);
       for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r)
-        phase_advance(G.mech, col, r, v[G.node[r]], dt, d.celsius);
+        phase_advance(G.mech, col, r, v[G.node[r]], t + dt, dt, d.celsius);
     }
     if (d.nrec) record_cell(d, c, k + 1 - d.rec_start);
     seg = deliver_cell(d, seg, seg_end, 2 * k + 1);

@@ -235,8 +235,17 @@ class MicroModel:
 
     # -- mechanisms ----------------------------------------------------------
     def _params(self, mech: Mechanism, params):
+        """(column, value) overrides; an array field takes a sequence of its extent."""
         items = params.items() if isinstance(params, dict) else params
-        return [(mech.field_index(str(k)), float(v)) for k, v in items]
+        out = []
+        for k, v in items:
+            if isinstance(v, (list, tuple, np.ndarray)):
+                if len(v) != mech.extent(str(k)) or mech.extent(str(k)) == 1 and str(k) in mech.fields:
+                    raise ValueError(f"{mech.name}.{k} takes {mech.extent(str(k))} values")
+                out.extend((mech.field_index(str(k), j + 1), float(x)) for j, x in enumerate(v))
+            else:
+                out.append((mech.field_index(str(k)), float(v)))
+        return out
 
     def insert(self, cell: Cell, mechanism, sections="all", params=None):
         """Insert a density mechanism where absent (model.jl:153-169)."""
@@ -482,9 +491,10 @@ def voltage_record(model, cell, section="soma", x=0.5, name=None):
     return RecordSpec(name or f"v_{cell.gid}", (0, cell.index, s, float(x), -1, -1, -1))
 
 
-def mechanism_record(model, target, *args, name=None, section="soma", x=0.5):
+def mechanism_record(model, target, *args, name=None, section="soma", x=0.5, index=1):
     """mechanism_record(model, point, field), mechanism_record(model, cell, mech, field)
-    or mechanism_record(model, cell_section, mech, field) (recording.jl:10-30, 56-63).
+    or mechanism_record(model, cell_section, mech, field) (recording.jl:10-30, 56-63);
+    `index` selects an element of an array field (1-based, references.jl:44-69).
     ROI and MICRO2MACRO fields: mind_amd.macro.mechanism_record."""
     from .mechanisms import BY_ID
     if isinstance(target, CellSection):
@@ -492,17 +502,17 @@ def mechanism_record(model, target, *args, name=None, section="soma", x=0.5):
         mech = resolve(mechanism)
         s = model._morph(target.cell).section_index(target.name)
         return RecordSpec(name or f"{target.cell.gid}.{target.name}.{fname}",
-                          (2, target.cell.index, s, float(x), -1, mech.id, mech.field_index(str(fname))))
+                          (2, target.cell.index, s, float(x), -1, mech.id, mech.field_index(str(fname), index)))
     if isinstance(target, PointProcess):
         (fname,) = args
         mech = BY_ID[model.points[target.index][0]]
         return RecordSpec(name or str(fname),
-                          (1, -1, -1, 0.0, target.index, mech.id, mech.field_index(str(fname))))
+                          (1, -1, -1, 0.0, target.index, mech.id, mech.field_index(str(fname), index)))
     mechanism, fname = args
     mech = resolve(mechanism)
     s = model._morph(target).section_index(section)
     return RecordSpec(name or f"{target.gid}.{fname}",
-                      (2, target.index, s, float(x), -1, mech.id, mech.field_index(str(fname))))
+                      (2, target.index, s, float(x), -1, mech.id, mech.field_index(str(fname), index)))
 
 
 # ion columns of the six mechanisms: hh reads ena/ek and writes ina/ik

@@ -42,7 +42,8 @@ EXEC_INPUTS = ("v", "t", "dt", "celsius")
 FUNCS = {"exp": "mind_exp", "log": "mind_log", "fabs": "mind_fabs", "sqrt": "sqrt"}
 BUILTIN_IDS = {"hh": "MIND_MECH_HH", "pas": "MIND_MECH_PAS", "ExpSyn": "MIND_MECH_EXPSYN",
                "Exp2Syn": "MIND_MECH_EXP2SYN", "IClamp": "MIND_MECH_ICLAMP", "NetStim": "MIND_MECH_NETSTIM",
-               "AlphaSynapse": "MIND_MECH_ALPHASYNAPSE", "SEClamp": "MIND_MECH_SECLAMP"}
+               "AlphaSynapse": "MIND_MECH_ALPHASYNAPSE", "SEClamp": "MIND_MECH_SECLAMP",
+               "VClamp": "MIND_MECH_VCLAMP"}
 
 
 class ModError(Exception):
@@ -61,7 +62,7 @@ class Tok:
 _TOKEN = re.compile(r"""
     (?P<num>(?:\d+\.\d*|\.\d+|\d+)(?:[eE][+-]?\d+)?)
   | (?P<name>[A-Za-z_][A-Za-z0-9_]*)
-  | (?P<op>==|!=|<=|>=|&&|\|\||[{}()<>,=+\-*/^!'~])
+  | (?P<op>==|!=|<=|>=|&&|\|\||[{}()\[\]<>,=+\-*/^!'~])
 """, re.X)
 
 
@@ -131,10 +132,24 @@ class Binary:
 
 
 @dataclass
+class Index:
+    id: str      # array field
+    index: object
+
+
+@dataclass
 class Assign:
     target: str
     value: object
     derivative: bool
+    line: int
+    index: object = None  # element of an array field
+
+
+@dataclass
+class While:
+    cond: object
+    body: list
     line: int
 
 
@@ -186,6 +201,7 @@ class Mechanism:
     decls: list = field(default_factory=list)         # (name, storage, default)
     blocks: dict = field(default_factory=dict)
     defines: dict = field(default_factory=dict)       # DEFINE name value (named constants)
+    extents: dict = field(default_factory=dict)       # array field -> element count
 
     @property
     def fields(self):
@@ -338,6 +354,13 @@ class Parser:
         self.expect("{")
         while not self.accept("}"):
             name = self.ident()
+            if self.accept("["):  # array field: name[N], N a number or a DEFINE
+                t = self.take()
+                n = float(t.text) if t.kind == "num" else self.m.defines.get(t.text)
+                if n is None or n != int(n) or n < 1:
+                    self.fail(t, "array size must be a positive integer or DEFINE")
+                self.expect("]")
+                self.m.extents[name] = int(n)
             default = None
             if self.accept("="):
                 neg = self.accept("-")
@@ -404,6 +427,11 @@ class Parser:
             return None
         if self.accept("UNITSOFF") or self.accept("UNITSON"):
             return None
+        if self.accept("while"):
+            self.expect("(")
+            cond = self.expr()
+            self.expect(")")
+            return While(cond, self.body(), t.line)
         if self.accept("if"):
             self.expect("(")
             cond = self.expr()
@@ -414,6 +442,11 @@ class Parser:
                 other = [self.statement()] if self.at("if") else self.body()
             return If(cond, then, other)
         name = self.ident()
+        if self.accept("["):
+            index = self.expr()
+            self.expect("]")
+            self.expect("=")
+            return Assign(name, self.expr(), False, t.line, index)
         if self.accept("'"):
             self.expect("=")
             return Assign(name, self.expr(), True, t.line)
@@ -489,6 +522,10 @@ class Parser:
         if t.kind == "name":
             if self.at("("):
                 return self.call_args(t.text)
+            if self.accept("["):
+                index = self.expr()
+                self.expect("]")
+                return Index(t.text, index)
             return Name(t.text)
         self.fail(t, f"unexpected {t.text!r} in expression")
 
@@ -509,7 +546,12 @@ class Gen:
     def __init__(self, m: Mechanism):
         self.m = m
         self.fields = m.fields
-        self.findex = {n: i for i, n in enumerate(self.fields)}
+        self.findex, off = {}, 0  # column of each field; an array's elements are consecutive
+        for n in self.fields:
+            self.findex[n] = off
+            off += m.extents.get(n, 1)
+        self.ncols = off
+        self.tmp = 0
         self.funcs = {b.name: b for (k, _), b in m.blocks.items() if k == "function"}
         self.procs = {b.name: b for (k, _), b in m.blocks.items() if k == "procedure"}
         self.derivs = {b.name: b for (k, _), b in m.blocks.items() if k == "derivative"}
@@ -535,7 +577,22 @@ class Gen:
             return f"({a} {x.op} {b})"
         if isinstance(x, Call):
             return self.call(x, scope)
+        if isinstance(x, Index):
+            return self.element(x.id, x.index, scope)
         self.err(f"unsupported expression {x!r}")
+
+    def element(self, name, index, scope):
+        """Column of element `index` (0-based, as in MOD) of an array field. A constant
+        index is checked here; a computed one is clamped to the array (gen_index)."""
+        n = self.m.extents.get(name)
+        if n is None or name in scope:
+            self.err(f"{name} is not an array field")
+        if isinstance(index, Num):
+            k = index.value
+            if k != int(k) or not 0 <= k < n:
+                self.err(f"index {k} is outside {name}[{n}]")
+            return f"c({self.p}_{name} + {int(k)}, r)"
+        return f"c({self.p}_{name} + gen_index({self.e(index, scope)}, {n}), r)"
 
     def call(self, c, scope):
         args = [self.e(a, scope) for a in c.args]
@@ -566,7 +623,7 @@ class Gen:
 
     def base_scope(self, ctx_v="x.v"):
         s = {n: lit(val) for n, val in self.m.defines.items()}
-        s.update({n: f"c({self.p}_{n}, r)" for n in self.fields})
+        s.update({n: f"c({self.p}_{n}, r)" for n in self.fields if n not in self.m.extents})
         s.update({"v": ctx_v, "t": "x.t", "dt": "x.dt", "celsius": "x.celsius"})
         for rv in self.m.randoms:
             s[rv] = "0.0"
@@ -583,16 +640,31 @@ class Gen:
                     cn = f"l_{n}"
                     out.append(f"{ind}double {cn} = 0.0;")
                     scope[n] = cn
+            elif isinstance(st, Assign) and st.index is not None:
+                if inline:
+                    self.err("array element writes in BREAKPOINT's current evaluation are not supported", st.line)
+                out.append(f"{ind}{self.element(st.target, st.index, scope)} = {self.e(st.value, scope)};")
+            elif isinstance(st, While):
+                if inline and self.user_calls(st.cond):
+                    self.err("a FUNCTION call in a while condition of the current evaluation", st.line)
+                cond = self.e(st.cond, scope)
+                if isinstance(st.cond, Binary) and st.cond.op != "^":
+                    cond = cond[1:-1]
+                out.append(f"{ind}while ({cond}) {{")
+                self.stmts(st.body, scope, out, ind + "  ", deriv_method, inline)
+                out.append(f"{ind}}}")
             elif isinstance(st, Assign):
                 if st.target not in scope or scope[st.target] in ("x.v", "x.t", "x.dt", "x.celsius"):
                     self.err(f"cannot assign {st.target}", st.line)
                 tgt = scope[st.target]
+                if inline:
+                    st = Assign(st.target, self.hoist(st.value, scope, out, ind, inline), st.derivative, st.line)
                 if st.derivative:
                     out.append(ind + self.derivative(st, scope, tgt, deriv_method))
                 else:
                     out.append(f"{ind}{tgt} = {self.e(st.value, scope)};")
             elif isinstance(st, If):
-                cond = self.e(st.cond, scope)
+                cond = self.e(self.hoist(st.cond, scope, out, ind, inline) if inline else st.cond, scope)
                 if isinstance(st.cond, Binary) and st.cond.op != "^":
                     cond = cond[1:-1]  # the if's own parentheses
                 out.append(f"{ind}if ({cond}) {{")
@@ -602,12 +674,71 @@ class Gen:
                     self.stmts(st.other, scope, out, ind + "  ", deriv_method, inline)
                 out.append(f"{ind}}}")
             elif isinstance(st, CallStmt) and inline and st.call.name in self.procs:
-                self.inline_proc(st, scope, out, ind, inline)
+                args = [self.hoist(a, scope, out, ind, inline) for a in st.call.args]
+                self.inline_proc(CallStmt(Call(st.call.name, args), st.line), scope, out, ind, inline)
+            elif isinstance(st, CallStmt) and inline and st.call.name in self.funcs:
+                self.hoist(st.call, scope, out, ind, inline)  # value unused
+            elif isinstance(st, CallStmt) and inline:
+                args = [self.hoist(a, scope, out, ind, inline) for a in st.call.args]
+                out.append(ind + self.call_stmt(CallStmt(Call(st.call.name, args), st.line), scope) + ";")
             elif isinstance(st, CallStmt):
                 out.append(ind + self.call_stmt(st, scope) + ";")
             elif isinstance(st, Solve):
                 self.err("SOLVE outside BREAKPOINT", st.line)
         return scope
+
+    def user_calls(self, x):
+        """FUNCTION calls in expression `x`, arguments before the call."""
+        if isinstance(x, Call):
+            inner = [c for a in x.args for c in self.user_calls(a)]
+            return inner + ([x] if x.name in self.funcs else [])
+        if isinstance(x, Unary):
+            return self.user_calls(x.x)
+        if isinstance(x, Binary):
+            return self.user_calls(x.a) + self.user_calls(x.b)
+        if isinstance(x, Index):
+            return self.user_calls(x.index)
+        return []
+
+    def hoist(self, x, scope, out, ind, chain):
+        """In the current evaluation, a FUNCTION may assign fields (held in local copies
+        there), so its calls are expanded before the statement that uses them, in
+        evaluation order, each into a temporary that replaces the call. Calls on the
+        right of && or || would lose their short-circuit and are refused."""
+        if isinstance(x, Call):
+            args = [self.hoist(a, scope, out, ind, chain) for a in x.args]
+            if x.name not in self.funcs:
+                return Call(x.name, args)
+            return Name(self.inline_func(Call(x.name, args), scope, out, ind, chain))
+        if isinstance(x, Unary):
+            return Unary(x.op, self.hoist(x.x, scope, out, ind, chain))
+        if isinstance(x, Binary):
+            if x.op in ("&&", "||") and self.user_calls(x.b):
+                self.err(f"a FUNCTION call on the right of {x.op} in the current evaluation")
+            return Binary(x.op, self.hoist(x.a, scope, out, ind, chain), self.hoist(x.b, scope, out, ind, chain))
+        if isinstance(x, Index):
+            return Index(x.id, self.hoist(x.index, scope, out, ind, chain))
+        return x
+
+    def inline_func(self, call, scope, out, ind, chain):
+        f = self.funcs[call.name]
+        if f.name in chain:
+            self.err(f"FUNCTION {f.name} calls itself", f.line)
+        if len(call.args) != len(f.args):
+            self.err(f"{f.name} takes {len(f.args)} arguments", f.line)
+        self.tmp += 1
+        tmp = f"f_{f.name}_{self.tmp}"
+        out.append(f"{ind}double {tmp} = 0.0;")
+        out.append(f"{ind}{{  // FUNCTION {f.name}")
+        inner = dict(scope)
+        for a, x in zip(f.args, call.args):
+            out.append(f"{ind}  const double p_{f.name}_{a}_{self.tmp} = {self.e(x, scope)};")
+            inner[a] = f"p_{f.name}_{a}_{self.tmp}"
+        inner[f.name] = tmp
+        self.stmts(f.body, inner, out, ind + "  ", None, tuple(chain) + (f.name,))
+        out.append(f"{ind}}}")
+        scope[tmp] = tmp
+        return tmp
 
     def inline_proc(self, st, scope, out, ind, chain):
         p = self.procs[st.call.name]
@@ -671,7 +802,8 @@ class Gen:
     def emit(self):
         m, p, out = self.m, self.p, []
         out.append(f"// ---- {m.name} ({os.path.basename(m.filename)}, sha256 {m.digest:016x}) ----")
-        out.append(f"enum {{ {', '.join(f'{p}_{n}' for n in self.fields)}, {p}_NFIELDS }};")
+        out.append(f"enum {{ {', '.join(f'{p}_{n} = {self.findex[n]}' for n in self.fields)}, "
+                   f"{p}_NFIELDS = {self.ncols} }};")
         if m.randoms:
             out.append(f"constexpr uint64_t {p}_digest = 0x{m.digest:016x}ull;")
         defaults = {n: d for n, st, d in m.decls if st == "parameter" and d is not None}
@@ -733,7 +865,7 @@ class Gen:
                 self.err(f"current {n} is not declared")
         lines = []
         lines.append("  for (int probe = 0; probe < 2; ++probe) {")
-        lines.append("    GenCtx x = gen_ctx(probe == 0 ? v + 0.001 : v, t);")
+        lines.append("    GenCtx x = gen_ctx(probe == 0 ? v + 0.001 : v, t, dt);")
         lines.append("    (void)x;")
         scope = self.base_scope()
         for n in written:  # assigned values of this evaluation, stored after the probe at v
@@ -776,7 +908,7 @@ class Gen:
         out.append("  return in;")
         out.append("}")
         out.append(f"MIND_HD void {p}_current_eval(const Cols& c, int64_t r, const {p}_cur_in& in, double v, "
-                   "double t, double& value, double& shifted, bool store) {")
+                   "double t, double dt, double& value, double& shifted, bool store) {")
         out.append("  (void)in;")
         for l in lines:
             if l not in stores:
@@ -784,9 +916,9 @@ class Gen:
                     l = l.replace(f"c({p}_{n}, r)", f"in.{n}")
             out.append(l)
         out.append("}")
-        out.append(f"MIND_HD void {p}_current(const Cols& c, int64_t r, double v, double t, double& value, "
-                   "double& shifted, bool store) {")
-        out.append(f"  {p}_current_eval(c, r, {p}_current_load(c, r), v, t, value, shifted, store);")
+        out.append(f"MIND_HD void {p}_current(const Cols& c, int64_t r, double v, double t, double dt, "
+                   "double& value, double& shifted, bool store) {")
+        out.append(f"  {p}_current_eval(c, r, {p}_current_load(c, r), v, t, dt, value, shifted, store);")
         out.append("}")
         return out
 
@@ -800,18 +932,35 @@ class Gen:
                     and st.target not in acc:
                 acc.append(st.target)
             elif isinstance(st, If):
+                self.collect_calls(st.cond, acc, chain)
                 self.collect_writes(st.then, acc, chain, local)
                 self.collect_writes(st.other, acc, chain, local)
+            elif isinstance(st, While):
+                self.collect_writes(st.body, acc, chain, local)
             elif isinstance(st, CallStmt) and st.call.name in self.procs:
                 p = self.procs[st.call.name]
                 if p.name in chain:
                     self.err(f"PROCEDURE {p.name} calls itself", st.line)
+                for a in st.call.args:
+                    self.collect_calls(a, acc, chain)
                 self.collect_writes(p.body, acc, chain + (p.name,), frozenset(p.args))
+            elif isinstance(st, CallStmt):
+                self.collect_calls(st.call, acc, chain)
+            if isinstance(st, Assign):
+                self.collect_calls(st.value, acc, chain)
+
+    def collect_calls(self, x, acc, chain):
+        for f in self.user_calls(x):
+            f = self.funcs[f.name]
+            if f.name in chain:
+                self.err(f"FUNCTION {f.name} calls itself", f.line)
+            self.collect_writes(f.body, acc, chain + (f.name,), frozenset(f.args) | {f.name})
 
     def advance(self):
         p = self.p
-        out = [f"MIND_HD void {p}_advance(const Cols& c, int64_t r, double v, double dt, double celsius) {{",
-               "  GenCtx x = gen_ctx(v, 0.0, dt, celsius);", "  (void)x;"]
+        out = [f"MIND_HD void {p}_advance(const Cols& c, int64_t r, double v, double t, double dt, "
+               "double celsius) {",
+               "  GenCtx x = gen_ctx(v, t, dt, celsius);", "  (void)x;"]
         bp = self.breakpoint()
         for st in (bp.body if bp else []):
             if not isinstance(st, Solve):
@@ -899,6 +1048,13 @@ struct GenCtx {
   NetStimOut out;
 };
 
+// Element of an array field addressed by a computed MOD index (0-based), kept
+// within the array.
+MIND_HD int gen_index(double j, int n) {
+  const int k = (int)j;
+  return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
 MIND_HD GenCtx gen_ctx(double v, double t, double dt = 0.0, double celsius = 0.0, Stream* s = nullptr,
                        uint64_t seed = 0) {
   GenCtx x;
@@ -949,9 +1105,9 @@ def dispatch(mechs):
     out.append(f'static_assert({checks}, "generated field layout differs from the runtime\'s");')
     out.append(f"constexpr uint64_t kNetStimDigest = NetStim_digest;")
     cases = lambda call: "\n".join(f"    case {BUILTIN_IDS[n]}: {call(n)}; break;" for n in BUILTIN_IDS)
-    out.append("MIND_HD void current(int mech, const Cols& c, int64_t r, double v, double t, double& value, "
-               "double& shifted, bool store) {\n  switch (mech) {\n"
-               + cases(lambda n: f"{n}_current(c, r, v, t, value, shifted, store)")
+    out.append("MIND_HD void current(int mech, const Cols& c, int64_t r, double v, double t, double dt, "
+               "double& value, double& shifted, bool store) {\n  switch (mech) {\n"
+               + cases(lambda n: f"{n}_current(c, r, v, t, dt, value, shifted, store)")
                + "\n    default: value = shifted = 0.0;\n  }\n}")
     # the two parts of nrn_cur per mechanism slot, for callers specialised on the mechanism
     out.append("template <int MECH>\nstruct CurOf;")
@@ -959,10 +1115,10 @@ def dispatch(mechs):
         out.append(f"template <>\nstruct CurOf<{BUILTIN_IDS[n]}> {{\n  using In = {n}_cur_in;\n"
                    f"  static MIND_HD In load(const Cols& c, int64_t r) {{ return {n}_current_load(c, r); }}\n"
                    f"  static MIND_HD void eval(const Cols& c, int64_t r, const In& in, double v, double t, "
-                   f"double& value, double& shifted, bool store) {{\n"
-                   f"    {n}_current_eval(c, r, in, v, t, value, shifted, store);\n  }}\n}};")
-    out.append("MIND_HD void advance(int mech, const Cols& c, int64_t r, double v, double dt, double celsius) {"
-               "\n  switch (mech) {\n" + cases(lambda n: f"{n}_advance(c, r, v, dt, celsius)")
+                   f"double dt, double& value, double& shifted, bool store) {{\n"
+                   f"    {n}_current_eval(c, r, in, v, t, dt, value, shifted, store);\n  }}\n}};")
+    out.append("MIND_HD void advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, "
+               "double celsius) {\n  switch (mech) {\n" + cases(lambda n: f"{n}_advance(c, r, v, t, dt, celsius)")
                + "\n    default: break;\n  }\n}")
     # membrane mechanisms only: mechanisms with RANDOM streams (NetStim) run their
     # INITIAL and NET_RECEIVE through the event queue with the row's stream

@@ -256,8 +256,9 @@ Template discretize(const std::vector<SectionIn>& secs) {
 // ------------------------------------------------------------ Mechanisms ----
 const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,    EXPSYN_NFIELDS,
                                       EXP2SYN_NFIELDS, ICLAMP_NFIELDS, NETSTIM_NFIELDS,
-                                      ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS};
-const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true, true, true};
+                                      ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS,
+                                      VCLAMP_NFIELDS};
+const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true, true, true, true};
 
 void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
   switch (mech) {
@@ -275,6 +276,9 @@ void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
       break;
     case MIND_MECH_ALPHASYNAPSE: f[ALPHASYNAPSE_TAU] = 0.1; break;  // AlphaSynapse.mod PARAMETER
     case MIND_MECH_SECLAMP: f[SECLAMP_RS] = 1.0; break;              // SEClamp.mod PARAMETER
+    case MIND_MECH_VCLAMP:                                             // VClamp.mod PARAMETER
+      f[VCLAMP_GAIN] = 1e5; f[VCLAMP_RSTIM] = 1.0; f[VCLAMP_TAU1] = 0.001;
+      break;
   }
 }
 
@@ -616,6 +620,20 @@ void deliver(Sim& s, double cutoff) {  // deliver_events! (events.jl:57-165)
 }
 
 // ----------------------------------------------------------- Membrane -------
+// VClamp.mod PROCEDURE clamp: the amplifier equations eliminated for e, vo, vi.
+void vclamp_clamp(Group& g, int r, double v, double dt, double stim, double e0, double vo0, double& e,
+                  double& vo, double& vi) {
+  const double t1 = g.at(VCLAMP_TAU1, r) / dt;
+  const double t2 = g.at(VCLAMP_TAU2, r) / dt;
+  const double fac = g.at(VCLAMP_FAC, r);
+  const double gain = g.at(VCLAMP_GAIN, r);
+  const double num = (1.0 + t1) * ((1.0 - fac) * v + fac * t2 * vo0 / (1.0 + t2)) - stim -
+                     t1 * (g.at(VCLAMP_VI0, r) - e0);
+  e = num / (2.0 + t1 + (1.0 + t1) * fac * gain / (1.0 + t2));
+  vo = (t2 * vo0 - gain * e) / (1.0 + t2);
+  vi = v + fac * vo - fac * v;
+}
+
 void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
   std::fill(s.rhs.begin(), s.rhs.end(), 0.0);
   std::fill(s.diag.begin(), s.diag.end(), 0.0);
@@ -680,6 +698,31 @@ void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
               sum = 0.0; sum += i;
               break;
             }
+            case MIND_MECH_VCLAMP: {  // VClamp.mod vstim(), then i = icur()
+              double tc = 0.0, stim = g.at(VCLAMP_STIM, r);
+              for (int j = 0; j < 3; ++j) {  // the first level whose end lies after t
+                stim = g.at(VCLAMP_AMP + j, r);
+                tc = tc + g.at(VCLAMP_DUR + j, r);
+                if (t < tc) {
+                  tc = tc + 100.0;
+                  break;
+                }
+              }
+              double e0 = g.at(VCLAMP_E0, r), vo0 = g.at(VCLAMP_VO0, r);
+              double e = g.at(VCLAMP_E, r), vo = g.at(VCLAMP_VO, r), vi = g.at(VCLAMP_VI, r), i = 0.0;
+              if (t > tc) {
+                e0 = 0.0;
+                vo0 = 0.0;
+              } else {
+                vclamp_clamp(g, r, vv, s.dt, stim, e0, vo0, e, vo, vi);
+                i = (vo - vv) / g.at(VCLAMP_RSTIM, r);
+              }
+              g.at(VCLAMP_TC, r) = tc; g.at(VCLAMP_STIM, r) = stim; g.at(VCLAMP_I, r) = i;
+              g.at(VCLAMP_E0, r) = e0; g.at(VCLAMP_VO0, r) = vo0;
+              g.at(VCLAMP_E, r) = e; g.at(VCLAMP_VO, r) = vo; g.at(VCLAMP_VI, r) = vi;
+              sum = 0.0; sum -= i;  // electrode current
+              break;
+            }
             case MIND_MECH_SECLAMP: {  // SEClamp.mod vstim() and icur(); at_time has no effect
               double on = 1.0, vc;
               if (t < g.at(SECLAMP_DUR1, r)) vc = g.at(SECLAMP_AMP1, r);
@@ -742,6 +785,7 @@ void cable(Sim& s) {  // assemble_cable_kernel!, solve_dhs_kernel!, update_volta
   }
 }
 
+
 void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
   const double dt = s.dt;
   for (Group& g : s.groups) {
@@ -764,6 +808,24 @@ void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
     } else if (g.mech == MIND_MECH_EXPSYN) {
       for (int r = 0; r < rows; ++r)
         g.at(EXPSYN_G, r) = relax(g.at(EXPSYN_G, r), 0.0, g.at(EXPSYN_TAU, r), dt);
+    } else if (g.mech == MIND_MECH_VCLAMP) {  // SOLVE update METHOD after_cvode (VClamp.mod update)
+      const double t = (double)s.step * dt + dt;  // time + dt
+      for (int r = 0; r < rows; ++r) {
+        const double v = s.v[g.node[r]];
+        double i = 0.0;  // FUNCTION icur
+        if (t > g.at(VCLAMP_TC, r)) {
+          g.at(VCLAMP_E0, r) = 0.0;
+          g.at(VCLAMP_VO0, r) = 0.0;
+        } else {
+          vclamp_clamp(g, r, v, dt, g.at(VCLAMP_STIM, r), g.at(VCLAMP_E0, r), g.at(VCLAMP_VO0, r),
+                       g.at(VCLAMP_E, r), g.at(VCLAMP_VO, r), g.at(VCLAMP_VI, r));
+          i = (g.at(VCLAMP_VO, r) - v) / g.at(VCLAMP_RSTIM, r);
+        }
+        g.at(VCLAMP_I, r) = i;
+        g.at(VCLAMP_E0, r) = g.at(VCLAMP_E, r);
+        g.at(VCLAMP_VO0, r) = g.at(VCLAMP_VO, r);
+        g.at(VCLAMP_VI0, r) = g.at(VCLAMP_VI, r);
+      }
     } else if (g.mech == MIND_MECH_SECLAMP) {  // SOLVE icur METHOD after_cvode: i at the new v
       for (int r = 0; r < rows; ++r) {
         double i = 0.0;
@@ -827,6 +889,16 @@ void initial_blocks(Sim& s) {  // INITIAL per group (initialize.jl:342-374)
         }
         case MIND_MECH_EXPSYN: g.at(EXPSYN_G, r) = 0.0; break;
         case MIND_MECH_ICLAMP: g.at(ICLAMP_I, r) = 0.0; break;
+        case MIND_MECH_VCLAMP: {  // VClamp.mod INITIAL
+          const double v = s.v[g.node[r]];
+          g.at(VCLAMP_E0, r) = 0.0;
+          g.at(VCLAMP_VO, r) = v;
+          g.at(VCLAMP_VO0, r) = v;
+          g.at(VCLAMP_VI, r) = v;
+          g.at(VCLAMP_VI0, r) = v;
+          g.at(VCLAMP_E, r) = 0.0;
+          break;
+        }
         case MIND_MECH_SECLAMP:  // SEClamp.mod INITIAL
           g.at(SECLAMP_TC2, r) = g.at(SECLAMP_DUR1, r) + g.at(SECLAMP_DUR2, r);
           g.at(SECLAMP_TC3, r) = g.at(SECLAMP_TC2, r) + g.at(SECLAMP_DUR3, r);

@@ -195,3 +195,23 @@ def test_seclamp_matches_oracle(gpu, method):
         assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
     assert got.traces["on"][-1] == 0.0 and got.traces["i"].min() < 0.0
     destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_vclamp_matches_oracle(gpu, method):
+    """VClamp at the soma of the single cell (levels switching off the step grid, the
+    clamp releasing into hh spiking), in both kernels: bit-identical to the oracle."""
+    model, info = configs.single_cell()
+    cell = info["cells"][0]
+    clamp = model.add_pointprocess(cell, "VClamp", section="soma", x=0.5,
+                                   params={"dur": [5.01, 10.0, 4.99], "amp": [-70.0, -10.0, -80.0],
+                                           "tau2": 0.002, "fac": 0.5})
+    recs = [voltage_record(model, cell, "soma", 0.5), voltage_record(model, cell, "dend", 0.5, name="vd"),
+            mechanism_record(model, clamp, "i"), mechanism_record(model, clamp, "e"),
+            mechanism_record(model, clamp, "vo"), mechanism_record(model, clamp, "stim")]
+    got, ref, vg, vr, _ = _compare(model, 60.0, recs, method=method)
+    assert np.array_equal(vg, vr)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    assert got.traces["i"][-1] == 0.0
+    destroy(model)

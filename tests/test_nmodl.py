@@ -18,7 +18,7 @@ from mind_amd import nmodl
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "mind.jl_amd", "csrc")
 MOD_DIR = "/root/reference/mod"
-BUILTINS = ("hh", "pas", "ExpSyn", "Exp2Syn", "IClamp", "NetStim", "AlphaSynapse", "SEClamp")
+BUILTINS = ("hh", "pas", "ExpSyn", "Exp2Syn", "IClamp", "NetStim", "AlphaSynapse", "SEClamp", "VClamp")
 
 LEAKGATE = textwrap.dedent("""
     TITLE toy gated leak
@@ -83,10 +83,10 @@ def test_generated_toy_mechanism_matches_closed_forms(tmp_path):
           Cols c{f, 1};
           gen::leakgate_initial(c, 0, -50.0, 6.3, nullptr, 0);
           double s0 = f[gen::leakgate_s];
-          gen::leakgate_advance(c, 0, -30.0, 0.025, 6.3);
+          gen::leakgate_advance(c, 0, -30.0, 0.025, 0.025, 6.3);
           double s1 = f[gen::leakgate_s];
           double val, sh;
-          gen::leakgate_current(c, 0, -30.0, 0.0, val, sh, true);
+          gen::leakgate_current(c, 0, -30.0, 0.0, 0.025, val, sh, true);
           std::printf("%.17g %.17g %.17g %.17g %.17g\\n", s0, s1, val, sh, f[gen::leakgate_i]);
         }
     """))
@@ -140,7 +140,7 @@ HARNESS = textwrap.dedent("""
       auto U = [&](double a, double b) { return std::uniform_real_distribution<double>(a, b)(g); };
       const int nf[MIND_MECH_COUNT] = {HH_NFIELDS, PAS_NFIELDS, EXPSYN_NFIELDS, EXP2SYN_NFIELDS,
                                        ICLAMP_NFIELDS, NETSTIM_NFIELDS, ALPHASYNAPSE_NFIELDS,
-                                       PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS};
+                                       PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS, VCLAMP_NFIELDS};
       for (int trial = 0; trial < 3000; ++trial) {
         for (int mech = 0; mech < MIND_MECH_COUNT; ++mech) {
           if (mech == MIND_MECH_NETSTIM) continue;  // through its stream, below
@@ -161,6 +161,13 @@ HARNESS = textwrap.dedent("""
                                            a[SECLAMP_AMP1] = U(-100, 50); a[SECLAMP_DUR2] = trial % 3 ? U(0, 100) : 0.0;
                                            a[SECLAMP_AMP2] = U(-100, 50); a[SECLAMP_DUR3] = U(0, 100);
                                            a[SECLAMP_AMP3] = U(-100, 50); }
+          if (mech == MIND_MECH_VCLAMP) { for (int k = 0; k < 3; ++k) { a[VCLAMP_DUR + k] = U(0, 100);
+                                                                        a[VCLAMP_AMP + k] = U(-100, 50); }
+                                          a[VCLAMP_GAIN] = U(1, 1e5); a[VCLAMP_RSTIM] = U(0.01, 10);
+                                          a[VCLAMP_TAU1] = U(1e-4, 0.1); a[VCLAMP_TAU2] = trial % 2 ? U(0, 0.1) : 0.0;
+                                          a[VCLAMP_FAC] = trial % 3 ? U(0, 10) : 0.0;
+                                          a[VCLAMP_E0] = U(-10, 10); a[VCLAMP_VO0] = U(-100, 50);
+                                          a[VCLAMP_VI0] = U(-100, 50); }
           std::memcpy(b, a, sizeof a);
           Cols ca{a, 1}, cb{b, 1};
           const double v = U(-100, 60), t = U(0, 250), dt = trial % 2 ? 0.025 : U(0.001, 0.2),
@@ -169,13 +176,13 @@ HARNESS = textwrap.dedent("""
           mech_initial(mech, ca, 0, v, cel);
           gen::initial(mech, cb, 0, v, cel);
           same(a, b, sizeof a, "initial", mech, trial);
-          mech_current(mech, ca, 0, v, t, va, sa, true);
-          gen::current(mech, cb, 0, v, t, vb, sb, true);
+          mech_current(mech, ca, 0, v, t, dt, va, sa, true);
+          gen::current(mech, cb, 0, v, t, dt, vb, sb, true);
           same(&va, &vb, 8, "current", mech, trial);
           same(&sa, &sb, 8, "current probe", mech, trial);
           same(a, b, sizeof a, "current fields", mech, trial);
-          mech_advance(mech, ca, 0, v, dt, cel);
-          gen::advance(mech, cb, 0, v, dt, cel);
+          mech_advance(mech, ca, 0, v, t + dt, dt, cel);
+          gen::advance(mech, cb, 0, v, t + dt, dt, cel);
           same(a, b, sizeof a, "advance", mech, trial);
           synapse_receive(mech, ca, 0, w);
           gen::receive(mech, cb, 0, w);
@@ -244,13 +251,15 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
           SHOW(hh, MIND_MECH_HH) SHOW(pas, MIND_MECH_PAS) SHOW(ExpSyn, MIND_MECH_EXPSYN)
           SHOW(Exp2Syn, MIND_MECH_EXP2SYN) SHOW(IClamp, MIND_MECH_ICLAMP) SHOW(NetStim, MIND_MECH_NETSTIM)
           SHOW(AlphaSynapse, MIND_MECH_ALPHASYNAPSE) SHOW(SEClamp, MIND_MECH_SECLAMP)
+          SHOW(VClamp, MIND_MECH_VCLAMP)
         }
     """))
     exe = tmp_path / "defaults"
     subprocess.run(["g++", "-O1", "-std=c++17", f"-I{CSRC}", f"-I{tmp_path}", "-o", str(exe), str(src),
                     os.path.join(CSRC, "layout.cpp")], check=True, capture_output=True, text=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
-    fields = {m.name: m.fields for m in (nmodl.parse(open(p).read(), p) for p in paths)}
+    fields = {m.name: [f"{n}[{k + 1}]" if n in m.extents else n for n in m.fields for k in range(m.extents.get(n, 1))]
+              for m in (nmodl.parse(open(p).read(), p) for p in paths)}
     for line in out.strip().split("\n"):
         name, k, gen_default, packed = line.split()
         fname = fields[name][int(k)]

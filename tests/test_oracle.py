@@ -380,6 +380,35 @@ def test_seclamp_on_a_passive_compartment_is_the_implicit_recurrence():
     assert r.traces["i"][-1] == 0.0  # off after dur1 + dur2 + dur3
 
 
+def test_vclamp_holds_each_level_and_balances_the_leak():
+    """VClamp (VClamp.mod, gain 1e5, rstim 1 MOhm, tau1 1 us) on an isopotential passive
+    compartment: at the end of each of the three levels the voltage sits at the level
+    and the electrode current (positive depolarises) equals the leak G (v - e) the
+    compartment loses there; after dur1 + dur2 + dur3 the clamp is off (i = 0) and the
+    compartment relaxes toward e. Array parameters are given as sequences."""
+    m, cell = passive_cell(v0=-65.0)
+    dur, amp = [2.01, 3.0, 1.0], [-40.0, -20.0, -50.0]
+    clamp = m.add_pointprocess(cell, "VClamp", params={"dur": dur, "amp": amp})
+    recs = [voltage_record(m, cell), mechanism_record(m, clamp, "i"), mechanism_record(m, clamp, "stim"),
+            mechanism_record(m, clamp, "amp", index=3, name="amp3")]
+    r = oracle.OracleSim(m).initialize().run(12.0, recs)
+    area = math.pi * 20.0 * 20.0
+    G, e = 1e-4 * area * 0.01, -65.0
+    v, i, t = r.traces["v_0"], r.traces["i"], r.time
+    for end, level in zip(np.cumsum(dur), amp):
+        k = int(np.searchsorted(t, end)) - 2  # the last sample inside the level
+        assert abs(v[k] - level) < 1e-3, (end, v[k])
+        assert abs(i[k] - G * (v[k] - e)) < 1e-3 * abs(G * (level - e)), (end, i[k])
+    assert r.traces["amp3"][0] == -50.0
+    off = t > sum(dur) + 0.05
+    assert np.all(i[off] == 0.0)
+    assert v[-1] < -50.0 and np.all(np.diff(v[off]) < 0)  # relaxing toward e = -65
+    with pytest.raises(ValueError):
+        m.add_pointprocess(cell, "VClamp", params={"dur": [1.0, 2.0]})  # dur has three elements
+    with pytest.raises(ValueError):
+        mechanism_record(m, clamp, "amp", index=4)
+
+
 def test_alpha_synapse_conductance_is_the_alpha_function():
     """AlphaSynapse (AlphaSynapse.mod): g = gmax x e^(1-x), x = (t - onset)/tau, at the
     current evaluation t + dt/2, zero outside 0 <= x <= 10; peak gmax at x = 1. The
