@@ -164,7 +164,7 @@ typedef struct mind_build_options {
   int32_t secondorder;    /* reference build! keyword (build.jl:120-131): 0 backward Euler;
                              1 and 2 Crank-Nicolson, the cable assembled with C/(dt/2) and
                              v += 2 dv (step.jl:112, cuda/micro.jl:660-679). 2 also corrects
-                             the per-node ion current columns, which none of the six
+                             the per-node ion current columns, which none of the ten
                              mechanisms reads (fixed ena/ek, no concentration dynamics),
                              so 2 behaves as 1. */
 } mind_build_options;
