@@ -177,3 +177,35 @@ def test_pattern_stim_matches_oracle(gpu, method):
     again = run(model, 20.0, recs)
     assert np.array_equal(again.spike_times, got[0].spike_times)
     destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_network_of_artificial_cells_only(gpu, method):
+    """A model without compartments: a PatternStim and two NetStims (one regular, one
+    Poisson). Rasters bit-identical to the oracle in both kernels. A connection into a
+    NetStim and a record of a NetStim field are refused by the device path with a clear
+    error."""
+    from mind_amd import MicroModel
+    from mind_amd._abi import MindError
+    m = MicroModel()
+    ps = m.add_pointprocess(m, "PatternStim", spike_times=[1.0, 2.5, 30.0])
+    regular = m.add_pointprocess(m, "NetStim", params={"interval": 5.0, "number": 3, "start": 0.0, "noise": 0.0})
+    poisson = m.add_pointprocess(m, "NetStim", params={"interval": 4.0, "number": 4, "start": 50.0, "noise": 0.5})
+    for k, p in enumerate((ps, regular, poisson)):
+        m.register_spike_source(p, sid=10 + k)
+    recs = []
+    print(f"case: artificial cells only {method}", flush=True)
+    build(m, dt_micro=0.025, seed=3, method=method)
+    initialize(m)
+    got = run(m, 100.0, recs)
+    ref = oracle.OracleSim(m, seed=3).initialize().run(100.0, recs)
+    assert np.array_equal(got.spike_times, ref.spike_times)
+    assert np.array_equal(got.spike_sources, ref.spike_sources)
+    assert got.spike_sources.tolist().count(10) == 3 and got.spike_sources.tolist().count(11) == 3
+    assert got.spike_sources.tolist().count(12) == 4
+    with pytest.raises(MindError, match="artificial"):
+        run(m, 110.0, [mechanism_record(m, poisson, "event")])
+    destroy(m)
+    m.connect(m.register_spike_source(ps), regular, weight=1.0, delay=0.5)
+    with pytest.raises(MindError, match="artificial"):
+        build(m, dt_micro=0.025, seed=3, method=method)
