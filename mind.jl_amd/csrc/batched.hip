@@ -120,7 +120,7 @@ struct BatchPlan {
 
 constexpr int kMaxThreads = 1024;
 constexpr int kLargeThreads = 512;  // block size cap of the launch of cells above the budget
-constexpr int kMaxGroups = 8;    // mechanism groups cached in LDS (the six of the step)
+constexpr int kMaxGroups = MIND_MECH_COUNT;  // mechanism groups cached in LDS: one per builtin at most
 constexpr int kMaxDetLds = 256;  // detectors of a workgroup cached in LDS
 // dynamic LDS of a workgroup, leaving room for its static tables
 constexpr size_t kLdsResident = 152 * 1024;

@@ -492,7 +492,9 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
   __syncthreads();
   for (int s = 0; s < n; ++s) {
     const int64_t b = (k1 + s) % d.ring_slots;
-    const int64_t cnt = (int64_t)d.ring_count[b];
+    // a writer that overflowed the bucket raised MIND_EQUEUE and stored nothing
+    // past ring_cap: read only the stored events
+    const int64_t cnt = std::min<int64_t>((int64_t)d.ring_count[b], d.ring_cap);
     for (int64_t i = tid; i < cnt; i += kBlock) {
       const int32_t cell = d.point_cell[d.edge_target[d.ring[b * d.ring_cap + i].edge]];
       atomicAdd(&d.seg_cursor[d.cell_bucket[cell]], 1ull);
@@ -522,7 +524,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
   // 3. scatter
   for (int s = 0; s < n; ++s) {
     const int64_t b = (k1 + s) % d.ring_slots;
-    const int64_t cnt = (int64_t)d.ring_count[b];
+    const int64_t cnt = std::min<int64_t>((int64_t)d.ring_count[b], d.ring_cap);
     for (int64_t i = tid; i < cnt; i += kBlock) {
       const QEvent ev = d.ring[b * d.ring_cap + i];
       const int32_t cell = d.point_cell[d.edge_target[ev.edge]];
@@ -550,6 +552,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
 
 // ============================================================ host side
 thread_local std::string g_error;
+constexpr int64_t kMaxWindow = 1000;          // steps per window at most
+constexpr int64_t kRingEvents = int64_t(1) << 27;  // default ring bound: 4 GiB of 32-byte events
 
 int fail(int code, const std::string& msg) {
   g_error = msg;
@@ -652,8 +656,10 @@ int setup_state(State& S, const mind_build_options& o) {
   }
   // window length: no event produced inside a window may be due inside it
   const double min_delay = L.fanout[L.first_virtual] == 0 ? INFINITY : L.min_micro_delay;
-  int64_t wmax = 1000;
-  if (std::isfinite(min_delay)) wmax = (int64_t)floor(min_delay / L.dt - 0.5);
+  // at most kMaxWindow steps per window: detector flags, emission buffers and the
+  // ring horizon grow with it
+  int64_t wmax = kMaxWindow;
+  if (std::isfinite(min_delay)) wmax = (int64_t)std::min(floor(min_delay / L.dt - 0.5), (double)kMaxWindow);
   if (o.window_steps > 0) wmax = std::min<int64_t>(wmax, o.window_steps);
   if (wmax < 1)
     return fail(MIND_EUNSUPPORTED,
@@ -690,8 +696,14 @@ int setup_state(State& S, const mind_build_options& o) {
   // delivery ring: one bucket per step up to the longest delay past a window
   const int64_t E = (int64_t)L.edge_target.size();
   d.ring_slots = (int64_t)ceil(L.max_delay / L.dt) + wmax + 4;
+  // default capacity per step bucket: every edge firing in one step, bounded so
+  // that the whole ring stays within kRingEvents events (long delays mean many
+  // buckets); an overflowing bucket raises MIND_EQUEUE and event_capacity sets
+  // the bucket size explicitly
   d.ring_cap = o.event_capacity > 0 ? o.event_capacity
                                     : std::min<int64_t>(std::max<int64_t>(E + 1024, 4096), 1 << 20);
+  if (o.event_capacity <= 0 && d.ring_slots * d.ring_cap > kRingEvents)
+    d.ring_cap = std::max<int64_t>(4096, kRingEvents / d.ring_slots);
   if ((rc = S.alloc(&d.ring, (size_t)d.ring_slots * d.ring_cap))) return rc;
   if ((rc = S.alloc(&d.ring_count, d.ring_slots))) return rc;
   HIP_TRY(hipMemset(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots));

@@ -585,7 +585,7 @@ def connection_weights(model, connection):
     return np.array(out[:n.value])
 
 
-def build(model, dt_micro=0.025, seed=1, method="plain", cells_per_group=0, window_steps=0,
+def build(model, dt_micro=0.025, seed=1, method="batched", cells_per_group=0, window_steps=0,
           event_capacity=0, device=0, secondorder=0):
     """Pack the model and allocate device state (build.jl:120-220 + allocate_simulation).
     secondorder 1 or 2 selects Crank-Nicolson cable steps as the reference's keyword does."""

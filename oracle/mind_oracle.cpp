@@ -320,6 +320,7 @@ struct Group {
   std::vector<double> data;      // field-major: data[field*rows + row]
   std::vector<double> initial;   // data snapshot after parameters (restored by initialize)
   std::vector<Rng> rng;
+  std::vector<int64_t> cell_rows;  // ncells+1: rows of cell c are [cell_rows[c], cell_rows[c+1])
   int rows() const { return (int)node.size(); }
   double& at(int field, int row) { return data[(size_t)field * node.size() + row]; }
 };
@@ -340,6 +341,7 @@ struct Sim {
   double dt = 0.025, celsius = 6.3;
   uint64_t seed = 1;
   int32_t secondorder = 0;  // build! keyword: 1, 2 Crank-Nicolson (step.jl:112, cuda/micro.jl:660-679)
+  int threads = 1;          // OpenMP threads of the membrane phases
   // nodes
   int64_t total = 0, compartments = 0;
   std::vector<int32_t> parent;
@@ -634,14 +636,17 @@ void vclamp_clamp(Group& g, int r, double v, double dt, double stim, double e0, 
   vi = v + fac * vo - fac * v;
 }
 
-void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
-  std::fill(s.rhs.begin(), s.rhs.end(), 0.0);
-  std::fill(s.diag.begin(), s.diag.end(), 0.0);
+// The membrane phases below work on a range of cells [c0, c1): a node's rhs and
+// diag receive contributions only from rows and nodes of its own cell, so cells
+// may run on separate threads and every node still sees the canonical order.
+void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (step.jl:208-224)
+  std::fill(s.rhs.begin() + s.cell_offset[c0], s.rhs.begin() + s.cell_offset[c1], 0.0);
+  std::fill(s.diag.begin() + s.cell_offset[c0], s.diag.begin() + s.cell_offset[c1], 0.0);
   for (int pass = 0; pass < 2; ++pass) {  // density groups, then point groups
     for (Group& g : s.groups) {
       if (kPoint[g.mech] != (pass == 1)) continue;
       if (g.mech == MIND_MECH_NETSTIM) continue;
-      for (int r = 0; r < g.rows(); ++r) {
+      for (int64_t r = g.cell_rows[c0]; r < g.cell_rows[c1]; ++r) {
         const int32_t node = g.node[r];
         if (node < 0) continue;
         const double v = s.v[node];
@@ -746,22 +751,23 @@ void currents(Sim& s, double t) {  // micro_currents! (step.jl:208-224)
   }
 }
 
-void cable(Sim& s) {  // assemble_cable_kernel!, solve_dhs_kernel!, update_voltage_kernel!
+// assemble_cable_kernel!, solve_dhs_kernel!, update_voltage_kernel!; returns
+// MIND_ESINGULAR on a zero or non-finite pivot
+int cable(Sim& s, int64_t c0, int64_t c1) {
   // assembly at dt, or dt/2 with secondorder (step.jl:112)
   const double dt = s.secondorder ? 0.5 * s.dt : s.dt;
-  const int64_t n = s.total;
-  for (int64_t i = 0; i < n; ++i) {  // each node pulls its own and its children's currents
+  const int64_t a = s.cell_offset[c0], n = s.cell_offset[c1];
+  int err = 0;
+  for (int64_t i = a; i < n; ++i) {  // each node pulls its own and its children's currents
     s.diag[i] = s.cap[i] / dt + s.diag[i] + s.axial_diag[i];
     double b = s.rhs[i];
     if (s.parent[i] >= 0) b += s.axial[i] * (s.v[s.parent[i]] - s.v[i]);
     for (int32_t c : s.children[i]) b -= s.axial[c] * (s.v[i] - s.v[c]);
     s.rhs[i] = b;
   }
-  for (int64_t i = n - 1; i >= 0; --i) {  // triang: children before parents
+  for (int64_t i = n - 1; i >= a; --i) {  // triang: children before parents
     const double pivot = s.diag[i];
-    if (pivot == 0.0 || !mind_isfinite(pivot)) {
-      if (s.error < MIND_ESINGULAR) s.error = MIND_ESINGULAR;
-    }
+    if (pivot == 0.0 || !mind_isfinite(pivot)) err = MIND_ESINGULAR;
     const int32_t p = s.parent[i];
     if (p >= 0) {
       const double factor = s.axial[i] / pivot;
@@ -769,7 +775,7 @@ void cable(Sim& s) {  // assemble_cable_kernel!, solve_dhs_kernel!, update_volta
       s.rhs[p] += factor * s.rhs[i];
     }
   }
-  for (int64_t i = 0; i < n; ++i) {  // bksub: parents before children
+  for (int64_t i = a; i < n; ++i) {  // bksub: parents before children
     double value = s.rhs[i];
     const int32_t p = s.parent[i];
     if (p >= 0) value += s.axial[i] * s.rhs[p];
@@ -779,19 +785,20 @@ void cable(Sim& s) {  // assemble_cable_kernel!, solve_dhs_kernel!, update_volta
   // secondorder 2 ion-current correction writes per-node ion current columns that none of
   // the six mechanisms reads)
   if (s.secondorder) {
-    for (int64_t i = 0; i < n; ++i) s.v[i] += 2.0 * s.rhs[i];
+    for (int64_t i = a; i < n; ++i) s.v[i] += 2.0 * s.rhs[i];
   } else {
-    for (int64_t i = 0; i < n; ++i) s.v[i] += s.rhs[i];
+    for (int64_t i = a; i < n; ++i) s.v[i] += s.rhs[i];
   }
+  return err;
 }
 
 
-void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
+void advance_states(Sim& s, int64_t c0, int64_t c1) {  // nrn_state at t + dt (step.jl:140)
   const double dt = s.dt;
   for (Group& g : s.groups) {
-    const int rows = g.rows();
+    const int64_t r0 = g.cell_rows[c0], rows = g.cell_rows[c1];
     if (g.mech == MIND_MECH_HH) {
-      for (int r = 0; r < rows; ++r) {
+      for (int64_t r = r0; r < rows; ++r) {
         double f[HH_NFIELDS];
         for (int k = 0; k < HH_NFIELDS; ++k) f[k] = g.at(k, r);
         hh_rates(f, s.v[g.node[r]], s.celsius);
@@ -801,16 +808,16 @@ void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
         for (int k = 0; k < HH_NFIELDS; ++k) g.at(k, r) = f[k];
       }
     } else if (g.mech == MIND_MECH_EXP2SYN) {
-      for (int r = 0; r < rows; ++r) {
+      for (int64_t r = r0; r < rows; ++r) {
         g.at(EXP2SYN_A, r) = relax(g.at(EXP2SYN_A, r), 0.0, g.at(EXP2SYN_TAU1, r), dt);
         g.at(EXP2SYN_B, r) = relax(g.at(EXP2SYN_B, r), 0.0, g.at(EXP2SYN_TAU2, r), dt);
       }
     } else if (g.mech == MIND_MECH_EXPSYN) {
-      for (int r = 0; r < rows; ++r)
+      for (int64_t r = r0; r < rows; ++r)
         g.at(EXPSYN_G, r) = relax(g.at(EXPSYN_G, r), 0.0, g.at(EXPSYN_TAU, r), dt);
     } else if (g.mech == MIND_MECH_VCLAMP) {  // SOLVE update METHOD after_cvode (VClamp.mod update)
       const double t = (double)s.step * dt + dt;  // time + dt
-      for (int r = 0; r < rows; ++r) {
+      for (int64_t r = r0; r < rows; ++r) {
         const double v = s.v[g.node[r]];
         double i = 0.0;  // FUNCTION icur
         if (t > g.at(VCLAMP_TC, r)) {
@@ -827,7 +834,7 @@ void advance_states(Sim& s) {  // nrn_state at t + dt (step.jl:140)
         g.at(VCLAMP_VI0, r) = g.at(VCLAMP_VI, r);
       }
     } else if (g.mech == MIND_MECH_SECLAMP) {  // SOLVE icur METHOD after_cvode: i at the new v
-      for (int r = 0; r < rows; ++r) {
+      for (int64_t r = r0; r < rows; ++r) {
         double i = 0.0;
         if (g.at(SECLAMP_ON, r) != 0.0) i = (g.at(SECLAMP_VC, r) - s.v[g.node[r]]) / g.at(SECLAMP_RS, r);
         g.at(SECLAMP_I, r) = i;
@@ -848,15 +855,30 @@ void detect(Sim& s, double time) {  // detect_spikes_kernel! (cuda/micro.jl:285-
   }
 }
 
+// currents at t + dt/2, cable, nrn_state for all cells; cells are independent
+// in these phases, so they are split over s.threads OpenMP threads (chunks of
+// consecutive cells), with results identical to one thread
+void membrane(Sim& s, double t_mid) {
+  const int64_t ncells = (int64_t)s.cell_offset.size() - 1;
+  const int64_t chunk = 16;
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s.threads) reduction(max : err) if (s.threads > 1)
+  for (int64_t c0 = 0; c0 < ncells; c0 += chunk) {
+    const int64_t c1 = std::min(ncells, c0 + chunk);
+    currents(s, t_mid, c0, c1);
+    err = std::max(err, cable(s, c0, c1));
+    advance_states(s, c0, c1);
+  }
+  if (err && s.error < err) s.error = err;
+}
+
 void advance_micro(Sim& s) {  // advance_micro! (step.jl:91-151)
   const double time = (double)s.step * s.dt;
   const double dt = s.dt;
   detect(s, time + 1.0e-10);
   flush(s);
   deliver(s, time + 0.5 * dt);
-  currents(s, time + 0.5 * dt);
-  cable(s);
-  advance_states(s);
+  membrane(s, time + 0.5 * dt);
   s.step += 1;
 }
 
@@ -941,6 +963,14 @@ int oracle_inject_spike(void* p, int64_t source, double time) {
   if (!(mind_isfinite(time) && time >= (double)s.step * s.dt))
     return fail(MIND_EARG, "spike time precedes the current time");
   enqueue(s, Event{time, 0.0, 0, 0, -(int32_t)(source + 1)});
+  return 0;
+}
+
+// OpenMP threads of the membrane phases (currents, cable, nrn_state); the event
+// queue stays serial. Results do not depend on the count.
+int oracle_set_threads(void* p, int32_t threads) {
+  if (threads < 1) return fail(MIND_EARG, "threads must be positive");
+  static_cast<Sim*>(p)->threads = threads;
   return 0;
 }
 
@@ -1096,6 +1126,10 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
         h.data[(size_t)d->param_field[q] * rows + r] = d->param_value[q];
     h.initial = h.data;
     h.rng.resize(rows);
+    h.cell_rows.assign(d->n_cells + 1, 0);  // NetStim rows (logical order) belong to no cell phase
+    if (g.mech != MIND_MECH_NETSTIM)
+      for (int c = 0; c <= d->n_cells; ++c)  // rows are sorted by node (artificial rows, -1, first)
+        h.cell_rows[c] = std::lower_bound(h.node.begin(), h.node.end(), (int32_t)s.cell_offset[c]) - h.node.begin();
     s.groups[gi] = h;
   }
   // 4. sources, detectors, fanout
@@ -1202,7 +1236,7 @@ int oracle_initialize(void* p) {  // initialize! (initialize.jl:270-429)
   initial_blocks(s);
   flush(s);
   deliver(s, 0.0);
-  currents(s, 0.0);
+  currents(s, 0.0, 0, (int64_t)s.cell_offset.size() - 1);
   flush(s);
   s.recording_spikes = false;
   s.initialized = s.error == 0;
@@ -1211,6 +1245,13 @@ int oracle_initialize(void* p) {  // initialize! (initialize.jl:270-429)
 }
 
 // Field read in canonical row order.
+int oracle_row_count(void* p, int mech, int64_t* rows) {
+  Sim& s = *static_cast<Sim*>(p);
+  const int gi = mech >= 0 && mech < MIND_MECH_COUNT ? s.group_of_mech[mech] : -1;
+  *rows = gi < 0 ? 0 : s.groups[gi].rows();
+  return 0;
+}
+
 int oracle_read_field(void* p, int mech, int field, double* out, int64_t n) {
   Sim& s = *static_cast<Sim*>(p);
   const int gi = s.group_of_mech[mech];

@@ -30,6 +30,8 @@ def lib():
         L.oracle_build.argtypes = [P(_abi.ModelDesc), C.c_double, C.c_uint64]
         L.oracle_free.argtypes = [C.c_void_p]
         L.oracle_set_secondorder.argtypes = [C.c_void_p, C.c_int32]
+        L.oracle_set_threads.argtypes = [C.c_void_p, C.c_int32]
+        L.oracle_row_count.argtypes = [C.c_void_p, C.c_int, P(C.c_int64)]
         L.oracle_inject_event.argtypes = [C.c_void_p, C.c_int64, C.c_double]
         L.oracle_inject_spike.argtypes = [C.c_void_p, C.c_int64, C.c_double]
         L.oracle_initialize.argtypes = [C.c_void_p]
@@ -62,10 +64,19 @@ def lib():
     return _lib
 
 
+def default_threads():
+    """Threads for the oracle's membrane phases: MIND_ORACLE_THREADS, else the
+    CPUs this process may use, at most 16."""
+    env = os.environ.get("MIND_ORACLE_THREADS")
+    if env:
+        return max(1, int(env))
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 class OracleSim:
     """The oracle's view of one MicroModel (same descriptor as libmind)."""
 
-    def __init__(self, model, dt=0.025, seed=1, secondorder=0):
+    def __init__(self, model, dt=0.025, seed=1, secondorder=0, threads=None):
         desc, keep = model.descriptor()
         self.model = model
         self.dt = dt
@@ -75,6 +86,13 @@ class OracleSim:
             raise RuntimeError("oracle build failed")
         if secondorder and lib().oracle_set_secondorder(self.h, int(secondorder)):
             raise ValueError(lib().oracle_last_error().decode())
+        self.set_threads(default_threads() if threads is None else threads)
+
+    def set_threads(self, threads):
+        """OpenMP threads of the membrane phases (results do not depend on it)."""
+        if lib().oracle_set_threads(self.h, int(threads)):
+            raise ValueError(lib().oracle_last_error().decode())
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -133,7 +151,13 @@ class OracleSim:
         lib().oracle_read_voltage(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n)
         return out
 
-    def field(self, mech_id, field, rows):
+    def rows(self, mech_id):
+        n = C.c_int64()
+        lib().oracle_row_count(self.h, int(mech_id), C.byref(n))
+        return n.value
+
+    def field(self, mech_id, field, rows=None):
+        rows = self.rows(mech_id) if rows is None else rows
         out = np.zeros(rows)
         rc = lib().oracle_read_field(self.h, mech_id, field,
                                      out.ctypes.data_as(C.POINTER(C.c_double)), rows)
