@@ -330,6 +330,16 @@ int mind_mlp_param_count(const mind_mlp* mlp, int64_t* n);
 int mind_mlp_set_params(mind_mlp* mlp, const float* params, int64_t n);
 int mind_mlp_get_params(mind_mlp* mlp, float* params, int64_t n);
 int mind_mlp_forward(mind_mlp* mlp, const float* x, int64_t batch, float* y);
+/* Allocation-free variant of mind_mlp_forward. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops). x (batch x n_in) and y (batch x n_out) are
+ * caller-owned DEVICE pointers on the MLP's device; the activations live in a
+ * static 4 MiB device array of the library (one per GPU), which limits batch to
+ * 4 MiB / (8 x widest padded layer) rows (2048 at width 256; larger batches are
+ * refused with MIND_EARG). stream NULL: run on the MLP's stream and return when
+ * y is written; otherwise only enqueue on that hipStream_t. Calls that overlap
+ * on one GPU share the array and race. Results equal mind_mlp_forward's. */
+int mind_mlp_forward_noalloc(const mind_mlp* mlp, const float* x, int64_t batch, float* y, void* stream);
 int mind_mlp_grad(mind_mlp* mlp, const float* x, const float* y, int64_t batch, float* loss, float* grads);
 int mind_mlp_train(mind_mlp* mlp, const float* X, const float* Y, int64_t n, int64_t batch, int32_t epochs,
                    float lr, uint64_t seed, float* epoch_loss);

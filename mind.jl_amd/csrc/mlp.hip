@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -147,12 +148,46 @@ __global__ void k_pack_weights(const float* w, int K, int N, int kp, int np, uin
   wb[t] = (k < K && j < N) ? f32_to_bf16(w[(int64_t)k * N + j]) : 0;
 }
 
+// y (rows x n_out, row-major) = the first n_out columns of z (row stride np) + b.
+__global__ void k_bias_out(const float* z, const float* b, int B, int N, int np, float* y) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * N) return;
+  const int i = t / N, j = t % N;
+  y[t] = z[(int64_t)i * np + j] + b[j];
+}
+
 inline unsigned nb(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+// Scratch memory of mind_mlp_forward_noalloc: two BF16 activation buffers and one
+// FP32 pre-activation buffer of kScratchRows rows each at the widest padded
+// layer. One array per GPU for the life of the process; every call on a GPU
+// shares it, so calls that overlap on one GPU race.
+constexpr int64_t kScratchBytes = int64_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_dev[64] = {};  // device address of `scratch` on each GPU
+
+// Device address of this GPU's `scratch` (looked up once per device, with that
+// device current).
+int scratch_address(int device, unsigned char** out) {
+  if (device < 0 || device >= 64) return fail(MIND_EARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MIND_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_dev[device]);
+  return 0;
+}
 
 }  // namespace
 
 struct Mlp {
   MlpShape sh{};
+  int device = 0;  // current device at creation: where every buffer lives
   int64_t max_batch = 0, nparam = 0;
   std::vector<int64_t> w_off, b_off;
   float *master = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
@@ -255,6 +290,10 @@ int mind_mlp_create(int32_t n_in, int32_t hidden, int32_t layers, int32_t n_out,
     return mind::fail(MIND_EARG, "invalid MLP shape");
   Mlp* M = new Mlp();
   M->sh = mind::MlpShape{n_in, hidden, layers, n_out};
+  if (hipGetDevice(&M->device) != hipSuccess) {
+    delete M;
+    return mind::fail(MIND_EHIP, "no current device");
+  }
   M->max_batch = (max_batch + 15) / 16 * 16;
   int rc = 0;
   if (hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -342,6 +381,52 @@ int mind_mlp_forward(mind_mlp* mlp, const float* x, int64_t batch, float* y) {
   MTRY(hipStreamSynchronize(M.stream));
   for (int i = 0; i < B; ++i)
     for (int j = 0; j < M.sh.n_out; ++j) y[(int64_t)i * M.sh.n_out + j] = out[(size_t)i * M.np(L - 1) + j];
+  return 0;
+}
+
+// mind_mlp_forward without allocation: x (batch x n_in) and y (batch x n_out)
+// are device pointers on the MLP's device, the activations live in the
+// library's static scratch array. With stream NULL the call runs on the MLP's
+// stream and returns when y is written; otherwise it is only enqueued on
+// `stream` (capturable). No hipMalloc, hipFree or host allocation.
+int mind_mlp_forward_noalloc(const mind_mlp* mlp, const float* x, int64_t batch, float* y, void* stream) {
+  if (!mlp || !x || !y) return mind::fail(MIND_EARG, "null argument");
+  const Mlp& M = *reinterpret_cast<const Mlp*>(mlp);
+  const int L = M.sh.nlin();
+  int64_t width = 0;  // widest padded layer edge
+  for (int l = 0; l < L; ++l) width = std::max<int64_t>(width, std::max(M.kp(l), M.np(l)));
+  const int64_t max_rows = mind::kScratchBytes / (width * (2 * (int64_t)sizeof(uint16_t) + (int64_t)sizeof(float)));
+  if (batch <= 0) return mind::fail(MIND_EARG, "batch must be positive");
+  if (batch > max_rows)
+    return mind::fail(MIND_EARG, "batch of " + std::to_string(batch) + " rows exceeds the scratch limit of " +
+                                     std::to_string(max_rows) + " rows for this MLP");
+  int cur = -1;
+  MTRY(hipGetDevice(&cur));
+  if (cur != M.device) MTRY(hipSetDevice(M.device));
+  unsigned char* base = nullptr;
+  int rc = mind::scratch_address(M.device, &base);
+  if (rc) return rc;
+  const int B = (int)batch;
+  uint16_t* act[2] = {reinterpret_cast<uint16_t*>(base),
+                      reinterpret_cast<uint16_t*>(base + (size_t)B * width * sizeof(uint16_t))};
+  float* z = reinterpret_cast<float*>(base + 2 * (size_t)B * width * sizeof(uint16_t));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : M.stream;
+  hipLaunchKernelGGL(mind::k_input_bf16, dim3(mind::nb((int64_t)B * M.kp(0))), dim3(256), 0, st, x, B, M.sh.n_in,
+                     M.kp(0), act[0]);
+  for (int l = 0; l < L; ++l) {
+    const int kp = M.kp(l), np = M.np(l);
+    hipLaunchKernelGGL(mind::k_gemm, dim3((B + 15) / 16, (np + 15) / 16), dim3(64), 0, st, act[l & 1], (int64_t)kp,
+                       (int64_t)1, M.wbf[l], (int64_t)np, (int64_t)1, z, (int64_t)np, B, np, kp, 0);
+    if (l < L - 1)
+      hipLaunchKernelGGL(mind::k_bias_act, dim3(mind::nb((int64_t)B * np)), dim3(256), 0, st, z,
+                         M.master + M.b_off[l], B, M.sh.n(l), np, act[(l + 1) & 1], 1);
+    else
+      hipLaunchKernelGGL(mind::k_bias_out, dim3(mind::nb((int64_t)B * M.sh.n_out)), dim3(256), 0, st, z,
+                         M.master + M.b_off[l], B, M.sh.n_out, np, y);
+  }
+  MTRY(hipGetLastError());
+  if (!stream) MTRY(hipStreamSynchronize(st));
+  if (cur != M.device) MTRY(hipSetDevice(cur));
   return 0;
 }
 

@@ -40,6 +40,7 @@ def _lib():
         L.mind_mlp_set_params.argtypes = [C.c_void_p, P(C.c_float), C.c_int64]
         L.mind_mlp_get_params.argtypes = [C.c_void_p, P(C.c_float), C.c_int64]
         L.mind_mlp_forward.argtypes = [C.c_void_p, P(C.c_float), C.c_int64, P(C.c_float)]
+        L.mind_mlp_forward_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.mind_mlp_grad.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64,
                                     P(C.c_float), P(C.c_float)]
         L.mind_mlp_train.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64, C.c_int64,
