@@ -11,10 +11,18 @@ the batched kernel with the model resident in HBM (build, autotune and
 initialize are outside the timed region); the plain one-cell-per-thread kernel
 is timed on the same model beside it.
 
+The timed region runs on the library's normal path (full windows replayed as
+a hipGraph, no per-launch timing) after an untimed warm-up of at least 80
+steps, so that events of the 1 ms minimum delay are delivered inside it; the
+events delivered there are reported.
+
 Also reported: the roofline of the dominant kernel (algorithmic bytes of its
-launch over its average duration measured with HIP events on the library's
-stream) and the CPU oracle timed on the host on a bounded sample of the same
-workload.
+launch over its average duration, measured with HIP events on the library's
+stream in a separate profiled pass after the timed region); the CPU oracle
+timed on the host on a bounded sample of the same workload, on one thread and
+on every usable core; the 1000-cell ball-and-stick network over its whole
+1000 ms in both kernels; and the micro-macro coupling, detailed against the
+learned surrogate.
 """
 from __future__ import annotations
 
@@ -51,6 +59,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--macro-ms", type=float, default=2000.0)
     ap.add_argument("--no-macro", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--plan", default="", metavar="BUDGET,THREADS,LAYOUT",
                     help="batched plan instead of the autotuner's (e.g. to profile a given plan)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -70,78 +79,42 @@ def main():
               "design-only", file=sys.stderr)
         sys.exit(2)
 
-    from mind_amd import build, configs, initialize, run, destroy
-    from mind_amd.model import (batch_plan, batch_work, kernel_time, set_batch_plan, set_profiling,
-                                synchronize, window_steps)
+    from mind_amd import configs, destroy
 
     dt = 0.025
     t0 = time.time()
     model, info = configs.tree_network(ncells=args.cells)
     log(f"generated model in {time.time() - t0:.1f}s; compartments {int(info['sizes'].sum())}")
-
-    def timed(method, steps, warmup, profile):
-        t1 = time.time()
-        build(model, dt_micro=dt, method=method)
-        if method == "batched" and args.plan:
-            set_batch_plan(model, *(int(x) for x in args.plan.split(",")))
-        initialize(model)
-        log(f"{method}: build+autotune+initialize {time.time() - t1:.1f}s")
-        if warmup:
-            run(model, warmup * dt)
-        set_profiling(model, profile)
-        synchronize(model)
-        t2 = time.perf_counter()
-        res = run(model, (warmup + steps) * dt)  # returns after its stream has drained
-        synchronize(model)
-        wall = time.perf_counter() - t2
-        kt = kernel_time(model) if profile else None
-        extra = {}
-        if method == "batched":
-            extra["plan"] = batch_plan(model)
-            extra["work"] = batch_work(model)
-        extra["window"] = window_steps(model)
-        extra["spikes"] = int(len(res.spike_times))
-        return wall, kt, extra
-
-    import ctypes as C
-
-    from mind_amd import _abi
-
-    wall, kt, extra = timed("batched", args.steps, args.warmup, True)
-    n, s = C.c_int64(), C.c_int64()
-    _abi.lib().mind_node_count(model.state, C.byref(n), C.byref(s))
-    nodes, comps = n.value, s.value
-    value = comps * args.steps / wall
-    launches, ksteps, kms = kt
-    work = extra["work"]
+    warm = max(args.warmup, MIN_WARMUP_STEPS)
+    b = timed_leg(model, "batched", args.steps, warm, plan=args.plan, profile_windows=3)
+    comps, nodes = b["compartments"], b["nodes"]
+    value = comps * args.steps / b["wall"]
+    launches, ksteps, kms = b["kernel_time"]
+    work = b["work"]
     bytes_per_step = sum(BYTES[k] * work[k] for k in BYTES)
     bytes_per_launch = bytes_per_step * (ksteps / launches)
     avg_launch_s = kms / launches / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    kernel = {0: "k_window", 4: "k_window_lean_hbm", 5: "k_window_lean"}[extra["plan"]["layout"]]
-    traffic, traffic_note = measured_traffic(kernel, extra["plan"], extra["window"], ksteps / launches,
+    kernel = {0: "k_window", 4: "k_window_lean_hbm", 5: "k_window_lean"}[b["plan"]["layout"]]
+    traffic, traffic_note = measured_traffic(kernel, b["plan"], b["window"], ksteps / launches,
                                              bytes_per_launch)
     destroy(model)
 
     plain = None
     if not args.no_plain:
-        pw, _, pextra = timed("plain", args.plain_steps, 0, False)
-        plain = {"value": comps * args.plain_steps / pw, "steps": args.plain_steps,
-                 "ms_per_step": 1e3 * pw / args.plain_steps}
+        pl = timed_leg(model, "plain", args.plain_steps, warm)
+        plain = {"value": comps * args.plain_steps / pl["wall"], "steps": args.plain_steps,
+                 "ms_per_step": 1e3 * pl["wall"] / args.plain_steps, "warmup": warm,
+                 "events_delivered": pl["delivered"]}
         destroy(model)
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
+        cpu = cpu_leg(model, comps, dt, skip=MIN_WARMUP_STEPS, steps_one=args.cpu_steps,
+                      steps_all=4 * args.cpu_steps,
+                      what=f"the same {args.cells}-cell model")
 
-        sim = oracle.OracleSim(model, dt=dt).initialize()
-        r = sim.run(args.cpu_steps * dt, record_spikes=False)
-        cpu = {"value": comps * args.cpu_steps / r.run_seconds, "unit": "compartment-steps/s",
-               "cores": 1, "kind": "port",
-               "sample": f"the same {args.cells}-cell model, first {args.cpu_steps} steps "
-                         f"after initialize, oracle (C++ FP64, -O3 -ffp-contract=off), 1 thread"}
-
+    config1 = None if args.no_config1 else config1_leg(dt, args.no_plain, args.no_cpu)
     macro = None if args.no_macro else macro_leg(args.macro_ms)
 
     line = {
@@ -151,7 +124,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * b["wall"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -160,9 +133,20 @@ def main():
         "config": {
             "workload": f"{args.cells} seed-generated branched cells (lognormal median 400 "
                         "compartments), hh/pas/Exp2Syn, in-degree 50, 5 Hz NetStim; batched "
-                        "window kernel",
+                        "window kernel replayed as a hipGraph",
             "cells": args.cells, "compartments": comps, "nodes": nodes, "dt_ms": dt,
-            "window_steps": extra["window"], "plan": extra["plan"],
+            "window_steps": b["window"], "plan": b["plan"],
+        },
+        "timed_region": {
+            "warmup_steps_run": warm,
+            "warmup_note": (f"untimed warm-up extended from {args.warmup} to {warm} steps so that "
+                            "events of the 1 ms minimum delay are delivered in the timed region")
+                           if warm > args.warmup else "as requested",
+            "start_ms": warm * dt, "end_ms": (warm + args.steps) * dt,
+            "events_delivered": b["delivered"], "events_routed": b["routed"],
+            "spikes": b["spikes"], "profiling": False,
+            "graph_replay": "every full window of the run" if args.steps >= b["window"]
+                            else f"no: {args.steps} steps are less than one {b['window']}-step window",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -170,15 +154,132 @@ def main():
             "traffic_source": traffic_note,
             "kernel": kernel, "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": kms / launches, "launches": launches,
+            "timing": f"HIP events around each {kernel} launch on the library's stream, in a "
+                      f"separate profiled pass of {ksteps} steps after the timed region",
         },
         "plain": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
-        "sim_seconds_per_wall_second": (args.steps * dt / 1e3) / wall,
-        "spikes_in_timed_run": extra["spikes"],
+        "sim_seconds_per_wall_second": (args.steps * dt / 1e3) / b["wall"],
+        "config1": config1,
         "macro": macro,
     }
     print(json.dumps(line), flush=True)
+
+
+# At least two 1 ms windows before timing: by then NetStim and recurrent events
+# (minimum delay 1 ms) are arriving every step.
+MIN_WARMUP_STEPS = 80
+
+
+def timed_leg(model, method, steps, warm, plan="", profile_windows=0):
+    """Build and initialize (untimed), `warm` untimed steps, then `steps` timed on
+    the library's normal path (hipGraph replay of full windows, profiling off);
+    with profile_windows, that many more windows with HIP-event timing of the
+    dominant launch."""
+    import ctypes as C
+
+    from mind_amd import _abi, build, initialize, run
+    from mind_amd.model import (batch_plan, batch_work, event_counts, kernel_time, set_batch_plan,
+                                set_profiling, synchronize, window_steps)
+
+    dt = 0.025
+    t1 = time.time()
+    build(model, dt_micro=dt, method=method)
+    if method == "batched" and plan:
+        set_batch_plan(model, *(int(x) for x in plan.split(",")))
+    initialize(model)
+    out = {"setup_s": time.time() - t1}
+    log(f"{method}: build+autotune+initialize {out['setup_s']:.1f}s")
+    if warm:
+        run(model, warm * dt)
+    set_profiling(model, False)
+    r0, d0 = event_counts(model)
+    synchronize(model)
+    t2 = time.perf_counter()
+    res = run(model, (warm + steps) * dt, record_spikes=True)  # returns after its stream has drained
+    synchronize(model)
+    out["wall"] = time.perf_counter() - t2
+    r1, d1 = event_counts(model)
+    out.update(routed=r1 - r0, delivered=d1 - d0, spikes=int(len(res.spike_times)), window=window_steps(model))
+    n, s = C.c_int64(), C.c_int64()
+    _abi.lib().mind_node_count(model.state, C.byref(n), C.byref(s))
+    out.update(nodes=n.value, compartments=s.value)
+    if method == "batched":
+        out["plan"] = batch_plan(model)
+        out["work"] = batch_work(model)
+    if profile_windows:
+        set_profiling(model, True)
+        run(model, (warm + steps + profile_windows * out["window"]) * dt, record_spikes=False)
+        out["kernel_time"] = kernel_time(model)
+        set_profiling(model, False)
+    return out
+
+
+def cpu_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
+
+
+def cpu_leg(model, comps, dt, skip, steps_one, steps_all, what):
+    """The oracle (C++ FP64 restatement, -O3 -ffp-contract=off) on the host: `skip`
+    untimed steps so that deliveries are under way, then `steps_one` steps on one
+    thread and `steps_all` more with OpenMP over cells on every usable core (the
+    event queue stays serial). `value` is the all-core rate."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = oracle.default_threads()
+    sim = oracle.OracleSim(model, dt=dt, threads=threads).initialize()
+    sim.run(skip * dt, record_spikes=False)
+    sim.set_threads(1)
+    r1 = sim.run((skip + steps_one) * dt, record_spikes=False)
+    sim.set_threads(threads)
+    rn = sim.run((skip + steps_one + steps_all) * dt, record_spikes=False)
+    cpu_model, nproc, usable = cpu_info()
+    one = comps * steps_one / r1.run_seconds
+    return {"value": comps * steps_all / rn.run_seconds, "unit": "compartment-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{what}, steps {skip + steps_one}-{skip + steps_one + steps_all} after initialize "
+                      f"(deliveries under way), oracle (C++ FP64, -O3 -ffp-contract=off), OpenMP over cells "
+                      f"on {threads} threads, event queue serial",
+            "one_thread": {"value": one, "steps": steps_one, "first_step": skip},
+            "nproc": nproc, "usable_cpus": usable, "cpu_model": cpu_model}
+
+
+def config1_leg(dt, no_plain, no_cpu):
+    """The 1000-cell ball-and-stick network (in-degree 100, 10 Hz Poisson drive)
+    over its whole 1000 ms in both kernels, and the oracle on a sample of it."""
+    from mind_amd import configs, destroy
+
+    model, info = configs.ball_stick_network()
+    stop_steps = int(round(info["tstop"] / dt))
+    out = {"workload": "1000 ball-and-stick cells (hh soma, 50-segment pas dendrite, Exp2Syn), in-degree "
+                       "100, lognormal weights, delays U(1,5) ms, 10 Hz Poisson NetStim per cell; 1000 ms",
+           "stop_ms": info["tstop"]}
+    legs = ["batched"] + ([] if no_plain else ["plain"])
+    for method in legs:
+        r = timed_leg(model, method, stop_steps, 0)
+        out[method] = {"value": r["compartments"] * stop_steps / r["wall"], "unit": "compartment-steps/s",
+                       "run_s": r["wall"], "sim_s_per_wall_s": info["tstop"] / 1e3 / r["wall"],
+                       "spikes": r["spikes"], "events_delivered": r["delivered"],
+                       "plan": r.get("plan")}
+        out["compartments"] = r["compartments"]
+        log(f"config-1 {method}: {info['tstop']:.0f} ms in {r['wall']:.2f}s")
+        destroy(model)
+    if "plain" in out:
+        out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    if not no_cpu:
+        out["cpu_baseline"] = cpu_leg(model, out["compartments"], dt, skip=MIN_WARMUP_STEPS, steps_one=1000,
+                                      steps_all=4000, what="the 1000-cell network")
+    return out
 
 
 def measured_traffic(kernel, plan, window, steps_per_launch, algorithmic_per_launch):

@@ -274,6 +274,10 @@ int mind_node_count(const mind_state* state, int64_t* nodes, int64_t* compartmen
 int mind_read_voltage(mind_state* state, double* out, int64_t n); /* canonical node order */
 int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out, int64_t n);
 int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows);
+/* Connection events routed (ticketed, including injected ones) and gathered for
+ * delivery since mind_initialize. Synchronises the state's stream. No reference
+ * counterpart (a measurement read-out). */
+int mind_event_counts(mind_state* state, int64_t* routed, int64_t* delivered);
 int mind_time(const mind_state* state, double* t, int64_t* step);
 /* read_value (src/runtime/state.jl:586-600): the current value of one record
  * selector (voltage, point-process field, density field); NetStim fields too.

@@ -32,6 +32,7 @@ struct Clock {
   uint64_t ns_seq;   // NetStim self-event insertion order
   int64_t emitted;   // spike emissions of the last exchange (in emit_all)
   int64_t mstep;     // macro step
+  int64_t delivered; // connection events gathered for delivery since initialize
 };
 
 struct GroupDev {

@@ -514,7 +514,10 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
     }
     running += total;
   }
-  if (tid == 0) d.seg_off[nb] = running;
+  if (tid == 0) {
+    d.seg_off[nb] = running;
+    if (running <= d.win_cap) d.clock->delivered += running;
+  }
   __syncthreads();
   if (running > d.win_cap) {
     if (tid == 0) raise_error(d.error, MIND_EQUEUE);
@@ -911,7 +914,7 @@ int mind_initialize(mind_state* state) {
     S.pending_spikes.push_back(mind::InjSpike{S.L.pattern_time[k], k + 1, S.L.pattern_source[k], 0});
   HIP_TRY(hipMemsetAsync(d.inj_count, 0, sizeof(int32_t), S.stream));
   HIP_TRY(hipMemsetAsync(d.error, 0, sizeof(int32_t), S.stream));
-  mind::Clock clock{0, 0, npattern, 0, 0};
+  mind::Clock clock{0, 0, npattern, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(d.clock, &clock, sizeof(clock), hipMemcpyHostToDevice, S.stream));
   HIP_TRY(hipMemsetAsync(d.ring_count, 0, sizeof(unsigned long long) * d.ring_slots, S.stream));
   for (size_t gi = 0; gi < S.groups_host.size(); ++gi)
@@ -928,7 +931,7 @@ int mind_initialize(mind_state* state) {
     hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
                        S.ns_logical, mind::kNetStimDigest, npattern);
     // the netstim sequence counter continues after the INITIAL insertions
-    mind::Clock c2{0, 0, npattern + (uint64_t)d.ns_rows, 0, 0};
+    mind::Clock c2{0, 0, npattern + (uint64_t)d.ns_rows, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(d.clock, &c2, sizeof(c2), hipMemcpyHostToDevice, S.stream));
   }
   // Spikes of the time-zero delivery are kept for the first run (step.jl:47-49).
@@ -1449,6 +1452,20 @@ int mind_restore(mind_state* state, const void* buffer, size_t size) {
     (void)hipGraphExecDestroy(S.graph);
     S.graph = nullptr;
   }
+  return 0;
+}
+
+// Connection events routed (ticketed) and gathered for delivery since
+// initialize, and spikes recorded by the last run (a device read; synchronises).
+int mind_event_counts(mind_state* state, int64_t* routed, int64_t* delivered) {
+  if (!state || !routed || !delivered) return fail(MIND_EARG, "null argument");
+  State& S = *reinterpret_cast<State*>(state);
+  HIP_TRY(hipSetDevice(S.device));
+  HIP_TRY(hipStreamSynchronize(S.stream));
+  mind::Clock c;
+  HIP_TRY(hipMemcpy(&c, S.d.clock, sizeof(c), hipMemcpyDeviceToHost));
+  *routed = (int64_t)c.ticket;
+  *delivered = c.delivered;
   return 0;
 }
 

@@ -162,6 +162,7 @@ def lib():
     L.mind_read_field.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _p(C.c_double), C.c_int64]
     L.mind_row_count.argtypes = [C.c_void_p, C.c_int32, _p(C.c_int64)]
     L.mind_time.argtypes = [C.c_void_p, _p(C.c_double), _p(C.c_int64)]
+    L.mind_event_counts.argtypes = [C.c_void_p, _p(C.c_int64), _p(C.c_int64)]
     L.mind_read_value.argtypes = [C.c_void_p, _p(Record), _p(C.c_double)]
     L.mind_checkpoint.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, _p(C.c_size_t)]
     L.mind_restore.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]

@@ -730,6 +730,14 @@ def batch_work(model):
     return dict(zip(keys, list(out)))
 
 
+def event_counts(model):
+    """(routed, delivered): connection events ticketed and gathered for delivery
+    since initialize."""
+    r, d = C.c_int64(), C.c_int64()
+    _abi.check(_abi.lib().mind_event_counts(model.state, C.byref(r), C.byref(d)))
+    return r.value, d.value
+
+
 def window_steps(model):
     w = C.c_int32()
     _abi.check(_abi.lib().mind_window_steps(model.state, C.byref(w)))

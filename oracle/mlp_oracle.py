@@ -98,3 +98,15 @@ def loss_and_grad(params, x, y, dims, bf16_ops=True):
         dz = dh * gelu_grad(zs[li - 1])
     flat = np.concatenate([np.concatenate([g[0].reshape(-1), g[1]]) for g in grads])
     return loss, flat
+
+
+def adam_step(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update in float64 (Kingma & Ba 2015, Algorithm 1): biased moments
+    m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, bias corrections 1 - b^t, then
+    w - lr * m_hat / (sqrt(v_hat) + eps). Returns (w, m, v)."""
+    w, g, m, v = (np.asarray(a, np.float64) for a in (w, g, m, v))
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    m_hat = m / (1.0 - b1 ** t)
+    v_hat = v / (1.0 - b2 ** t)
+    return w - lr * m_hat / (np.sqrt(v_hat) + eps), m, v

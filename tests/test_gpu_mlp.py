@@ -57,3 +57,30 @@ def test_training_fits_a_smooth_map(gpu):
     # and beat the best linear fit of the same data
     coef = S.fit_linear(x, y)
     assert np.mean((ref - y) ** 2) < 0.5 * np.mean((S.predict_linear(coef, x) - y) ** 2)
+
+
+def test_adam_steps_match_oracle(gpu):
+    """Two training steps through mind_mlp_train (k_adam on FP32 master weights)
+    against the float64 Adam oracle applied to the GPU's own gradients. The batch
+    is one row repeated, so the shuffled minibatch has the gradient that
+    mind_mlp_grad returns for it. A missing bias correction or swapped betas moves
+    weights by O(lr); FP32 rounding stays below 1e-7."""
+    dims, B, lr = (52, 256, 3, 2), 16, 1e-3
+    print(f"case: adam dims={dims} batch={B}", flush=True)
+    g = np.random.default_rng(4)
+    p0 = MO.init_params(*dims, seed=9)
+    p0 = (p0 + 0.01 * g.standard_normal(p0.size)).astype(np.float32)
+    net = S.MLP(*dims, max_batch=64)
+    net.set_params(p0)
+    w, m, v = p0.astype(np.float64), 0.0, 0.0
+    for t in (1, 2):
+        x = np.repeat(g.standard_normal((1, dims[0])).astype(np.float32), B, axis=0)
+        y = np.repeat(g.standard_normal((1, dims[3])).astype(np.float32), B, axis=0)
+        _, grad = net.grad(x, y)
+        assert np.count_nonzero(grad) > 0.5 * grad.size
+        net.train(x, y, batch=B, epochs=1, lr=lr, seed=t)
+        got = net.params().astype(np.float64)
+        w, m, v = MO.adam_step(w, grad, m, v, t, lr)
+        np.testing.assert_allclose(got, w, rtol=0, atol=1e-7)
+        w = got  # continue from the device's FP32 weights
+    assert np.max(np.abs(got - p0)) > 1.5 * lr

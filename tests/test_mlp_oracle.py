@@ -74,3 +74,21 @@ def test_bf16_mode_close_to_exact():
     b, _, _ = MO.forward(p, x, DIMS, bf16_ops=False)
     assert np.max(np.abs(a - b)) < 0.05 * np.max(np.abs(b))
     assert not np.array_equal(a, b)  # the rounding is really applied
+
+
+def test_adam_oracle_closed_forms():
+    """Adam with bias correction (Kingma & Ba 2015): the first step moves every
+    weight by lr * g / (|g| + eps); a constant gradient keeps that step; a gradient
+    that flips sign on step 2 gives m_hat = -g / 19 and v_hat = g^2 exactly
+    (m2 = 0.09 g - 0.1 g over 1 - 0.81, v2 = 0.001999 g^2 over 1 - 0.998001)."""
+    g = np.array([0.3, -2.0, 1e-3, 0.0, 5e-9])
+    w0 = np.array([0.1, -0.2, 0.3, 0.4, 0.5])
+    lr = 1e-3
+    w1, m1, v1 = MO.adam_step(w0, g, 0.0, 0.0, 1, lr)
+    np.testing.assert_allclose(w0 - w1, lr * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-18)
+    w2, m2, v2 = MO.adam_step(w1, g, m1, v1, 2, lr)
+    np.testing.assert_allclose(w1 - w2, lr * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-18)
+    w2f, _, _ = MO.adam_step(w1, -g, m1, v1, 2, lr)
+    np.testing.assert_allclose(w1 - w2f, lr * (-g / 19.0) / (np.abs(g) + 1e-8), rtol=1e-11, atol=1e-18)
+    # without the bias correction the first step would be 0.1 / sqrt(0.001) = 3.16 times larger
+    assert abs((w0 - w1)[0]) < 1.01 * lr

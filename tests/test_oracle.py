@@ -557,3 +557,125 @@ def test_micro_macro_runs_and_is_reproducible():
     assert np.array_equal(a.E, b.E) and np.array_equal(a.spike_times, b.spike_times)
     assert a.E.shape == (3, 301)
     assert np.all(np.isfinite(a.E)) and np.all(a.rate >= 0)
+
+
+# --- ExpSyn (mod/ExpSyn.mod:1-42) -------------------------------------------------
+def test_expsyn_jump_is_weight_and_decay_is_exact():
+    """ExpSyn: NET_RECEIVE g = g + w, then DERIVATIVE g' = -g/tau by cnexp. One event
+    of weight w due at 3.0 ms (NetStim at 2.0 + delay 1.0) is applied by the
+    delivery that follows the 3.0 ms sample, so g(3.0) = 0 in the trace, g = w just
+    after, and g(3.0 + k dt) = w exp(-k dt / tau); i = g (v - e)."""
+    m, cell = passive_cell(v0=-65.0)
+    tau, e, w, dt = 2.0, 0.0, 3e-3, 0.025
+    syn = m.add_pointprocess(cell, "ExpSyn", params={"tau": tau, "e": e})
+    ns = m.add_pointprocess(m, "NetStim", params={"interval": 10.0, "number": 1, "start": 2.0, "noise": 0.0})
+    m.connect(ns, syn, weight=w, delay=1.0)
+    recs = [mechanism_record(m, syn, "g"), mechanism_record(m, syn, "i"), voltage_record(m, cell)]
+    sim = oracle.OracleSim(m).initialize()
+    r = sim.run(3.0, recs)
+    assert np.all(r.traces["g"] == 0.0)
+    from mind_amd.mechanisms import builtin
+
+    mech = builtin("ExpSyn")
+    assert sim.field(mech.id, mech.field_index("g"))[0] == w  # the jump, exactly w
+    r2 = sim.run(8.0, recs)
+    k = np.arange(1, len(r2.time))
+    np.testing.assert_allclose(r2.traces["g"][1:], w * np.exp(-k * dt / tau), rtol=1e-12)
+    # the current is evaluated at t + dt/2 from g before that step's decay
+    g_prev = np.concatenate([[w], r2.traces["g"][1:-1]])
+    v_prev = r2.traces["v_0"][:-1]
+    np.testing.assert_allclose(r2.traces["i"][1:], g_prev * (v_prev - e), rtol=1e-6)
+
+
+# --- MACRO2MICRO PoissonDrive and MICRO2MACRO PopRate (project MIND roles) --------
+def _held_macro(n_roi, cells, rate_hz, k_micro=0.0, clamp_amps=None, bin_ms=1.0):
+    """ROIs whose Wilson-Cowan state is held (tau 1e15 ms: E + h dE rounds to E), so
+    the drive rate is rate_scale * E exactly; each ROI owns `cells` one-compartment
+    cells with an ExpSyn of weight 0 fed by PoissonDrive, a detector at -20 mV and
+    PopRate. With clamp_amps the cells are hh with an IClamp of that amplitude."""
+    W = np.zeros((n_roi, n_roi))
+    mm = M.MacroModel(M.Connectivity(W, np.full((n_roi, n_roi), 2.0)))
+    micro = MicroModel()
+    for r in range(n_roi):
+        M.use_macro(mm.roi(r), "WilsonCowan", params={"tau_e": 1e15, "tau_i": 1e15, "k_micro": k_micro,
+                                                      "rate_scale": rate_hz / 0.5},
+                    initial_state={"E": 0.5, "I": 0.1})
+    sources = []
+    for r in range(n_roi):
+        for c in range(cells):
+            (cell,) = micro.add_population([Section("soma", label="soma", length=20.0, diameter=20.0)], 1)
+            if clamp_amps is None:
+                micro.insert(cell, "pas", params={"g": 1e-4, "e": -65.0})
+            else:
+                micro.insert(cell, "hh")
+                micro.add_pointprocess(cell, "IClamp", params={"del": 1.0 + c, "dur": 1e9,
+                                                               "amp": clamp_amps[r][c]})
+            syn = micro.add_pointprocess(cell, "ExpSyn", params={"tau": 2.0, "e": 0.0})
+            src = micro.register_spike_source(cell, section="soma", x=0.5, threshold=-20.0)
+            sources.append((r, src))
+            if r == 0 and c == 0:
+                M.use_micro(mm.roi(0), micro)
+            elif c == 0:
+                M.use_micro(mm.roi(r), micro)
+            M.macro2micro(mm.roi(r), "PoissonDrive", target=syn, weight=0.0, delay=0.0)
+            M.micro2macro(mm.roi(r), "PopRate", source=src, params={"bin": bin_ms})
+    return mm, sources
+
+
+def test_poisson_drive_counts_and_intervals():
+    """PoissonDrive at a held rate of 200 Hz per binding (NET_RECEIVE: tn = start +
+    negexp * 1000 / rate while tn < stop, rekeyed every macro step): events per
+    0.1 ms step have mean and variance rate * dt / 1000 = 0.02, consecutive steps
+    are uncorrelated (a stream that was not rekeyed would repeat itself every
+    step), and the intervals of each binding's train are exponential with mean
+    5 ms (Kolmogorov-Smirnov). A Hz/kHz slip changes the mean 1000-fold."""
+    from scipy import stats
+
+    n_roi, cells, rate, stop = 2, 6, 200.0, 500.0
+    mm, _ = _held_macro(n_roi, cells, rate)
+    res = oracle.OracleMacro(mm).initialize().run(stop)
+    assert np.all(res.E == 0.5)
+    virtual = res.spike_sources < 0
+    times, sids = res.spike_times[virtual], -res.spike_sources[virtual] - 1
+    nb = n_roi * cells
+    assert set(np.unique(sids)) == set(range(nb))
+    steps = int(round(stop / 0.1))
+    counts = np.zeros((nb, steps))
+    np.add.at(counts, (sids, np.minimum((times / 0.1).astype(int), steps - 1)), 1)
+    lam = rate * 0.1 / 1000.0
+    n = counts.size
+    assert abs(counts.mean() - lam) < 5 * math.sqrt(lam / n)
+    assert abs(counts.var() / counts.mean() - 1.0) < 0.08
+    a, b = counts[:, :-1].ravel(), counts[:, 1:].ravel()
+    assert abs(np.corrcoef(a, b)[0, 1]) < 5 / math.sqrt(a.size)
+    isi = np.concatenate([np.diff(np.sort(times[sids == k])) for k in range(nb)])
+    assert stats.kstest(isi, "expon", args=(0, 1000.0 / rate)).pvalue > 1e-3
+    # reproducible for a seed, different for another
+    again = oracle.OracleMacro(mm).initialize().run(stop)
+    assert np.array_equal(again.spike_times, res.spike_times)
+    other = oracle.OracleMacro(mm, seed=2).initialize().run(stop)
+    assert not np.array_equal(other.spike_times[other.spike_sources < 0], times)
+
+
+def test_pop_rate_bins_match_hand_count():
+    """PopRate: at the end of every 1 ms bin an ROI publishes (spikes of its sources
+    in the bin) / (sources x 1 ms) in Hz, held until the next bin end. Counted here
+    from the raster by hand, for regularly firing hh cells of different rates."""
+    amps = [[0.3, 0.6, 1.2], [0.8, 0.0, 2.0]]
+    mm, sources = _held_macro(2, 3, 0.0, clamp_amps=amps)
+    stop = 60.0
+    res = oracle.OracleMacro(mm).initialize().run(stop)
+    cells = res.spike_sources >= 0
+    t, sid = res.spike_times[cells], res.spike_sources[cells]
+    assert len(t) > 20
+    sid_roi = {src.sid: r for r, src in sources}
+    for r in range(2):
+        mine = np.array([sid_roi[s] == r for s in sid])
+        for b in range(int(stop)):
+            n = np.count_nonzero(mine & (t >= b) & (t < b + 1))
+            expect = n / (3 * 1.0) * 1000.0
+            sample = 10 * (b + 1)  # the macro step that closes the bin
+            assert res.rate[r, sample] == expect, (r, b)
+            if b + 1 < int(stop):
+                assert np.all(res.rate[r, sample:sample + 10] == expect)
+    assert res.rate.max() > 0
