@@ -1,255 +1,283 @@
-"""Neurolucida ASCII morphology import (load_asc, src/model/micro/morphology.jl:344-398;
-parse_asc, asc_tree_label, asc_branches, asc_soma_points in
-src/model/micro/utils/morphology.jl:124-298).
+"""Neurolucida ASCII morphology import: the reference's `load_asc`
+(src/model/micro/morphology.jl:344-398, helpers in src/model/micro/utils/morphology.jl:124-298).
 
-The file is a sequence of parenthesised trees. Within a tree:
+The format is a sequence of parenthesised S-expressions:
 
-- `(x y z d)` is a sample (d is a diameter: its magnitude is used, and a diameter of
-  at most 1e-6 becomes 1);
-- a parenthesised list that starts with a list is a split: its `|`-separated parts
-  are the child branches of the current branch;
-- other lists (`(Dendrite)`, `(Color Red)`, `(CellBody)`) and strings label or
-  decorate; `<...>` markup, `;` comments and commas are ignored.
+* `(x y z d)` is a sample; d is a diameter whose magnitude is used (at most 1e-6
+  becomes 1; a missing d is 1);
+* a list whose first item is a list is a split: its parts, separated by `|`, are
+  the child branches of the branch that contains it;
+* other lists and strings label or decorate (`(Dendrite)`, `(Color Red)`,
+  `"CellBody"`, words such as `Normal`); `<...>` markup, `;` comments to the end
+  of a line and commas are ignored.
 
-A tree's label is the last of CellBody/Axon/Dendrite/Apical (case and spaces
-ignored) named by one of its strings or by the first item of one of its lists;
-trees with any other label (markers, spines) are skipped. One CellBody contour at
-most becomes the soma: a sphere-like section along the contour's principal axis
-(see `soma_points`). Every axon/dend/apic branch becomes a section named
-<label>_<ordinal>; a root branch attaches to the soma at 0.5 (or, without a soma,
-is a root), and a child branch starts with a copy of its parent's last sample
-(with the child's first diameter) unless it already starts there.
+A top-level tree is classified by the last of CellBody / Axon / Dendrite / Apical
+(case and blanks ignored) that it names in a string item or as the first word of
+one of its lists; other trees (markers, spines) are skipped. At most one CellBody
+contour is allowed; it becomes the soma section `soma_0`, a chain of 21 points
+along the contour's principal axis with the contour's width there as diameter
+(`soma_points`). Each axon/dend/apic branch becomes a section `<label>_<k>`: a
+root branch hangs from the soma at 0.5 (or is a root without a soma); a child
+branch hangs from the end of its parent and starts with a copy of the parent's
+last sample (with the child's first diameter) unless it starts there already; a
+single-sample branch becomes three points one diameter long along x.
 """
 from __future__ import annotations
 
 import math
-import re
 
 import numpy as np
 
 from .model import Section
 
-_TOKEN = re.compile(r';[^\n]*|"[^"]*"|[()|<>,]|[^\s()|<>,;]+')
-_NUMBER = re.compile(r"[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?$")
-_NONFINITE = re.compile(r"[+-]?(inf|infinity|nan)$", re.IGNORECASE)
-_LABELS = {"CELLBODY": "soma", "AXON": "axon", "DENDRITE": "dend", "APICAL": "apic"}
+KINDS = {"cellbody": "soma", "axon": "axon", "dendrite": "dend", "apical": "apic"}
+SPLIT = "|"
+_DIGITS = set("0123456789")
+
+
+# ------------------------------------------------------------------ reading ----
+def _is_number(word):
+    """Decimal literal: optional sign, digits with at most one point (at least one
+    digit), optional exponent."""
+    body = word[1:] if word[:1] in "+-" else word
+    mantissa, _, exponent = body.replace("E", "e").partition("e")
+    if "e" in body.lower() and not exponent:
+        return False
+    if exponent:
+        exp = exponent[1:] if exponent[:1] in "+-" else exponent
+        if not exp or not set(exp) <= _DIGITS:
+            return False
+    whole, dot, frac = mantissa.partition(".")
+    if "." in frac:
+        return False
+    digits = whole + frac
+    return bool(digits) and set(digits) <= _DIGITS
+
+
+def _nonfinite(word):
+    return word.lower().lstrip("+-") in ("inf", "infinity", "nan")
+
+
+def _tokens(text, filename):
+    """(kind, value, line) tokens: '(' ')' '|' 'num' 'str'."""
+    i, n, line = 0, len(text), 1
+    stops = set("()|<>,;\" \t\r\n")
+    while i < n:
+        ch = text[i]
+        if ch == "\n":
+            line += 1
+            i += 1
+        elif ch in " \t\r,":
+            i += 1
+        elif ch == ";":  # comment to the end of the line
+            j = text.find("\n", i)
+            i = n if j < 0 else j
+        elif ch == "<":  # markup to the next '>'
+            j = text.find(">", i)
+            j = n if j < 0 else j
+            line += text.count("\n", i, j)
+            i = j + 1
+        elif ch in "()|":
+            yield ch, ch, line
+            i += 1
+        elif ch == '"':
+            j = text.find('"', i + 1)
+            j = n if j < 0 else j
+            yield "str", text[i + 1:j], line
+            line += text.count("\n", i, j)
+            i = j + 1
+        elif ch == ">":
+            i += 1
+        else:
+            j = i
+            while j < n and text[j] not in stops:
+                j += 1
+            word = text[i:j]
+            if _is_number(word):
+                yield "num", float(word), line
+            elif _nonfinite(word):
+                raise ValueError(f"{filename}:{line}: ASC coordinate {word} is not finite")
+            else:
+                yield "str", word, line
+            i = j
 
 
 def parse(text, filename="<asc>"):
-    """The top-level parenthesised expressions of `text` as nested lists of floats,
-    strings and lists (parse_asc, utils/morphology.jl:132-193)."""
-    tokens, lines = [], []
-    line, last = 1, 0
-    for m in _TOKEN.finditer(text):
-        line += text.count("\n", last, m.start())
-        last = m.start()
-        tok = m.group(0)
-        if tok.startswith(";"):
-            continue
-        tokens.append(tok)
-        lines.append(line)
-    pos = 0
-
-    def expression():
-        nonlocal pos
-        items, start = [], pos
-        pos += 1
-        while pos < len(tokens):
-            tok = tokens[pos]
-            if tok == ")":
-                pos += 1
-                return items
-            if tok == "(":
-                items.append(expression())
-            elif tok == "<":  # markup up to the closing '>'
-                while pos < len(tokens) and tokens[pos] != ">":
-                    pos += 1
-                pos += 1
-            else:
-                if _NUMBER.match(tok):
-                    items.append(float(tok))
-                elif _NONFINITE.match(tok):
-                    raise ValueError(f"{filename}:{lines[pos]}: non-finite ASC coordinate")
-                elif tok.startswith('"'):
-                    items.append(tok[1:-1])
-                elif tok != ",":
-                    items.append(tok)
-                pos += 1
-        raise ValueError(f"{filename}:{lines[start]}: unclosed ASC expression")
-
-    roots = []
-    while pos < len(tokens):
-        if tokens[pos] == "(":
-            roots.append(expression())
-        else:
-            pos += 1
-    return roots
+    """Top-level S-expressions of `text` as nested Python lists of floats, strings
+    (the split marker is the string '|') and lists."""
+    top, stack = [], []  # stack of (list, line where it opened)
+    for kind, value, line in _tokens(text, filename):
+        if kind == "(":
+            stack.append(([], line))
+        elif kind == ")":
+            if stack:
+                done, _ = stack.pop()
+                (stack[-1][0] if stack else top).append(done)
+        elif stack:
+            stack[-1][0].append(value)
+    if stack:
+        raise ValueError(f"{filename}:{stack[0][1]}: ASC list opened here is never closed")
+    return top
 
 
-def _label(item):
-    if not isinstance(item, str):
-        return None
-    return _LABELS.get(re.sub(r"\s+", "", item.upper()))
+def _kind_of(word):
+    return KINDS.get("".join(word.split()).lower()) if isinstance(word, str) else None
 
 
-def tree_label(root):
-    """asc_tree_label (utils/morphology.jl:202-212): the last label a tree names."""
-    label = None
-    for item in root:
-        candidate = _label(item)
-        if isinstance(item, list) and item:
-            candidate = _label(item[0])
-        if candidate is not None:
-            label = candidate
-    return label
+def classify(tree):
+    """Label of a top-level tree: the last kind word among its string items and the
+    first items of its lists (None when it names none)."""
+    found = None
+    for item in tree:
+        word = item[0] if isinstance(item, list) and item else item
+        kind = _kind_of(word)
+        if kind is not None:
+            found = kind
+    return found
 
 
-class Branch:
+class Arbor:
+    """A branch: its samples and the branches it splits into."""
+
     __slots__ = ("points", "children")
 
-    def __init__(self):
-        self.points, self.children = [], []
+    def __init__(self, points=None, children=None):
+        self.points = points if points is not None else []
+        self.children = children if children is not None else []
 
 
-def branches(items):
-    """Branches of a tree or split (asc_branches, utils/morphology.jl:214-238)."""
-    out = [Branch()]
+def _sample(item):
+    if len(item) < 3 or not all(isinstance(v, float) for v in item[:3]):
+        raise ValueError("an ASC sample needs three numeric coordinates")
+    d = abs(item[3]) if len(item) > 3 and isinstance(item[3], float) else 1.0
+    return (item[0], item[1], item[2], d if d > 1e-6 else 1.0)
+
+
+def arbors(items):
+    """Branches of a tree's (or a split's) items: `|` starts the next branch;
+    samples extend the current one; a split adds child branches to it. Branches
+    with neither samples nor children are left out."""
+    parts = [Arbor()]
     for item in items:
-        if isinstance(item, str) and item == "|":
-            out.append(Branch())
+        if item == SPLIT:
+            parts.append(Arbor())
         elif isinstance(item, list) and item:
-            if isinstance(item[0], float):
-                if len(item) < 3 or not all(isinstance(x, float) for x in item[:3]):
-                    raise ValueError("ASC point requires three coordinates")
-                d = abs(item[3]) if len(item) >= 4 and isinstance(item[3], float) else 1.0
-                if d <= 1e-6:
-                    d = 1.0
-                out[-1].points.append((item[0], item[1], item[2], d))
-            elif isinstance(item[0], list):
-                out[-1].children.extend(branches(item))
-    return [b for b in out if b.points or b.children]
+            head = item[0]
+            if isinstance(head, float):
+                parts[-1].points.append(_sample(item))
+            elif isinstance(head, list):
+                parts[-1].children += arbors(item)
+    return [a for a in parts if a.points or a.children]
 
 
-def _interp(x, y, at):
-    """Piecewise-linear y(at) over increasing x, clamped at the ends
-    (interpolate_samples, utils/morphology.jl:240-248)."""
-    x, y, at = np.asarray(x), np.asarray(y), np.asarray(at)
-    out = np.empty(at.shape)
-    for k, p in enumerate(at):
-        if p <= x[0]:
-            out[k] = y[0]
-        elif p >= x[-1]:
-            out[k] = y[-1]
-        else:
-            right = int(np.searchsorted(x, p, side="left"))
-            left = right - 1
-            frac = (p - x[left]) / (x[right] - x[left])
-            out[k] = y[left] + frac * (y[right] - y[left])
-    return out
+# ------------------------------------------------------------------ soma -------
+def _resample(curve, count):
+    """`count` points at equal arc length along a polyline (ends kept)."""
+    steps = np.linalg.norm(np.diff(curve, axis=0), axis=1)
+    arc = np.concatenate([[0.0], np.cumsum(steps)])
+    keep = np.concatenate([[True], steps > 0])  # coincident samples add no arc
+    arc, curve = arc[keep], curve[keep]
+    at = np.linspace(0.0, arc[-1], count)
+    return np.column_stack([np.interp(at, arc, curve[:, j]) for j in range(3)])
 
 
-def _monotone(x, y):
-    keep = [0]
-    for k in range(1, len(x)):
-        if x[k] > x[keep[-1]]:
+def _rising(x, y):
+    """The samples of (x, y) at which x exceeds every earlier x."""
+    best, keep = -math.inf, []
+    for k, v in enumerate(x):
+        if v > best:
+            best = v
             keep.append(k)
     return x[keep], y[keep]
 
 
-def soma_points(contour):
-    """The soma of a CellBody contour (asc_soma_points, utils/morphology.jl:250-298):
-    the contour resampled at 101 equal arc-length steps; its centroid and principal
-    axis (largest eigenvector of the scatter matrix, its largest component made
-    positive); the in-plane minor direction (middle eigenvector with z dropped); the
-    outline split at its far ends along the major axis into two monotone halves; and
-    21 points along the axis between the second-smallest and second-largest
-    positions, each with the distance between the halves as its diameter (ends
-    averaged with their neighbours, diameters of at most 1e-6 set to 1). Empty when
-    the contour is too short or degenerate."""
-    pts = np.asarray([p[:3] for p in contour], dtype=np.float64)
-    if len(pts) < 3:
+def soma_points(contour, count=101, stations=21):
+    """Soma section of a CellBody contour: the contour resampled at `count` equal
+    arc-length steps; the centroid and principal axes of those samples (scatter
+    matrix eigenvectors; the major one with its largest component positive; the
+    middle one with its z component dropped as in-plane minor direction, or a
+    perpendicular of the major one when that vanishes); the outline split at its
+    far ends along the major axis into two halves, each made monotone along it;
+    `stations` points on the major axis between the second-lowest and
+    second-highest outline positions, with the distance between the halves as
+    diameter (the end values averaged with their neighbours, at most 1e-6 set to
+    1). Empty when the contour is too short or degenerate."""
+    xyz = np.asarray([p[:3] for p in contour], np.float64)
+    if len(xyz) < 3:
         return []
-    arcs = np.zeros(len(pts))
-    for k in range(1, len(pts)):
-        arcs[k] = arcs[k - 1] + math.sqrt(float(np.sum((pts[k] - pts[k - 1]) ** 2)))
-    if not arcs[-1] > 1e-12:
+    if not np.sum(np.linalg.norm(np.diff(xyz, axis=0), axis=1)) > 1e-12:
         return []
-    at = np.linspace(0.0, arcs[-1], 101)
-    sampled = np.stack([_interp(arcs, pts[:, j], at) for j in range(3)], axis=1)
-    center = sampled.sum(axis=0) / len(sampled)
-    centered = sampled - center
-    _, vectors = np.linalg.eigh(centered.T @ centered)
-    major = vectors[:, 2].copy()
+    ring = _resample(xyz, count)
+    center = ring.sum(axis=0) / count
+    rel = ring - center
+    _, axes = np.linalg.eigh(rel.T @ rel)  # ascending eigenvalues
+    major = axes[:, 2].copy()
     if major[np.argmax(np.abs(major))] < 0.0:
         major = -major
-    minor = vectors[:, 1].copy()
-    minor[2] = 0.0
+    minor = np.array([axes[0, 1], axes[1, 1], 0.0])
     if np.linalg.norm(minor) < 1e-6:
         minor = np.array([-major[1], major[0], 0.0])
         if np.linalg.norm(minor) < 1e-12:
             minor = np.array([1.0, 0.0, 0.0])
-    minor = minor / np.linalg.norm(minor)
-    longitudinal, radial = centered @ major, centered @ minor
-    first = int(np.argmax(longitudinal))
-    longitudinal, radial = np.roll(longitudinal, -first), np.roll(radial, -first)
-    split = int(np.argmin(longitudinal))
-    if not 0 < split < len(longitudinal) - 1:
+    minor /= np.linalg.norm(minor)
+    along, across = rel @ major, rel @ minor
+    start = int(np.argmax(along))
+    along, across = np.roll(along, -start), np.roll(across, -start)
+    turn = int(np.argmin(along))  # from the far end back: one half, then the other
+    if turn <= 0 or turn >= count - 1:
         return []
-    x1, y1 = _monotone(longitudinal[:split][::-1], radial[:split][::-1])
-    x2, y2 = _monotone(longitudinal[split:], radial[split:])
-    if min(len(x1), len(x2)) < 2:
+    x1, y1 = _rising(along[:turn][::-1], across[:turn][::-1])
+    x2, y2 = _rising(along[turn:], across[turn:])
+    if len(x1) < 2 or len(x2) < 2:
         return []
-    merged = np.sort(np.concatenate([x1, x2]))
-    if not merged[-2] > merged[1]:
+    pos = np.sort(np.concatenate([x1, x2]))
+    lo, hi = pos[1], pos[-2]
+    if not hi > lo:
         return []
-    coords = np.linspace(merged[1], merged[-2], 21)
-    diam = np.abs(_interp(x1, y1, coords) - _interp(x2, y2, coords))
-    diam[0] = 0.5 * (diam[0] + diam[1])
-    diam[-1] = 0.5 * (diam[-1] + diam[-2])
-    out = []
-    for c, d in zip(coords, diam):
-        x, y, z = center + major * c
-        out.append((float(x), float(y), float(z), float(d) if d > 1e-6 else 1.0))
-    return out
+    at = np.linspace(lo, hi, stations)
+    width = np.abs(np.interp(at, x1, y1) - np.interp(at, x2, y2))
+    width[0] = (width[0] + width[1]) / 2
+    width[-1] = (width[-1] + width[-2]) / 2
+    width[width <= 1e-6] = 1.0
+    return [tuple(float(v) for v in center + major * c) + (float(w),) for c, w in zip(at, width)]
 
 
+# ------------------------------------------------------------------ sections ---
 def load_asc(path, nseg=1, Ra=100.0, cm=1.0):
     """Sections of the Neurolucida ASCII morphology in `path` (list of Section)."""
     with open(path) as f:
-        roots = parse(f.read(), str(path))
-    trees = [(tree_label(root), branches(root)) for root in roots]
-    contours = [b.points for label, bs in trees if label == "soma" for b in bs if b.points]
+        trees = parse(f.read(), str(path))
+    kinds = [(classify(t), arbors(t)) for t in trees]
+    contours = [a.points for kind, arbs in kinds if kind == "soma" for a in arbs if a.points]
     if len(contours) > 1:
-        raise ValueError("ASC supports one CellBody contour")
-    sections = []
-    parent = None
+        raise ValueError(f"{path}: more than one CellBody contour (only one soma is supported)")
+    out = []
+    soma = None
     if contours:
-        points = soma_points(contours[0])
-        if len(points) >= 2:
-            parent = "soma_0"
-            sections.append(Section(parent, label="soma", points=points, nseg=nseg, Ra=Ra, cm=cm))
-    counters = {"axon": 0, "dend": 0, "apic": 0}
-
-    def emit(branch, label, parent, terminal, parent_x):
-        points = list(branch.points)
-        here = terminal if not points else points[-1]
-        if terminal is not None and points and tuple(terminal[:3]) != tuple(points[0][:3]):
-            points.insert(0, (terminal[0], terminal[1], terminal[2], points[0][3]))
-        if points:
-            if len(points) == 1:
-                x, y, z, d = points[0]
-                points = [(x - 0.5 * d, y, z, d), points[0], (x + 0.5 * d, y, z, d)]
-            name = f"{label}_{counters[label]}"
-            counters[label] += 1
-            sections.append(Section(name, label=label, parent=parent, parent_x=parent_x, points=points,
-                                    nseg=nseg, Ra=Ra, cm=cm))
-            parent, parent_x = name, 1.0
-        for child in branch.children:
-            emit(child, label, parent, here, parent_x)
-
-    for label, bs in trees:
-        if label not in counters:
+        pts = soma_points(contours[0])
+        if len(pts) >= 2:
+            soma = "soma_0"
+            out.append(Section(soma, label="soma", points=pts, nseg=nseg, Ra=Ra, cm=cm))
+    serial = {"axon": 0, "dend": 0, "apic": 0}
+    for kind, arbs in kinds:
+        if kind not in serial:
             continue
-        for b in bs:
-            emit(b, label, parent, None, 1.0 if parent is None else 0.5)
-    return sections
+        # (arbor, parent section, parent's end sample, attachment) in depth-first order
+        todo = [(a, soma, None, 0.5 if soma else 1.0) for a in reversed(arbs)]
+        while todo:
+            arbor, parent, end, x = todo.pop()
+            pts = list(arbor.points)
+            if pts and end is not None and pts[0][:3] != end[:3]:
+                pts.insert(0, end[:3] + (pts[0][3],))
+            tail = pts[-1] if pts else end
+            if pts:
+                if len(pts) == 1:
+                    px, py, pz, d = pts[0]
+                    pts = [(px - d / 2, py, pz, d), pts[0], (px + d / 2, py, pz, d)]
+                name = f"{kind}_{serial[kind]}"
+                serial[kind] += 1
+                out.append(Section(name, label=kind, parent=parent, parent_x=x, points=pts, nseg=nseg,
+                                   Ra=Ra, cm=cm))
+                parent, x = name, 1.0
+            todo.extend((c, parent, tail, x) for c in reversed(arbor.children))
+    return out

@@ -16,6 +16,8 @@ src/model/build.jl:230-295) and run for macro models (src/runtime/step.jl:4-74,
 """
 from __future__ import annotations
 
+import csv
+
 import ctypes as C
 from dataclasses import dataclass, field
 
@@ -72,71 +74,53 @@ class Connectivity:
             raise ValueError("delays must be finite and nonnegative")
 
 
-def _csv_fields(line):
-    """Comma-separated fields with double-quote quoting ("" inside quotes is a quote),
-    each stripped of surrounding blanks (csv_fields, macro/utils/model.jl:73-101)."""
-    if '"' not in line:
-        return [f.strip() for f in line.split(",")]
-    out, buf, quoted, i = [], [], False, 0
-    while i < len(line):
-        ch = line[i]
-        if ch == '"':
-            if quoted and i + 1 < len(line) and line[i + 1] == '"':
-                buf.append('"')
-                i += 1
-            else:
-                quoted = not quoted
-        elif ch == "," and not quoted:
-            out.append("".join(buf).strip())
-            buf = []
-        else:
-            buf.append(ch)
-        i += 1
-    out.append("".join(buf).strip())
-    return out
+def _csv_rows(path):
+    """Non-blank rows of a CSV file, fields stripped of surrounding blanks (RFC 4180
+    quoting by the standard library: a quoted field may hold commas, "" is a quote)."""
+    with open(path, newline="") as f:
+        return [[x.strip() for x in row] for row in csv.reader(f, skipinitialspace=True)
+                if any(x.strip() for x in row)]
+
+
+def _matrix_block(rows, name, path):
+    """The `matrix,<name>` block: header row of ROI labels, then one row per target
+    ROI (its label, then one value per source ROI)."""
+    starts = [k for k, r in enumerate(rows) if r[:2] == ["matrix", name]]
+    if not starts:
+        raise ValueError(f"{path}: no matrix,{name} block")
+    at = starts[0] + 1
+    if at >= len(rows):
+        raise ValueError(f"{path}: matrix,{name} has no header row")
+    labels = rows[at][1:]
+    body = rows[at + 1:at + 1 + len(labels)]
+    if len(body) < len(labels):
+        raise ValueError(f"{path}: matrix,{name} has {len(body)} of {len(labels)} rows")
+    values = np.empty((len(labels), len(labels)))
+    for k, (label, row) in enumerate(zip(labels, body)):
+        if len(row) != len(labels) + 1 or row[0] != label:
+            raise ValueError(f"{path}: matrix,{name} row {k + 1} does not match the header's labels or size")
+        values[k] = [float(x) for x in row[1:]]
+    return labels, values
 
 
 def load_connectivity(path) -> Connectivity:
-    """Connectome in MIND_Sim's labelled CSV format (load_connectivity,
-    src/model/macro/model.jl:22-54): a `matrix,weights` block and a `matrix,delays_ms`
-    block, each a header row of ROI labels then one row per target ROI (label, then the
-    value from each source). Zero diagonal; a nonpositive weight must have zero delay."""
-    with open(path) as f:
-        rows = [r for r in (_csv_fields(line.rstrip("\r\n")) for line in f) if any(x != "" for x in r)]
-
-    def block(name):
-        marker = next((k for k, r in enumerate(rows) if len(r) >= 2 and r[0] == "matrix" and r[1] == name), None)
-        if marker is None:
-            raise ValueError(f"missing matrix,{name} in {path}")
-        if marker + 1 >= len(rows):
-            raise ValueError(f"empty matrix,{name}")
-        labels = rows[marker + 1][1:]
-        n = len(labels)
-        if marker + n + 1 > len(rows) - 1:
-            raise ValueError(f"incomplete matrix,{name}")
-        values = np.empty((n, n))
-        for k in range(n):
-            r = rows[marker + k + 2]
-            if len(r) != n + 1 or r[0] != labels[k]:
-                raise ValueError(f"matrix,{name} labels or dimensions differ")
-            values[k] = [float(x) for x in r[1:]]
-        return labels, values
-
-    labels, weights = block("weights")
-    dlabels, delays = block("delays_ms")
-    if labels != dlabels:
-        raise ValueError("weight and delay labels differ")
-    n = len(labels)
-    for t in range(n):
-        for q in range(n):
-            if t == q and (weights[t, q] != 0.0 or delays[t, q] != 0.0):
-                raise ValueError("connectivity CSV diagonal must be zero")
-            if weights[t, q] <= 0.0 and delays[t, q] != 0.0:
-                raise ValueError("CSV delay must be zero when weight is nonpositive")
-    if not all(labels) or len(set(labels)) != n:
-        raise ValueError("ROI labels must be nonempty and unique")
+    """Connectome in MIND_Sim's labelled CSV format, the reference's `load_connectivity`
+    (src/model/macro/model.jl:22-54): a `matrix,weights` block and a `matrix,delays_ms`
+    block with the same ROI labels. The diagonal is zero, and an absent (nonpositive)
+    weight has delay zero."""
+    rows = _csv_rows(path)
+    labels, weights = _matrix_block(rows, "weights", path)
+    delay_labels, delays = _matrix_block(rows, "delays_ms", path)
+    if delay_labels != labels:
+        raise ValueError(f"{path}: the weight and delay blocks have different ROI labels")
+    if np.any(np.diag(weights) != 0.0) or np.any(np.diag(delays) != 0.0):
+        raise ValueError(f"{path}: the diagonal (self-connections) must be zero")
+    if np.any((weights <= 0.0) & (delays != 0.0)):
+        raise ValueError(f"{path}: a delay must be zero where the weight is not positive")
+    if "" in labels or len(set(labels)) != len(labels):
+        raise ValueError(f"{path}: ROI labels must be nonempty and unique")
     if not (np.all(np.isfinite(weights)) and np.all(np.isfinite(delays))):
-        raise ValueError("connectivity must be finite with nonnegative delays")
+        raise ValueError(f"{path}: weights and delays must be finite")
     return Connectivity(weights, delays, list(labels))
 
 

@@ -42,8 +42,8 @@ def test_connectome_csv_round_trip(tmp_path, quote):
 
 
 @pytest.mark.parametrize("defect,message", [
-    ("diagonal", "diagonal"), ("delay", "delay must be zero"), ("labels", "labels differ"),
-    ("missing", "missing matrix,delays_ms"), ("short", "labels or dimensions"), ("dup", "unique"),
+    ("diagonal", "diagonal"), ("delay", "delay must be zero"), ("labels", "different ROI labels"),
+    ("missing", "no matrix,delays_ms"), ("short", "labels or size"), ("dup", "unique"),
 ])
 def test_connectome_csv_rejections(tmp_path, defect, message):
     W, D = _matrices()
@@ -141,12 +141,46 @@ def test_swc_branches_and_zero_length_sections(tmp_path):
     assert m._morph(cell).section_index("dend_3") == 4
 
 
+def test_swc_soma_chain_attachments(tmp_path):
+    """A three-sample soma chain (not the three-point sphere: the ends continue the
+    root): a neurite leaving its middle sample hangs at 0.5 without repeating the
+    soma sample; one leaving its last sample hangs at 1 and starts with a copy of
+    that sample carrying the neurite's own diameter."""
+    text = ("1 1 0 0 0 4 -1\n2 1 0 6 0 4 1\n3 1 0 12 0 4 2\n"
+            "4 3 5 6 0 1 2\n5 3 15 6 0 1 4\n6 2 0 20 0 0.5 3\n7 2 0 30 0 0.5 6\n")
+    s = _by_name(load_swc(_swc(tmp_path, text)))
+    assert sorted(s) == ["axon_0", "dend_0", "soma_0"]
+    assert s["soma_0"].points == [(0.0, 0.0, 0.0, 8.0), (0.0, 6.0, 0.0, 8.0), (0.0, 12.0, 0.0, 8.0)]
+    assert s["dend_0"].parent == "soma_0" and s["dend_0"].parent_x == 0.5
+    assert s["dend_0"].points == [(5.0, 6.0, 0.0, 2.0), (15.0, 6.0, 0.0, 2.0)]
+    assert s["axon_0"].parent == "soma_0" and s["axon_0"].parent_x == 1.0
+    assert s["axon_0"].points == [(0.0, 12.0, 0.0, 1.0), (0.0, 20.0, 0.0, 1.0), (0.0, 30.0, 0.0, 1.0)]
+
+
+def test_swc_neurite_root_with_proximal_sibling(tmp_path):
+    """A root that is not soma with two children of its type: the first continues its
+    section; the second starts a section at the proximal end (x = 0) that repeats
+    the root sample; a child of another type under a continuing sample starts a
+    section at x = 1; unknown types are named sec_<type>_<k>."""
+    text = ("1 2 0 0 0 1 -1\n2 2 10 0 0 1 1\n3 2 0 10 0 1 1\n4 2 20 0 0 1 2\n"
+            "5 7 20 10 0 0.5 4\n")
+    secs = load_swc(_swc(tmp_path, text))
+    s = _by_name(secs)
+    assert [x.name for x in secs] == ["axon_0", "axon_1", "sec_7_0"]  # branches in sample order along the run
+    assert s["axon_0"].parent is None
+    assert s["axon_0"].points == [(0.0, 0.0, 0.0, 2.0), (10.0, 0.0, 0.0, 2.0), (20.0, 0.0, 0.0, 2.0)]
+    assert s["axon_1"].parent == "axon_0" and s["axon_1"].parent_x == 0.0
+    assert s["axon_1"].points == [(0.0, 0.0, 0.0, 2.0), (0.0, 10.0, 0.0, 2.0)]
+    assert s["sec_7_0"].parent == "axon_0" and s["sec_7_0"].parent_x == 1.0
+    assert s["sec_7_0"].label == "7" and s["sec_7_0"].points[0] == (20.0, 0.0, 0.0, 2.0)
+
+
 @pytest.mark.parametrize("text,message", [
-    ("1 1 0 0 0 5\n", "seven SWC columns"),
-    ("1 1 0 0 0 5 -1\n1 3 0 1 0 1 1\n", "duplicate"),
-    ("1 1 0 0 0 5 -1\n2 3 0 1 0 1 9\n", "missing parent"),
-    ("1 3 0 0 0 1 2\n2 3 0 1 0 1 1\n", "unrooted cycle"),
-    ("\n", "empty"),
+    ("1 1 0 0 0 5\n", "7 columns"),
+    ("1 1 0 0 0 5 -1\n1 3 0 1 0 1 1\n", "appears twice"),
+    ("1 1 0 0 0 5 -1\n2 3 0 1 0 1 9\n", "is not a sample"),
+    ("1 3 0 0 0 1 2\n2 3 0 1 0 1 1\n", "cycle"),
+    ("\n", "no SWC samples"),
 ])
 def test_swc_rejections(tmp_path, text, message):
     with pytest.raises(ValueError, match=message):
@@ -212,7 +246,7 @@ def test_asc_branches_terminals_and_labels(tmp_path):
     assert d["dend_3"].parent == "dend_2"
     assert d["dend_3"].points == [(20.0, -5.0, 0.0, 1.0), (25.0, -5.0, 0.0, 1.0)]
     assert d["dend_4"].points == [(20.0, -5.0, 0.0, 1.0), (20.0, -15.0, 0.0, 1.0)]
-    one = A.branches(A.parse("((Axon) (4 5 6 2))")[0])
+    one = A.arbors(A.parse("((Axon) (4 5 6 2))")[0])
     assert [b.points for b in one] == [[(4.0, 5.0, 6.0, 2.0)]]
     f2 = tmp_path / "one.asc"
     f2.write_text("((Axon) (4 5 6 2))\n")
@@ -262,10 +296,10 @@ def test_asc_soma_direction_and_degenerate_contours():
 
 
 @pytest.mark.parametrize("text,message", [
-    ("((Dendrite) (0 0 0 1) (1 2 3 1)", "unclosed"),
-    ("((Dendrite) (0 0 inf 1))", "non-finite"),
-    ("((Dendrite) (0 0 1) (1 x 2 1))", "three coordinates"),
-    ('("CellBody" (0 0 0 1) (1 0 0 1) (0 1 0 1)) ("CellBody" (5 0 0 1) (6 0 0 1) (5 1 0 1))', "one CellBody"),
+    ("((Dendrite) (0 0 0 1) (1 2 3 1)", "never closed"),
+    ("((Dendrite) (0 0 inf 1))", "not finite"),
+    ("((Dendrite) (0 0 1) (1 x 2 1))", "three numeric coordinates"),
+    ('("CellBody" (0 0 0 1) (1 0 0 1) (0 1 0 1)) ("CellBody" (5 0 0 1) (6 0 0 1) (5 1 0 1))', "more than one CellBody"),
 ])
 def test_asc_rejections(tmp_path, text, message):
     f = tmp_path / "bad.asc"
