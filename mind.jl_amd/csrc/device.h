@@ -52,6 +52,31 @@ struct Emit {  // one spike emission: source index and spike time
   int32_t pad;
 };
 
+// A pending event of the event-exact path (ScheduledEvent, src/runtime/state.jl:18-24):
+// target > 0 is point target-1, target < 0 is source -target-1; edge is the
+// 1-based connection edge (0 for none); order is its insertion order.
+struct SEvent {
+  double time, flag;
+  int64_t edge, order;
+  int32_t target, pad;
+};
+
+// Queue state of the event-exact path (serial.hip).
+struct SerialDev {
+  SEvent* heap;
+  int64_t heap_cap;
+  int64_t* heap_n;
+  SEvent* outbox;
+  int64_t outbox_cap;
+  int64_t* outbox_n;
+  int64_t* sequences;           // per point: net_send / net_event count
+  const int64_t* target_rank;   // per point
+  const int64_t* source_rank;   // per source
+  const int64_t* point_src_off; // sources of each point (CSR)
+  const int32_t* point_src;
+  const int32_t* ns_point;      // point of each NetStim row
+};
+
 struct Dev {
   double dt, celsius;
   uint64_t seed;
@@ -131,6 +156,10 @@ struct Dev {
   double* rec_buf;
   int64_t rec_stride;
   int64_t rec_start;
+  // records of artificial-cell fields (NetStim): sampled by the event kernels,
+  // after the midpoint delivery of each step, before its end-of-step delivery
+  const int32_t* art_rec;
+  int64_t n_art_rec;
   // injected source spikes due in the next window, and their count (the window's
   // exchange consumes them and clears the count)
   InjSpike* inj;
@@ -162,5 +191,15 @@ __host__ __device__ inline int64_t call_of(double t, int64_t jmin, double dt) {
 __device__ inline int64_t emit_key(int64_t j) { return j < 0 ? -1 : 3 * (j / 2) + 1 + (j & 1); }
 
 __device__ inline Cols cols(const GroupDev& g) { return Cols{g.data, g.rows}; }
+
+// Samples the artificial-cell records for step k (sample k + 1 of the run).
+__device__ inline void record_artificial(const Dev& d, int64_t k) {
+  const int64_t sample = k + 1 - d.rec_start;
+  for (int64_t q = 0; q < d.n_art_rec; ++q) {
+    const int32_t rec = d.art_rec[q];
+    const GroupDev& G = d.g[d.rec_group[rec]];
+    d.rec_buf[rec * d.rec_stride + sample] = G.data[d.rec_field[rec] * G.rows + d.rec_index[rec]];
+  }
+}
 
 }  // namespace mind

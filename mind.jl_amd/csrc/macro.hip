@@ -248,6 +248,9 @@ void macro_release(State& S) {
 
 int macro_setup(State& S, const mind_macro_desc& md) {
   const Layout& L = S.L;
+  if (S.serial)
+    return fail(MIND_EUNSUPPORTED, "micro-macro coupling needs micro NetCon delays of at least 1.5 dt and no "
+                                   "connections into NetStims");
   MacroHost* H = new MacroHost();
   S.macro = H;
   MacroDev& m = H->m;

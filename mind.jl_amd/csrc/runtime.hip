@@ -319,10 +319,17 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     const Cols col = d.ns_group >= 0 ? cols(d.g[d.ns_group]) : Cols{nullptr, 0};
     int64_t count = 0;
     uint64_t seq = d.clock->ns_seq;
+    // artificial-cell records of step k take the state after the events of
+    // calls <= 2k (the record precedes the end-of-step delivery)
+    int64_t sampled = n > 0 ? k0 : INT64_MAX;
     while (m > 0) {
       const HeapItem it = heap_pop(heap, m);
       const int32_t r = it.row;
       const double t = it.t;
+      if (d.n_art_rec) {
+        const int64_t jt = call_of(t, call_lo, dt);
+        for (; sampled < k0 + n && 2 * sampled < jt; ++sampled) record_artificial(d, sampled);
+      }
       if (r < 0) {  // injected source spike: routed at its delivery call, like a net_event
         const int64_t j = call_of(t, call_lo, dt);
         if (count < d.emit_cap) d.emit_ns[count] = Emit{emit_key(j), t, d.inj[-r - 1].source, 0};
@@ -362,6 +369,8 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     }
     d.clock->ns_seq = seq;
     n_ns = count < d.emit_cap ? count : d.emit_cap;
+    if (d.n_art_rec)
+      for (; sampled < k0 + n; ++sampled) record_artificial(d, sampled);
   }
   __syncthreads();
   // B. Detector spikes of the window in (step, detector) order, compacted by
@@ -556,6 +565,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
 // ============================================================ host side
 thread_local std::string g_error;
 constexpr int64_t kMaxWindow = 1000;          // steps per window at most
+constexpr int64_t kSerialWindow = 16;         // steps per replayed graph on the event-exact path
 constexpr int64_t kRingEvents = int64_t(1) << 27;  // default ring bound: 4 GiB of 32-byte events
 
 int fail(int code, const std::string& msg) {
@@ -664,9 +674,17 @@ int setup_state(State& S, const mind_build_options& o) {
   int64_t wmax = kMaxWindow;
   if (std::isfinite(min_delay)) wmax = (int64_t)std::min(floor(min_delay / L.dt - 0.5), (double)kMaxWindow);
   if (o.window_steps > 0) wmax = std::min<int64_t>(wmax, o.window_steps);
-  if (wmax < 1)
-    return fail(MIND_EUNSUPPORTED,
-                "NetCon delays shorter than 1.5 dt are not supported on the device path");
+  // delays shorter than 1.5 dt (so that an event can fall due in the step that
+  // produced it) or connections into artificial cells (NetStim inputs are
+  // feedback receivers) run on the event-exact path, one step per window
+  bool into_artificial = false;
+  for (int32_t t : L.edge_target) {
+    if (L.point_group[t] < 0)  // PatternStim has no NET_RECEIVE
+      return fail(MIND_EARG, "a connection targets a PatternStim, which receives no events");
+    if (L.point_cell[t] < 0) into_artificial = true;
+  }
+  S.serial = wmax < 1 || into_artificial;
+  if (S.serial) wmax = kSerialWindow;
   S.wmax = (int32_t)wmax;
   if ((rc = S.alloc(&d.det_flags, (size_t)d.ndet * wmax))) return rc;
   // edges
@@ -678,10 +696,6 @@ int setup_state(State& S, const mind_build_options& o) {
     if ((rc = S.upload(&pd, L.edge_weight))) return rc; d.edge_weight = pd;
     if ((rc = S.upload(&p64, L.source_sid))) return rc; d.source_sid = p64;
   }
-  // connections must target synapses of cells (NetStim inputs are not on the device path)
-  for (int32_t t : L.edge_target)
-    if (L.point_cell[t] < 0)
-      return fail(MIND_EUNSUPPORTED, "connections into artificial cells are not supported on the device path");
   // NetStim rows
   d.ns_rows = d.ns_group >= 0 ? L.groups[d.ns_group].rows : 0;
   if ((rc = S.alloc(&d.ns_stream, d.ns_rows))) return rc;
@@ -732,6 +746,16 @@ int setup_state(State& S, const mind_build_options& o) {
   if ((rc = S.alloc(&d.inj_count, 1))) return rc;
   HIP_TRY(hipMemset(d.inj_count, 0, sizeof(int32_t)));
   if ((rc = S.alloc(&S.inj_ev, kInjCap))) return rc;
+  if (S.serial) {
+    double* pd;
+    int32_t* p32;
+    if ((rc = S.upload(&pd, L.pattern_time))) return rc;
+    S.pattern_time = pd;
+    if ((rc = S.upload(&p32, L.pattern_source))) return rc;
+    S.pattern_source = p32;
+    const int64_t cap = o.event_capacity > 0 ? o.event_capacity : std::max<int64_t>(int64_t(1) << 20, 8 * E);
+    if ((rc = serial_setup(S, cap))) return rc;
+  }
   return 0;
 }
 
@@ -848,19 +872,23 @@ int stage_injections(State& S, int32_t n) {
   return 0;
 }
 
+// The cell kernel of n steps (plain or batched).
+int cells_launch(State& S, int32_t n) {
+  if (S.method == MIND_METHOD_BATCHED) return batched_launch(S, n);
+  int rc = 0;
+  if (S.profiling && (rc = timer_start(S))) return rc;
+  hipLaunchKernelGGL(k_plain, dim3(blocks_for(S.d.ncells, 64)), dim3(64), 0, S.stream, S.d, n);
+  if (S.profiling && (rc = timer_stop(S))) return rc;
+  if (S.profiling) S.kernel_steps += n;
+  return 0;
+}
+
 int launch_window(State& S, int32_t n) {
+  if (S.serial) return serial_launch(S, n);
   Dev& d = S.d;
   hipLaunchKernelGGL(k_gather, dim3(1), dim3(kBlock), 0, S.stream, d, n);
-  if (S.method == MIND_METHOD_BATCHED) {
-    int rc = batched_launch(S, n);
-    if (rc) return rc;
-  } else {
-    int rc = 0;
-    if (S.profiling && (rc = timer_start(S))) return rc;
-    hipLaunchKernelGGL(k_plain, dim3(blocks_for(d.ncells, 64)), dim3(64), 0, S.stream, d, n);
-    if (S.profiling && (rc = timer_stop(S))) return rc;
-    if (S.profiling) S.kernel_steps += n;
-  }
+  int rc = cells_launch(S, n);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_exchange, dim3(1), dim3(kBlock), 0, S.stream, d, n);
   HIP_TRY(hipGetLastError());
   return 0;
@@ -910,8 +938,9 @@ int mind_initialize(mind_state* state) {
   // PatternStim spikes: source events enqueued before any INITIAL block runs
   // (reset_events!), so they take the first insertion orders
   const uint64_t npattern = S.L.pattern_time.size();
-  for (uint64_t k = 0; k < npattern; ++k)
-    S.pending_spikes.push_back(mind::InjSpike{S.L.pattern_time[k], k + 1, S.L.pattern_source[k], 0});
+  if (!S.serial)  // (the event-exact path enqueues them itself)
+    for (uint64_t k = 0; k < npattern; ++k)
+      S.pending_spikes.push_back(mind::InjSpike{S.L.pattern_time[k], k + 1, S.L.pattern_source[k], 0});
   HIP_TRY(hipMemsetAsync(d.inj_count, 0, sizeof(int32_t), S.stream));
   HIP_TRY(hipMemsetAsync(d.error, 0, sizeof(int32_t), S.stream));
   mind::Clock clock{0, 0, npattern, 0, 0, 0};
@@ -927,7 +956,7 @@ int mind_initialize(mind_state* state) {
     hipLaunchKernelGGL(mind::k_init_group, dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0,
                        S.stream, d, gi);
   }
-  if (d.ns_rows > 0) {
+  if (d.ns_rows > 0 && !S.serial) {
     hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
                        S.ns_logical, mind::kNetStimDigest, npattern);
     // the netstim sequence counter continues after the INITIAL insertions
@@ -938,8 +967,12 @@ int mind_initialize(mind_state* state) {
   d.record_spikes = 1;
   HIP_TRY(hipMemsetAsync(d.spike_count, 0, sizeof(unsigned long long), S.stream));
   // time-zero delivery (NetStim events and pattern spikes at t <= 0), then currents at t = 0
-  if (int rcs = mind::stage_injections(S, 0)) return rcs;
-  hipLaunchKernelGGL(mind::k_exchange, dim3(1), dim3(mind::kBlock), 0, S.stream, d, 0);
+  if (S.serial) {
+    if (int rcs = mind::serial_initialize(S)) return rcs;
+  } else {
+    if (int rcs = mind::stage_injections(S, 0)) return rcs;
+    hipLaunchKernelGGL(mind::k_exchange, dim3(1), dim3(mind::kBlock), 0, S.stream, d, 0);
+  }
   for (int gi = 0; gi < d.ngroups; ++gi) {
     if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
     hipLaunchKernelGGL(mind::k_currents_fields, dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0,
@@ -967,9 +1000,8 @@ static int resolve_records(State& S, const mind_record* recs, int64_t n, std::ve
       cell.push_back(r.cell);
     } else if (r.kind == 1) {
       if (r.point < 0 || r.point >= (int32_t)L.point_group.size()) return fail(MIND_EARG, "unknown point process");
-      if (L.point_cell[r.point] < 0)
-        return fail(MIND_EUNSUPPORTED, "records of artificial-cell fields are not supported on the device path");
       const int g = L.point_group[r.point];
+      if (g < 0) return fail(MIND_EARG, "a PatternStim has no fields to record");
       if (r.field < 0 || r.field >= mind::field_count(L.groups[g].mech)) return fail(MIND_EARG, "unknown field");
       kind.push_back(1); group.push_back(g); field.push_back(r.field);
       index.push_back(L.point_row[r.point]);
@@ -1029,13 +1061,19 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
     return 0;
   };
   const int64_t samples = count + 1;
+  // records of cells, by cell (sampled by the cell kernels); records of
+  // artificial cells (cell -1), sampled by the event kernels
   std::vector<int64_t> cell_rec_off(S.L.ncells + 1, 0);
-  std::vector<int32_t> cell_rec(n_records);
-  for (int64_t q = 0; q < n_records; ++q) cell_rec_off[cell[q] + 1] += 1;
+  std::vector<int32_t> cell_rec, art_rec;
+  for (int64_t q = 0; q < n_records; ++q)
+    if (cell[q] >= 0) cell_rec_off[cell[q] + 1] += 1;
+    else art_rec.push_back((int32_t)q);
   for (int c = 0; c < S.L.ncells; ++c) cell_rec_off[c + 1] += cell_rec_off[c];
+  cell_rec.resize(cell_rec_off[S.L.ncells]);
   {
     std::vector<int64_t> cur(cell_rec_off.begin(), cell_rec_off.end() - 1);
-    for (int64_t q = 0; q < n_records; ++q) cell_rec[cur[cell[q]]++] = (int32_t)q;
+    for (int64_t q = 0; q < n_records; ++q)
+      if (cell[q] >= 0) cell_rec[cur[cell[q]]++] = (int32_t)q;
   }
   {
     int32_t* p32; int64_t* p64;
@@ -1045,6 +1083,8 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
     if ((rc = run_upload(&p64, index))) return rc; d.rec_index = p64;
     if ((rc = run_upload(&p64, cell_rec_off))) return rc; d.cell_rec_off = p64;
     if ((rc = run_upload(&p32, cell_rec))) return rc; d.cell_rec = p32;
+    if ((rc = run_upload(&p32, art_rec))) return rc; d.art_rec = p32;
+    d.n_art_rec = (int64_t)art_rec.size();
     void* p = nullptr;
     if (hipMalloc(&p, std::max<int64_t>(1, n_records * samples) * sizeof(double)) != hipSuccess)
       return fail(MIND_EOOM, "device allocation failed");
@@ -1123,6 +1163,7 @@ int mind_step(mind_state* state) {
   HIP_TRY(hipSetDevice(S.device));
   if (!S.initialized) return fail(MIND_EARG, "simulation is not initialized");
   S.d.nrec = 0;
+  S.d.n_art_rec = 0;
   S.d.record_spikes = 0;
   if (S.graph) {
     hipGraphExecDestroy(S.graph);
@@ -1241,6 +1282,8 @@ int mind_inject_event(mind_state* state, int64_t connection, double time) {
   if (!(std::isfinite(time) && time >= now)) return fail(MIND_EARG, "event arrival precedes the current time");
   const int64_t edge = L.connection_edge[connection];
   HIP_TRY(hipSetDevice(S.device));
+  if (S.serial)  // enqueued now, with the next insertion order
+    return mind::serial_inject(S, mind::SEvent{time, 0.0, edge + 1, 0, L.edge_target[edge] + 1, 0});
   HIP_TRY(hipStreamSynchronize(S.stream));
   mind::Clock c;
   HIP_TRY(hipMemcpy(&c, S.d.clock, sizeof(c), hipMemcpyDeviceToHost));
@@ -1264,6 +1307,7 @@ int mind_inject_spike(mind_state* state, int64_t source, double time) {
   const double now = (double)S.step * L.dt;
   if (!(std::isfinite(time) && time >= now)) return fail(MIND_EARG, "spike time precedes the current time");
   HIP_TRY(hipSetDevice(S.device));
+  if (S.serial) return mind::serial_inject(S, mind::SEvent{time, 0.0, 0, 0, -(int32_t)(source + 1), 0});
   HIP_TRY(hipStreamSynchronize(S.stream));
   mind::Clock c;
   HIP_TRY(hipMemcpy(&c, S.d.clock, sizeof(c), hipMemcpyDeviceToHost));
@@ -1332,6 +1376,10 @@ std::vector<std::pair<void*, size_t>> ckpt_arrays(State& S) {
   a.push_back({d.ns_edge, sizeof(int32_t) * d.ns_rows});
   a.push_back({d.ns_valid, sizeof(int32_t) * d.ns_rows});
   a.push_back({d.clock, sizeof(mind::Clock)});
+  if (S.serial) {  // the event-exact path's queue counters (its heap follows the arrays)
+    a.push_back({S.q.sequences, sizeof(int64_t) * S.L.point_group.size()});
+    a.push_back({S.q.heap_n, sizeof(int64_t)});
+  }
   return a;
 }
 
@@ -1359,6 +1407,13 @@ int mind_checkpoint(mind_state* state, void* buffer, size_t capacity, size_t* si
     const size_t at = blob.size();
     blob.resize(at + n);
     if (n) HIP_TRY(hipMemcpy(blob.data() + at, p, n, hipMemcpyDeviceToHost));
+  }
+  if (S.serial) {
+    int64_t hn = 0;
+    HIP_TRY(hipMemcpy(&hn, S.q.heap_n, sizeof hn, hipMemcpyDeviceToHost));
+    std::vector<mind::SEvent> heap(hn);
+    if (hn) HIP_TRY(hipMemcpy(heap.data(), S.q.heap, sizeof(mind::SEvent) * hn, hipMemcpyDeviceToHost));
+    put(blob, heap.data(), hn);
   }
   std::vector<unsigned long long> counts(d.ring_slots);
   HIP_TRY(hipMemcpy(counts.data(), d.ring_count, sizeof(unsigned long long) * d.ring_slots, hipMemcpyDeviceToHost));
@@ -1417,6 +1472,14 @@ int mind_restore(mind_state* state, const void* buffer, size_t size) {
     if (at + n > size) return fail(MIND_EARG, "truncated checkpoint");
     if (n) HIP_TRY(hipMemcpy(p, b + at, n, hipMemcpyHostToDevice));
     at += n;
+  }
+  if (S.serial) {  // heap_n came with the arrays
+    int64_t hn = 0;
+    HIP_TRY(hipMemcpy(&hn, S.q.heap_n, sizeof hn, hipMemcpyDeviceToHost));
+    if (hn < 0 || hn > S.q.heap_cap) return fail(MIND_EARG, "corrupt checkpoint");
+    std::vector<mind::SEvent> heap(hn);
+    if (!take(heap.data(), sizeof(mind::SEvent) * hn)) return fail(MIND_EARG, "truncated checkpoint");
+    if (hn) HIP_TRY(hipMemcpy(S.q.heap, heap.data(), sizeof(mind::SEvent) * hn, hipMemcpyHostToDevice));
   }
   std::vector<unsigned long long> counts(d.ring_slots, 0);
   std::vector<mind::QEvent> ev;

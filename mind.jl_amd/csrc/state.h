@@ -45,6 +45,11 @@ struct State {
   // batched kernel plan (cell groups, chain units and level schedules)
   BatchPlan* batch = nullptr;
   MacroHost* macro = nullptr;
+  // event-exact path (delays under 1.5 dt or connections into NetStims)
+  bool serial = false;
+  SerialDev q{};
+  double* pattern_time = nullptr;
+  int32_t* pattern_source = nullptr;
   std::vector<int32_t> cell_bucket;  // event bucket of each cell: the cell, or its group
   // injected input not yet on the device: connection events beyond the delivery
   // ring's horizon, source spikes due in a later window (inject_event!, inject_spike!)
@@ -106,5 +111,10 @@ int timer_start(State& S);
 int timer_stop(State& S);
 int batched_setup(State& S, int32_t cells_per_group);
 int batched_launch(State& S, int32_t n);
+int cells_launch(State& S, int32_t n);
+int serial_setup(State& S, int64_t capacity);
+int serial_initialize(State& S);
+int serial_launch(State& S, int32_t n);
+int serial_inject(State& S, const SEvent& e);
 
 }  // namespace mind

@@ -182,11 +182,10 @@ def test_pattern_stim_matches_oracle(gpu, method):
 @pytest.mark.parametrize("method", ["plain", "batched"])
 def test_network_of_artificial_cells_only(gpu, method):
     """A model without compartments: a PatternStim and two NetStims (one regular, one
-    Poisson). Rasters bit-identical to the oracle in both kernels. A connection into a
-    NetStim and a record of a NetStim field are refused by the device path with a clear
-    error."""
+    Poisson). Rasters and NetStim field records bit-identical to the oracle in both
+    kernels, on the window path and, once a connection goes into a NetStim, on the
+    event-exact path."""
     from mind_amd import MicroModel
-    from mind_amd._abi import MindError
     m = MicroModel()
     ps = m.add_pointprocess(m, "PatternStim", spike_times=[1.0, 2.5, 30.0])
     regular = m.add_pointprocess(m, "NetStim", params={"interval": 5.0, "number": 3, "start": 0.0, "noise": 0.0})
@@ -203,9 +202,25 @@ def test_network_of_artificial_cells_only(gpu, method):
     assert np.array_equal(got.spike_sources, ref.spike_sources)
     assert got.spike_sources.tolist().count(10) == 3 and got.spike_sources.tolist().count(11) == 3
     assert got.spike_sources.tolist().count(12) == 4
-    with pytest.raises(MindError, match="artificial"):
-        run(m, 110.0, [mechanism_record(m, poisson, "event")])
+    recs = [mechanism_record(m, poisson, "event", name="event"), mechanism_record(m, regular, "on", name="on"),
+            mechanism_record(m, poisson, "ispike", name="ispike")]
+    initialize(m)
+    got = run(m, 110.0, recs)
+    ref = oracle.OracleSim(m, seed=3).initialize().run(110.0, recs)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    assert got.traces["ispike"][-1] == 4.0
     destroy(m)
+    # an input into a NetStim (a feedback receiver) moves the model to the event-exact path
     m.connect(m.register_spike_source(ps), regular, weight=1.0, delay=0.5)
-    with pytest.raises(MindError, match="artificial"):
-        build(m, dt_micro=0.025, seed=3, method=method)
+    build(m, dt_micro=0.025, seed=3, method=method)
+    initialize(m)
+    got = run(m, 110.0, recs)
+    ref = oracle.OracleSim(m, seed=3).initialize().run(110.0, recs)
+    assert np.array_equal(got.spike_times, ref.spike_times)
+    assert np.array_equal(got.spike_sources, ref.spike_sources)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    with pytest.raises(ValueError, match="NET_RECEIVE"):  # a PatternStim receives no events
+        m.connect(m.register_spike_source(regular), ps, weight=1.0, delay=1.0)
+    destroy(m)

@@ -679,3 +679,53 @@ def test_pop_rate_bins_match_hand_count():
             if b + 1 < int(stop):
                 assert np.all(res.rate[r, sample:sample + 10] == expect)
     assert res.rate.max() > 0
+
+
+# --- short and zero NetCon delays, NetStim inputs (events.jl:57-165) ---------------
+def test_delivery_call_of_short_delays():
+    """A spike detected at the start of step k (time k dt + 1e-10) reaches ExpSyns over
+    delays 0, 0.6 dt, 1.4 dt and 1.6 dt. The reference delivers through t + dt/2
+    before the currents and through t + dt after recording, so g first shows the
+    event (decayed once by nrn_state) at sample k+1 for delay 0 (midpoint of step k),
+    k+2 for 0.6 dt (end of step k) and 1.4 dt (midpoint of step k+1), and k+3 for
+    1.6 dt (end of step k+1)."""
+    dt, w, tau = 0.025, 1e-3, 2.0
+    model, info = configs.single_cell()
+    src = model.register_spike_source(info["cells"][0], section="soma", x=0.5, threshold=-20.0)
+    syns = []
+    for factor in (0.0, 0.6, 1.4, 1.6):
+        (cell,) = model.add_population([Section("soma", label="soma", length=20.0, diameter=20.0)], 1)
+        model.insert(cell, "pas", params={"g": 1e-4, "e": -65.0})
+        syn = model.add_pointprocess(cell, "ExpSyn", params={"tau": tau, "e": 0.0})
+        model.connect(src, syn, weight=w, delay=factor * dt)
+        syns.append(syn)
+    recs = [mechanism_record(model, s, "g", name=f"g{j}") for j, s in enumerate(syns)]
+    r = oracle.OracleSim(model).initialize().run(30.0, recs)
+    first = [t for t, sid in zip(r.spike_times, r.spike_sources) if sid == src.sid][0]
+    k = int(round((first - 1e-10) / dt))
+    assert first == k * dt + 1e-10
+    for j, lag in enumerate((1, 2, 2, 3)):
+        g = r.traces[f"g{j}"]
+        assert np.all(g[:k + lag] == 0.0), j
+        assert g[k + lag] == pytest.approx(w * math.exp(-dt / tau), rel=1e-12), j
+
+
+def test_netstim_input_starts_and_stops_a_train():
+    """NetStim NET_RECEIVE with flag 0 (NetStim.mod:77-105): an event of positive weight
+    at 5 ms switches on a NetStim that INITIAL left off (start < 0); with noise 0 its
+    first self-event is due at once (event - interval = 0), so it fires at 5, 7, ...
+    every interval; an event of negative weight at 8 ms switches it off and the 9 ms
+    self-event then does nothing. Zero delays keep every event inside its step."""
+    m = MicroModel()
+    ns = m.add_pointprocess(m, "NetStim", params={"interval": 2.0, "number": 10, "start": -1.0, "noise": 0.0})
+    out = m.register_spike_source(ns, sid=3)
+    on, off = m.register_spike_source(m, sid=1), m.register_spike_source(m, sid=2)
+    m.connect(on, ns, weight=1.0, delay=0.0)
+    m.connect(off, ns, weight=-1.0, delay=0.0)
+    sim = oracle.OracleSim(m).initialize()
+    sim.inject_spike(on, 5.0)
+    sim.inject_spike(off, 8.0)
+    r = sim.run(20.0)
+    ns_times = [t for t, s in zip(r.spike_times, r.spike_sources) if s == out.sid]
+    assert ns_times == [5.0, 7.0]
+    assert sorted(r.spike_sources[r.spike_sources != out.sid]) == [1, 2]
