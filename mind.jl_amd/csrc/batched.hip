@@ -26,7 +26,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -537,7 +541,7 @@ __device__ __forceinline__ void solve_lean(const Dev& d, const BatchDev& b, int 
 template <int MECH, int U, typename C>
 __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, const int32_t* node_of, int64_t r0,
                                                  int64_t r1, int64_t n0, double tc, const double* v, C rhs, C diag,
-                                                 int tid, int T) {
+                                                 int tid, int T, bool store) {
   using Ops = CurOps<MECH>;
   for (int64_t r = r0 + tid; r < r1; r += (int64_t)U * T) {
     int64_t rr[U];
@@ -560,7 +564,7 @@ __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, 
       if (r + (int64_t)u * T < r1) {
         const int li = nd[u] - (int)n0;
         double value, shifted;
-        Ops::eval(col, rr[u], in[u], v[li], tc, d.dt, value, shifted, true);
+        Ops::eval(col, rr[u], in[u], v[li], tc, d.dt, value, shifted, store);
         const double scale = area[u] * 0.01;
         double acc_r = rhs[li], acc_d = diag[li];
         acc_r -= scale * value;
@@ -570,6 +574,13 @@ __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, 
       }
     }
   }
+}
+
+// Mechanisms whose current-phase stores are outputs only: no other phase and
+// no NET_RECEIVE reads them.
+__device__ __forceinline__ bool mech_outputs_only(int32_t mech) {
+  return mech == MIND_MECH_HH || mech == MIND_MECH_PAS || mech == MIND_MECH_EXPSYN || mech == MIND_MECH_EXP2SYN ||
+         mech == MIND_MECH_ICLAMP || mech == MIND_MECH_ALPHASYNAPSE;
 }
 
 // Mechanisms with a state update (nrn_state).
@@ -683,8 +694,13 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       diag[i] = 0.0;
     }
     __syncthreads();
-    // nrn_cur at t + dt/2, one mechanism group after another
+    // nrn_cur at t + dt/2, one mechanism group after another. The assigned
+    // outputs of a current phase that no other phase reads (i, g of hh, pas,
+    // ExpSyn, Exp2Syn, IClamp, AlphaSynapse) are stored on the window's last
+    // step, which leaves memory as it would be after every step, or on every
+    // step when the run records values.
     const double tc = t + 0.5 * dt;
+    const bool store_outputs = s == n - 1 || d.nrec > 0;
     for (int gi = 0; gi < ngr; ++gi) {
       const GroupDev& G = sg[gi];
       if (G.mech == MIND_MECH_NETSTIM) continue;
@@ -692,12 +708,12 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       if (r0 == r1) continue;
       const Cols col = cols(G);
       if (G.mech == MIND_MECH_PAS && !G.point) {
-        density_currents<MIND_MECH_PAS, 4>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T);
+        density_currents<MIND_MECH_PAS, 4>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
         __syncthreads();
         continue;
       }
       if (G.mech == MIND_MECH_HH && !G.point) {
-        density_currents<MIND_MECH_HH, 1>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T);
+        density_currents<MIND_MECH_HH, 1>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
         __syncthreads();
         continue;
       }
@@ -709,7 +725,8 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         int64_t q = r;
         do {  // a run of rows on one node, in row order
           double value, shifted;
-          phase_current(G.mech, col, q, v[li], tc, dt, value, shifted, true);
+          phase_current(G.mech, col, q, v[li], tc, dt, value, shifted,
+                        store_outputs || !mech_outputs_only(G.mech));
           const double scale = G.point ? 1.0 : d.area[node] * 0.01;
           acc_r -= scale * value;
           acc_d += scale * (shifted - value) / 0.001;
@@ -1750,6 +1767,76 @@ int install_plan(State& S, int64_t budget, int32_t threads, int32_t layout) {
 
 // Chooses the node budget and block size by timing the window kernel on this
 // model (initialize! must follow, which restores every state the trials touched).
+namespace {
+
+// Autotuned plans by model topology: this process's choices, and optionally a
+// file named by MIND_PLAN_CACHE (lines "key budget threads layout").
+struct PlanChoice { int64_t budget; int threads; int layout; };
+std::mutex g_plan_mu;
+std::map<uint64_t, PlanChoice> g_plans;
+
+uint64_t mix(uint64_t h, const void* p, size_t n) {
+  const uint8_t* b = static_cast<const uint8_t*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  return h;
+}
+
+// What the plan's cost depends on: the cells' trees, the mechanism rows per
+// cell, the detectors and the window length.
+uint64_t topology_key(const State& S) {
+  const Layout& L = S.L;
+  uint64_t h = 0xcbf29ce484222325ull;
+  const int32_t version = 2;  // of the candidate list
+  h = mix(h, &version, sizeof version);
+  h = mix(h, L.cell_off.data(), L.cell_off.size() * sizeof(int64_t));
+  h = mix(h, L.parent.data(), L.parent.size() * sizeof(int32_t));
+  for (const MechGroup& G : L.groups) {
+    h = mix(h, &G.mech, sizeof G.mech);
+    h = mix(h, G.cell_rows.data(), G.cell_rows.size() * sizeof(int64_t));
+  }
+  h = mix(h, L.cell_det_off.data(), L.cell_det_off.size() * sizeof(int64_t));
+  h = mix(h, &S.wmax, sizeof S.wmax);
+  return h;
+}
+
+bool cached_plan(uint64_t key, PlanChoice* out) {
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) {
+    *out = it->second;
+    return true;
+  }
+  const char* path = std::getenv("MIND_PLAN_CACHE");
+  if (!path || !*path) return false;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  unsigned long long k = 0;
+  long long b = 0;
+  int t = 0, l = 0;
+  bool found = false;
+  while (std::fscanf(f, "%llx %lld %d %d", &k, &b, &t, &l) == 4)
+    if (k == key) {
+      *out = PlanChoice{b, t, l};
+      found = true;
+    }
+  std::fclose(f);
+  if (found) g_plans[key] = *out;
+  return found;
+}
+
+void remember_plan(uint64_t key, const PlanChoice& c) {
+  std::lock_guard<std::mutex> lock(g_plan_mu);
+  g_plans[key] = c;
+  const char* path = std::getenv("MIND_PLAN_CACHE");
+  if (!path || !*path) return;
+  if (FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "%llx %lld %d %d\n", (unsigned long long)key, (long long)c.budget, c.threads, c.layout);
+    std::fclose(f);
+  }
+}
+
+}  // namespace
+
 int batched_setup(State& S, int32_t cells_per_group) {
   HIP_TRY(hipStreamCreateWithFlags(&S.aux, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
@@ -1766,14 +1853,20 @@ int batched_setup(State& S, int32_t cells_per_group) {
   for (int32_t c = 0; c < S.L.ncells; ++c) max_cell = std::max<int64_t>(max_cell, S.L.cell_off[c + 1] - S.L.cell_off[c]);
   struct Cand { int64_t budget; int threads; int layout; };
   std::vector<Cand> cands;
+  const uint64_t key = topology_key(S);
+  PlanChoice known{};
+  const bool log = std::getenv("MIND_AUTOTUNE_LOG") != nullptr;
   if (cells_per_group > 0) {
     cands.push_back({std::max<int64_t>(1, (int64_t)cells_per_group * max_cell), 256, 0});
+  } else if (cached_plan(key, &known)) {
+    cands.push_back({known.budget, known.threads, known.layout});
   } else {
     for (int threads : {256, 512}) {
       for (int64_t bud : {1024, 1700, 2750}) cands.push_back({bud, threads, 0});
       for (int64_t bud : {1800, 2400, 3000, 3600, 4800, 6000}) cands.push_back({bud, threads, 5});
       for (int64_t bud : {3000, 3600, 4500, 5400, 7200}) cands.push_back({bud, threads, 4});
     }
+    for (int64_t bud : {4000, 5000}) cands.push_back({bud, 512, 4});
     for (int64_t bud : {4800, 6000}) cands.push_back({bud, 1024, 5});
     for (int64_t bud : {7200, 9000}) cands.push_back({bud, 1024, 4});
   }
@@ -1787,16 +1880,29 @@ int batched_setup(State& S, int32_t cells_per_group) {
     BatchPlan* P = S.batch;
     if (!tune) return 0;
     if ((rc2 = mind_initialize(reinterpret_cast<mind_state*>(&S)))) return rc2;
+    (void)P;
+    // whole windows as a run replays them (gather, cells, exchange captured in
+    // one graph): two to warm up, then four timed
     const int32_t n = S.wmax;
-    for (int rep = 0; rep < 2; ++rep) {  // first repetition warms up
-      HIP_TRY(hipStreamSynchronize(S.stream));
+    hipGraph_t g;
+    hipGraphExec_t exec;
+    HIP_TRY(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+    rc2 = launch_window(S, n);
+    const hipError_t e = hipStreamEndCapture(S.stream, &g);
+    if (rc2) return rc2;
+    if (e != hipSuccess) return fail(MIND_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    for (int rep = 0; rep < 2 && !rc2; ++rep) {
+      if (hipStreamSynchronize(S.stream) != hipSuccess) rc2 = fail(MIND_EHIP, "autotune synchronisation");
       const auto t0 = std::chrono::steady_clock::now();
-      for (int w = 0; w < 3; ++w)
-        if ((rc2 = launch_cells(S, *P, n))) return rc2;
-      HIP_TRY(hipStreamSynchronize(S.stream));
+      for (int w = 0; w < 2 + 2 * rep && !rc2; ++w)
+        if (hipGraphLaunch(exec, S.stream) != hipSuccess) rc2 = fail(MIND_EHIP, "autotune graph launch");
+      if (!rc2 && hipStreamSynchronize(S.stream) != hipSuccess) rc2 = fail(MIND_EHIP, "autotune synchronisation");
       secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    return 0;
+    (void)hipGraphExecDestroy(exec);
+    return rc2;
   };
   // every candidate once, then the three fastest twice more: the pick has the
   // lowest mean of its three timings
@@ -1805,10 +1911,13 @@ int batched_setup(State& S, int32_t cells_per_group) {
     double secs = 0;
     if ((rc = trial(cands[k], secs))) return rc;
     timed.push_back({secs, k});
+    if (log && tune)
+      std::fprintf(stderr, "autotune: budget %lld threads %d layout %d: %.3f ms per window\n",
+                   (long long)cands[k].budget, cands[k].threads, cands[k].layout, 1e3 * secs / 4);
   }
   if (tune) {
     std::sort(timed.begin(), timed.end());
-    for (size_t q = 0; q < std::min<size_t>(3, timed.size()); ++q) {
+    for (size_t q = 0; q < std::min<size_t>(4, timed.size()); ++q) {
       double total = timed[q].first;
       for (int rep = 0; rep < 2; ++rep) {
         double secs = 0;
@@ -1820,10 +1929,12 @@ int batched_setup(State& S, int32_t cells_per_group) {
         pick = cands[timed[q].second];
       }
     }
-  }
-  if (tune) {
     double secs = 0;
     if ((rc = trial(pick, secs))) return rc;
+    if (cells_per_group <= 0) remember_plan(key, PlanChoice{pick.budget, pick.threads, pick.layout});
+    if (log)
+      std::fprintf(stderr, "autotune: picked budget %lld threads %d layout %d\n", (long long)pick.budget,
+                   pick.threads, pick.layout);
   }
   HIP_TRY(hipMemset(S.d.error, 0, sizeof(int32_t)));
   S.initialized = false;
