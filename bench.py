@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--macro-ms", type=float, default=2000.0)
     ap.add_argument("--no-macro", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--eval-seeds", type=int, default=16,
+                    help="closed-loop surrogate evaluation over seeds 1..N of the micro-macro workload")
     ap.add_argument("--plan", default="", metavar="BUDGET,THREADS,LAYOUT",
                     help="batched plan instead of the autotuner's (e.g. to profile a given plan)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -115,7 +117,7 @@ def main():
                       what=f"the same {args.cells}-cell model")
 
     config1 = None if args.no_config1 else config1_leg(dt, args.no_plain, args.no_cpu)
-    macro = None if args.no_macro else macro_leg(args.macro_ms)
+    macro = None if args.no_macro else macro_leg(args.macro_ms, args.eval_seeds)
 
     line = {
         "metric": "compartment-steps/s (batched vs plain); macro sim-s/wall-s surrogate vs plain",
@@ -313,40 +315,53 @@ def measured_traffic(kernel, plan, window, steps_per_launch, algorithmic_per_lau
     return ratio * algorithmic_per_launch, f"profiles/{f}, plan {rec['plan']}, scaled by measured/algorithmic bytes"
 
 
-def macro_leg(stop):
-    """16 Wilson-Cowan ROIs x 256 ball-and-stick cells (evaluation seed 1): the
-    detailed micro-macro run (batched micro kernel) against the surrogate run of
-    the same ROI graph, in simulated s per wall s of the run loop."""
+def macro_leg(stop, eval_seeds):
+    """16 Wilson-Cowan ROIs x 256 ball-and-stick cells. The detailed micro-macro run
+    (batched micro kernel, one population-rate bin per window) against the
+    surrogate run of the same ROI graph for evaluation seed 1, in simulated s per
+    wall s of the run loop; then, closed loop on evaluation seeds 1..eval_seeds,
+    the per-ROI time-averaged rate error of the surrogate and of its linear map
+    alone (the method without the network)."""
     import numpy as np
 
     from mind_amd import configs, destroy
     from mind_amd import macro as M
     from mind_amd import surrogate as S
 
-    mm, _ = configs.micro_macro(seed=1)
-    t0 = time.time()
-    M.build(mm, seed=1, method="batched")
-    M.initialize(mm)
-    log(f"micro-macro: build+autotune+initialize {time.time() - t0:.1f}s")
-    plain = M.run(mm, stop, record_spikes=False)
+    def detailed(seed):
+        mm, _ = configs.micro_macro(seed=seed)
+        t0 = time.time()
+        M.build(mm, seed=1, method="batched")
+        M.initialize(mm)
+        setup = time.time() - t0
+        res = M.run(mm, stop, record_spikes=False)
+        destroy(mm)
+        mm.micro.state = None
+        return mm, res, setup
+
+    def errors(plain, sur):
+        rp = plain.rate[:, 10::10].mean(axis=1)
+        rs = sur.rate[:, 10::10].mean(axis=1)
+        verr = np.abs(sur.mean_v[:, 10::10].mean(axis=1) - plain.mean_v[:, 10::10].mean(axis=1))
+        return np.abs(rs - rp) / rp, verr
+
+    mm, plain, setup = detailed(1)
+    log(f"micro-macro: build+autotune+initialize {setup:.1f}s")
     log(f"micro-macro: {stop:.0f} ms in {plain.run_seconds:.2f}s")
-    destroy(mm)
-    mm.micro.state = None
     out = {"workload": "16 Wilson-Cowan ROIs (RK4, dt 0.1 ms, delayed all-to-all coupling) x 256 "
                        "200-compartment ball-and-stick cells, Poisson drive, 1 ms binned rate; seed 1",
            "stop_ms": stop, "plain_sim_s_per_wall_s": stop / 1e3 / plain.run_seconds,
-           "plain_run_s": plain.run_seconds}
+           "plain_run_s": plain.run_seconds,
+           "plain_windows": "one 1 ms rate bin (10 macro steps, 40 micro steps) per replayed window"}
     if not os.path.exists(S.WEIGHTS):
         out["surrogate"] = None
         return out
     net = S.Network.load()
+    lin = S.linear_only(net)
     cs = S.CoupledSurrogate(mm, net).initialize()
     sur = cs.run(stop)
     cs.close()
-    rp = plain.rate[:, 10::10].mean(axis=1)
-    rs = sur.result.rate[:, 10::10].mean(axis=1)
-    err = np.abs(rs - rp) / rp
-    verr = np.abs(sur.result.mean_v[:, 10::10].mean(axis=1) - plain.mean_v[:, 10::10].mean(axis=1))
+    err, verr = errors(plain, sur.result)
     out.update({
         "surrogate_sim_s_per_wall_s": stop / 1e3 / sur.result.run_seconds,
         "surrogate_run_s": sur.result.run_seconds,
@@ -356,6 +371,29 @@ def macro_leg(stop):
         "ood_windows": sur.ood_windows,
     })
     log(f"surrogate: {stop:.0f} ms in {sur.result.run_seconds:.4f}s; rate error max {err.max():.4f}")
+    net_err, lin_err, ood = [], [], 0
+    for seed in range(1, eval_seeds + 1):
+        if seed == 1:
+            mms, ps = mm, plain
+        else:
+            mms, ps, _ = detailed(seed)
+        for model_net, acc in ((net, net_err), (lin, lin_err)):
+            cs = S.CoupledSurrogate(mms, model_net).initialize()
+            r = cs.run(stop)
+            cs.close()
+            if model_net is net:
+                ood += r.ood_windows
+            acc.append(errors(ps, r.result)[0])
+        log(f"seed {seed}: rate error network max {net_err[-1].max():.4f}, linear map {lin_err[-1].max():.4f}")
+    ne, le = np.concatenate(net_err), np.concatenate(lin_err)
+    out["closed_loop_eval"] = {
+        "seeds": f"1-{eval_seeds}", "stop_ms": stop, "rois": int(ne.size),
+        "network_rate_rel_err": {"max": float(ne.max()), "mean": float(ne.mean())},
+        "linear_map_rate_rel_err": {"max": float(le.max()), "mean": float(le.mean())},
+        "network_beats_linear_map_mean": bool(ne.mean() < le.mean()),
+        "seeds_within_5pct": int(sum(float(e.max()) <= 0.05 for e in net_err)),
+        "ood_windows": int(ood),
+    }
     return out
 
 

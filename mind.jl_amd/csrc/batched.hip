@@ -816,6 +816,14 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         d.rec_buf[rec * d.rec_stride + sample] = x;
       }
     }
+    if (d.snap && (s + 1) % d.snap_every == 0) {
+      const int64_t m = (s + 1) / d.snap_every - 1;
+      for (int64_t q = det0 + tid; q < det1; q += T) {
+        const int32_t kk = d.cell_det[q];
+        const int32_t si = d.det_snap[kk];
+        if (si >= 0) d.snap[m * d.n_snap + si] = v[d.det_node[kk] - n0];
+      }
+    }
     __syncthreads();
     if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k + 1);  // end-of-step delivery
   }

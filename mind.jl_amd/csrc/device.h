@@ -160,6 +160,13 @@ struct Dev {
   // after the midpoint delivery of each step, before its end-of-step delivery
   const int32_t* art_rec;
   int64_t n_art_rec;
+  // soma-voltage snapshots of the MICRO2MACRO outputs at every macro-step end of
+  // a window (micro-macro windows longer than one macro step): snap[m * n_snap +
+  // det_snap[detector]] after window step (m + 1) * snap_every - 1
+  double* snap;
+  const int32_t* det_snap;
+  int32_t snap_every;
+  int64_t n_snap;
   // injected source spikes due in the next window, and their count (the window's
   // exchange consumes them and clears the count)
   InjSpike* inj;

@@ -30,6 +30,13 @@ struct MacroDev {
   int64_t* in_base;
   double* trace;             // [5][n][samples]
   int64_t samples, trace_start;
+  // windows of several macro steps (one population-rate bin): macro steps per
+  // full window, the E, I and coupling of each step of a window ([3][kwin][n]),
+  // and the outputs' soma-voltage snapshots ([kwin][n_out], filled by the cells)
+  int32_t kwin;
+  double* traj;
+  const double* snap;
+  int64_t n_out;
 };
 
 // Wilson-Cowan REGION: S(x) = 1/(1 + exp(-a (x - theta))) - 1/(1 + exp(a theta)).
@@ -56,6 +63,7 @@ __host__ __device__ inline void wc_rk4(const double* p, double& e, double& i, do
 
 struct MacroHost {
   MacroDev m{};
+  bool bins = false;  // windows of one bin (kwin macro steps) instead of one macro step
   std::vector<void*> allocations;
   std::vector<double> wc, E0, I0;
   hipGraphExec_t graph = nullptr;

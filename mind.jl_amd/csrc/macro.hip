@@ -221,6 +221,175 @@ __global__ void k_macro_record(MacroDev m, int64_t sample) {
   if (r < m.n) record_macro(m, r, sample);
 }
 
+// Window of K macro steps s .. s+K-1 inside one population-rate bin (the last
+// step at most ends the bin). The population rate stays fixed until the bin
+// ends and every coupling delay is at least K macro steps, so the ROI states of
+// the window's steps follow from what is known at its start: for each step in
+// order, the delayed coupling, the Poisson events of every MACRO2MICRO binding
+// (tickets in step, then binding order) and, but for the last step, the RK4
+// advance and its publication into the history ring. The same operations as
+// one macro step at a time, in the same order per ROI.
+__global__ void __launch_bounds__(kB) k_macro_pre_bin(Dev d, MacroDev m, int32_t K) {
+  __shared__ int64_t scan[kB / 64 + 1];
+  const int tid = threadIdx.x;
+  const int64_t s0 = d.clock->mstep;
+  const int64_t k0 = d.clock->step;
+  const uint64_t ticket0 = d.clock->ticket;
+  int64_t routed = 0;
+  double* tE = m.traj;
+  double* tI = m.traj + (int64_t)m.kwin * m.n;
+  double* tC = m.traj + 2 * (int64_t)m.kwin * m.n;
+  for (int mi = 0; mi < K; ++mi) {
+    const int64_t s = s0 + mi;
+    const double start = (double)s * m.dt_macro, stop = (double)(s + 1) * m.dt_macro;
+    const int64_t slot = s % m.cap;
+    for (int i = tid; i < m.n; i += kB) {
+      double c = 0.0;
+      for (int j = 0; j < m.n; ++j) {
+        const double w = m.W[(int64_t)i * m.n + j];
+        if (w == 0.0) continue;
+        const int64_t sl = ((s - m.dsteps[(int64_t)i * m.n + j]) % m.cap + m.cap) % m.cap;
+        c = c + w * m.hist[sl * m.n + j];
+      }
+      tC[(int64_t)mi * m.n + i] = c;
+    }
+    for (int64_t b = tid; b < m.nb; b += kB) {
+      const double rate = input_rate(m, b, slot);
+      int64_t cnt = 0;
+      if (rate > 0.0) {
+        Stream st = input_stream(d, m, b, s);
+        const double scale = 1000.0 / rate;
+        double tn = start + -mind_log(stream_uniform(st, 0)) * 1.0 * scale;
+        while (tn < stop) {
+          ++cnt;
+          tn = tn + -mind_log(stream_uniform(st, 0)) * 1.0 * scale;
+        }
+      }
+      m.in_count[b] = cnt;
+    }
+    __syncthreads();
+    int64_t running = 0;
+    for (int64_t base = 0; base < m.nb; base += kB) {
+      const int64_t b = base + tid;
+      int64_t total;
+      const int64_t pre = scan_block(b < m.nb ? m.in_count[b] : 0, scan, &total);
+      if (b < m.nb) m.in_base[b] = running + pre;
+      running += total;
+    }
+    for (int64_t b = tid; b < m.nb; b += kB) {
+      const double rate = input_rate(m, b, slot);
+      if (!(rate > 0.0)) continue;
+      Stream st = input_stream(d, m, b, s);
+      const double scale = 1000.0 / rate;
+      const int64_t src = m.first_virtual + b;
+      const int64_t e = d.fanout[src];
+      int64_t q = routed + m.in_base[b];
+      double tn = start + -mind_log(stream_uniform(st, 0)) * 1.0 * scale;
+      while (tn < stop) {
+        const double T = tn + d.edge_delay[e];
+        const int64_t j = call_of(T, 2 * k0, d.dt);
+        const int64_t bucket = (j / 2) % d.ring_slots;
+        const unsigned long long pos = atomicAdd(&d.ring_count[bucket], 1ull);
+        if ((int64_t)pos < d.ring_cap)
+          d.ring[bucket * d.ring_cap + (int64_t)pos] = QEvent{T, ticket0 + (uint64_t)q + 1, j, (int32_t)e, 0};
+        else
+          raise_error(d.error, MIND_EQUEUE);
+        if (d.record_spikes) {
+          const unsigned long long sp = atomicAdd(d.spike_count, 1ull);
+          if ((int64_t)sp < d.spike_cap) {
+            d.spike_t[sp] = tn;
+            d.spike_sid[sp] = d.source_sid[src];
+          } else {
+            raise_error(d.error, MIND_EQUEUE);
+          }
+        }
+        ++q;
+        tn = tn + -mind_log(stream_uniform(st, 0)) * 1.0 * scale;
+      }
+    }
+    routed += running;
+    __syncthreads();
+    if (mi + 1 < K) {  // the advance of a step that does not end the bin
+      for (int r = tid; r < m.n; r += kB) {
+        const double* p = m.wc + (int64_t)r * WC_NPARAMS;
+        const double u = tC[(int64_t)mi * m.n + r] + p[WC_K_MICRO] * (m.rate[r] / p[WC_RATE_SCALE]);
+        double e = m.E[r], i = m.I[r];
+        wc_rk4(p, e, i, u, m.dt_macro);
+        m.E[r] = e;
+        m.I[r] = i;
+        m.hist[((s + 1) % m.cap) * m.n + r] = e;
+        tE[(int64_t)mi * m.n + r] = e;
+        tI[(int64_t)mi * m.n + r] = i;
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) d.clock->ticket = ticket0 + (uint64_t)routed;
+}
+
+// After the micro window of K macro steps: per step, the outputs' spike counts
+// (emissions of the step's micro steps within its time span), the bin's rate at
+// its end, the mean soma voltage from the step's snapshot, the last step's RK4
+// advance and publication, and the macro records of every step.
+__global__ void __launch_bounds__(kB) k_macro_post_bin(Dev d, MacroDev m, int32_t K) {
+  extern __shared__ int lds_counts[];  // [K][n]
+  const int tid = threadIdx.x;
+  const int64_t s0 = d.clock->mstep;
+  const int64_t k0 = d.clock->step - (int64_t)K * m.ratio;
+  for (int q = tid; q < K * m.n; q += kB) lds_counts[q] = 0;
+  __syncthreads();
+  const int64_t ne = d.clock->emitted;
+  for (int64_t q = tid; q < ne; q += kB) {
+    const Emit em = d.emit_all[q];
+    const int r = m.src_roi[em.source];
+    if (r < 0) continue;
+    const int64_t mi = (em.key / 3 - k0) / m.ratio;
+    if (mi < 0 || mi >= K) continue;
+    const double start = (double)(s0 + mi) * m.dt_macro, stop = (double)(s0 + mi + 1) * m.dt_macro;
+    if (em.t >= start && em.t < stop) atomicAdd(&lds_counts[mi * m.n + r], 1);
+  }
+  __syncthreads();
+  const double* tE = m.traj;
+  const double* tI = m.traj + (int64_t)m.kwin * m.n;
+  const double* tC = m.traj + 2 * (int64_t)m.kwin * m.n;
+  for (int r = tid; r < m.n; r += kB) {
+    const double* p = m.wc + (int64_t)r * WC_NPARAMS;
+    for (int mi = 0; mi < K; ++mi) {
+      const int64_t s = s0 + mi;
+      m.acc[r] = m.acc[r] + (double)lds_counts[mi * m.n + r];
+      if ((s + 1) % m.bin_steps[r] == 0) {
+        const double nsrc = (double)(m.out_off[r + 1] - m.out_off[r]);
+        const double bin = (double)m.bin_steps[r] * m.dt_macro;
+        m.rate[r] = nsrc > 0.0 ? m.acc[r] / (nsrc * bin) * 1000.0 : 0.0;
+        m.acc[r] = 0.0;
+      }
+      double sum = 0.0;
+      for (int64_t q = m.out_off[r]; q < m.out_off[r + 1]; ++q) sum += m.snap[(int64_t)mi * m.n_out + q];
+      const int64_t cnt = m.out_off[r + 1] - m.out_off[r];
+      m.meanv[r] = cnt > 0 ? sum / (double)cnt : 0.0;
+      m.coupling[r] = tC[(int64_t)mi * m.n + r];
+      if (mi + 1 == K) {
+        const double u = m.coupling[r] + p[WC_K_MICRO] * (m.rate[r] / p[WC_RATE_SCALE]);
+        double e = m.E[r], i = m.I[r];
+        wc_rk4(p, e, i, u, m.dt_macro);
+        m.E[r] = e;
+        m.I[r] = i;
+        m.hist[((s + 1) % m.cap) * m.n + r] = e;
+        record_macro(m, r, s + 1 - m.trace_start);
+      } else if (m.trace) {
+        const int64_t S = m.samples, at = s + 1 - m.trace_start;
+        m.trace[(0 * (int64_t)m.n + r) * S + at] = tE[(int64_t)mi * m.n + r];
+        m.trace[(1 * (int64_t)m.n + r) * S + at] = tI[(int64_t)mi * m.n + r];
+        m.trace[(2 * (int64_t)m.n + r) * S + at] = m.rate[r];
+        m.trace[(3 * (int64_t)m.n + r) * S + at] = m.coupling[r];
+        m.trace[(4 * (int64_t)m.n + r) * S + at] = m.meanv[r];
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) d.clock->mstep = s0 + K;
+}
+
 namespace {
 
 template <typename T>
@@ -264,6 +433,7 @@ int macro_setup(State& S, const mind_macro_desc& md) {
   m.ratio = (int32_t)std::nearbyint(ratio);
   if (S.wmax < m.ratio)
     return fail(MIND_EUNSUPPORTED, "micro NetCon delays must cover a whole macro step on the device path");
+  const int32_t micro_window = S.wmax;  // longest micro window the micro delays allow
   S.wmax = m.ratio;
   std::vector<double> W(md.weights, md.weights + (size_t)n * n), wc(md.wc_params, md.wc_params + (size_t)n * WC_NPARAMS);
   std::vector<int64_t> ds((size_t)n * n);
@@ -276,6 +446,20 @@ int macro_setup(State& S, const mind_macro_desc& md) {
   m.cap = maxd + 1;
   std::vector<int64_t> bins(n);
   for (int r = 0; r < n; ++r) bins[r] = std::max<int64_t>(1, (int64_t)std::nearbyint(md.bin_ms[r] / md.dt_macro));
+  // Windows of one bin: every ROI with the same bin of K >= 2 macro steps, every
+  // coupling delay at least K macro steps (so no step of a window reads an E the
+  // window produces), and micro delays that allow a window of K macro steps.
+  int64_t min_coupling = INT64_MAX;
+  for (size_t q = 0; q < ds.size(); ++q)
+    if (W[q] != 0.0) min_coupling = std::min(min_coupling, ds[q]);
+  const int64_t K = bins[0];
+  bool same = true;
+  for (int r = 0; r < n; ++r) same = same && bins[r] == K;
+  H->bins = same && K >= 2 && min_coupling >= K && (int64_t)micro_window >= K * m.ratio &&
+            K * m.ratio <= (int64_t)1 << 20 && K * (int64_t)n * (int64_t)sizeof(int) <= 64 * 1024 &&
+            (S.window_steps_opt == 0 || S.window_steps_opt >= K * m.ratio);
+  m.kwin = H->bins ? (int32_t)K : 1;
+  if (H->bins) S.wmax = (int32_t)(K * m.ratio);
   const int64_t nb = md.input_offsets[n];
   std::vector<int32_t> in_roi(nb);
   std::vector<uint64_t> sid(nb);
@@ -330,6 +514,23 @@ int macro_setup(State& S, const mind_macro_desc& md) {
   if ((rc = mupload(*H, &pu, sid))) return rc; m.stream_id = pu;
   if ((rc = mupload(*H, &m.in_count, std::vector<int64_t>(nb, 0)))) return rc;
   if ((rc = mupload(*H, &m.in_base, std::vector<int64_t>(nb, 0)))) return rc;
+  if (H->bins) {
+    m.n_out = out_off[n];
+    if ((rc = mupload(*H, &m.traj, std::vector<double>((size_t)3 * m.kwin * n, 0.0)))) return rc;
+    double* snap;
+    if ((rc = mupload(*H, &snap, std::vector<double>((size_t)std::max<int64_t>(1, m.kwin * m.n_out), 0.0)))) return rc;
+    m.snap = snap;
+    std::vector<int32_t> det_snap(L.det_source.size(), -1);
+    for (size_t k = 0; k < L.det_source.size(); ++k)
+      for (int64_t q = 0; q < m.n_out; ++q)
+        if (md.output_source[q] == L.det_source[k]) det_snap[k] = (int32_t)q;
+    if ((rc = mupload(*H, &p32, det_snap))) return rc;
+    S.d.snap = snap;
+    S.d.det_snap = p32;
+    S.d.snap_every = m.ratio;
+    S.d.n_snap = m.n_out;
+  }
+  // E0 and I0 stay the last two uploads (macro_initialize)
   if ((rc = mupload(*H, &pd, H->E0))) return rc;
   if ((rc = mupload(*H, &pd, H->I0))) return rc;
   return 0;
@@ -352,6 +553,17 @@ int launch_macro_step(State& S) {
   int rc = launch_window(S, H.m.ratio);
   if (rc) return rc;
   hipLaunchKernelGGL(k_macro_post, dim3(1), dim3(kB), sizeof(int) * H.m.n, S.stream, S.d, H.m);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// K macro steps inside one rate bin as one micro window.
+int launch_macro_window(State& S, int32_t K) {
+  MacroHost& H = *S.macro;
+  hipLaunchKernelGGL(k_macro_pre_bin, dim3(1), dim3(kB), 0, S.stream, S.d, H.m, K);
+  int rc = launch_window(S, K * H.m.ratio);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_macro_post_bin, dim3(1), dim3(kB), sizeof(int) * (size_t)K * H.m.n, S.stream, S.d, H.m, K);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -427,13 +639,16 @@ int mind_run_macro(mind_state* state, double stop, int32_t record_spikes, mind_m
   HIP_TRY(hipStreamSynchronize(S.stream));
   const auto t0 = std::chrono::steady_clock::now();
   int rc = 0;
-  for (int64_t j = 0; j < count; ++j) {
-    if ((rc = mind::stage_injections(S, H.m.ratio))) break;
-    if (!S.profiling) {
+  // windows: one macro step, or (bins) the macro steps up to the next bin end
+  for (int64_t j = 0, ms = mstep; j < count;) {
+    const int32_t K = H.bins ? (int32_t)std::min<int64_t>(H.m.kwin - ms % H.m.kwin, count - j) : 1;
+    if ((rc = mind::stage_injections(S, K * H.m.ratio))) break;
+    const bool full = K == H.m.kwin;
+    if (!S.profiling && full) {
       if (!H.graph) {
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
-        rc = mind::launch_macro_step(S);
+        rc = H.bins ? mind::launch_macro_window(S, K) : mind::launch_macro_step(S);
         hipError_t e = hipStreamEndCapture(S.stream, &g);
         if (rc) break;
         if (e != hipSuccess) {
@@ -444,10 +659,12 @@ int mind_run_macro(mind_state* state, double stop, int32_t record_spikes, mind_m
         (void)hipGraphDestroy(g);
       }
       HIP_TRY(hipGraphLaunch(H.graph, S.stream));
-    } else if ((rc = mind::launch_macro_step(S))) {
+    } else if ((rc = H.bins ? mind::launch_macro_window(S, K) : mind::launch_macro_step(S))) {
       break;
     }
-    S.step += H.m.ratio;
+    S.step += (int64_t)K * H.m.ratio;
+    j += K;
+    ms += K;
   }
   HIP_TRY(hipStreamSynchronize(S.stream));
   const auto t1 = std::chrono::steady_clock::now();

@@ -211,6 +211,13 @@ __global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
         phase_advance(G.mech, col, r, v[G.node[r]], t + dt, dt, d.celsius);
     }
     if (d.nrec) record_cell(d, c, k + 1 - d.rec_start);
+    if (d.snap && (s + 1) % d.snap_every == 0) {
+      const int64_t m = (s + 1) / d.snap_every - 1;
+      for (int64_t q = d.cell_det_off[c]; q < d.cell_det_off[c + 1]; ++q) {
+        const int32_t kk = d.cell_det[q];
+        if (d.det_snap[kk] >= 0) d.snap[m * d.n_snap + d.det_snap[kk]] = v[d.det_node[kk]];
+      }
+    }
     seg = deliver_cell(d, seg, seg_end, 2 * k + 1);
   }
 }
@@ -601,6 +608,7 @@ int setup_state(State& S, const mind_build_options& o) {
   Dev& d = S.d;
   S.device = o.device;
   S.method = o.method;
+  S.window_steps_opt = o.window_steps;
   HIP_TRY(hipSetDevice(o.device));
   HIP_TRY(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
   d.dt = L.dt;

@@ -38,6 +38,7 @@ struct State {
   double* v_init = nullptr;
   uint64_t* ns_logical = nullptr;
   int32_t wmax = 1;
+  int32_t window_steps_opt = 0;  // build option window_steps (0: as long as the model allows)
   int64_t step = 0;
   bool initialized = false;
   hipGraphExec_t graph = nullptr;
@@ -107,6 +108,7 @@ int check_device_error(State& S);
 int collect_spikes(State& S, int64_t* n, double** times, int64_t** sids);
 int macro_setup(State& S, const mind_macro_desc& md);
 int macro_initialize(State& S);
+int launch_macro_window(State& S, int32_t K);
 int timer_start(State& S);
 int timer_stop(State& S);
 int batched_setup(State& S, int32_t cells_per_group);

@@ -306,8 +306,11 @@ def _lib():
 
 
 def build(model: MacroModel, dt_micro=0.025, dt_macro=0.1, seed=1, method="batched",
-          cells_per_group=0, device=0):
-    """build! of a macro model: pack the micro network, the ROI graph and transforms."""
+          cells_per_group=0, device=0, window_steps=0):
+    """build! of a macro model: pack the micro network, the ROI graph and transforms.
+    The device advances one population-rate bin per window when the model allows
+    it (equal bins, coupling delays of at least a bin, micro delays of at least a
+    bin); window_steps = dt_macro / dt_micro keeps one macro step per window."""
     ratio = dt_macro / dt_micro
     if abs(ratio - round(ratio)) > 1e-9 or round(ratio) < 1:
         raise ValueError("dt_macro must be an integer multiple of dt_micro")
@@ -322,7 +325,7 @@ def build(model: MacroModel, dt_micro=0.025, dt_macro=0.1, seed=1, method="batch
     opts.seed = int(seed)
     opts.method = {"plain": 0, "batched": 1}[method]
     opts.cells_per_group = int(cells_per_group)
-    opts.window_steps = 0
+    opts.window_steps = int(window_steps)
     opts.event_capacity = 0
     opts.device = int(device)
     opts.secondorder = 0

@@ -351,3 +351,16 @@ def fit_linear(X, Y):
 
 def predict_linear(coef, X):
     return np.concatenate([X, np.ones((X.shape[0], 1))], axis=1) @ coef
+
+
+def linear_only(net: Network) -> Network:
+    """The same trained surrogate with its MLP switched off (output layer zero), so
+    that only the linear skip map drives the ROIs: the method without the network,
+    for closed-loop comparisons."""
+    if not net.residual or net.linear is None:
+        raise ValueError("the network has no linear skip map")
+    p = np.array(net.params, np.float32, copy=True)
+    n_in, hidden, layers, n_out = net.dims
+    last = p.size - (hidden * n_out + n_out)  # W then b of the output layer, last in the flat order
+    p[last:] = 0.0
+    return Network(p, net.norm, net.dims, net.v_rest, net.linear, dict(net.meta), net.residual)
