@@ -16,7 +16,9 @@
 #ifndef MIND_H
 #define MIND_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
+#endif
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -52,6 +54,10 @@ enum {
   MIND_MECH_VCLAMP = 9,      /* two-electrode three-level voltage clamp (amplifier model) */
   MIND_MECH_COUNT = 10
 };
+
+/* Mechanism slots: the builtins above, then mechanisms compiled at run time by
+ * mind_compile_mod (ids MIND_MECH_COUNT .. MIND_MECH_SLOTS - 1). */
+enum { MIND_MECH_SLOTS = 64 };
 
 /* Field columns of each builtin mechanism (parameters, states, assigned). */
 enum { HH_GNABAR, HH_GKBAR, HH_GL, HH_EL, HH_M, HH_H, HH_N, HH_ENA, HH_EK, HH_GNA, HH_GK,
@@ -239,6 +245,49 @@ typedef struct mind_macro_result {
   int64_t* spike_sids;
   double run_seconds;
 } mind_macro_result;
+
+/* ---- Mechanisms (compile_mod / load_mod / builtin, src/compiler/frontend.jl:58-110) ----
+ * A MOD text is lowered at run time into the next free mechanism slot; a model
+ * that inserts it gets its cell kernels compiled for the device with hipRTC at
+ * mind_build (once per set of compiled mechanisms and device, then reused).
+ * Runtime mechanisms are membrane mechanisms: SUFFIX density mechanisms and
+ * POINT_PROCESSes whose NET_RECEIVE (connection weight only) changes their own
+ * fields; na, k and ca ions with fixed reversal potentials and concentrations
+ * (default_ions, src/model/micro/utils/biophysics.jl:31-42); no RANDOM,
+ * net_send or net_event (those are the builtin NetStim's). */
+enum { MIND_KIND_DENSITY = 0, MIND_KIND_POINT = 1, MIND_KIND_ARTIFICIAL = 2 };
+typedef struct mind_mech_desc {
+  const char* name;      /* NEURON block name (owned by the library) */
+  const char* filename;  /* as given to mind_compile_mod; "" for builtins */
+  int32_t kind;          /* MIND_KIND_* */
+  int32_t n_fields;      /* field columns (an array field takes one per element) */
+  int32_t has_net_receive;
+  int32_t has_state;     /* BREAKPOINT has SOLVE statements (nrn_state) */
+  int32_t compiled;      /* 1 for a runtime slot, 0 for a builtin */
+  uint64_t digest;       /* first 16 hex digits of the SHA-256 of the MOD text (0 for builtins) */
+} mind_mech_desc;
+/* compile_mod(source; filename) (frontend.jl:74-110): lowers `length` bytes of MOD
+ * text into a mechanism slot and returns its id in *mech. Compiling the same text
+ * and filename again returns the same id. Errors (MIND_EARG) name the file and
+ * line in mind_last_error; MIND_EQUEUE when every slot is taken. */
+int mind_compile_mod(const char* source, size_t length, const char* filename, int32_t* mech);
+int mind_mech_info(int32_t mech, mind_mech_desc* out);
+/* Field names (library-owned strings; element k of array field a is "a[k]",
+ * 1-based) and default values of a mechanism's columns, up to `capacity`;
+ * *count is the number of columns. */
+int mind_mech_fields(int32_t mech, const char** names, double* defaults, int32_t capacity, int32_t* count);
+/* builtin(name) / a compiled mechanism by name: the most recently compiled one of
+ * that name, else the builtin. */
+int mind_mech_lookup(const char* name, int32_t* mech);
+/* C++ text of a compiled mechanism's phases (namespace mind::user, functions
+ * u<id>_current/_advance/_initial/_receive over Cols), as hipRTC compiles them;
+ * for host builds of the same lowering. */
+int mind_mech_lowering(int32_t mech, const char** text);
+/* Host only (no device): compiles the cell kernels for the compiled mechanisms
+ * mechs[0..n) and GPU architecture `arch` (e.g. "gfx950") with hipRTC, as
+ * mind_build does for a model that uses them, without loading the code. No
+ * counterpart in the reference (a build check). */
+int mind_rtc_check(const int32_t* mechs, int32_t n, const char* arch);
 
 /* Pack and upload a model. Replaces build!/pack_micro (src/model/build.jl:120-220,
  * src/model/micro/model.jl:420-902) and allocate_simulation

@@ -1,9 +1,11 @@
 // Device-side data structures shared by the window kernels (see runtime.hip).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include "layout.h"
+#endif
 #include "mechanisms.h"
 
 namespace mind {

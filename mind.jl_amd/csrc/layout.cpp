@@ -9,6 +9,7 @@
 // (src/model/micro/utils/morphology.jl:28-63). Compiled with -ffp-contract=off:
 // these numbers feed a bit-exact comparison.
 #include "layout.h"
+#include "mechreg.h"
 
 #include <math.h>
 
@@ -22,13 +23,23 @@ int field_count(int mech) {
                                          EXP2SYN_NFIELDS, ICLAMP_NFIELDS,  NETSTIM_NFIELDS,
                                          ALPHASYNAPSE_NFIELDS, PATTERNSTIM_NFIELDS, SECLAMP_NFIELDS,
                                       VCLAMP_NFIELDS};
-  return n[mech];
+  if (mech >= 0 && mech < MIND_MECH_COUNT) return n[mech];
+  const UserSlot* u = user_slot(mech);
+  return u ? (int)u->m.columns.size() : 0;
 }
 
-bool mech_is_point(int mech) { return mech >= MIND_MECH_EXPSYN; }
+// Point processes and artificial cells (created by add_pointprocess!).
+bool mech_is_point(int mech) {
+  if (mech >= 0 && mech < MIND_MECH_COUNT) return mech >= MIND_MECH_EXPSYN;
+  return mech_known(mech) && mech_kind(mech) != MIND_KIND_DENSITY;
+}
 
 void field_defaults(int mech, double* f) {
   for (int i = 0; i < field_count(mech); ++i) f[i] = 0.0;
+  if (const UserSlot* u = user_slot(mech)) {  // PARAMETER defaults and ion values of a compiled mechanism
+    for (size_t i = 0; i < u->m.defaults.size(); ++i) f[i] = u->m.defaults[i];
+    return;
+  }
   switch (mech) {
     case MIND_MECH_HH:  // hh.mod PARAMETER block; ena/ek from the default ions
       f[HH_GNABAR] = 0.12; f[HH_GKBAR] = 0.036; f[HH_GL] = 0.0003; f[HH_EL] = -54.3;
@@ -270,7 +281,7 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
       if (L.parent[i] >= 0) L.child[cur[L.parent[i]]++] = (int32_t)i;
   }
   // 3. Mechanism groups: first use over density insertions, then point processes.
-  for (int k = 0; k < MIND_MECH_COUNT; ++k) L.group_of_mech[k] = -1;
+  for (int k = 0; k < MIND_MECH_SLOTS; ++k) L.group_of_mech[k] = -1;
   struct Pending { int64_t p0, p1; };
   std::vector<std::vector<Pending>> params;
   auto group_of = [&](int mech) {
@@ -282,10 +293,10 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
     }
     return L.group_of_mech[mech];
   };
-  std::vector<std::vector<uint8_t>> seen(MIND_MECH_COUNT);
+  std::vector<std::vector<uint8_t>> seen(MIND_MECH_SLOTS);
   for (int64_t k = 0; k < d.n_density; ++k) {
     const int mech = d.dens_mech[k];
-    if (mech < 0 || mech >= MIND_MECH_COUNT || mech_is_point(mech))
+    if (mech < 0 || mech >= MIND_MECH_SLOTS || !mech_known(mech) || mech_is_point(mech))
       return "cell insertion requires a distributed micro mechanism";
     const int c = d.dens_cell[k];
     if (c < 0 || c >= d.n_cells) return "density insertion on an unknown cell";
@@ -311,7 +322,7 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
   std::vector<std::pair<double, int32_t>> pattern;  // (time, point)
   for (int64_t p = 0; p < d.n_points; ++p) {
     const int mech = d.pt_mech[p];
-    if (mech < 0 || mech >= MIND_MECH_COUNT || !mech_is_point(mech))
+    if (mech < 0 || mech >= MIND_MECH_SLOTS || !mech_is_point(mech))
       return "point creation requires a point process";
     const int64_t s0 = d.pt_spike_offsets ? d.pt_spike_offsets[p] : 0;
     const int64_t s1 = d.pt_spike_offsets ? d.pt_spike_offsets[p + 1] : 0;
@@ -360,8 +371,8 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
     H.rows = rows;
     H.node.resize(rows); H.point.resize(rows); H.logical.resize(rows);
     const int nf = field_count(G.mech);
-    double def[32];
-    field_defaults(G.mech, def);
+    std::vector<double> def(std::max(nf, 1));
+    field_defaults(G.mech, def.data());
     H.data.resize((size_t)nf * rows);
     for (int f = 0; f < nf; ++f) std::fill(H.data.begin() + f * rows, H.data.begin() + (f + 1) * rows, def[f]);
     for (int64_t r = 0; r < rows; ++r) {
@@ -427,8 +438,7 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
     if (d.con_source[e] < 0 || d.con_source[e] >= d.n_sources) return "connection from an unknown source";
     if (d.con_target[e] < 0 || d.con_target[e] >= d.n_points) return "connection to an unknown point";
     const int tm = d.pt_mech[d.con_target[e]];
-    if (tm != MIND_MECH_EXPSYN && tm != MIND_MECH_EXP2SYN && tm != MIND_MECH_NETSTIM)
-      return "target has no NET_RECEIVE block";
+    if (!mech_has_receive(tm)) return "target has no NET_RECEIVE block";
     if (!(d.con_delay[e] >= 0.0) || !isfinite(d.con_delay[e]) || !isfinite(d.con_weight[e]))
       return "connection weight and delay must be finite, delay nonnegative";
     L.fanout[d.con_source[e] + 1] += 1;
@@ -487,7 +497,7 @@ std::string pack_inputs(const mind_model_desc& d, const mind_macro_desc& m, Layo
     const int32_t p = m.input_point[b];
     if (p < 0 || p >= d.n_points) return "macro2micro target is not a point process";
     const int tm = d.pt_mech[p];
-    if (tm != MIND_MECH_EXPSYN && tm != MIND_MECH_EXP2SYN) return "macro2micro target has no NET_RECEIVE";
+    if (tm == MIND_MECH_NETSTIM || !mech_has_receive(tm)) return "macro2micro target has no NET_RECEIVE";
     if (L.point_cell[p] < 0) return "macro2micro target must sit on a cell";
     if (!(m.input_delay[b] >= 0.0) || !isfinite(m.input_delay[b]) || !isfinite(m.input_weight[b]))
       return "macro2micro weight and delay must be finite, delay nonnegative";

@@ -53,7 +53,7 @@ struct Layout {
   std::vector<int32_t> child;
 
   std::vector<MechGroup> groups;   // canonical order: first use, densities first
-  int group_of_mech[MIND_MECH_COUNT];
+  int group_of_mech[MIND_MECH_SLOTS];
   std::vector<int32_t> point_group, point_row, point_cell;
 
   int64_t nsources = 0;

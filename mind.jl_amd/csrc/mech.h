@@ -10,7 +10,11 @@
 // results match the CPU restatement bit for bit (-ffp-contract=off).
 #pragma once
 
+#ifdef __HIPCC_RTC__
+#include "mind.h"
+#else
 #include "../../include/mind.h"
+#endif
 #include "mind_math.h"
 
 namespace mind {

@@ -10,6 +10,21 @@
 #include "mech_gen.h"
 #endif
 
+// Runtime-compiled mechanisms (slots MIND_MECH_COUNT and up): a module built
+// with hipRTC for a model that uses them defines MIND_USER_MECHANISMS and
+// provides user_mechs.h, whose mind::user::{current, advance, initial, receive,
+// has_state} dispatch over its slots.
+#ifdef MIND_USER_MECHANISMS
+#include "user_mechs.h"
+#define MIND_USER_CASE(call) \
+  if (mech >= MIND_MECH_COUNT) { call; return; }
+#define MIND_USER_VALUE(call) \
+  if (mech >= MIND_MECH_COUNT) return call;
+#else
+#define MIND_USER_CASE(call)
+#define MIND_USER_VALUE(call)
+#endif
+
 namespace mind {
 
 #ifdef MIND_GENERATED_MECHANISMS
@@ -20,6 +35,7 @@ constexpr const char* kMechanismSource = gen::kSource;
 constexpr uint64_t kNetStimDigest = gen::kNetStimDigest;
 MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double t, double dt, double& value,
                            double& shifted, bool store) {
+  MIND_USER_CASE(user::current(mech, c, r, v, t, dt, value, shifted, store))
   gen::current(mech, c, r, v, t, dt, value, shifted, store);
 }
 // nrn_cur of mechanism slot MECH in two parts: the column reads of a row (load),
@@ -27,12 +43,20 @@ MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double 
 template <int MECH>
 using CurOps = gen::CurOf<MECH>;
 MIND_HD void phase_advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, double celsius) {
+  MIND_USER_CASE(user::advance(mech, c, r, v, t, dt, celsius))
   gen::advance(mech, c, r, v, t, dt, celsius);
 }
 MIND_HD void phase_initial(int mech, const Cols& c, int64_t r, double v, double celsius) {
+  MIND_USER_CASE(user::initial(mech, c, r, v, celsius))
   gen::initial(mech, c, r, v, celsius);
 }
-MIND_HD void phase_receive(int mech, const Cols& c, int64_t r, double weight) { gen::receive(mech, c, r, weight); }
+// NET_RECEIVE of a connection event delivered at time t (the builtin receivers
+// other than NetStim do not read t).
+MIND_HD void phase_receive(int mech, const Cols& c, int64_t r, double t, double weight) {
+  MIND_USER_CASE(user::receive(mech, c, r, t, weight))
+  (void)t;
+  gen::receive(mech, c, r, weight);
+}
 MIND_HD NetStimOut phase_netstim_initial(const Cols& c, int64_t r, Stream& s, uint64_t seed) {
   return gen::NetStim_initial(c, r, 0.0, 0.0, &s, seed);
 }
@@ -47,6 +71,7 @@ constexpr const char* kMechanismSource = "hand lowering (mech.h)";
 constexpr uint64_t kNetStimDigest = 0xa186833b569ff3b2ull;
 MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double t, double dt, double& value,
                            double& shifted, bool store) {
+  MIND_USER_CASE(user::current(mech, c, r, v, t, dt, value, shifted, store))
   mech_current(mech, c, r, v, t, dt, value, shifted, store);
 }
 // nrn_cur of mechanism slot MECH in two parts; the hand lowering reads its
@@ -61,12 +86,20 @@ struct CurOps {
   }
 };
 MIND_HD void phase_advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, double celsius) {
+  MIND_USER_CASE(user::advance(mech, c, r, v, t, dt, celsius))
   mech_advance(mech, c, r, v, t, dt, celsius);
 }
 MIND_HD void phase_initial(int mech, const Cols& c, int64_t r, double v, double celsius) {
+  MIND_USER_CASE(user::initial(mech, c, r, v, celsius))
   mech_initial(mech, c, r, v, celsius);
 }
-MIND_HD void phase_receive(int mech, const Cols& c, int64_t r, double weight) { synapse_receive(mech, c, r, weight); }
+// NET_RECEIVE of a connection event delivered at time t (the builtin receivers
+// other than NetStim do not read t).
+MIND_HD void phase_receive(int mech, const Cols& c, int64_t r, double t, double weight) {
+  MIND_USER_CASE(user::receive(mech, c, r, t, weight))
+  (void)t;
+  synapse_receive(mech, c, r, weight);
+}
 MIND_HD NetStimOut phase_netstim_initial(const Cols& c, int64_t r, Stream& s, uint64_t seed) {
   return netstim_initial(c, r, s, seed);
 }
@@ -75,5 +108,11 @@ MIND_HD NetStimOut phase_netstim_receive(const Cols& c, int64_t r, Stream& s, ui
   return netstim_receive(c, r, s, seed, t, w, flag);
 }
 #endif
+
+// Whether a mechanism has a state update (nrn_state).
+MIND_HD bool phase_has_state(int mech) {
+  MIND_USER_VALUE(user::has_state(mech))
+  return mech_has_state(mech);
+}
 
 }  // namespace mind

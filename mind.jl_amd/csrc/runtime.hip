@@ -36,6 +36,7 @@
 
 #include "device.h"
 #include "state.h"
+#include "cell_kernels.h"
 
 namespace mind {
 
@@ -44,18 +45,6 @@ __global__ void k_init_nodes(Dev d, const double* v_init) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < d.nodes) d.v[i] = v_init[i];
   if (i < d.ndet) d.det_above[i] = 0;
-}
-
-__global__ void k_init_group(Dev d, int gi) {
-  const GroupDev& G = d.g[gi];
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= G.rows) return;
-  if (G.mech == MIND_MECH_NETSTIM) {
-    // initialize_random_kernel! (cuda/micro.jl:79-91): keys from the logical id
-    // and the NetStim source digest, then the INITIAL block.
-    return;
-  }
-  phase_initial(G.mech, cols(G), r, d.v[G.node[r]], d.celsius);
 }
 
 __global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest, uint64_t seq0) {
@@ -84,142 +73,12 @@ __global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest, 
   }
 }
 
-// micro_currents!(0.0) at the end of initialize!: sets the assigned current fields.
-__global__ void k_currents_fields(Dev d, int gi, double t) {
-  const GroupDev& G = d.g[gi];
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= G.rows) return;
-  double value, shifted;
-  phase_current(G.mech, cols(G), r, d.v[G.node[r]], t, d.dt, value, shifted, true);
-}
-
 __global__ void k_record_sample(Dev d, int64_t sample) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.nrec) return;
   const int64_t idx = d.rec_index[q];
   const double x = d.rec_kind[q] == 0 ? d.v[idx] : d.g[d.rec_group[q]].data[d.rec_field[q] * d.g[d.rec_group[q]].rows + idx];
   d.rec_buf[q * d.rec_stride + sample] = x;
-}
-
-// ------------------------------------------------------------ shared pieces
-__device__ inline void record_cell(const Dev& d, int64_t c, int64_t sample) {
-  for (int64_t q = d.cell_rec_off[c]; q < d.cell_rec_off[c + 1]; ++q) {
-    const int32_t rec = d.cell_rec[q];
-    const int64_t idx = d.rec_index[rec];
-    const double x = d.rec_kind[rec] == 0
-                         ? d.v[idx]
-                         : d.g[d.rec_group[rec]].data[d.rec_field[rec] * d.g[d.rec_group[rec]].rows + idx];
-    d.rec_buf[rec * d.rec_stride + sample] = x;
-  }
-}
-
-// Applies this cell's events of delivery call j; returns the new cursor.
-__device__ inline int64_t deliver_cell(const Dev& d, int64_t seg, int64_t end, int64_t j) {
-  while (seg < end && d.win[seg].call == j) {
-    const int32_t e = d.win[seg].edge;
-    const int32_t target = d.edge_target[e];
-    const GroupDev& G = d.g[d.point_group[target]];
-    phase_receive(G.mech, cols(G), d.point_row[target], d.edge_weight[e]);
-    ++seg;
-  }
-  return seg;
-}
-
-__device__ inline void detect_cell(const Dev& d, int64_t c, int s) {
-  for (int64_t q = d.cell_det_off[c]; q < d.cell_det_off[c + 1]; ++q) {
-    const int32_t k = d.cell_det[q];
-    const bool cur = d.v[d.det_node[k]] > d.det_thr[k];
-    d.det_flags[(int64_t)s * d.ndet + k] = (cur && !d.det_above[k]) ? 1 : 0;
-    d.det_above[k] = cur;
-  }
-}
-
-// ------------------------------------------------------------ plain kernel
-// One cell per thread; every per-node array lives in HBM. This is the
-// reference's one-instance-per-thread execution shape, kept as the baseline
-// the batched kernel is measured against.
-__global__ void __launch_bounds__(64) k_plain(Dev d, int32_t n) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.ncells) return;
-  const int64_t k0 = d.clock->step;
-  const int64_t n0 = d.cell_off[c], n1 = d.cell_off[c + 1];
-  int64_t seg = d.seg_off[d.cell_bucket[c]];
-  const int64_t seg_end = d.seg_off[d.cell_bucket[c] + 1];
-  const double dt = d.dt;
-  double* v = d.v;
-  double* rhs = d.rhs;
-  double* diag = d.diag;
-  for (int s = 0; s < n; ++s) {
-    const int64_t k = k0 + s;
-    const double t = (double)k * dt;
-    detect_cell(d, c, s);
-    seg = deliver_cell(d, seg, seg_end, 2 * k);
-    // nrn_cur at t + dt/2
-    const double tc = t + 0.5 * dt;
-    for (int64_t i = n0; i < n1; ++i) { rhs[i] = 0.0; diag[i] = 0.0; }
-    for (int gi = 0; gi < d.ngroups; ++gi) {
-      const GroupDev& G = d.g[gi];
-      if (G.mech == MIND_MECH_NETSTIM) continue;
-      const Cols col = cols(G);
-      for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r) {
-        const int32_t node = G.node[r];
-        double value, shifted;
-        phase_current(G.mech, col, r, v[node], tc, dt, value, shifted, true);
-        const double scale = G.point ? 1.0 : d.area[node] * 0.01;
-        rhs[node] -= scale * value;
-        diag[node] += scale * (shifted - value) / 0.001;
-      }
-    }
-    // cable assembly: own axial current first, children in increasing index
-    for (int64_t i = n0; i < n1; ++i) {
-      diag[i] = d.cdt[i] + diag[i] + d.adiag[i];
-      const int32_t p = d.parent[i];
-      if (p >= 0) {
-        const double cur = d.axial[i] * (v[p] - v[i]);
-        rhs[i] += cur;
-        rhs[p] -= cur;
-      }
-    }
-    // Hines elimination (children before parents) and back substitution
-    bool bad = false;
-    for (int64_t i = n1 - 1; i >= n0; --i) {
-      const double piv = diag[i];
-      bad |= (piv == 0.0) || !mind_isfinite(piv);
-      const int32_t p = d.parent[i];
-      if (p >= 0) {
-        const double g = d.axial[i];
-        const double f = g / piv;
-        diag[p] -= f * g;
-        rhs[p] += f * rhs[i];
-      }
-    }
-    if (bad) raise_error(d.error, MIND_ESINGULAR);
-    for (int64_t i = n0; i < n1; ++i) {
-      double x = rhs[i];
-      const int32_t p = d.parent[i];
-      if (p >= 0) x += d.axial[i] * rhs[p];
-      x = x / diag[i];
-      rhs[i] = x;
-      v[i] += d.vscale * x;
-    }
-    // nrn_state at t + dt
-    for (int gi = 0; gi < d.ngroups; ++gi) {
-      const GroupDev& G = d.g[gi];
-      if (!mech_has_state(G.mech)) continue;
-      const Cols col = cols(G);
-      for (int64_t r = G.cell_rows[c]; r < G.cell_rows[c + 1]; ++r)
-        phase_advance(G.mech, col, r, v[G.node[r]], t + dt, dt, d.celsius);
-    }
-    if (d.nrec) record_cell(d, c, k + 1 - d.rec_start);
-    if (d.snap && (s + 1) % d.snap_every == 0) {
-      const int64_t m = (s + 1) / d.snap_every - 1;
-      for (int64_t q = d.cell_det_off[c]; q < d.cell_det_off[c + 1]; ++q) {
-        const int32_t kk = d.cell_det[q];
-        if (d.det_snap[kk] >= 0) d.snap[m * d.n_snap + d.det_snap[kk]] = v[d.det_node[kk]];
-      }
-    }
-    seg = deliver_cell(d, seg, seg_end, 2 * k + 1);
-  }
 }
 
 // ------------------------------------------------------------ block helpers
@@ -611,6 +470,8 @@ int setup_state(State& S, const mind_build_options& o) {
   S.window_steps_opt = o.window_steps;
   HIP_TRY(hipSetDevice(o.device));
   HIP_TRY(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  int rck = rtc_kernels(S);  // the cell kernels of a model with runtime-compiled mechanisms
+  if (rck) return rck;
   d.dt = L.dt;
   d.celsius = L.celsius;
   d.seed = L.seed;
@@ -885,7 +746,7 @@ int cells_launch(State& S, int32_t n) {
   if (S.method == MIND_METHOD_BATCHED) return batched_launch(S, n);
   int rc = 0;
   if (S.profiling && (rc = timer_start(S))) return rc;
-  hipLaunchKernelGGL(k_plain, dim3(blocks_for(S.d.ncells, 64)), dim3(64), 0, S.stream, S.d, n);
+  HIP_TRY(launch(S, K_PLAIN, (const void*)k_plain, dim3(blocks_for(S.d.ncells, 64)), dim3(64), 0, S.stream, S.d, n));
   if (S.profiling && (rc = timer_stop(S))) return rc;
   if (S.profiling) S.kernel_steps += n;
   return 0;
@@ -961,8 +822,8 @@ int mind_initialize(mind_state* state) {
                      S.stream, d, S.v_init);
   for (int gi = 0; gi < d.ngroups; ++gi) {
     if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
-    hipLaunchKernelGGL(mind::k_init_group, dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0,
-                       S.stream, d, gi);
+    HIP_TRY(mind::launch(S, mind::K_INIT_GROUP, (const void*)mind::k_init_group,
+                         dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0, S.stream, d, gi));
   }
   if (d.ns_rows > 0 && !S.serial) {
     hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
@@ -983,8 +844,8 @@ int mind_initialize(mind_state* state) {
   }
   for (int gi = 0; gi < d.ngroups; ++gi) {
     if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
-    hipLaunchKernelGGL(mind::k_currents_fields, dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0,
-                       S.stream, d, gi, 0.0);
+    HIP_TRY(mind::launch(S, mind::K_CURRENTS_FIELDS, (const void*)mind::k_currents_fields,
+                         dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0, S.stream, d, gi, 0.0));
   }
   HIP_TRY(hipGetLastError());
   d.record_spikes = 0;
@@ -1015,7 +876,7 @@ static int resolve_records(State& S, const mind_record* recs, int64_t n, std::ve
       index.push_back(L.point_row[r.point]);
       cell.push_back(L.point_cell[r.point]);
     } else if (r.kind == 2) {
-      if (r.mech < 0 || r.mech >= MIND_MECH_COUNT || L.group_of_mech[r.mech] < 0)
+      if (r.mech < 0 || r.mech >= MIND_MECH_SLOTS || L.group_of_mech[r.mech] < 0)
         return fail(MIND_EARG, "mechanism not in model");
       const int g = L.group_of_mech[r.mech];
       const int64_t node = L.range_node(r.cell, r.section, r.x);
@@ -1210,14 +1071,14 @@ int mind_read_voltage(mind_state* state, double* out, int64_t n) {
 
 int mind_row_count(const mind_state* state, int32_t mech, int64_t* rows) {
   const State& S = *reinterpret_cast<const State*>(state);
-  if (mech < 0 || mech >= MIND_MECH_COUNT || S.L.group_of_mech[mech] < 0) return fail(MIND_EARG, "mechanism not in model");
+  if (mech < 0 || mech >= MIND_MECH_SLOTS || S.L.group_of_mech[mech] < 0) return fail(MIND_EARG, "mechanism not in model");
   *rows = S.L.groups[S.L.group_of_mech[mech]].rows;
   return 0;
 }
 
 int mind_read_field(mind_state* state, int32_t mech, int32_t field, double* out, int64_t n) {
   State& S = *reinterpret_cast<State*>(state);
-  if (mech < 0 || mech >= MIND_MECH_COUNT || S.L.group_of_mech[mech] < 0) return fail(MIND_EARG, "mechanism not in model");
+  if (mech < 0 || mech >= MIND_MECH_SLOTS || S.L.group_of_mech[mech] < 0) return fail(MIND_EARG, "mechanism not in model");
   const int g = S.L.group_of_mech[mech];
   const int64_t rows = S.L.groups[g].rows;
   if (n != rows || field < 0 || field >= mind::field_count(mech)) return fail(MIND_EARG, "size or field mismatch");

@@ -21,6 +21,19 @@ int fail(int code, const std::string& msg);
 struct BatchPlan;
 struct MacroHost;
 struct State;
+
+// Kernels that run mechanism phases. A model with runtime-compiled mechanisms
+// launches them from a module that hipRTC builds for its set of mechanisms
+// (rtc.hip); every other model launches the library's own.
+enum KernelId {
+  K_PLAIN, K_INIT_GROUP, K_CURRENTS_FIELDS, K_WINDOW, K_WINDOW_LEAN, K_WINDOW_LEAN_HBM, K_WINDOW_LARGE,
+  K_SERIAL_STEP, K_COUNT
+};
+struct KernelSet {
+  hipModule_t module = nullptr;
+  hipFunction_t fn[K_COUNT] = {};
+  int max_groups = 0;  // mechanism groups its batched kernels cache in LDS
+};
 void batched_release(State& S);
 void macro_release(State& S);
 
@@ -46,6 +59,7 @@ struct State {
   // batched kernel plan (cell groups, chain units and level schedules)
   BatchPlan* batch = nullptr;
   MacroHost* macro = nullptr;
+  const KernelSet* ks = nullptr;  // runtime-compiled kernels (owned by the module cache), or null
   // event-exact path (delays under 1.5 dt or connections into NetStims)
   bool serial = false;
   SerialDev q{};
@@ -100,6 +114,19 @@ struct State {
   }
 };
 
+
+// Launch of kernel `id`: `fn` (the library's) or the state's module function.
+hipError_t launch_kernel(const State& S, int id, const void* fn, dim3 grid, dim3 block, size_t shmem,
+                         hipStream_t stream, void** args);
+template <typename... A>
+hipError_t launch(const State& S, int id, const void* fn, dim3 grid, dim3 block, size_t shmem, hipStream_t stream,
+                  A... a) {
+  void* args[] = {(void*)&a...};
+  return launch_kernel(S, id, fn, grid, block, shmem, stream, args);
+}
+// Builds (or takes from the process-wide cache) the module of the state's
+// runtime-compiled mechanisms; leaves S.ks null when the model has none.
+int rtc_kernels(State& S);
 
 int launch_window(State& S, int32_t n);
 int stage_injections(State& S, int32_t n);

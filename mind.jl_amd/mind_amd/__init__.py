@@ -4,7 +4,7 @@ Mirrors the reference's model-building API (MicroModel, Section,
 add_population, insert, add_pointprocess, register_spike_source, connect,
 build, initialize, run, step, *_record) over libmind's C ABI.
 """
-from .mechanisms import Mechanism, builtin  # noqa: F401
+from .mechanisms import Mechanism, builtin, compile_mod, load_mod  # noqa: F401
 from .model import (  # noqa: F401
     Cell, CellSection, MicroModel, Morphology, PointProcess, Section, SimulationResult, SpikeSource,
     add_pointprocess, add_population, batch_plan, build, checkpoint, connect, connection_weights, destroy,

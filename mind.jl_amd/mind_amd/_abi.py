@@ -118,6 +118,19 @@ class Result(C.Structure):
     ]
 
 
+class MechDesc(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("filename", C.c_char_p),
+        ("kind", C.c_int32),
+        ("n_fields", C.c_int32),
+        ("has_net_receive", C.c_int32),
+        ("has_state", C.c_int32),
+        ("compiled", C.c_int32),
+        ("digest", C.c_uint64),
+    ]
+
+
 def as_ptr(array: np.ndarray, ctype):
     return array.ctypes.data_as(C.POINTER(ctype))
 
@@ -184,6 +197,12 @@ def lib():
     L.mind_last_error.restype = C.c_char_p
     L.mind_build_info.restype = C.c_char_p
     L.mind_copy_bandwidth.argtypes = [C.c_int32, C.c_int64, C.c_int32, _p(C.c_double)]
+    L.mind_compile_mod.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, _p(C.c_int32)]
+    L.mind_mech_info.argtypes = [C.c_int32, _p(MechDesc)]
+    L.mind_mech_fields.argtypes = [C.c_int32, _p(C.c_char_p), _p(C.c_double), C.c_int32, _p(C.c_int32)]
+    L.mind_mech_lookup.argtypes = [C.c_char_p, _p(C.c_int32)]
+    L.mind_mech_lowering.argtypes = [C.c_int32, _p(C.c_char_p)]
+    L.mind_rtc_check.argtypes = [_p(C.c_int32), C.c_int32, C.c_char_p]
     _lib = L
     return L
 

@@ -1,12 +1,17 @@
-"""Builtin mechanism table: names, ids and field columns.
+"""Mechanisms: the builtin table and runtime compilation of MOD files.
 
-Mirrors the reference's `builtin(:name)` (src/compiler/frontend.jl:58-72) for
-the ten mechanisms of the simulation step (array fields take one column per
-element, named name[1], name[2], ...). Field order is the column order of
-the packed mechanism data (include/mind.h).
+Mirrors the reference's `builtin(:name)`, `compile_mod` and `load_mod`
+(src/compiler/frontend.jl:58-110). The ten builtins are the mechanisms of the
+simulation step; `compile_mod` / `load_mod` lower a MOD text through libmind's
+NMODL frontend (mind_compile_mod) into a new mechanism slot, whose device
+phases are compiled with hipRTC when a model that uses it is built. Array
+fields take one column per element, named name[1], name[2], ...; field order
+is the column order of the packed mechanism data (include/mind.h).
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 
@@ -18,6 +23,11 @@ class Mechanism:
     fields: tuple
     has_net_receive: bool = False
     index: dict = field(default_factory=dict, compare=False, hash=False)
+    compiled: bool = False    # a runtime slot (compile_mod / load_mod)
+    has_state: bool = False   # compiled: BREAKPOINT has SOLVE statements
+    defaults: tuple = ()      # compiled: column defaults (PARAMETER values, ion values)
+    digest: int = 0           # compiled: SHA-256 prefix of the MOD text
+    filename: str = ""
 
     def field_index(self, name: str, index: int = 1) -> int:
         """Column of a field; of element `index` (1-based, as the reference's
@@ -67,6 +77,57 @@ _TABLE = {
 }
 
 BY_ID = {m.id: m for m in _TABLE.values()}
+
+
+_KINDS = {0: "density", 1: "point", 2: "artificial"}
+
+
+def compile_mod(source: str, filename: str = "<mod>") -> Mechanism:
+    """Lower MOD text into a runtime mechanism slot (compile_mod, frontend.jl:74-110).
+    Compiling the same text and filename again returns the same mechanism.
+    Malformed or unsupported input raises ValueError naming the file and line."""
+    from . import _abi
+
+    L = _abi.lib()
+    raw = source.encode()
+    mech = C.c_int32(-1)
+    code = L.mind_compile_mod(raw, len(raw), str(filename).encode(), C.byref(mech))
+    if code != 0:
+        message = L.mind_last_error().decode(errors="replace")
+        if code == 1:
+            raise ValueError(message)
+        raise _abi.MindError(code, message)
+    if mech.value in BY_ID:
+        return BY_ID[mech.value]
+    desc = _abi.MechDesc()
+    _abi.check(L.mind_mech_info(mech.value, C.byref(desc)))
+    n = desc.n_fields
+    names = (C.c_char_p * max(n, 1))()
+    defaults = (C.c_double * max(n, 1))()
+    count = C.c_int32(0)
+    _abi.check(L.mind_mech_fields(mech.value, names, defaults, n, C.byref(count)))
+    m = Mechanism(desc.name.decode(), mech.value, _KINDS[desc.kind], tuple(names[k].decode() for k in range(n)),
+                  bool(desc.has_net_receive), compiled=True, has_state=bool(desc.has_state),
+                  defaults=tuple(defaults[k] for k in range(n)), digest=int(desc.digest),
+                  filename=desc.filename.decode())
+    BY_ID[m.id] = m
+    return m
+
+
+def load_mod(path) -> Mechanism:
+    """compile_mod of a MOD file (load_mod, frontend.jl:74-80)."""
+    with open(path, encoding="utf-8", newline="") as f:
+        return compile_mod(f.read(), os.fspath(path))
+
+
+def lowering(mechanism: Mechanism) -> str:
+    """C++ text of a compiled mechanism's phases (namespace mind::user), as the
+    device compiles them."""
+    from . import _abi
+
+    text = C.c_char_p()
+    _abi.check(_abi.lib().mind_mech_lowering(mechanism.id, C.byref(text)))
+    return text.value.decode()
 
 
 def builtin(name) -> Mechanism:

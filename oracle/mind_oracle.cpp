@@ -260,7 +260,32 @@ const int kFields[MIND_MECH_COUNT] = {HH_NFIELDS,     PAS_NFIELDS,    EXPSYN_NFI
                                       VCLAMP_NFIELDS};
 const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true, true, true, true};
 
+// A runtime-compiled mechanism: its phases come from a host build (g++) of the
+// library's lowering of its MOD text, registered by oracle_register_mech.
+using UserCurrent = void (*)(double* data, int64_t rows, int64_t r, double v, double t, double dt, double* value,
+                             double* shifted, int32_t store);
+using UserAdvance = void (*)(double* data, int64_t rows, int64_t r, double v, double t, double dt, double celsius);
+using UserInitial = void (*)(double* data, int64_t rows, int64_t r, double v, double celsius);
+using UserReceive = void (*)(double* data, int64_t rows, int64_t r, double t, double weight);
+struct UserMech {
+  int nfields = 0;
+  bool point = false, has_state = false;
+  std::vector<double> defaults;
+  UserCurrent current = nullptr;
+  UserAdvance advance = nullptr;
+  UserInitial initial = nullptr;
+  UserReceive receive = nullptr;
+};
+UserMech g_user[MIND_MECH_SLOTS];
+
+int field_count(int mech) { return mech < MIND_MECH_COUNT ? kFields[mech] : g_user[mech].nfields; }
+bool is_point(int mech) { return mech < MIND_MECH_COUNT ? kPoint[mech] : g_user[mech].point; }
+
 void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
+  if (mech >= MIND_MECH_COUNT) {
+    for (int k = 0; k < g_user[mech].nfields; ++k) f[k] = g_user[mech].defaults[k];
+    return;
+  }
   switch (mech) {
     case MIND_MECH_HH:
       f[HH_GNABAR] = 0.12; f[HH_GKBAR] = 0.036; f[HH_GL] = 0.0003; f[HH_EL] = -54.3;
@@ -581,7 +606,9 @@ void receive(Sim& s, int gi, int row, const Event& ev) {
     }
     case MIND_MECH_EXPSYN: g.at(EXPSYN_G, row) = g.at(EXPSYN_G, row) + w; break;
     case MIND_MECH_NETSTIM: netstim_receive(s, g, row, t, edge, w, flag); break;
-    default: break;
+    default:
+      if (g.mech >= MIND_MECH_COUNT) g_user[g.mech].receive(g.data.data(), g.rows(), row, t, w);
+      break;
   }
 }
 
@@ -644,13 +671,20 @@ void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (s
   std::fill(s.diag.begin() + s.cell_offset[c0], s.diag.begin() + s.cell_offset[c1], 0.0);
   for (int pass = 0; pass < 2; ++pass) {  // density groups, then point groups
     for (Group& g : s.groups) {
-      if (kPoint[g.mech] != (pass == 1)) continue;
+      if (is_point(g.mech) != (pass == 1)) continue;
       if (g.mech == MIND_MECH_NETSTIM) continue;
       for (int64_t r = g.cell_rows[c0]; r < g.cell_rows[c1]; ++r) {
         const int32_t node = g.node[r];
         if (node < 0) continue;
         const double v = s.v[node];
         double shifted = 0.0, value = 0.0;
+        if (g.mech >= MIND_MECH_COUNT) {  // both probes inside the compiled phase
+          g_user[g.mech].current(g.data.data(), g.rows(), r, v, t, s.dt, &value, &shifted, 1);
+          const double scale = is_point(g.mech) ? 1.0 : s.area[node] * 0.01;
+          s.rhs[node] -= scale * value;
+          s.diag[node] += scale * (shifted - value) / 0.001;
+          continue;
+        }
         for (int probe = 0; probe < 2; ++probe) {
           const double vv = probe == 0 ? v + 0.001 : v;
           double sum = 0.0;
@@ -743,7 +777,7 @@ void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (s
           }
           if (probe == 0) shifted = sum; else value = sum;
         }
-        const double scale = kPoint[g.mech] ? 1.0 : s.area[node] * 0.01;
+        const double scale = is_point(g.mech) ? 1.0 : s.area[node] * 0.01;
         s.rhs[node] -= scale * value;
         s.diag[node] += scale * (shifted - value) / 0.001;
       }
@@ -797,7 +831,12 @@ void advance_states(Sim& s, int64_t c0, int64_t c1) {  // nrn_state at t + dt (s
   const double dt = s.dt;
   for (Group& g : s.groups) {
     const int64_t r0 = g.cell_rows[c0], rows = g.cell_rows[c1];
-    if (g.mech == MIND_MECH_HH) {
+    if (g.mech >= MIND_MECH_COUNT) {
+      if (!g_user[g.mech].has_state) continue;
+      const double t = (double)s.step * dt + dt;
+      for (int64_t r = r0; r < rows; ++r)
+        g_user[g.mech].advance(g.data.data(), g.rows(), r, s.v[g.node[r]], t, dt, s.celsius);
+    } else if (g.mech == MIND_MECH_HH) {
       for (int64_t r = r0; r < rows; ++r) {
         double f[HH_NFIELDS];
         for (int k = 0; k < HH_NFIELDS; ++k) f[k] = g.at(k, r);
@@ -886,6 +925,10 @@ void initial_blocks(Sim& s) {  // INITIAL per group (initialize.jl:342-374)
   for (size_t gi = 0; gi < s.groups.size(); ++gi) {
     Group& g = s.groups[gi];
     for (int r = 0; r < g.rows(); ++r) {
+      if (g.mech >= MIND_MECH_COUNT) {
+        g_user[g.mech].initial(g.data.data(), g.rows(), r, s.v[g.node[r]], s.celsius);
+        continue;
+      }
       switch (g.mech) {
         case MIND_MECH_HH: {
           double f[HH_NFIELDS];
@@ -942,6 +985,23 @@ extern "C" {
 const char* oracle_last_error(void) { return g_error.c_str(); }
 
 void oracle_free(void* p) { delete static_cast<Sim*>(p); }
+
+// Registers the host build of a runtime-compiled mechanism's phases under its
+// slot id (before oracle_build of a model that uses it).
+int oracle_register_mech(int32_t mech, int32_t nfields, int32_t point, int32_t has_state, const double* defaults,
+                         void* current, void* advance, void* initial, void* receive) {
+  if (mech < MIND_MECH_COUNT || mech >= MIND_MECH_SLOTS || nfields < 0) return fail(MIND_EARG, "bad mechanism slot");
+  UserMech& u = g_user[mech];
+  u.nfields = nfields;
+  u.point = point != 0;
+  u.has_state = has_state != 0;
+  u.defaults.assign(defaults, defaults + nfields);
+  u.current = (UserCurrent)current;
+  u.advance = (UserAdvance)advance;
+  u.initial = (UserInitial)initial;
+  u.receive = (UserReceive)receive;
+  return 0;
+}
 
 // inject_event! (state.jl:618-636): the connection's event at `time` without its
 // delay, enqueued (ticketed) now.
@@ -1041,7 +1101,7 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
     return s.cell_offset[cell] + (range ? range_node(t, sec, x) : location_node(t, sec, x));
   };
   // 3. mechanism groups in first-use order: density insertions, then points
-  s.group_of_mech.assign(MIND_MECH_COUNT, -1);
+  s.group_of_mech.assign(MIND_MECH_SLOTS, -1);
   auto group_for = [&](int mech) {
     if (s.group_of_mech[mech] < 0) {
       s.group_of_mech[mech] = (int)s.groups.size();
@@ -1053,7 +1113,7 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
   };
   struct RowInit { int64_t poff0, poff1; };
   std::vector<std::vector<RowInit>> row_params;
-  std::vector<uint8_t> has_density(MIND_MECH_COUNT * (size_t)s.total, 0);
+  std::vector<uint8_t> has_density(MIND_MECH_SLOTS * (size_t)s.total, 0);
   for (int64_t k = 0; k < d->n_density; ++k) {
     const int mech = d->dens_mech[k];
     const int gi = group_for(mech);
@@ -1115,10 +1175,10 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
     }
     for (int r = 0; r < rows; ++r)
       if (h.target[r] > 0) s.point_row[h.target[r] - 1] = r;
-    const int nf = kFields[g.mech];
+    const int nf = field_count(g.mech);
     h.data.assign((size_t)nf * rows, 0.0);
-    double def[32] = {0};
-    defaults(g.mech, def);
+    std::vector<double> def(std::max(nf, 32), 0.0);
+    defaults(g.mech, def.data());
     for (int f = 0; f < nf; ++f)
       for (int r = 0; r < rows; ++r) h.data[(size_t)f * rows + r] = def[f];
     for (int r = 0; r < rows; ++r)
@@ -1247,13 +1307,14 @@ int oracle_initialize(void* p) {  // initialize! (initialize.jl:270-429)
 // Field read in canonical row order.
 int oracle_row_count(void* p, int mech, int64_t* rows) {
   Sim& s = *static_cast<Sim*>(p);
-  const int gi = mech >= 0 && mech < MIND_MECH_COUNT ? s.group_of_mech[mech] : -1;
+  const int gi = mech >= 0 && mech < MIND_MECH_SLOTS ? s.group_of_mech[mech] : -1;
   *rows = gi < 0 ? 0 : s.groups[gi].rows();
   return 0;
 }
 
 int oracle_read_field(void* p, int mech, int field, double* out, int64_t n) {
   Sim& s = *static_cast<Sim*>(p);
+  if (mech < 0 || mech >= MIND_MECH_SLOTS) return fail(MIND_EARG, "unknown mechanism");
   const int gi = s.group_of_mech[mech];
   if (gi < 0) return fail(MIND_EARG, "mechanism not in model");
   Group& g = s.groups[gi];

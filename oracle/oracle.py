@@ -57,11 +57,81 @@ def lib():
                                    P(C.c_double)]
         L.oracle_exp.restype = C.c_double
         L.oracle_exp.argtypes = [C.c_double]
+        L.oracle_register_mech.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_double),
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_log.restype = C.c_double
         L.oracle_log.argtypes = [C.c_double]
         L.oracle_last_error.restype = C.c_char_p
         _lib = L
     return _lib
+
+
+_registered = {}  # runtime mechanism id -> its host build (kept loaded)
+
+_WRAPPERS = """
+extern "C" {{
+void oracle_u{i}_current(double* data, int64_t rows, int64_t r, double v, double t, double dt, double* value,
+                        double* shifted, int32_t store) {{
+  mind::user::u{i}_current(mind::Cols{{data, rows}}, r, v, t, dt, *value, *shifted, store != 0);
+}}
+void oracle_u{i}_advance(double* data, int64_t rows, int64_t r, double v, double t, double dt, double celsius) {{
+  mind::user::u{i}_advance(mind::Cols{{data, rows}}, r, v, t, dt, celsius);
+}}
+void oracle_u{i}_initial(double* data, int64_t rows, int64_t r, double v, double celsius) {{
+  mind::user::u{i}_initial(mind::Cols{{data, rows}}, r, v, celsius, nullptr, 0);
+}}
+void oracle_u{i}_receive(double* data, int64_t rows, int64_t r, double t, double weight) {{
+  mind::user::u{i}_receive(mind::Cols{{data, rows}}, r, t, weight, 0.0, nullptr, 0);
+}}
+}}
+"""
+
+
+def _host_build(mech):
+    """g++ build of libmind's lowering of a runtime-compiled mechanism (the text the
+    device compiles with hipRTC), cached by content under the temp directory."""
+    import hashlib
+    import subprocess
+    import tempfile
+
+    from mind_amd import mechanisms
+
+    csrc = os.path.join(os.path.dirname(HERE), "mind.jl_amd", "csrc")
+    src = '#include "mech.h"\n' + mechanisms.lowering(mech) + _WRAPPERS.format(i=mech.id)
+    key = hashlib.sha256(src.encode()).hexdigest()[:16]
+    cache = os.path.join(tempfile.gettempdir(), f"mind_oracle_mechs_{os.getuid()}")
+    os.makedirs(cache, exist_ok=True)
+    so = os.path.join(cache, f"u{mech.id}_{key}.so")
+    if not os.path.exists(so):
+        cpp = os.path.join(cache, f"u{mech.id}_{key}.cpp")
+        with open(cpp, "w") as f:
+            f.write(src)
+        tmp = f"{so}.{os.getpid()}.tmp"
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+                        f"-I{csrc}", "-o", tmp, cpp], check=True, capture_output=True, text=True)
+        os.replace(tmp, so)
+    return C.CDLL(so)
+
+
+def register_mechanisms(model):
+    """Registers with the oracle every runtime-compiled mechanism the model uses."""
+    from mind_amd.mechanisms import BY_ID
+
+    ids = {mid for (_, _, mid, _) in model.density} | {p[0] for p in model.points}
+    for mid in sorted(i for i in ids if i >= 10):
+        if mid in _registered:
+            continue
+        mech = BY_ID[mid]
+        h = _host_build(mech)
+        fns = [C.cast(getattr(h, f"oracle_u{mid}_{ph}"), C.c_void_p)
+               for ph in ("current", "advance", "initial", "receive")]
+        n = len(mech.fields)
+        defaults = (C.c_double * max(1, n))(*mech.defaults)
+        rc = lib().oracle_register_mech(mid, n, 1 if mech.kind == "point" else 0, int(mech.has_state), defaults,
+                                        *fns)
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        _registered[mid] = h
 
 
 def default_threads():
@@ -77,6 +147,7 @@ class OracleSim:
     """The oracle's view of one MicroModel (same descriptor as libmind)."""
 
     def __init__(self, model, dt=0.025, seed=1, secondorder=0, threads=None):
+        register_mechanisms(model)
         desc, keep = model.descriptor()
         self.model = model
         self.dt = dt
@@ -240,6 +311,7 @@ class OracleMacro:
 
         L = _macro_lib()
         self.model = model
+        register_mechanisms(model.micro)
         d, k1 = model.micro.descriptor()
         m, k2 = M.macro_descriptor(model, dt_macro)
         self.h = L.oracle_build_macro(C.byref(d), C.byref(m), float(dt), int(seed))

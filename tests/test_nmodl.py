@@ -242,6 +242,8 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
         #include "mech_gen.h"
         #include "layout.h"
         #include <cstdio>
+        #include <string>
+        namespace mind { int fail(int code, const std::string&) { return code; } }
         using namespace mind;
         int main() {
           double f[32], h[32];
@@ -256,7 +258,8 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
     """))
     exe = tmp_path / "defaults"
     subprocess.run(["g++", "-O1", "-std=c++17", f"-I{CSRC}", f"-I{tmp_path}", "-o", str(exe), str(src),
-                    os.path.join(CSRC, "layout.cpp")], check=True, capture_output=True, text=True)
+                    *(os.path.join(CSRC, f) for f in ("layout.cpp", "mechreg.cpp", "nmodl.cpp"))],
+                   check=True, capture_output=True, text=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     fields = {m.name: [f"{n}[{k + 1}]" if n in m.extents else n for n in m.fields for k in range(m.extents.get(n, 1))]
               for m in (nmodl.parse(open(p).read(), p) for p in paths)}
