@@ -111,13 +111,25 @@ __global__ void k_to_bf16(const float* x, int64_t n, uint16_t* y) {
   if (t < n) y[t] = f32_to_bf16(x[t]);
 }
 
-// Bias gradient: column sums in batch order (deterministic).
-__global__ void k_colsum(const float* dz, int B, int N, int np, float* gb) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+// Bias gradient: column sums. A workgroup of kColSlices x 64 threads takes 64
+// columns; slice q sums rows q, q + kColSlices, ... (coalesced across the
+// columns), then the slices' partials are added in slice order: a fixed order,
+// so the result is deterministic.
+constexpr int kColSlices = 16;
+__global__ void __launch_bounds__(64 * kColSlices) k_colsum(const float* dz, int B, int N, int np, float* gb) {
+  __shared__ float part[kColSlices][64];
+  const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + c;
   float s = 0.0f;
-  for (int i = 0; i < B; ++i) s += dz[(int64_t)i * np + j];
-  gb[j] = s;
+  if (j < N)
+    for (int i = q; i < B; i += kColSlices) s += dz[(int64_t)i * np + j];
+  part[q][c] = s;
+  __syncthreads();
+  if (q == 0 && j < N) {
+    float t = part[0][c];
+    for (int k = 1; k < kColSlices; ++k) t += part[k][c];
+    gb[j] = t;
+  }
 }
 
 // dz = dh * gelu'(z) on the real columns of a hidden layer.
@@ -260,7 +272,8 @@ void backward(Mlp& M, int B) {
     hipLaunchKernelGGL(k_to_bf16, dim3(nb((int64_t)B * np)), dim3(256), 0, M.stream, M.dz, (int64_t)B * np, M.dzb);
     // dW = act^T dz  (K x N, written unpadded into the flat gradient)
     gemm(M, M.act[l], 1, kp, M.dzb, np, 1, M.grad + M.w_off[l], N, K, N, B, 0);
-    hipLaunchKernelGGL(k_colsum, dim3(nb(N)), dim3(256), 0, M.stream, M.dz, B, N, np, M.grad + M.b_off[l]);
+    hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(64 * kColSlices), 0, M.stream, M.dz, B, N, np,
+                       M.grad + M.b_off[l]);
     if (l == 0) break;
     // dh = dz W^T  (B x kp), then through the previous layer's GELU
     gemm(M, M.dzb, np, 1, M.wbf[l], 1, np, M.dh, kp, B, kp, np, 0);
