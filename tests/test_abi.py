@@ -58,6 +58,9 @@ def _c_structs():
 @pytest.mark.parametrize("cname,pyname,jlname", [
     ("mind_model_desc", "ModelDesc", "ModelDesc"), ("mind_build_options", "BuildOptions", "BuildOptions"),
     ("mind_record", "Record", "RecordSpec"), ("mind_result", "Result", "ResultC"),
+    ("mind_mech_desc", "MechDesc", "MechDesc"), ("mind_macro_desc", "macro.MacroDesc", "MacroDesc"),
+    ("mind_macro_result", "macro.MacroResultC", "MacroResultC"),
+    ("mind_surrogate_desc", "surrogate.SurrogateDescC", "SurrogateDesc"),
 ])
 def test_bindings_match_the_header_structs(cname, pyname, jlname):
     """The ctypes structures of mind_amd and the Julia structs of MINDAMD.jl lay out the
@@ -65,15 +68,18 @@ def test_bindings_match_the_header_structs(cname, pyname, jlname):
     import ctypes as C
     import re
 
-    from mind_amd import _abi
+    import importlib
+
     want = _c_structs()[cname]
     kinds = {C.c_double: "f8", C.c_int32: "i4", C.c_int64: "i8", C.c_uint64: "u8"}
+    module, _, cls = pyname.rpartition(".")
+    mod = importlib.import_module(f"mind_amd.{module or '_abi'}")
     py = [(n, kinds.get(t, "p" if issubclass(t, (C._Pointer, C.c_void_p, C.c_char_p)) else "?"))
-          for n, t in getattr(_abi, pyname)._fields_]
+          for n, t in getattr(mod, cls)._fields_]
     assert py == want
     jl = open(os.path.join(ROOT, "mind.jl_amd", "julia", "MINDAMD.jl")).read()
     body = re.search(rf"^struct {jlname}\n(.*?)^end", jl, flags=re.S | re.M).group(1)
     jkinds = {"Float64": "f8", "Int32": "i4", "Int64": "i8", "UInt64": "u8"}
-    fields = [(n, "p" if t.startswith("Ptr") else jkinds[t])
+    fields = [(n, "p" if t.startswith("Ptr") or t == "Cstring" else jkinds[t])
               for n, t in re.findall(r"^\s+(\w+)::(\S+)", body, flags=re.M)]
     assert fields == want
