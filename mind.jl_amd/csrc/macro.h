@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "device.h"
+#include "macro_gen.h"
 
 namespace mind {
 
@@ -39,26 +40,70 @@ struct MacroDev {
   int64_t n_out;
 };
 
-// Wilson-Cowan REGION: S(x) = 1/(1 + exp(-a (x - theta))) - 1/(1 + exp(a theta)).
-__host__ __device__ inline double wc_sigmoid(double x, double a, double theta) {
-  return 1.0 / (1.0 + mind_exp(-a * (x - theta))) - 1.0 / (1.0 + mind_exp(a * theta));
+// The macro step's MIND-role mechanisms, lowered by nmodlc from
+// mind.jl_amd/mod/{WilsonCowan,LinearDelay,PoissonDrive,PopRate}.mod
+// (macro_gen.h). The kernels call them through these wrappers, each on one row
+// held in registers.
+static_assert(gen_macro::WilsonCowan_tau_e == WC_TAU_E && gen_macro::WilsonCowan_tau_i == WC_TAU_I &&
+                  gen_macro::WilsonCowan_c_ee == WC_C_EE && gen_macro::WilsonCowan_c_ei == WC_C_EI &&
+                  gen_macro::WilsonCowan_c_ie == WC_C_IE && gen_macro::WilsonCowan_c_ii == WC_C_II &&
+                  gen_macro::WilsonCowan_a_e == WC_A_E && gen_macro::WilsonCowan_theta_e == WC_THETA_E &&
+                  gen_macro::WilsonCowan_a_i == WC_A_I && gen_macro::WilsonCowan_theta_i == WC_THETA_I &&
+                  gen_macro::WilsonCowan_P == WC_P && gen_macro::WilsonCowan_Q == WC_Q &&
+                  gen_macro::WilsonCowan_k_micro == WC_K_MICRO && gen_macro::WilsonCowan_rate_scale == WC_RATE_SCALE &&
+                  gen_macro::WilsonCowan_E0 == WC_E0 && gen_macro::WilsonCowan_I0 == WC_I0,
+              "WilsonCowan.mod's parameters must be the WC_* columns in order");
+
+// REGION: one macro step of length dt of the ROI whose parameters are p (WC_*
+// order), from (e, i), with the delayed coupling and the micro population rate
+// (Hz) as inputs.
+MIND_HD void region_advance(const double* p, double& e, double& i, double coupling, double rate, double dt) {
+  using namespace gen_macro;
+  double f[WilsonCowan_NFIELDS];
+  for (int k = 0; k < WC_NPARAMS; ++k) f[k] = p[k];
+  f[WilsonCowan_E] = e;
+  f[WilsonCowan_I] = i;
+  f[WilsonCowan_coupling] = coupling;
+  f[WilsonCowan_rate] = rate;
+  f[WilsonCowan_dE] = 0.0;
+  f[WilsonCowan_dI] = 0.0;
+  WilsonCowan_advance(Cols{f, 1}, 0, 0.0, 0.0, dt, 0.0);
+  e = f[WilsonCowan_E];
+  i = f[WilsonCowan_I];
 }
 
-__host__ __device__ inline void wc_deriv(const double* p, double e, double i, double u, double& de, double& di) {
-  de = (-e + wc_sigmoid(p[WC_C_EE] * e - p[WC_C_EI] * i + p[WC_P] + u, p[WC_A_E], p[WC_THETA_E])) /
-       p[WC_TAU_E];
-  di = (-i + wc_sigmoid(p[WC_C_IE] * e - p[WC_C_II] * i + p[WC_Q], p[WC_A_I], p[WC_THETA_I])) /
-       p[WC_TAU_I];
+// MACRO2MACRO: an edge's contribution to its target's coupling, from the
+// source's delayed E.
+MIND_HD double edge_value(double weight, double source_e) {
+  using namespace gen_macro;
+  double f[LinearDelay_NFIELDS] = {0.0};
+  f[LinearDelay_weight] = weight;
+  f[LinearDelay_x] = source_e;
+  LinearDelay_evaluate(Cols{f, 1}, 0, 0.0, 0.0);
+  return f[LinearDelay_y];
 }
 
-__host__ __device__ inline void wc_rk4(const double* p, double& e, double& i, double u, double h) {
-  double k1e, k1i, k2e, k2i, k3e, k3i, k4e, k4i;
-  wc_deriv(p, e, i, u, k1e, k1i);
-  wc_deriv(p, e + 0.5 * h * k1e, i + 0.5 * h * k1i, u, k2e, k2i);
-  wc_deriv(p, e + 0.5 * h * k2e, i + 0.5 * h * k2i, u, k3e, k3i);
-  wc_deriv(p, e + h * k3e, i + h * k3i, u, k4e, k4i);
-  e = e + h / 6.0 * (k1e + 2.0 * k2e + 2.0 * k3e + k4e);
-  i = i + h / 6.0 * (k1i + 2.0 * k2i + 2.0 * k3i + k4i);
+// MACRO2MICRO: the Poisson rate (Hz) of a binding from its ROI's E and rate_scale.
+MIND_HD double drive_rate(double rate_scale, double e) {
+  using namespace gen_macro;
+  double f[PoissonDrive_NFIELDS] = {0.0};
+  f[PoissonDrive_e] = e;
+  f[PoissonDrive_scale] = rate_scale;
+  PoissonDrive_evaluate(Cols{f, 1}, 0, 0.0, 0.0);
+  return f[PoissonDrive_rate];
+}
+
+// MICRO2MACRO at a bin end: the population rate (Hz) of `count` spikes of
+// `sources` cells over `width` ms. PopRate's NET_RECEIVE adds one per spike; the
+// kernels add a macro step's spikes at once, the same exact integer sum.
+MIND_HD double bin_rate(double count, double sources, double width) {
+  using namespace gen_macro;
+  double f[PopRate_NFIELDS] = {0.0};
+  f[PopRate_count] = count;
+  f[PopRate_sources] = sources;
+  f[PopRate_width] = width;
+  PopRate_evaluate(Cols{f, 1}, 0, 0.0, 0.0);
+  return f[PopRate_rate];
 }
 
 struct MacroHost {

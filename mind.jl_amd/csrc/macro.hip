@@ -41,7 +41,7 @@ __device__ inline Stream input_stream(const Dev& d, const MacroDev& m, int64_t b
 
 __device__ inline double input_rate(const MacroDev& m, int64_t b, int64_t slot) {
   const int r = m.in_roi[b];
-  return m.wc[(int64_t)r * WC_NPARAMS + WC_RATE_SCALE] * m.hist[slot * m.n + r];
+  return drive_rate(m.wc[(int64_t)r * WC_NPARAMS + WC_RATE_SCALE], m.hist[slot * m.n + r]);
 }
 
 constexpr int kB = 1024;
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kB) k_macro_pre(Dev d, MacroDev m) {
       const double w = m.W[(int64_t)i * m.n + j];
       if (w == 0.0) continue;
       const int64_t sl = ((s - m.dsteps[(int64_t)i * m.n + j]) % m.cap + m.cap) % m.cap;
-      c = c + w * m.hist[sl * m.n + j];
+      c = c + edge_value(w, m.hist[sl * m.n + j]);
     }
     m.coupling[i] = c;
   }
@@ -187,14 +187,13 @@ __global__ void __launch_bounds__(kB) k_macro_post(Dev d, MacroDev m) {
     if ((s + 1) % m.bin_steps[r] == 0) {
       const double nsrc = (double)(m.out_off[r + 1] - m.out_off[r]);
       const double bin = (double)m.bin_steps[r] * m.dt_macro;
-      m.rate[r] = nsrc > 0.0 ? m.acc[r] / (nsrc * bin) * 1000.0 : 0.0;
+      m.rate[r] = bin_rate(m.acc[r], nsrc, bin);
       m.acc[r] = 0.0;
     }
     m.meanv[r] = mean_soma(d, m, r);
     const double* p = m.wc + (int64_t)r * WC_NPARAMS;
-    const double u = m.coupling[r] + p[WC_K_MICRO] * (m.rate[r] / p[WC_RATE_SCALE]);
     double e = m.E[r], i = m.I[r];
-    wc_rk4(p, e, i, u, m.dt_macro);
+    region_advance(p, e, i, m.coupling[r], m.rate[r], m.dt_macro);
     m.E[r] = e;
     m.I[r] = i;
     m.hist[((s + 1) % m.cap) * m.n + r] = e;
@@ -249,7 +248,7 @@ __global__ void __launch_bounds__(kB) k_macro_pre_bin(Dev d, MacroDev m, int32_t
         const double w = m.W[(int64_t)i * m.n + j];
         if (w == 0.0) continue;
         const int64_t sl = ((s - m.dsteps[(int64_t)i * m.n + j]) % m.cap + m.cap) % m.cap;
-        c = c + w * m.hist[sl * m.n + j];
+        c = c + edge_value(w, m.hist[sl * m.n + j]);
       }
       tC[(int64_t)mi * m.n + i] = c;
     }
@@ -312,9 +311,8 @@ __global__ void __launch_bounds__(kB) k_macro_pre_bin(Dev d, MacroDev m, int32_t
     if (mi + 1 < K) {  // the advance of a step that does not end the bin
       for (int r = tid; r < m.n; r += kB) {
         const double* p = m.wc + (int64_t)r * WC_NPARAMS;
-        const double u = tC[(int64_t)mi * m.n + r] + p[WC_K_MICRO] * (m.rate[r] / p[WC_RATE_SCALE]);
         double e = m.E[r], i = m.I[r];
-        wc_rk4(p, e, i, u, m.dt_macro);
+        region_advance(p, e, i, tC[(int64_t)mi * m.n + r], m.rate[r], m.dt_macro);
         m.E[r] = e;
         m.I[r] = i;
         m.hist[((s + 1) % m.cap) * m.n + r] = e;
@@ -360,7 +358,7 @@ __global__ void __launch_bounds__(kB) k_macro_post_bin(Dev d, MacroDev m, int32_
       if ((s + 1) % m.bin_steps[r] == 0) {
         const double nsrc = (double)(m.out_off[r + 1] - m.out_off[r]);
         const double bin = (double)m.bin_steps[r] * m.dt_macro;
-        m.rate[r] = nsrc > 0.0 ? m.acc[r] / (nsrc * bin) * 1000.0 : 0.0;
+        m.rate[r] = bin_rate(m.acc[r], nsrc, bin);
         m.acc[r] = 0.0;
       }
       double sum = 0.0;
@@ -369,9 +367,8 @@ __global__ void __launch_bounds__(kB) k_macro_post_bin(Dev d, MacroDev m, int32_
       m.meanv[r] = cnt > 0 ? sum / (double)cnt : 0.0;
       m.coupling[r] = tC[(int64_t)mi * m.n + r];
       if (mi + 1 == K) {
-        const double u = m.coupling[r] + p[WC_K_MICRO] * (m.rate[r] / p[WC_RATE_SCALE]);
         double e = m.E[r], i = m.I[r];
-        wc_rk4(p, e, i, u, m.dt_macro);
+        region_advance(p, e, i, m.coupling[r], m.rate[r], m.dt_macro);
         m.E[r] = e;
         m.I[r] = i;
         m.hist[((s + 1) % m.cap) * m.n + r] = e;

@@ -346,8 +346,17 @@ struct Ion {
   std::vector<std::string> reads, writes;
 };
 
+// A READ_SOURCE / READ_TARGET / WRITE_SOURCE / WRITE_TARGET binding of a
+// MIND-role mechanism: an ROI exposure and the mechanism variable it maps to.
+struct Binding {
+  std::string direction, exposure, variable;
+};
+
 struct Mechanism {
   std::string filename, source, kind, name;  // kind: suffix, point_process, artificial_cell
+  std::string role;                          // MIND block ROLE (lower case), empty for micro mechanisms
+  std::vector<std::string> exposures;
+  std::vector<Binding> bindings;
   std::vector<Ion> ions;
   std::vector<std::string> nonspecific, electrode, ranges, randoms;
   std::vector<Decl> decls;
@@ -410,6 +419,8 @@ class Parser {
         m_.defines[name] = strtod(v.text.c_str(), nullptr);
       } else if (k == "LOCAL") {
         names_list();
+      } else if (k == "MIND") {
+        mind_block();
       } else if (k == "INITIAL" || k == "BREAKPOINT" || k == "NET_RECEIVE" || k == "DERIVATIVE" ||
                  k == "PROCEDURE" || k == "FUNCTION") {
         block(k, t);
@@ -522,6 +533,40 @@ class Parser {
         fail(t, k + " is not supported by the device lowering");
       } else {
         fail(t, "unsupported NEURON statement " + quoted(k));
+      }
+    }
+  }
+
+  // MIND { ROLE r  EXPOSURE a, b  READ_SOURCE x AS y, ... } (frontend.jl:514-548)
+  void mind_block() {
+    expect("{");
+    auto keyword = [](const std::string& w) {
+      return w == "ROLE" || w == "EXPOSURE" || w == "READ_SOURCE" || w == "READ_TARGET" || w == "WRITE_SOURCE" ||
+             w == "WRITE_TARGET";
+    };
+    while (!accept("}")) {
+      const Tok t = take();
+      if (t.text == "ROLE") {
+        std::string role = ident();
+        for (char& ch : role)
+          if (ch >= 'A' && ch <= 'Z') ch = (char)(ch - 'A' + 'a');
+        if (role != "region" && role != "macro2macro" && role != "macro2micro" && role != "micro2macro")
+          fail(t, "unknown MIND role " + role);
+        m_.role = role;
+      } else if (t.text == "EXPOSURE") {
+        append(m_.exposures, names_list());
+      } else if (t.text == "READ_SOURCE" || t.text == "READ_TARGET" || t.text == "WRITE_SOURCE" ||
+                 t.text == "WRITE_TARGET") {
+        while (peek().kind == Tok::NAME && !keyword(peek().text)) {
+          Binding b;
+          b.direction = t.text;
+          b.exposure = ident();
+          b.variable = accept("AS") ? ident() : b.exposure;
+          m_.bindings.push_back(b);
+          accept(",");
+        }
+      } else {
+        fail(t, "unknown MIND declaration " + t.text);
       }
     }
   }
@@ -813,7 +858,10 @@ class Gen {
     routines.insert(routines.end(), procs_.begin(), procs_.end());
     for (const Block* b : routines) out.push_back(signature(*b) + ";");
     for (const Block* b : routines) routine(*b, out);
-    current(out);
+    if (m_.role.empty())
+      current(out);
+    else
+      evaluate(out);
     advance(out);
     initial(out);
     receive(out);
@@ -1313,6 +1361,22 @@ class Gen {
     out.push_back("}");
   }
 
+  // MIND-role mechanisms: BREAKPOINT without its SOLVE statements, run once on the
+  // fields (no voltage probes, no membrane current).
+  void evaluate(std::vector<std::string>& out) {
+    out.push_back("MIND_HD void " + p_ + "_evaluate(const Cols& c, int64_t r, double t, double dt) {");
+    out.push_back("  GenCtx x = gen_ctx(0.0, t, dt);");
+    out.push_back("  (void)x;");
+    const Block* bp = m_.block("breakpoint", "breakpoint");
+    if (bp) {
+      std::vector<S> body;
+      for (const S& s : bp->body)
+        if (s->kind != Stmt::SOLVE) body.push_back(s);
+      stmts(body, base_scope(), out, "  ");
+    }
+    out.push_back("}");
+  }
+
   void advance(std::vector<std::string>& out) {
     out.push_back("MIND_HD void " + p_ + "_advance(const Cols& c, int64_t r, double v, double t, double dt, "
                   "double celsius) {");
@@ -1523,11 +1587,33 @@ std::string builtin_header(const std::vector<std::pair<std::string, std::string>
 
 std::string user_prelude() { return kGenHelpers; }
 
+std::string macro_header(const std::vector<std::pair<std::string, std::string>>& sources) {
+  std::vector<Mechanism> mechs;
+  for (const auto& s : sources) mechs.push_back(parse(s.second, s.first));
+  std::set<std::string> names;
+  std::vector<std::string> srcs;
+  for (const Mechanism& m : mechs) {
+    if (m.role.empty()) raise_at(m.filename, 1, "no MIND ROLE: " + m.name + " is not a macro-step mechanism");
+    if (!names.insert(m.name).second) raise_at("<input>", 1, "duplicate mechanism names");
+    srcs.push_back(basename_of(m.filename) + " (" + m.role + ", sha256 " + hex16(m.digest()) + ")");
+  }
+  std::vector<std::string> parts;
+  parts.push_back(std::string("// Generated by nmodlc from MIND-role MOD sources; do not edit.\n#pragma once\n\n"
+                              "#include \"mech.h\"\n\nnamespace mind {\nnamespace gen_macro {\n\n") +
+                  kGenHelpers);
+  for (const Mechanism& m : mechs) parts.push_back(Gen(m, m.name).emit());
+  parts.push_back("constexpr const char* kSource = \"NMODL lowering of " + join(srcs, ", ") + "\";");
+  parts.push_back("\n}  // namespace gen_macro\n}  // namespace mind\n");
+  return join(parts, "\n");
+}
+
 UserMechanism lower_user(const std::string& filename, const std::string& text, const std::string& prefix) {
   const Mechanism m = parse(text, filename);
   auto refuse = [&](const std::string& msg) { raise_at(filename, 1, msg); };
   if (m.kind == "artificial_cell")
     refuse("ARTIFICIAL_CELL " + m.name + ": runtime mechanisms are membrane mechanisms (SUFFIX or POINT_PROCESS)");
+  if (!m.role.empty())
+    refuse("MIND ROLE " + m.role + ": the macro step's role mechanisms are lowered into its kernels at library build");
   if (!m.randoms.empty()) refuse("RANDOM variables are supported for the builtin NetStim only");
   for (const Block& b : m.blocks)
     if (calls(b.body, "net_send") || calls(b.body, "net_event"))

@@ -68,6 +68,13 @@ struct UserMechanism {
 // ion concentrations or reversal potentials, reads of ion currents.
 UserMechanism lower_user(const std::string& filename, const std::string& text, const std::string& prefix);
 
+// The MIND-role mechanisms of the macro step (a MIND block with ROLE REGION,
+// MACRO2MACRO, MACRO2MICRO or MICRO2MACRO) lowered to the macro_gen.h header of
+// libmind, namespace mind::gen_macro: per mechanism its column enum, defaults,
+// routines, <name>_evaluate (BREAKPOINT without SOLVE, once), _advance (the
+// SOLVE statements), _initial and _receive.
+std::string macro_header(const std::vector<std::pair<std::string, std::string>>& sources);
+
 // Definitions the lowered phases of runtime slots use (GenCtx, gen_ctx,
 // gen_index), for namespace mind::user.
 std::string user_prelude();

@@ -114,10 +114,10 @@ __global__ void __launch_bounds__(kThreads) k_surrogate_run(SurDev s, int64_t co
         const double w = s.W[(int64_t)i * n + q];
         if (w == 0.0) continue;
         const int64_t sl = ((ms - s.dsteps[(int64_t)i * n + q]) % s.cap + s.cap) % s.cap;
-        c = c + w * s.hist[sl * n + q];
+        c = c + edge_value(w, s.hist[sl * n + q]);
       }
       s.coupling[i] = c;
-      s.acc[i] = s.acc[i] + s.wc[(int64_t)i * WC_NPARAMS + WC_RATE_SCALE] * s.hist[slot * n + i];
+      s.acc[i] = s.acc[i] + drive_rate(s.wc[(int64_t)i * WC_NPARAMS + WC_RATE_SCALE], s.hist[slot * n + i]);
     }
     if ((ms + 1) % s.bin_steps == 0) {
       for (int i = tid; i < n; i += kThreads) {
@@ -181,9 +181,8 @@ __global__ void __launch_bounds__(kThreads) k_surrogate_run(SurDev s, int64_t co
     }
     for (int i = tid; i < n; i += kThreads) {
       const double* p = s.wc + (int64_t)i * WC_NPARAMS;
-      const double u = s.coupling[i] + p[WC_K_MICRO] * (s.rate[i] / p[WC_RATE_SCALE]);
       double e = s.E[i], ii = s.I[i];
-      wc_rk4(p, e, ii, u, s.dt_macro);
+      region_advance(p, e, ii, s.coupling[i], s.rate[i], s.dt_macro);
       s.E[i] = e;
       s.I[i] = ii;
       s.hist[((ms + 1) % s.cap) * n + i] = e;

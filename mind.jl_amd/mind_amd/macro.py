@@ -3,7 +3,9 @@
 Mirrors Connectivity / MacroModel / roi / use_macro / use_micro / connect /
 micro2macro / macro2micro (src/model/macro/model.jl:4-162,
 src/model/build.jl:230-295) and run for macro models (src/runtime/step.jl:4-74,
-156-204). The MIND-role mechanisms are builtins of this project:
+156-204). The MIND-role mechanisms are MOD files of this project
+(mind.jl_amd/mod/*.mod, MIND blocks with ROLE REGION / MACRO2MACRO / MACRO2MICRO /
+MICRO2MACRO), lowered into the macro kernels by the library's NMODL frontend:
 
   WilsonCowan   REGION: E/I populations, tau dE/dt = -E + S_E(c_EE E - c_EI I + P + u),
                 tau_I dI/dt = -I + S_I(c_IE E - c_II I + Q), S(x) = 1/(1+exp(-a(x-theta)))
