@@ -28,6 +28,14 @@ struct InjSpike {
 };
 constexpr int kInjCap = 1024;  // injected spikes handed to one window, events per staging copy
 
+// A NetStim self-event or injected source spike waiting in the window's heap
+// (row < 0: injected spike -row - 1), ordered by (time, insertion order).
+struct HeapItem {
+  double t;
+  uint64_t seq;
+  int32_t row;
+};
+
 struct Clock {
   int64_t step;
   uint64_t ticket;   // routed events so far
@@ -124,6 +132,7 @@ struct Dev {
   int32_t* ns_edge;
   int32_t* ns_valid;
   const int32_t* ns_source;
+  HeapItem* ns_heap;  // ns_rows + kInjCap entries: the window's heap when LDS cannot hold it
   // delivery ring
   int64_t ring_slots, ring_cap;
   QEvent* ring;

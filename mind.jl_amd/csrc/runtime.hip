@@ -112,8 +112,7 @@ __device__ inline int64_t block_scan(int64_t x, int64_t* lds, int64_t* total) {
 }
 
 // ------------------------------------------------------------ exchange
-struct HeapItem { double t; uint64_t seq; int32_t row; };
-constexpr int kHeapCap = 2048;
+constexpr int kHeapCap = 2048;  // NetStim and injected events of a window held in LDS; more use Dev::ns_heap
 
 __device__ inline bool heap_less(const HeapItem& a, const HeapItem& b) {
   return a.t < b.t || (a.t == b.t && a.seq < b.seq);
@@ -162,26 +161,31 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
   const int32_t ninj = *d.inj_count;
   if (tid == 0) { heap_n = 0; n_ns = 0; }
   __syncthreads();
+  // every due row and injection gets a slot in HBM (one pending event per row,
+  // at most kInjCap injections, so ns_heap never overflows); the first kHeapCap
+  // also in LDS, used when all of them fit
   for (int64_t r = tid; r < d.ns_rows; r += kBlock) {
     if (d.ns_valid[r] && d.ns_t[r] <= cut_hi) {
       const int slot = atomicAdd(&heap_n, 1);
-      if (slot < kHeapCap) heap[slot] = HeapItem{d.ns_t[r], d.ns_seq[r], (int32_t)r};
+      const HeapItem it{d.ns_t[r], d.ns_seq[r], (int32_t)r};
+      if (slot < kHeapCap) heap[slot] = it;
+      d.ns_heap[slot] = it;
     }
   }
   for (int k = tid; k < ninj && k < kInjCap; k += kBlock) {
     const int slot = atomicAdd(&heap_n, 1);
-    if (slot < kHeapCap) heap[slot] = HeapItem{d.inj[k].t, d.inj[k].seq, -(k + 1)};
+    const HeapItem it{d.inj[k].t, d.inj[k].seq, -(k + 1)};
+    if (slot < kHeapCap) heap[slot] = it;
+    d.ns_heap[slot] = it;
   }
   __syncthreads();
   if (tid == 0 && (d.ns_rows > 0 || ninj > 0)) {
-    int hn = heap_n;
-    if (hn > kHeapCap) {
-      raise_error(d.error, MIND_EUNSUPPORTED);
-      hn = 0;
-    }
-    // heapify
+    const int hn = heap_n;
+    // a popped row pushes at most its own next event, so the heap never grows
+    // beyond its initial size
+    HeapItem* hp = hn <= kHeapCap ? heap : d.ns_heap;
     int m = 0;
-    for (int i = 0; i < hn; ++i) heap_push(heap, m, heap[i]);
+    for (int i = 0; i < hn; ++i) heap_push(hp, m, hp[i]);
     const Cols col = d.ns_group >= 0 ? cols(d.g[d.ns_group]) : Cols{nullptr, 0};
     int64_t count = 0;
     uint64_t seq = d.clock->ns_seq;
@@ -189,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
     // calls <= 2k (the record precedes the end-of-step delivery)
     int64_t sampled = n > 0 ? k0 : INT64_MAX;
     while (m > 0) {
-      const HeapItem it = heap_pop(heap, m);
+      const HeapItem it = heap_pop(hp, m);
       const int32_t r = it.row;
       const double t = it.t;
       if (d.n_art_rec) {
@@ -227,10 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
         d.ns_seq[r] = ++seq;
         d.ns_edge[r] = edge;
         d.ns_valid[r] = 1;
-        if (tn <= cut_hi) {
-          if (m < kHeapCap) heap_push(heap, m, HeapItem{tn, seq, r});
-          else raise_error(d.error, MIND_EUNSUPPORTED);
-        }
+        if (tn <= cut_hi) heap_push(hp, m, HeapItem{tn, seq, r});
       }
     }
     d.clock->ns_seq = seq;
@@ -569,6 +570,7 @@ int setup_state(State& S, const mind_build_options& o) {
   d.ns_rows = d.ns_group >= 0 ? L.groups[d.ns_group].rows : 0;
   if ((rc = S.alloc(&d.ns_stream, d.ns_rows))) return rc;
   if ((rc = S.alloc(&d.ns_t, d.ns_rows))) return rc;
+  if ((rc = S.alloc(&d.ns_heap, d.ns_rows + kInjCap))) return rc;
   if ((rc = S.alloc(&d.ns_flag, d.ns_rows))) return rc;
   if ((rc = S.alloc(&d.ns_seq, d.ns_rows))) return rc;
   if ((rc = S.alloc(&d.ns_edge, d.ns_rows))) return rc;
@@ -644,7 +646,7 @@ int check_device_error(State& S) {
     case MIND_ENONFINITE: return fail(code, "non-finite numerical state");
     case MIND_EQUEUE: return fail(code, "event ring, window list or spike buffer capacity exceeded");
     case MIND_EUNSUPPORTED:
-      return fail(code, "an event fell due inside the window that produced it, or too many NetStims were due in one window");
+      return fail(code, "an event fell due inside the window that produced it");
     default: return fail(code, "device error " + std::to_string(code));
   }
 }

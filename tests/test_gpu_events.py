@@ -142,3 +142,44 @@ def test_injection_and_checkpoint_on_event_exact_path(gpu, method):
     for r in recs:
         assert np.array_equal(again.traces[r.name], rest.traces[r.name]), r.name
     destroy(m2)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_more_netstims_due_in_a_window_than_lds_holds(gpu, method):
+    """Window path (delays of 1 ms): 3000 NetStims fall due together at 1 ms and every
+    2 ms after, half of them with noise, more than the 2048 events the exchange kernel
+    orders in LDS, so it orders them in its HBM heap. Rasters, voltages, synapse
+    states and a NetStim record bit for bit."""
+    m = MicroModel()
+    cells = m.add_population(configs.ball_and_stick_sections(10), 20)
+    syns, stims = [], []
+    for c in cells:
+        m.insert(c, builtin("hh"), sections="soma")
+        m.insert(c, builtin("pas"), sections="dend", params={"g": 1e-4, "e": -65.0})
+        syns.append(m.add_pointprocess(c, "Exp2Syn", section="dend", x=0.5, params={"tau1": 0.5, "tau2": 5.0,
+                                                                                  "e": 0.0}))
+    for k in range(3000):
+        ns = m.add_pointprocess(m, "NetStim", params={"interval": 2.0, "number": 1e9, "start": 1.0,
+                                                      "noise": 0.5 * (k % 2)})
+        m.connect(ns, syns[k % len(syns)], weight=2e-4, delay=1.0)
+        stims.append(ns)
+    recs = [voltage_record(m, cells[k], "soma", 0.5, name=f"v{k}") for k in range(2)]
+    recs.append(mechanism_record(m, stims[1], "ispike", name="ispike"))
+    print(f"case: 3000 NetStims due in one window, {method}", flush=True)
+    build(m, dt_micro=DT, seed=3, method=method)
+    initialize(m)
+    got = run(m, 12.0, recs)
+    v_got, a_got = read_voltage(m), read_field(m, "Exp2Syn", "A")
+    destroy(m)
+    sim = oracle.OracleSim(m, seed=3).initialize()
+    ref = sim.run(12.0, recs)
+    exp2 = builtin("Exp2Syn")
+    assert np.array_equal(got.spike_times, ref.spike_times)
+    assert np.array_equal(got.spike_sources, ref.spike_sources)
+    assert np.array_equal(v_got, sim.voltage())
+    assert np.array_equal(a_got, sim.field(exp2.id, exp2.field_index("A")))
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    first = m.register_spike_source(stims[0]).sid  # the source its connection made
+    assert np.count_nonzero(got.spike_sources == first) == 6  # noiseless: 1, 3, ..., 11 ms
+    assert len(got.spike_times) > 3000 * 5
