@@ -150,6 +150,31 @@ typedef struct mind_model_desc {
    * (reset_events!, src/runtime/utils/events.jl:92-110). */
   const int64_t* pt_spike_offsets;
   const double* pt_spike_times;
+
+  /* Ion species (MicroModel(; ions) with IonSpecies, src/model/micro/biophysics.jl:3-25):
+   * n_species entries, or 0 for default_ions (na, k, ca; src/model/micro/utils/biophysics.jl:
+   * 31-42). A species a mechanism uses must be listed (with the valence its USEION VALENCE
+   * declares, if any). Ion rows exist per (compartment, species) where a mechanism uses the
+   * species or a setting below names it; each holds the reversal potential, the internal
+   * and external concentrations, the current and its voltage derivative (model.jl:710-855). */
+  int32_t n_species;
+  const char* const* species_name;
+  const double* species_valence;
+  const double* species_internal;  /* mM */
+  const double* species_external;  /* mM */
+  const double* species_reversal;  /* mV */
+  const int32_t* species_nernst;   /* 1: reversal from the Nernst equation */
+  /* Section ion settings (add_population!(...; ions), utils/model.jl:280-320), applied in
+   * order on every segment of (cell, section): what 0, 1, 2 set the internal concentration,
+   * external concentration or reversal of species `species` (index into the species list)
+   * to value; 3 sets its reversal policy (value 0 automatic, 1 fixed, 2 initial, 3 advance,
+   * 4 dynamic); 4 sets its ion style bits (IonStyle encoding, ion_style_bits). */
+  int64_t n_ion_settings;
+  const int32_t* ionset_cell;
+  const int32_t* ionset_section;
+  const int32_t* ionset_species;
+  const int32_t* ionset_what;
+  const double* ionset_value;
 } mind_model_desc;
 
 /* Kernel families for the micro step. */
@@ -169,18 +194,20 @@ typedef struct mind_build_options {
   int32_t device;         /* HIP device ordinal */
   int32_t secondorder;    /* reference build! keyword (build.jl:120-131): 0 backward Euler;
                              1 and 2 Crank-Nicolson, the cable assembled with C/(dt/2) and
-                             v += 2 dv (step.jl:112, cuda/micro.jl:660-679). 2 also corrects
-                             the per-node ion current columns, which none of the ten
-                             mechanisms reads (fixed ena/ek, no concentration dynamics),
-                             so 2 behaves as 1. */
+                             v += 2 dv (step.jl:112, cuda/micro.jl:660-679); 2 also adds
+                             di/dv * dv to the ion currents before nrn_state. */
 } mind_build_options;
 
 typedef struct mind_state mind_state;
 
-/* Record selectors (reference voltage_record / mechanism_record, recording.jl:4-76).
+/* Record selectors (reference voltage_record / mechanism_record / ion_record,
+ * recording.jl:4-76).
  * kind 0: membrane voltage at (cell, section, x);
  * kind 1: field of point process `point`;
- * kind 2: field of density mechanism `mech` at (cell, section, x). */
+ * kind 2: field of density mechanism `mech` at (cell, section, x);
+ * kind 3: ion column `field` (MIND_ION_*) of species `mech` (index into the species list)
+ *         at (cell, section, x) (ion_ref, references.jl:156-178). */
+enum { MIND_ION_E = 0, MIND_ION_CI = 1, MIND_ION_CO = 2, MIND_ION_I = 3, MIND_ION_DIDV = 4 };
 typedef struct mind_record {
   int32_t kind;
   int32_t cell;
@@ -252,9 +279,9 @@ typedef struct mind_macro_result {
  * mind_build (once per set of compiled mechanisms and device, then reused).
  * Runtime mechanisms are membrane mechanisms: SUFFIX density mechanisms and
  * POINT_PROCESSes whose NET_RECEIVE (connection weight only) changes their own
- * fields; na, k and ca ions with fixed reversal potentials and concentrations
- * (default_ions, src/model/micro/utils/biophysics.jl:31-42); no RANDOM,
- * net_send or net_event (those are the builtin NetStim's). */
+ * fields, with USEION reads and writes of reversal potentials, concentrations and
+ * currents through the model's ion rows; no RANDOM, net_send or net_event (those
+ * are the builtin NetStim's). */
 enum { MIND_KIND_DENSITY = 0, MIND_KIND_POINT = 1, MIND_KIND_ARTIFICIAL = 2 };
 typedef struct mind_mech_desc {
   const char* name;      /* NEURON block name (owned by the library) */
@@ -279,6 +306,16 @@ int mind_mech_fields(int32_t mech, const char** names, double* defaults, int32_t
 /* builtin(name) / a compiled mechanism by name: the most recently compiled one of
  * that name, else the builtin. */
 int mind_mech_lookup(const char* name, int32_t* mech);
+/* USEION statement `index` of a mechanism (declaration order; hh: na, k): the ion name
+ * (library-owned), its VALENCE (*has_valence 0 when none), and per ion column MIND_ION_E,
+ * _CI, _CO, _I the field column it reads into / writes from (-1 none); reads include
+ * written concentrations. *count is the number of statements. */
+int mind_mech_ions(int32_t mech, int32_t index, const char** name, int32_t* has_valence, double* valence,
+                   int32_t read[4], int32_t write[4], int32_t* count);
+/* Per column of a compiled mechanism: whether it is a STATE variable and whether its
+ * NET_RECEIVE block touches it (0 for the builtins), up to `capacity`; *count is the
+ * number of columns. */
+int mind_mech_columns(int32_t mech, uint8_t* is_state, uint8_t* in_receive, int32_t capacity, int32_t* count);
 /* C++ text of a compiled mechanism's phases (namespace mind::user, functions
  * u<id>_current/_advance/_initial/_receive over Cols), as hipRTC compiles them;
  * for host builds of the same lowering. */

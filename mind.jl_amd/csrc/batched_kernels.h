@@ -109,8 +109,7 @@ __device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_e
   while (seg < seg_end && d.win[seg].call == call) {
     const int32_t e = d.win[seg].edge;
     const int32_t target = d.edge_target[e];
-    const GroupDev& G = d.g[d.point_group[target]];
-    phase_receive(G.mech, cols(G), d.point_row[target], d.win[seg].t, d.edge_weight[e]);
+    receive_event(d, d.g[d.point_group[target]], d.point_row[target], d.win[seg].t, d.edge_weight[e]);
     ++seg;
   }
   return seg;
@@ -515,7 +514,7 @@ __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, 
       if (r + (int64_t)u * T < r1) {
         const int li = nd[u] - (int)n0;
         double value, shifted;
-        Ops::eval(col, rr[u], in[u], v[li], tc, d.dt, value, shifted, store);
+        Ops::eval(col, rr[u], in[u], v[li], tc, d.dt, value, shifted, store, d.celsius);
         const double scale = area[u] * 0.01;
         double acc_r = rhs[li], acc_d = diag[li];
         acc_r -= scale * value;
@@ -644,6 +643,8 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       rhs[i] = 0.0;
       diag[i] = 0.0;
     }
+    if (d.nion)
+      for (int64_t q = d.node_ion_off[n0] + tid; q < d.node_ion_off[n1]; q += T) ion_prepare(d, q, 0x40);
     __syncthreads();
     // nrn_cur at t + dt/2, one mechanism group after another. The assigned
     // outputs of a current phase that no other phase reads (i, g of hh, pas,
@@ -658,12 +659,12 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       const int64_t r0 = sr0[gi], r1 = sr1[gi];
       if (r0 == r1) continue;
       const Cols col = cols(G);
-      if (G.mech == MIND_MECH_PAS && !G.point) {
+      if (G.mech == MIND_MECH_PAS && !G.point && !G.nslot) {
         density_currents<MIND_MECH_PAS, 4>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
         __syncthreads();
         continue;
       }
-      if (G.mech == MIND_MECH_HH && !G.point) {
+      if (G.mech == MIND_MECH_HH && !G.point && !G.nslot) {
         density_currents<MIND_MECH_HH, 1>(d, col, G.node, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
         __syncthreads();
         continue;
@@ -676,8 +677,11 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         int64_t q = r;
         do {  // a run of rows on one node, in row order
           double value, shifted;
-          phase_current(G.mech, col, q, v[li], tc, dt, value, shifted,
-                        store_outputs || !mech_outputs_only(G.mech));
+          if (G.nslot)
+            ion_current(d, G, col, q, v[li], tc, d.area[node], value, shifted);
+          else
+            phase_current(G.mech, col, q, v[li], tc, dt, value, shifted, store_outputs || !mech_outputs_only(G.mech),
+                          d.celsius, nullptr);
           const double scale = G.point ? 1.0 : d.area[node] * 0.01;
           acc_r -= scale * value;
           acc_d += scale * (shifted - value) / 0.001;
@@ -735,6 +739,28 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       solve_waves<M>(d, b, n0, nn, sched, hn, oh, od, ax, rhs, diag, v);
     }
     __syncthreads();
+    if (d.ion_correct && d.nion) {  // secondorder 2: ion currents += di/dv * dv
+      for (int i = tid; i < nn; i += T) ion_correct_node(d, n0 + i, rhs[i]);
+      __syncthreads();
+    }
+    if (d.ion_ordered) {
+      // nrn_state at t + dt group by group, as the reference's launches run: a
+      // mechanism reads what an earlier one wrote to the ion rows; rows sharing a
+      // node run in row order
+      for (int gi = 0; gi < ngr; ++gi) {
+        const GroupDev& G = sg[gi];
+        if (!stateful(G.mech)) continue;
+        for (int64_t r = sr0[gi] + tid; r < sr1[gi]; r += T) {
+          if (G.point && !G.run_start[r]) continue;
+          int64_t q = r;
+          do {
+            advance_row(d, G, q, v[G.node[q] - n0], t + dt);
+            ++q;
+          } while (G.point && q < sr1[gi] && G.node[q] == G.node[r]);
+        }
+        __syncthreads();
+      }
+    } else {
     // nrn_state at t + dt: the rows of all stateful groups as one index space
     // (rows are independent), so that the groups' rows run side by side
     {
@@ -752,18 +778,16 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         }
         const GroupDev& G = sg[gi];
         r += sr0[gi];
-        phase_advance(G.mech, cols(G), r, v[G.node[r] - n0], t + dt, dt, d.celsius);
+        advance_row(d, G, r, v[G.node[r] - n0], t + dt);
       }
+    }
     }
     if (rec1 > rec0) {
       __syncthreads();
       const int64_t sample = k + 1 - d.rec_start;
       for (int64_t q = rec0 + tid; q < rec1; q += T) {
         const int32_t rec = d.cell_rec[q];
-        const int64_t idx = d.rec_index[rec];
-        const double x = d.rec_kind[rec] == 0
-                             ? v[idx - n0]
-                             : d.g[d.rec_group[rec]].data[d.rec_field[rec] * d.g[d.rec_group[rec]].rows + idx];
+        const double x = d.rec_kind[rec] == 0 ? v[d.rec_index[rec] - n0] : stored_value(d, rec);
         d.rec_buf[rec * d.rec_stride + sample] = x;
       }
     }

@@ -53,6 +53,11 @@ struct GroupDev {
   const int32_t* node;
   const int64_t* cell_rows;
   const uint8_t* run_start;
+  // ion transfers: ion_row[r * nslot + s] (-1 none); ion_bind: nread loads, nwrite
+  // stores, nrecv NET_RECEIVE exchanges
+  const int32_t* ion_row;
+  const IonBind* ion_bind;
+  int16_t nslot, nread, nwrite, nrecv;
 };
 
 struct Emit {  // one spike emission: source index and spike time
@@ -158,10 +163,10 @@ struct Dev {
   int32_t record_spikes;
   // value recording
   int64_t nrec;
-  const int32_t* rec_kind;   // 0 voltage, 1 field
+  const int32_t* rec_kind;   // 0 voltage, 1 and 2 field, 3 ion column
   const int32_t* rec_group;
   const int32_t* rec_field;
-  const int64_t* rec_index;  // node or row
+  const int64_t* rec_index;  // node, row or ion row
   const int64_t* cell_rec_off;
   const int32_t* cell_rec;
   double* rec_buf;
@@ -178,6 +183,17 @@ struct Dev {
   const int32_t* det_snap;
   int32_t snap_every;
   int64_t n_snap;
+  // ion rows (node, species), sorted by node: columns MIND_ION_* of row q at
+  // ion[column * nion + q]
+  int64_t nion;
+  double* ion;
+  const uint8_t* ion_style;
+  const int32_t* ion_species;
+  const double* ion_valence;     // per species
+  const double* ion_conc0;       // per species: internal, external (cinit)
+  const int64_t* node_ion_off;   // nodes + 1
+  int32_t ion_ordered;           // a mechanism writes a reversal or concentration
+  int32_t ion_correct;           // secondorder 2: currents += di/dv * dv
   // injected source spikes due in the next window, and their count (the window's
   // exchange consumes them and clears the count)
   InjSpike* inj;
@@ -209,6 +225,102 @@ __host__ __device__ inline int64_t call_of(double t, int64_t jmin, double dt) {
 __device__ inline int64_t emit_key(int64_t j) { return j < 0 ? -1 : 3 * (j / 2) + 1 + (j & 1); }
 
 __device__ inline Cols cols(const GroupDev& g) { return Cols{g.data, g.rows}; }
+
+// ------------------------------------------------------------------ ions
+// prepare_currents_kernel! (src/runtime/cuda/micro.jl:601-629) for ion row q:
+// clears the current and di/dv; the Nernst reversal where style & mask (0x40
+// every step, 0x20 at initialization).
+__device__ inline void ion_prepare(const Dev& d, int64_t q, uint8_t mask) {
+  d.ion[MIND_ION_I * d.nion + q] = 0.0;
+  d.ion[MIND_ION_DIDV * d.nion + q] = 0.0;
+  if (d.ion_style[q] & mask)
+    d.ion[MIND_ION_E * d.nion + q] = mind_nernst(d.ion[MIND_ION_CI * d.nion + q], d.ion[MIND_ION_CO * d.nion + q],
+                                                 d.ion_valence[d.ion_species[q]], d.celsius);
+}
+
+// load_ions! / store_ions! (cuda/micro.jl:376-392): the mechanism fields read from
+// the row's ion rows; the fields written back, the current column excepted.
+__device__ inline void ion_load(const Dev& d, const GroupDev& G, const Cols& c, int64_t r) {
+  for (int k = 0; k < G.nread; ++k) {
+    const IonBind b = G.ion_bind[k];
+    const int32_t q = G.ion_row[r * G.nslot + b.slot];
+    if (q >= 0) c(b.field, r) = d.ion[b.column * d.nion + q];
+  }
+}
+__device__ inline void ion_store(const Dev& d, const GroupDev& G, const Cols& c, int64_t r) {
+  for (int k = G.nread; k < G.nread + G.nwrite; ++k) {
+    const IonBind b = G.ion_bind[k];
+    const int32_t q = G.ion_row[r * G.nslot + b.slot];
+    if (q >= 0 && b.column != MIND_ION_I) d.ion[b.column * d.nion + q] = c(b.field, r);
+  }
+}
+
+// nrn_cur of a row with ion transfers (mechanism_current_instance!, cuda/micro.jl:
+// 427-500): loads, both probes (fields stored), then per written ion current
+// i += s * i_row and di/dv += s * (i(v + 0.001) - i_row) / 0.001 with s = 1 for
+// density rows and 100 / area for point processes; other writes stored.
+__device__ inline void ion_current(const Dev& d, const GroupDev& G, const Cols& c, int64_t r, double v, double t,
+                                   double area, double& value, double& shifted) {
+  ion_load(d, G, c, r);
+  double ish[kMaxIonCurrents];
+  phase_current(G.mech, c, r, v, t, d.dt, value, shifted, true, d.celsius, ish);
+  for (int k = G.nread; k < G.nread + G.nwrite; ++k) {
+    const IonBind b = G.ion_bind[k];
+    const int32_t q = G.ion_row[r * G.nslot + b.slot];
+    if (q < 0) continue;
+    if (b.column == MIND_ION_I) {
+      const double s = G.point ? 100.0 / area : 1.0;
+      const double x = c(b.field, r);
+      d.ion[MIND_ION_I * d.nion + q] += s * x;
+      d.ion[MIND_ION_DIDV * d.nion + q] += s * (ish[b.aux] - x) / 0.001;
+    } else {
+      d.ion[b.column * d.nion + q] = c(b.field, r);
+    }
+  }
+}
+
+// nrn_state of a row: loads, SOLVE statements, stores.
+__device__ inline void advance_row(const Dev& d, const GroupDev& G, int64_t r, double v, double t) {
+  const Cols c = cols(G);
+  if (G.nslot) ion_load(d, G, c, r);
+  phase_advance(G.mech, c, r, v, t, d.dt, d.celsius);
+  if (G.nslot) ion_store(d, G, c, r);
+}
+
+// NET_RECEIVE of a connection event: the concentrations NET_RECEIVE mentions are
+// imported first and the STATE ones published after (receive_ions!,
+// cuda/micro.jl:324-337).
+__device__ inline void receive_event(const Dev& d, const GroupDev& G, int64_t r, double t, double weight) {
+  const Cols c = cols(G);
+  const int k0 = G.nread + G.nwrite, k1 = k0 + G.nrecv;
+  for (int k = k0; k < k1; ++k) {
+    const IonBind b = G.ion_bind[k];
+    const int32_t q = G.ion_row[r * G.nslot + b.slot];
+    if (q >= 0) c(b.field, r) = d.ion[b.column * d.nion + q];
+  }
+  phase_receive(G.mech, c, r, t, weight);
+  for (int k = k0; k < k1; ++k) {
+    const IonBind b = G.ion_bind[k];
+    const int32_t q = G.ion_row[r * G.nslot + b.slot];
+    if (q >= 0 && b.aux) d.ion[b.column * d.nion + q] = c(b.field, r);
+  }
+}
+
+// update_voltage_kernel!'s secondorder-2 correction (cuda/micro.jl:660-679): the
+// ion currents of node n move by di/dv times the solved increment dv.
+__device__ inline void ion_correct_node(const Dev& d, int64_t n, double dv) {
+  for (int64_t q = d.node_ion_off[n]; q < d.node_ion_off[n + 1]; ++q)
+    d.ion[MIND_ION_I * d.nion + q] += d.ion[MIND_ION_DIDV * d.nion + q] * dv;
+}
+
+// A recorded value other than a voltage: a mechanism field (kinds 1, 2) or an
+// ion column (3).
+__device__ inline double stored_value(const Dev& d, int32_t rec) {
+  const int64_t idx = d.rec_index[rec];
+  if (d.rec_kind[rec] == 3) return d.ion[d.rec_field[rec] * d.nion + idx];
+  const GroupDev& G = d.g[d.rec_group[rec]];
+  return G.data[d.rec_field[rec] * G.rows + idx];
+}
 
 // Samples the artificial-cell records for step k (sample k + 1 of the run).
 __device__ inline void record_artificial(const Dev& d, int64_t k) {

@@ -14,7 +14,9 @@
 #include <math.h>
 
 #include <algorithm>
+#include <map>
 #include <numeric>
+#include <tuple>
 
 namespace mind {
 
@@ -59,6 +61,14 @@ void field_defaults(int mech, double* f) {
     default: break;
   }
 }
+
+struct IonParam {  // a mechanism parameter (group, row, field, value), maybe an ion variable
+  int group;
+  int64_t row;
+  int field;
+  double value;
+};
+std::string pack_ions(const mind_model_desc& d, const std::vector<IonParam>& params, Layout& L);
 
 namespace {
 
@@ -219,6 +229,256 @@ std::string build_template(const mind_model_desc& d, int m, Template& t) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- ions ----
+// IonStyle bits (ion_style_bits, src/model/micro/utils/biophysics.jl:3-4): c_style
+// (bits 0-1), cinit 0x04, e_style (bits 3-4), einit 0x20, eadvance 0x40.
+uint8_t ion_style_bits(int c, int e, bool einit, bool eadvance, bool cinit) {
+  return (uint8_t)(c + 4 * (cinit ? 1 : 0) + 8 * e + 32 * (einit ? 1 : 0) + 64 * (eadvance ? 1 : 0));
+}
+// promote_ion_style (utils/biophysics.jl:9-22)
+uint8_t promote_ion_style(uint8_t style, int concentration, int reversal) {
+  concentration = std::max((int)(style & 0x03), concentration);
+  reversal = std::max((int)((style >> 3) & 0x03), reversal);
+  if (concentration > 0 && reversal < 2) reversal = 2;
+  return ion_style_bits(concentration, reversal, concentration > 0 && reversal == 2,
+                        concentration == 3 && reversal == 2, concentration == 3);
+}
+
+int64_t Layout::ion_row_at(int64_t node, int32_t sp) const {
+  if (node < 0 || node >= nodes) return -1;
+  for (int64_t q = node_ion_off[node]; q < node_ion_off[node + 1]; ++q)
+    if (ion_species[q] == sp) return q;
+  return -1;
+}
+
+// Ion species, rows, styles and initial values, and each group's transfers
+// (src/model/micro/model.jl:672-855, src/model/utils/build.jl:111-135).
+std::string pack_ions(const mind_model_desc& d, const std::vector<IonParam>& params, Layout& L) {
+  std::vector<Species>& sp = L.species;
+  if (d.n_species == 0) {  // default_ions (utils/biophysics.jl:31-42)
+    sp = {{"na", 1.0, 10.0, 140.0, 50.0, false},
+          {"k", 1.0, 54.4, 2.5, -77.0, false},
+          {"ca", 2.0, 5.0e-5, 2.0, 132.4579341637009, false}};
+  } else {
+    if (!d.species_name || !d.species_valence || !d.species_internal || !d.species_external ||
+        !d.species_reversal || !d.species_nernst)
+      return "ion species arrays missing";
+    for (int32_t k = 0; k < d.n_species; ++k) {
+      Species s;
+      s.name = d.species_name[k] ? d.species_name[k] : "";
+      s.valence = d.species_valence[k];
+      s.internal = d.species_internal[k];
+      s.external = d.species_external[k];
+      s.reversal = d.species_reversal[k];
+      s.nernst = d.species_nernst[k] != 0;
+      if (s.name.empty()) return "ion species without a name";
+      if (!isfinite(s.valence)) return "ion valence must be finite";
+      if (!(isfinite(s.internal) && isfinite(s.external) && isfinite(s.reversal)))
+        return "non-finite ion parameter";
+      if (s.internal < 0.0 || s.external < 0.0) return "ion concentrations must be nonnegative";
+      for (const Species& o : sp)
+        if (o.name == s.name) return "duplicate ion species " + s.name;
+      sp.push_back(s);
+    }
+  }
+  auto species_of = [&](const std::string& name) {
+    for (size_t k = 0; k < sp.size(); ++k)
+      if (sp[k].name == name) return (int32_t)k;
+    return (int32_t)-1;
+  };
+  // species of every USEION, checked against its VALENCE; a species only a
+  // mechanism declares starts at 1 mM on both sides (model.jl:690-703)
+  std::vector<std::vector<int32_t>> slot_species(L.groups.size());
+  for (size_t g = 0; g < L.groups.size(); ++g) {
+    const std::vector<nmodl::IonUse>& uses = mech_ions(L.groups[g].mech);
+    int ncur = 0;
+    for (const nmodl::IonUse& u : uses) {
+      int32_t k = species_of(u.name);
+      if (k < 0) {
+        if (!u.has_valence) return "USEION " + u.name + " needs a VALENCE declaration";
+        Species s;
+        s.name = u.name;
+        s.valence = u.valence;
+        sp.push_back(s);
+        k = (int32_t)sp.size() - 1;
+      } else if (u.has_valence && sp[k].valence != u.valence) {
+        return "inconsistent valence for ion " + u.name;
+      }
+      slot_species[g].push_back(k);
+      ncur += u.write[MIND_ION_I] >= 0;
+    }
+    if (ncur > kMaxIonCurrents) return "a mechanism writes more than 8 ion currents";
+  }
+  std::vector<int32_t> rank(sp.size());  // species by name
+  {
+    std::vector<int32_t> order(sp.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return sp[a].name < sp[b].name; });
+    for (size_t k = 0; k < order.size(); ++k) rank[order[k]] = (int32_t)k;
+  }
+  // the segments (not the endpoint) of a cell's section
+  auto segments = [&](int c, int sec, std::vector<int64_t>& out) -> bool {
+    if (c < 0 || c >= L.ncells) return false;
+    const Template& t = L.templates[L.cell_template[c]];
+    if (sec < 0 || sec >= (int)t.sec_nodes.size()) return false;
+    out.clear();
+    for (size_t j = 0; j + 1 < t.sec_nodes[sec].size(); ++j) out.push_back(L.cell_off[c] + t.sec_nodes[sec][j]);
+    return true;
+  };
+  // 1. occupied (node, species) pairs
+  std::vector<std::pair<int64_t, int32_t>> occupied;  // (node, rank)
+  std::vector<int64_t> seg;
+  for (size_t g = 0; g < L.groups.size(); ++g) {
+    const MechGroup& G = L.groups[g];
+    for (int32_t k : slot_species[g])
+      for (int64_t r = 0; r < G.rows; ++r)
+        if (G.node[r] >= 0) occupied.push_back({G.node[r], rank[k]});
+  }
+  // a point process's ions cover its whole section (NEURON need_memb)
+  for (int64_t p = 0; p < d.n_points; ++p) {
+    if (d.pt_cell[p] < 0 || L.point_group[p] < 0) continue;
+    for (int32_t k : slot_species[L.point_group[p]]) {
+      if (!segments(d.pt_cell[p], d.pt_section[p], seg)) return "unknown section";
+      for (int64_t n : seg) occupied.push_back({n, rank[k]});
+    }
+  }
+  if (d.n_ion_settings > 0 &&
+      (!d.ionset_cell || !d.ionset_section || !d.ionset_species || !d.ionset_what || !d.ionset_value))
+    return "ion setting arrays missing";
+  for (int64_t q = 0; q < d.n_ion_settings; ++q) {
+    const int32_t k = d.ionset_species[q];
+    if (k < 0 || k >= (int32_t)sp.size()) return "ion setting of an unknown species";
+    if (!segments(d.ionset_cell[q], d.ionset_section[q], seg)) return "ion setting on an unknown section";
+    for (int64_t n : seg) occupied.push_back({n, rank[k]});
+  }
+  std::sort(occupied.begin(), occupied.end());
+  occupied.erase(std::unique(occupied.begin(), occupied.end()), occupied.end());
+  std::vector<int32_t> by_rank(sp.size());
+  for (size_t k = 0; k < sp.size(); ++k) by_rank[rank[k]] = (int32_t)k;
+  L.nion = (int64_t)occupied.size();
+  const int64_t n_ion = L.nion;
+  L.ion_species.resize(n_ion);
+  L.ion_style.assign(n_ion, 0);
+  L.ion_data.assign(5 * n_ion, 0.0);
+  L.node_ion_off.assign(L.nodes + 1, 0);
+  for (int64_t q = 0; q < n_ion; ++q) {
+    L.ion_species[q] = by_rank[occupied[q].second];
+    L.node_ion_off[occupied[q].first + 1] += 1;
+  }
+  for (int64_t n = 0; n < L.nodes; ++n) L.node_ion_off[n + 1] += L.node_ion_off[n];
+  // 2. initial values: the species' defaults (a Nernst species' reversal from its
+  // concentrations), then ion parameters of mechanisms, then section values
+  for (int64_t q = 0; q < n_ion; ++q) {
+    const Species& s = sp[L.ion_species[q]];
+    L.ion_data[MIND_ION_E * n_ion + q] =
+        s.nernst ? mind_nernst(s.internal, s.external, s.valence, L.celsius) : s.reversal;
+    L.ion_data[MIND_ION_CI * n_ion + q] = s.internal;
+    L.ion_data[MIND_ION_CO * n_ion + q] = s.external;
+    L.ion_style[q] = s.nernst ? ion_style_bits(1, 2, true, true, false) : 0;
+  }
+  for (const IonParam& ip : params) {
+    const MechGroup& G = L.groups[ip.group];
+    if (G.node[ip.row] < 0) continue;
+    const std::vector<nmodl::IonUse>& uses = mech_ions(G.mech);
+    for (size_t s = 0; s < uses.size(); ++s)
+      for (int k = 0; k < 4; ++k) {
+        if (uses[s].read[k] != ip.field && uses[s].write[k] != ip.field) continue;
+        if ((k == MIND_ION_CI || k == MIND_ION_CO) && ip.value < 0.0)
+          return "negative ion concentration of " + uses[s].name;
+        L.ion_data[k * n_ion + L.ion_row_at(G.node[ip.row], slot_species[ip.group][s])] = ip.value;
+      }
+  }
+  for (int64_t q = 0; q < d.n_ion_settings; ++q) {
+    const int what = d.ionset_what[q];
+    if (what < 0 || what > 4) return "unknown ion setting";
+    if (what > 2) continue;
+    const double v = d.ionset_value[q];
+    if (!isfinite(v) || (what < 2 && v < 0.0)) return what < 2 ? "negative ion concentration" : "non-finite ion value";
+    const int column = what == 0 ? MIND_ION_CI : what == 1 ? MIND_ION_CO : MIND_ION_E;
+    segments(d.ionset_cell[q], d.ionset_section[q], seg);
+    for (int64_t n : seg) L.ion_data[column * n_ion + L.ion_row_at(n, d.ionset_species[q])] = v;
+  }
+  // 3. transfers and styles; one writer per concentration and node
+  std::vector<std::tuple<int64_t, int32_t, int>> owners;
+  for (size_t g = 0; g < L.groups.size(); ++g) {
+    MechGroup& G = L.groups[g];
+    const std::vector<nmodl::IonUse>& uses = mech_ions(G.mech);
+    G.nslot = (int32_t)uses.size();
+    G.ion_row.assign((size_t)G.rows * G.nslot, -1);
+    G.ion_bind.clear();
+    if (!G.nslot) continue;
+    for (int64_t r = 0; r < G.rows; ++r)
+      for (int s = 0; s < G.nslot; ++s)
+        G.ion_row[(size_t)r * G.nslot + s] = (int32_t)L.ion_row_at(G.node[r], slot_species[g][s]);
+    int ncur = 0;
+    std::vector<IonBind> reads, writes, recv;
+    const std::vector<int>& rf = mech_receive_fields(G.mech);
+    for (int s = 0; s < G.nslot; ++s) {
+      const nmodl::IonUse& u = uses[s];
+      for (int k = 0; k < 4; ++k) {
+        if (u.read[k] >= 0) reads.push_back({u.read[k], k, s, 0});
+        if (u.read[k] >= 0 && (k == MIND_ION_CI || k == MIND_ION_CO) &&
+            std::find(rf.begin(), rf.end(), u.read[k]) != rf.end())
+          recv.push_back({u.read[k], k, s, mech_state_column(G.mech, u.read[k]) ? 1 : 0});
+      }
+      for (int k = 0; k < 4; ++k)
+        if (u.write[k] >= 0) {
+          writes.push_back({u.write[k], k, s, k == MIND_ION_I ? ncur : -1});
+          if (k != MIND_ION_I) L.ion_ordered = true;
+        }
+      ncur += u.write[MIND_ION_I] >= 0;
+      const bool wconc = u.write[MIND_ION_CI] >= 0 || u.write[MIND_ION_CO] >= 0;
+      const int cstyle = wconc ? 3 : (u.read[MIND_ION_CI] >= 0 || u.read[MIND_ION_CO] >= 0) ? 1 : 0;
+      const int estyle = u.write[MIND_ION_E] >= 0 ? 3 : u.read[MIND_ION_E] >= 0 ? 1 : 0;
+      for (int64_t r = 0; r < G.rows; ++r) {
+        const int32_t q = G.ion_row[(size_t)r * G.nslot + s];
+        if (q < 0) continue;
+        L.ion_style[q] = promote_ion_style(L.ion_style[q], cstyle, estyle);
+        for (int k : {MIND_ION_CI, MIND_ION_CO})
+          if (u.write[k] >= 0) owners.push_back({G.node[r], slot_species[g][s], k});
+      }
+    }
+    G.nread = (int32_t)reads.size();
+    G.nwrite = (int32_t)writes.size();
+    G.nrecv = (int32_t)recv.size();
+    G.ion_bind = reads;
+    G.ion_bind.insert(G.ion_bind.end(), writes.begin(), writes.end());
+    G.ion_bind.insert(G.ion_bind.end(), recv.begin(), recv.end());
+  }
+  std::sort(owners.begin(), owners.end());
+  for (size_t k = 1; k < owners.size(); ++k)
+    if (owners[k] == owners[k - 1])
+      return "multiple writers of " + sp[std::get<1>(owners[k])].name +
+             (std::get<2>(owners[k]) == MIND_ION_CI ? "i" : "o") + " at node " + std::to_string(std::get<0>(owners[k]));
+  // a Nernst species updates its reversal at initialization and every step unless a
+  // mechanism owns the reversal (model.jl:841-847)
+  for (int64_t q = 0; q < n_ion; ++q)
+    if (sp[L.ion_species[q]].nernst && ((L.ion_style[q] >> 3) & 0x03) != 0x03) L.ion_style[q] |= 0x60;
+  // section overrides, the last one per (cell, section, species) (ion_style_override,
+  // utils/biophysics.jl:24-29)
+  std::map<std::tuple<int, int, int>, std::pair<int, double>> over;
+  for (int64_t q = 0; q < d.n_ion_settings; ++q)
+    if (d.ionset_what[q] >= 3)
+      over[{d.ionset_cell[q], d.ionset_section[q], d.ionset_species[q]}] = {d.ionset_what[q], d.ionset_value[q]};
+  for (const auto& [key, o] : over) {
+    segments(std::get<0>(key), std::get<1>(key), seg);
+    for (int64_t n : seg) {
+      const int64_t q = L.ion_row_at(n, std::get<2>(key));
+      if (o.first == 4) {
+        const int b = (int)o.second;
+        if (!(o.second >= 0.0 && o.second < 128.0 && o.second == (double)b)) return "invalid ion style";
+        L.ion_style[q] = (uint8_t)b;
+      } else {
+        const int policy = (int)o.second;
+        if (!(policy >= 0 && policy <= 4 && o.second == (double)policy)) return "unknown ion reversal policy";
+        static const uint8_t flags[5] = {0, 0x00, 0x20, 0x40, 0x60};
+        if (policy > 0) L.ion_style[q] = (uint8_t)((L.ion_style[q] & 0x1f) | flags[policy]);
+      }
+    }
+  }
+  return "";
+}
+
 int64_t Layout::location_node(int32_t cell, int32_t section, double x) const {
   return cell_off[cell] + template_location(templates[cell_template[cell]], section, x);
 }
@@ -357,6 +617,7 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
     params[g].push_back({d.pt_param_offsets[p], d.pt_param_offsets[p + 1]});
   }
   // Sort rows by node (stable) and fill parameter columns.
+  std::vector<IonParam> ion_params;
   for (size_t g = 0; g < L.groups.size(); ++g) {
     MechGroup& G = L.groups[g];
     const int64_t rows = (int64_t)G.node.size();
@@ -384,6 +645,7 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
         const int f = d.param_field[q];
         if (f < 0 || f >= nf) return "unknown mechanism field";
         H.data[(size_t)f * rows + r] = d.param_value[q];
+        ion_params.push_back({(int)g, r, f, d.param_value[q]});
       }
       if (H.point[r] >= 0) L.point_row[H.point[r]] = (int32_t)r;
     }
@@ -399,6 +661,10 @@ std::string pack(const mind_model_desc& d, double dt, uint64_t seed, Layout& L) 
       for (int64_t q = 1; q < rows; ++q) H.run_start[q] = H.node[q] != H.node[q - 1];
     }
     L.groups[g] = std::move(H);
+  }
+  {
+    const std::string e = pack_ions(d, ion_params, L);
+    if (!e.empty()) return e;
   }
   // 4. Sources, detectors (one per voltage source, in source order) and fanout.
   L.nsources = d.n_sources;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/mind.h"
+#include "mech.h"
 
 namespace mind {
 
@@ -26,6 +27,19 @@ struct MechGroup {
   std::vector<double> data;       // field-major: data[field * rows + row]
   std::vector<int64_t> cell_rows; // rows of cell c: [cell_rows[c], cell_rows[c+1])
   std::vector<uint8_t> run_start; // point groups: row starts a run of equal nodes
+  // ion transfers (USEION statements): ion_row[r * nslot + s] is row r's ion row
+  // for statement s (-1 for artificial cells); ion_bind holds the nread loads,
+  // the nwrite stores and the nrecv NET_RECEIVE exchanges
+  int32_t nslot = 0, nread = 0, nwrite = 0, nrecv = 0;
+  std::vector<int32_t> ion_row;
+  std::vector<IonBind> ion_bind;
+};
+
+// IonSpecies (src/model/micro/biophysics.jl:3-25).
+struct Species {
+  std::string name;
+  double valence = 1.0, internal = 1.0, external = 1.0, reversal = 0.0;
+  bool nernst = false;
 };
 
 struct Template {  // one discretized morphology
@@ -74,6 +88,18 @@ struct Layout {
   std::vector<double> edge_delay, edge_weight;
   std::vector<int64_t> connection_edge;  // edge of each connection, in creation order
   double min_delay = 0.0, max_delay = 0.0;
+
+  // ion rows, one per (node, species) pair in use, sorted by node and then by
+  // species name (src/model/micro/model.jl:710-855); columns MIND_ION_E, _CI,
+  // _CO, _I, _DIDV of row q at ion_data[column * nion + q]
+  std::vector<Species> species;       // the description's order
+  int64_t nion = 0;
+  std::vector<int32_t> ion_species;
+  std::vector<uint8_t> ion_style;     // IonStyle bits (ion_style_bits)
+  std::vector<double> ion_data;
+  std::vector<int64_t> node_ion_off;  // nodes + 1
+  bool ion_ordered = false;           // some mechanism writes a reversal or a concentration
+  int64_t ion_row_at(int64_t node, int32_t species) const;
 
   std::vector<int32_t> netstim_source;  // per NetStim row: its source index or -1
   // MACRO2MICRO bindings are virtual sources [first_virtual, nsources), one edge each

@@ -60,6 +60,16 @@ struct Cols {
   MIND_HD double& operator()(int field, int64_t row) const { return data[field * rows + row]; }
 };
 
+// One ion transfer of a mechanism group: mechanism field `field` and ion column
+// `column` (MIND_ION_*) of the row's USEION statement `slot`. aux: for a current
+// write, its index among the written ion currents (the di/dv probe); for a
+// NET_RECEIVE exchange, 1 if the field is a STATE (published back).
+struct IonBind {
+  int32_t field, column, slot, aux;
+};
+// Written ion currents one mechanism may have (USEION statements writing i).
+constexpr int kMaxIonCurrents = 8;
+
 // VClamp.mod clamp(): the three amplifier equations solved for e, vo and vi from
 // the previous step's e0, vo0, vi0 (VClamp.mod PROCEDURE clamp).
 MIND_HD void vclamp_amplifier(const Cols& c, int64_t r, double v, double dt, double stim, double e0, double vo0,
@@ -79,8 +89,10 @@ MIND_HD void vclamp_amplifier(const Cols& c, int64_t r, double v, double dt, dou
 // v + 0.001; stores the assigned fields of the evaluation at v.
 // Current fields sum NONSPECIFIC currents first, then ion writes (hh: il, ina,
 // ik), and subtract ELECTRODE currents (src/runtime/initialize.jl:496-502).
+// ish (if not null) receives the written ion currents at v + 0.001 in USEION
+// order (hh: ina, ik), for the ion rows' di/dv.
 MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t, double dt,
-                          double& value, double& shifted, bool store) {
+                          double& value, double& shifted, bool store, double* ish) {
   (void)dt;
   for (int probe = 0; probe < 2; ++probe) {
     const double vv = probe == 0 ? v + 0.001 : v;
@@ -96,6 +108,10 @@ MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t
         sum += il;
         sum += ina;
         sum += ik;
+        if (probe == 0 && ish) {
+          ish[0] = ina;
+          ish[1] = ik;
+        }
         if (probe == 1 && store) {
           c(HH_GNA, r) = gna; c(HH_INA, r) = ina; c(HH_GK, r) = gk; c(HH_IK, r) = ik;
           c(HH_IL, r) = il;

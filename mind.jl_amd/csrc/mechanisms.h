@@ -34,9 +34,9 @@ static_assert(gen::kComplete,
 constexpr const char* kMechanismSource = gen::kSource;
 constexpr uint64_t kNetStimDigest = gen::kNetStimDigest;
 MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double t, double dt, double& value,
-                           double& shifted, bool store) {
-  MIND_USER_CASE(user::current(mech, c, r, v, t, dt, value, shifted, store))
-  gen::current(mech, c, r, v, t, dt, value, shifted, store);
+                           double& shifted, bool store, double celsius, double* ish) {
+  MIND_USER_CASE(user::current(mech, c, r, v, t, dt, value, shifted, store, celsius, ish))
+  gen::current(mech, c, r, v, t, dt, value, shifted, store, celsius, ish);
 }
 // nrn_cur of mechanism slot MECH in two parts: the column reads of a row (load),
 // then the evaluation and stores (eval).
@@ -70,9 +70,10 @@ constexpr const char* kMechanismSource = "hand lowering (mech.h)";
 // SHA-256 of NetStim.mod (src/compiler/ir.jl:316-324).
 constexpr uint64_t kNetStimDigest = 0xa186833b569ff3b2ull;
 MIND_HD void phase_current(int mech, const Cols& c, int64_t r, double v, double t, double dt, double& value,
-                           double& shifted, bool store) {
-  MIND_USER_CASE(user::current(mech, c, r, v, t, dt, value, shifted, store))
-  mech_current(mech, c, r, v, t, dt, value, shifted, store);
+                           double& shifted, bool store, double celsius, double* ish) {
+  MIND_USER_CASE(user::current(mech, c, r, v, t, dt, value, shifted, store, celsius, ish))
+  (void)celsius;  // no builtin reads celsius in BREAKPOINT
+  mech_current(mech, c, r, v, t, dt, value, shifted, store, ish);
 }
 // nrn_cur of mechanism slot MECH in two parts; the hand lowering reads its
 // columns in the evaluation.
@@ -81,8 +82,9 @@ struct CurOps {
   struct In {};
   static MIND_HD In load(const Cols&, int64_t) { return In{}; }
   static MIND_HD void eval(const Cols& c, int64_t r, const In&, double v, double t, double dt, double& value,
-                           double& shifted, bool store) {
-    mech_current(MECH, c, r, v, t, dt, value, shifted, store);
+                           double& shifted, bool store, double celsius) {
+    (void)celsius;
+    mech_current(MECH, c, r, v, t, dt, value, shifted, store, nullptr);
   }
 };
 MIND_HD void phase_advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, double celsius) {

@@ -5,6 +5,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <deque>
 #include <mutex>
 
@@ -77,6 +78,35 @@ bool mech_has_receive(int id) {
   if (id >= 0 && id < MIND_MECH_COUNT) return builtins()[id].receive;
   const UserSlot* s = user_slot(id);
   return s && s->m.has_net_receive;
+}
+
+const std::vector<nmodl::IonUse>& mech_ions(int id) {
+  static const std::vector<nmodl::IonUse> none;
+  // hh.mod: USEION na READ ena WRITE ina, USEION k READ ek WRITE ik
+  static const std::vector<nmodl::IonUse> hh = [] {
+    nmodl::IonUse na, k;
+    na.name = "na";
+    na.read[0] = HH_ENA;
+    na.write[3] = HH_INA;
+    k.name = "k";
+    k.read[0] = HH_EK;
+    k.write[3] = HH_IK;
+    return std::vector<nmodl::IonUse>{na, k};
+  }();
+  if (id == MIND_MECH_HH) return hh;
+  const UserSlot* s = user_slot(id);
+  return s ? s->m.ions : none;
+}
+
+bool mech_state_column(int id, int col) {
+  const UserSlot* s = user_slot(id);
+  return s && col >= 0 && col < (int)s->m.state_column.size() && s->m.state_column[col];
+}
+
+const std::vector<int>& mech_receive_fields(int id) {
+  static const std::vector<int> none;
+  const UserSlot* s = user_slot(id);
+  return s ? s->m.receive_fields : none;
 }
 
 }  // namespace mind
@@ -173,5 +203,37 @@ extern "C" int mind_mech_lowering(int32_t mech, const char** text) {
   const UserSlot* s = user_slot(mech);
   if (!s) return fail(MIND_EARG, "mechanism " + std::to_string(mech) + " is not a compiled mechanism");
   *text = s->lowering.c_str();
+  return 0;
+}
+
+extern "C" int mind_mech_ions(int32_t mech, int32_t index, const char** name, int32_t* has_valence, double* valence,
+                              int32_t read[4], int32_t write[4], int32_t* count) {
+  if (!count) return fail(MIND_EARG, "mind_mech_ions: null count");
+  if (!mech_known(mech)) return fail(MIND_EARG, "unknown mechanism " + std::to_string(mech));
+  const std::vector<nmodl::IonUse>& ions = mech_ions(mech);
+  *count = (int32_t)ions.size();
+  if (index < 0 || index >= (int32_t)ions.size()) return 0;
+  const nmodl::IonUse& u = ions[index];
+  if (name) *name = u.name.c_str();
+  if (has_valence) *has_valence = u.has_valence ? 1 : 0;
+  if (valence) *valence = u.valence;
+  for (int k = 0; k < 4; ++k) {
+    if (read) read[k] = u.read[k];
+    if (write) write[k] = u.write[k];
+  }
+  return 0;
+}
+
+extern "C" int mind_mech_columns(int32_t mech, uint8_t* is_state, uint8_t* in_receive, int32_t capacity,
+                                 int32_t* count) {
+  if (!count) return fail(MIND_EARG, "mind_mech_columns: null count");
+  if (!mech_known(mech)) return fail(MIND_EARG, "unknown mechanism " + std::to_string(mech));
+  const int n = field_count(mech);
+  const std::vector<int>& rf = mech_receive_fields(mech);
+  for (int k = 0; k < n && k < capacity; ++k) {
+    if (is_state) is_state[k] = mech_state_column(mech, k) ? 1 : 0;
+    if (in_receive) in_receive[k] = std::find(rf.begin(), rf.end(), k) != rf.end() ? 1 : 0;
+  }
+  *count = n;
   return 0;
 }

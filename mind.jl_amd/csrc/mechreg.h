@@ -27,5 +27,11 @@ bool mech_known(int id);
 // MIND_KIND_* of a known mechanism.
 int mech_kind(int id);
 bool mech_has_receive(int id);
+// USEION statements of a known mechanism (hh: na and k; the other builtins none).
+const std::vector<nmodl::IonUse>& mech_ions(int id);
+// Whether column `col` of a known mechanism is a STATE variable, and the columns
+// its NET_RECEIVE block touches.
+bool mech_state_column(int id, int col);
+const std::vector<int>& mech_receive_fields(int id);
 
 }  // namespace mind

@@ -150,3 +150,27 @@ MIND_HD double mind_fabs(double x) { return mind_from_bits(mind_bits(x) & 0x7fff
 MIND_HD int mind_isfinite(double x) {
   return (mind_bits(x) & 0x7ff0000000000000ull) != 0x7ff0000000000000ull;
 }
+
+// Nernst potential (mV) of an ion of valence z at concentrations ci, co (mM),
+// and the GHK current factor (mA/cm2 per cm/s) at voltage v (mV), in the
+// operation order of src/model/micro/utils/biophysics.jl:43-70 (CODATA 2018
+// exact k, N_A and e). The small-argument branch x/(exp(x) - 1) ~ 1 - x/2 for
+// |x| < 1e-4 is NEURON's.
+MIND_HD double mind_nernst(double ci, double co, double z, double celsius) {
+  if (z == 0.0) return 0.0;
+  if (ci <= 0.0) return 1.0e6;
+  if (co <= 0.0) return -1.0e6;
+  const double gas = 1.380649e-23 * 6.02214076e23;
+  const double faraday = 1.602176634e-19 * 6.02214076e23;
+  return (1000.0 * gas * (celsius + 273.15) / faraday) / z * mind_log(co / ci);
+}
+MIND_HD double mind_ghk_quotient(double x) {
+  return mind_fabs(x) < 1.0e-4 ? 1.0 - x / 2.0 : x / (mind_exp(x) - 1.0);
+}
+MIND_HD double mind_ghk(double v, double ci, double co, double z, double celsius) {
+  const double gas = 1.380649e-23 * 6.02214076e23;
+  const double faraday = 1.602176634e-19 * 6.02214076e23;
+  const double thermal = 1000.0 * gas * (celsius + 273.15) / faraday;
+  const double scaled = z * v / thermal;
+  return 0.001 * z * faraday * (ci * mind_ghk_quotient(-scaled) - co * mind_ghk_quotient(scaled));
+}

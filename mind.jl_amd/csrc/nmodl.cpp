@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cctype>
 #include <map>
 #include <memory>
@@ -344,6 +345,8 @@ struct Decl {
 struct Ion {
   std::string ion;
   std::vector<std::string> reads, writes;
+  bool has_valence = false;
+  double valence = 0.0;
 };
 
 // A READ_SOURCE / READ_TARGET / WRITE_SOURCE / WRITE_TARGET binding of a
@@ -512,8 +515,11 @@ class Parser {
         while (peek().text == "READ" || peek().text == "WRITE" || peek().text == "VALENCE") {
           const std::string w = take().text;
           if (w == "VALENCE") {
-            accept("-");
-            take();
+            const bool neg = accept("-");
+            const Tok v = take();
+            if (v.kind != Tok::NUM) fail(v, "VALENCE takes a number");
+            ion.has_valence = true;
+            ion.valence = neg ? -strtod(v.text.c_str(), nullptr) : strtod(v.text.c_str(), nullptr);
             continue;
           }
           append(w == "READ" ? ion.reads : ion.writes, names_list());
@@ -947,6 +953,14 @@ class Gen {
       need_random(c);
       return "stream_uniform(*x.s, x.seed)";
     }
+    if ((c->id == "nernst" || c->id == "nrn_nernst") && !func(c->id)) {  // codegen.jl:112-115
+      if (args.size() != 3) err(c->id + " takes (internal, external, valence)");
+      return "gen_nernst(" + args[0] + ", " + args[1] + ", " + args[2] + ", x.celsius)";
+    }
+    if ((c->id == "ghk" || c->id == "nrn_ghk") && !func(c->id)) {
+      if (args.size() != 4) err(c->id + " takes (v, internal, external, valence)");
+      return "gen_ghk(" + args[0] + ", " + args[1] + ", " + args[2] + ", " + args[3] + ", x.celsius)";
+    }
     if (c->id == "at_time") {  // 1 in the step whose interval (t - dt, t] holds the event (codegen.jl:102-110)
       if (args.size() != 1) err("at_time takes one argument");
       return "(((" + args[0] + " - 1.0e-11) <= x.t && (" + args[0] + " - 1.0e-11) > x.t - x.dt) ? 1.0 : 0.0)";
@@ -1281,16 +1295,20 @@ class Gen {
     std::vector<std::string> written;
     collect_writes(body, written, {}, {});
     std::vector<std::string> currents = m_.nonspecific;
+    std::vector<std::string> ion_currents;  // written ion currents in USEION order (the ish outputs)
     for (const Ion& ion : m_.ions)
       for (const std::string& w : ion.writes)
-        if (!w.empty() && w[0] == 'i') currents.push_back(w);
+        if (w == "i" + ion.ion) {
+          currents.push_back(w);
+          ion_currents.push_back(w);
+        }
     std::vector<std::string> all = currents;
     all.insert(all.end(), m_.electrode.begin(), m_.electrode.end());
     for (const std::string& n : all)
       if (!findex_.count(n)) err("current " + n + " is not declared");
     std::vector<std::string> lines;
     lines.push_back("  for (int probe = 0; probe < 2; ++probe) {");
-    lines.push_back("    GenCtx x = gen_ctx(probe == 0 ? v + 0.001 : v, t, dt);");
+    lines.push_back("    GenCtx x = gen_ctx(probe == 0 ? v + 0.001 : v, t, dt, celsius);");
     lines.push_back("    (void)x;");
     Scope scope = base_scope();
     for (const std::string& n : written) {
@@ -1308,6 +1326,12 @@ class Gen {
     for (const std::string& n : m_.electrode) lines.push_back("    sum = (sum - " + value_of(n) + ");");
     lines.push_back("    if (probe == 0) {");
     lines.push_back("      shifted = sum;");
+    if (!ion_currents.empty()) {
+      lines.push_back("      if (ish) {");
+      for (size_t k = 0; k < ion_currents.size(); ++k)
+        lines.push_back("        ish[" + std::to_string(k) + "] = " + value_of(ion_currents[k]) + ";");
+      lines.push_back("      }");
+    }
     lines.push_back("    } else {");
     lines.push_back("      value = sum;");
     std::vector<std::string> stores;
@@ -1342,8 +1366,10 @@ class Gen {
     out.push_back("  return in;");
     out.push_back("}");
     out.push_back("MIND_HD void " + p_ + "_current_eval(const Cols& c, int64_t r, const " + p_ +
-                  "_cur_in& in, double v, double t, double dt, double& value, double& shifted, bool store) {");
+                  "_cur_in& in, double v, double t, double dt, double& value, double& shifted, bool store, "
+                  "double celsius, double* ish) {");
     out.push_back("  (void)in;");
+    out.push_back("  (void)ish;");
     for (std::string l : lines) {
       if (!is_store(l)) {
         for (const std::string& n : reads) {
@@ -1356,8 +1382,9 @@ class Gen {
     }
     out.push_back("}");
     out.push_back("MIND_HD void " + p_ + "_current(const Cols& c, int64_t r, double v, double t, double dt, "
-                  "double& value, double& shifted, bool store) {");
-    out.push_back("  " + p_ + "_current_eval(c, r, " + p_ + "_current_load(c, r), v, t, dt, value, shifted, store);");
+                  "double& value, double& shifted, bool store, double celsius, double* ish) {");
+    out.push_back("  " + p_ + "_current_eval(c, r, " + p_ +
+                  "_current_load(c, r), v, t, dt, value, shifted, store, celsius, ish);");
     out.push_back("}");
   }
 
@@ -1485,6 +1512,16 @@ MIND_HD GenCtx gen_ctx(double v, double t, double dt = 0.0, double celsius = 0.0
   x.out = NetStimOut{0, 0.0, 0, 0.0, 0.0};
   return x;
 }
+
+// Nernst potential (mV) and GHK current factor of the nernst / ghk intrinsics
+// (src/model/micro/utils/biophysics.jl:43-70), the operations in the
+// reference's order.
+MIND_HD double gen_nernst(double ci, double co, double z, double celsius) {
+  return mind_nernst(ci, co, z, celsius);
+}
+MIND_HD double gen_ghk(double v, double ci, double co, double z, double celsius) {
+  return mind_ghk(v, ci, co, z, celsius);
+}
 )";
 
 // Builtin mechanisms and their runtime slots (include/mind.h).
@@ -1533,8 +1570,8 @@ std::string dispatch(const std::vector<Mechanism>& mechs) {
   };
   out.push_back(
       "MIND_HD void current(int mech, const Cols& c, int64_t r, double v, double t, double dt, double& value, "
-      "double& shifted, bool store) {\n  switch (mech) {\n" +
-      cases("_current", "(c, r, v, t, dt, value, shifted, store)", false) +
+      "double& shifted, bool store, double celsius, double* ish) {\n  switch (mech) {\n" +
+      cases("_current", "(c, r, v, t, dt, value, shifted, store, celsius, ish)", false) +
       "\n    default: value = shifted = 0.0;\n  }\n}");
   out.push_back("template <int MECH>\nstruct CurOf;");
   for (const auto& b : builtin_ids()) {
@@ -1542,8 +1579,8 @@ std::string dispatch(const std::vector<Mechanism>& mechs) {
     out.push_back("template <>\nstruct CurOf<" + b.second + "> {\n  using In = " + n +
                   "_cur_in;\n  static MIND_HD In load(const Cols& c, int64_t r) { return " + n +
                   "_current_load(c, r); }\n  static MIND_HD void eval(const Cols& c, int64_t r, const In& in, double "
-                  "v, double t, double dt, double& value, double& shifted, bool store) {\n    " +
-                  n + "_current_eval(c, r, in, v, t, dt, value, shifted, store);\n  }\n};");
+                  "v, double t, double dt, double& value, double& shifted, bool store, double celsius) {\n    " +
+                  n + "_current_eval(c, r, in, v, t, dt, value, shifted, store, celsius, nullptr);\n  }\n};");
   }
   out.push_back("MIND_HD void advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, "
                 "double celsius) {\n  switch (mech) {\n" +
@@ -1619,40 +1656,66 @@ UserMechanism lower_user(const std::string& filename, const std::string& text, c
     if (calls(b.body, "net_send") || calls(b.body, "net_event"))
       raise_at(filename, b.line, "net_send and net_event are supported for the builtin NetStim only");
   struct IonDefaults { const char* ion; double e, in, out; };
-  // default_ions (src/model/micro/utils/biophysics.jl:31-42)
+  // default_ions (src/model/micro/utils/biophysics.jl:31-42); a species the
+  // model declares itself starts at 1 mM on both sides (model.jl:696-703)
   static const IonDefaults kIons[] = {{"na", 50.0, 10.0, 140.0},
                                       {"k", -77.0, 54.4, 2.5},
                                       {"ca", 132.4579341637009, 5.0e-5, 2.0}};
+  Gen g(m, prefix);
   std::map<std::string, double> ion_value;
+  std::vector<IonUse> uses;
   for (const Ion& ion : m.ions) {
     const IonDefaults* d = nullptr;
     for (const IonDefaults& k : kIons)
       if (ion.ion == k.ion) d = &k;
-    if (!d) refuse("ion " + ion.ion + " has no default species (na, k and ca are known)");
-    for (const std::string& w : ion.writes)
-      if (w != "i" + ion.ion)
-        refuse("USEION " + ion.ion + " WRITE " + w + ": only ion currents may be written (no concentration dynamics)");
-    for (const std::string& r : ion.reads) {
-      if (r == "e" + ion.ion)
-        ion_value[r] = d->e;
-      else if (r == ion.ion + "i")
-        ion_value[r] = d->in;
-      else if (r == ion.ion + "o")
-        ion_value[r] = d->out;
-      else
-        refuse("USEION " + ion.ion + " READ " + r + ": only reversal potentials and concentrations may be read");
+    const double start[4] = {d ? d->e : 0.0, d ? d->in : 1.0, d ? d->out : 1.0, 0.0};
+    // ion columns: reversal, internal and external concentration, current
+    const std::string names[4] = {"e" + ion.ion, ion.ion + "i", ion.ion + "o", "i" + ion.ion};
+    auto slot_of = [&](const std::string& n) {
+      for (int k = 0; k < 4; ++k)
+        if (n == names[k]) return k;
+      return -1;
+    };
+    auto field_of = [&](const std::string& n, const char* access) {
+      bool scalar = false;
+      for (const std::string& f : g.fields()) scalar |= f == n && !m.extents.count(n);
+      if (!scalar) refuse("USEION " + ion.ion + " " + access + " " + n + ": not a declared scalar variable");
+      return g.column(n);
+    };
+    IonUse u;
+    u.name = ion.ion;
+    u.has_valence = ion.has_valence;
+    u.valence = ion.valence;
+    for (const std::string& w : ion.writes) {
+      const int k = slot_of(w);
+      if (k < 0) refuse("USEION " + ion.ion + " WRITE " + w + ": not an ion variable of " + ion.ion);
+      u.write[k] = field_of(w, "WRITE");
     }
+    // written concentrations are loaded too (utils/build.jl:118-124)
+    std::vector<std::string> reads = ion.reads;
+    for (int k : {1, 2})
+      if (u.write[k] >= 0 && std::find(reads.begin(), reads.end(), names[k]) == reads.end()) reads.push_back(names[k]);
+    for (const std::string& r : reads) {
+      const int k = slot_of(r);
+      if (k < 0) refuse("USEION " + ion.ion + " READ " + r + ": not an ion variable of " + ion.ion);
+      u.read[k] = field_of(r, "READ");
+      ion_value[r] = start[k];
+    }
+    uses.push_back(u);
   }
-  Gen g(m, prefix);
   UserMechanism u;
   u.name = m.name;
   u.filename = filename;
   u.kind = m.kind == "point_process" ? 1 : 0;
   u.digest = m.digest();
   u.code = g.emit();
+  u.ions = uses;
   std::map<std::string, double> defaults;
   for (const Decl& d : m.decls)
     if (d.storage == "parameter" && d.has_default) defaults[d.name] = d.value;
+  std::set<std::string> states;
+  for (const Decl& d : m.decls)
+    if (d.storage == "state") states.insert(d.name);
   for (const std::string& n : g.fields()) {
     const int ext = m.extents.count(n) ? m.extents.at(n) : 0;
     for (int k = 0; k < std::max(ext, 1); ++k) {
@@ -1661,7 +1724,18 @@ UserMechanism lower_user(const std::string& filename, const std::string& text, c
       if (k == 0 && defaults.count(n)) v = defaults[n];
       if (ion_value.count(n)) v = ion_value[n];
       u.defaults.push_back(v);
+      u.state_column.push_back(states.count(n) ? 1 : 0);
     }
+  }
+  // the fields NET_RECEIVE touches, from its lowered text
+  {
+    const std::string head = "MIND_HD NetStimOut " + prefix + "_receive(";
+    const size_t at = u.code.find(head);
+    const std::string body = at == std::string::npos ? "" : u.code.substr(at, u.code.find("\n}", at) - at);
+    if (body.find("x.celsius") != std::string::npos)
+      refuse("celsius in NET_RECEIVE: a runtime mechanism's NET_RECEIVE sees the connection weight only");
+    for (const std::string& n : g.fields())
+      if (body.find("c(" + prefix + "_" + n + ", r)") != std::string::npos) u.receive_fields.push_back(g.column(n));
   }
   u.has_net_receive = m.block("net_receive", "net_receive") != nullptr;
   const Block* bp = m.block("breakpoint", "breakpoint");

@@ -50,6 +50,17 @@ uint64_t digest(const std::string& text);
 // The builtin set (filename, text) lowered to the mech_gen.h header of libmind.
 std::string builtin_header(const std::vector<std::pair<std::string, std::string>>& sources);
 
+// One USEION statement: per ion column (0 reversal, 1 internal and 2 external
+// concentration, 3 current) the field column it reads into or writes from, -1
+// if none. Reads include written concentrations (NMODL loads them).
+struct IonUse {
+  std::string name;
+  bool has_valence = false;
+  double valence = 0.0;
+  int read[4] = {-1, -1, -1, -1};
+  int write[4] = {-1, -1, -1, -1};
+};
+
 // A mechanism lowered for a runtime slot.
 struct UserMechanism {
   std::string name, filename;
@@ -60,12 +71,14 @@ struct UserMechanism {
   bool has_net_receive = false;
   bool has_state = false;            // BREAKPOINT has SOLVE statements
   std::string code;                  // C++ phases named <prefix>_*, for namespace mind::user
+  std::vector<IonUse> ions;          // USEION statements in declaration order
+  std::vector<uint8_t> state_column; // per column: 1 for a STATE variable
+  std::vector<int> receive_fields;   // columns NET_RECEIVE reads or writes
 };
 
 // Lowers one MOD text for slot functions named `prefix` (a C identifier).
 // Refuses what the device path cannot run for a runtime mechanism: artificial
-// cells, RANDOM, net_send / net_event, ions other than na, k and ca, writes of
-// ion concentrations or reversal potentials, reads of ion currents.
+// cells, RANDOM, net_send / net_event, celsius in NET_RECEIVE.
 UserMechanism lower_user(const std::string& filename, const std::string& text, const std::string& prefix);
 
 // The MIND-role mechanisms of the macro step (a MIND block with ROLE REGION,

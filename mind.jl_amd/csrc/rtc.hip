@@ -59,8 +59,8 @@ std::string user_header(const std::vector<int>& slots) {
   };
   h += "namespace mind {\nnamespace user {\n";
   h += "MIND_HD void current(int mech, const Cols& c, int64_t r, double v, double t, double dt, double& value, "
-       "double& shifted, bool store) {\n  switch (mech) {\n" +
-       cases("_current", "(c, r, v, t, dt, value, shifted, store)", false) +
+       "double& shifted, bool store, double celsius, double* ish) {\n  switch (mech) {\n" +
+       cases("_current", "(c, r, v, t, dt, value, shifted, store, celsius, ish)", false) +
        "    default: value = shifted = 0.0;\n  }\n}\n";
   h += "MIND_HD void advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, double celsius) {\n"
        "  switch (mech) {\n" +

@@ -76,8 +76,7 @@ __global__ void k_init_netstim(Dev d, const uint64_t* logical, uint64_t digest, 
 __global__ void k_record_sample(Dev d, int64_t sample) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.nrec) return;
-  const int64_t idx = d.rec_index[q];
-  const double x = d.rec_kind[q] == 0 ? d.v[idx] : d.g[d.rec_group[q]].data[d.rec_field[q] * d.g[d.rec_group[q]].rows + idx];
+  const double x = d.rec_kind[q] == 0 ? d.v[d.rec_index[q]] : stored_value(d, (int32_t)q);
   d.rec_buf[q * d.rec_stride + sample] = x;
 }
 
@@ -514,6 +513,37 @@ int setup_state(State& S, const mind_build_options& o) {
     if ((rc = S.upload(&p32, G.node))) return rc; H.node = p32;
     if ((rc = S.upload(&p64, G.cell_rows))) return rc; H.cell_rows = p64;
     if ((rc = S.upload(&p8, G.run_start))) return rc; H.run_start = p8;
+    H.nslot = (int16_t)G.nslot;
+    H.nread = (int16_t)G.nread;
+    H.nwrite = (int16_t)G.nwrite;
+    H.nrecv = (int16_t)G.nrecv;
+    H.ion_row = nullptr;
+    H.ion_bind = nullptr;
+    if (G.nslot) {
+      IonBind* pb;
+      if ((rc = S.upload(&p32, G.ion_row))) return rc; H.ion_row = p32;
+      if ((rc = S.upload(&pb, G.ion_bind))) return rc; H.ion_bind = pb;
+    }
+  }
+  // ion rows
+  d.nion = L.nion;
+  d.ion_ordered = L.ion_ordered ? 1 : 0;
+  d.ion_correct = L.secondorder == 2 ? 1 : 0;
+  {
+    std::vector<double> valence, conc0;
+    for (const Species& sp : L.species) {
+      valence.push_back(sp.valence);
+      conc0.push_back(sp.internal);
+      conc0.push_back(sp.external);
+    }
+    double* pd; int32_t* p32; int64_t* p64; uint8_t* p8;
+    if ((rc = S.alloc(&d.ion, std::max<size_t>(1, L.ion_data.size())))) return rc;
+    if ((rc = S.upload(&S.ion_initial, L.ion_data))) return rc;
+    if ((rc = S.upload(&p8, L.ion_style))) return rc; d.ion_style = p8;
+    if ((rc = S.upload(&p32, L.ion_species))) return rc; d.ion_species = p32;
+    if ((rc = S.upload(&pd, valence))) return rc; d.ion_valence = pd;
+    if ((rc = S.upload(&pd, conc0))) return rc; d.ion_conc0 = pd;
+    if ((rc = S.upload(&p64, L.node_ion_off))) return rc; d.node_ion_off = p64;
   }
   {
     GroupDev* pg;
@@ -822,10 +852,22 @@ int mind_initialize(mind_state* state) {
                            sizeof(double) * S.L.groups[gi].data.size(), hipMemcpyDeviceToDevice, S.stream));
   hipLaunchKernelGGL(mind::k_init_nodes, dim3(mind::blocks_for(std::max(d.nodes, d.ndet), 256)), dim3(256), 0,
                      S.stream, d, S.v_init);
+  if (d.nion) {  // ion rows restored, concentrations reset (cinit), initial reversals
+    HIP_TRY(hipMemcpyAsync(d.ion, S.ion_initial, sizeof(double) * S.L.ion_data.size(), hipMemcpyDeviceToDevice,
+                           S.stream));
+    hipLaunchKernelGGL(mind::k_init_ions, dim3(mind::blocks_for(d.nion, 256)), dim3(256), 0, S.stream, d);
+  }
   for (int gi = 0; gi < d.ngroups; ++gi) {
-    if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
+    const mind::MechGroup& G = S.L.groups[gi];
+    if (G.mech == MIND_MECH_NETSTIM) continue;
     HIP_TRY(mind::launch(S, mind::K_INIT_GROUP, (const void*)mind::k_init_group,
-                         dim3(mind::blocks_for(S.L.groups[gi].rows, 256)), dim3(256), 0, S.stream, d, gi));
+                         dim3(mind::blocks_for(G.rows, 256)), dim3(256), 0, S.stream, d, gi));
+    bool writes_concentration = false;
+    for (int k = G.nread; k < G.nread + G.nwrite; ++k)
+      writes_concentration |= G.ion_bind[k].column == MIND_ION_CI || G.ion_bind[k].column == MIND_ION_CO;
+    if (writes_concentration)
+      hipLaunchKernelGGL(mind::k_init_reversals, dim3(mind::blocks_for(G.rows, 256)), dim3(256), 0, S.stream, d,
+                         gi);
   }
   if (d.ns_rows > 0 && !S.serial) {
     hipLaunchKernelGGL(mind::k_init_netstim, dim3(mind::blocks_for(d.ns_rows, 256)), dim3(256), 0, S.stream, d,
@@ -844,6 +886,8 @@ int mind_initialize(mind_state* state) {
     if (int rcs = mind::stage_injections(S, 0)) return rcs;
     hipLaunchKernelGGL(mind::k_exchange, dim3(1), dim3(mind::kBlock), 0, S.stream, d, 0);
   }
+  if (d.nion)
+    hipLaunchKernelGGL(mind::k_prepare_ions, dim3(mind::blocks_for(d.nion, 256)), dim3(256), 0, S.stream, d);
   for (int gi = 0; gi < d.ngroups; ++gi) {
     if (S.L.groups[gi].mech == MIND_MECH_NETSTIM) continue;
     HIP_TRY(mind::launch(S, mind::K_CURRENTS_FIELDS, (const void*)mind::k_currents_fields,
@@ -889,6 +933,15 @@ static int resolve_records(State& S, const mind_record* recs, int64_t n, std::ve
       if (r.field < 0 || r.field >= mind::field_count(r.mech)) return fail(MIND_EARG, "unknown field");
       kind.push_back(1); group.push_back(g); field.push_back(r.field);
       index.push_back(it - nodes.begin());
+      cell.push_back(r.cell);
+    } else if (r.kind == 3) {  // ion_ref (references.jl:156-178)
+      if (r.cell < 0 || r.cell >= L.ncells) return fail(MIND_EARG, "record on an unknown cell");
+      if (r.mech < 0 || r.mech >= (int32_t)L.species.size()) return fail(MIND_EARG, "unknown ion species");
+      if (r.field < MIND_ION_E || r.field > MIND_ION_DIDV) return fail(MIND_EARG, "unknown ion field");
+      const int64_t q = L.ion_row_at(L.range_node(r.cell, r.section, r.x), r.mech);
+      if (q < 0) return fail(MIND_EARG, "ion " + L.species[r.mech].name + " is not present at this location");
+      kind.push_back(3); group.push_back(-1); field.push_back(r.field);
+      index.push_back(q);
       cell.push_back(r.cell);
     } else {
       return fail(MIND_EARG, "unknown record kind");
@@ -1119,7 +1172,9 @@ int mind_read_value(mind_state* state, const mind_record* record, double* out) {
   }
   HIP_TRY(hipSetDevice(S.device));
   HIP_TRY(hipStreamSynchronize(S.stream));
-  const double* src = kind == 0 ? S.d.v + index : S.groups_host[g].data + (size_t)field * L.groups[g].rows + index;
+  const double* src = kind == 0   ? S.d.v + index
+                      : kind == 3 ? S.d.ion + (size_t)field * L.nion + index
+                                  : S.groups_host[g].data + (size_t)field * L.groups[g].rows + index;
   HIP_TRY(hipMemcpy(out, src, sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
@@ -1227,6 +1282,7 @@ uint64_t fingerprint(const State& S) {
   h = fnv(h, L.edge_target.data(), L.edge_target.size() * sizeof(int32_t));
   h = fnv(h, L.edge_delay.data(), L.edge_delay.size() * sizeof(double));
   h = fnv(h, &S.d.ndet, sizeof S.d.ndet);
+  h = fnv(h, &L.nion, sizeof L.nion);
   h = fnv(h, &S.d.ring_slots, sizeof S.d.ring_slots);
   h = fnv(h, &S.d.ring_cap, sizeof S.d.ring_cap);
   return h;
@@ -1240,6 +1296,7 @@ std::vector<std::pair<void*, size_t>> ckpt_arrays(State& S) {
   for (size_t gi = 0; gi < S.groups_host.size(); ++gi)
     a.push_back({S.groups_host[gi].data, sizeof(double) * S.L.groups[gi].data.size()});
   a.push_back({d.det_above, sizeof(uint8_t) * d.ndet});
+  a.push_back({d.ion, sizeof(double) * S.L.ion_data.size()});
   a.push_back({d.ns_stream, sizeof(mind::Stream) * d.ns_rows});
   a.push_back({d.ns_t, sizeof(double) * d.ns_rows});
   a.push_back({d.ns_flag, sizeof(double) * d.ns_rows});

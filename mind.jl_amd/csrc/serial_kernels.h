@@ -149,7 +149,7 @@ struct Engine {  // lane 0's view of the queue
     const double w = ev.edge == 0 ? 0.0 : d.edge_weight[ev.edge - 1];
     d.clock->delivered += 1;
     if (G.mech != MIND_MECH_NETSTIM) {
-      phase_receive(G.mech, cols(G), row, ev.time, w);
+      receive_event(d, G, row, ev.time, w);
       return false;
     }
     Stream s = d.ns_stream[row];

@@ -48,6 +48,7 @@ struct State {
   std::vector<void*> allocations;
   std::vector<GroupDev> groups_host;
   std::vector<double*> group_initial;  // parameter snapshot restored by initialize
+  double* ion_initial = nullptr;       // packed ion rows, restored by initialize
   double* v_init = nullptr;
   uint64_t* ns_logical = nullptr;
   int32_t wmax = 1;

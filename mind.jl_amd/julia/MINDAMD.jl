@@ -85,6 +85,19 @@ struct ModelDesc
     con_delay::Ptr{Float64}
     pt_spike_offsets::Ptr{Int64}   # PatternStim spike times (CSR over points), or NULL
     pt_spike_times::Ptr{Float64}
+    n_species::Int32
+    species_name::Ptr{Cstring}
+    species_valence::Ptr{Float64}
+    species_internal::Ptr{Float64}
+    species_external::Ptr{Float64}
+    species_reversal::Ptr{Float64}
+    species_nernst::Ptr{Int32}
+    n_ion_settings::Int64
+    ionset_cell::Ptr{Int32}
+    ionset_section::Ptr{Int32}
+    ionset_species::Ptr{Int32}
+    ionset_what::Ptr{Int32}
+    ionset_value::Ptr{Float64}
 end
 
 struct BuildOptions
@@ -168,6 +181,19 @@ Base.@kwdef mutable struct ModelArrays
     con_delay::Vector{Float64} = Float64[]
     pt_spike_offsets::Vector{Int64} = Int64[]   # empty, or one offset per point + 1
     pt_spike_times::Vector{Float64} = Float64[]
+    # ion species (empty: default_ions) and section ion settings (include/mind.h)
+    species_name::Vector{String} = String[]
+    species_valence::Vector{Float64} = Float64[]
+    species_internal::Vector{Float64} = Float64[]
+    species_external::Vector{Float64} = Float64[]
+    species_reversal::Vector{Float64} = Float64[]
+    species_nernst::Vector{Int32} = Int32[]
+    ionset_cell::Vector{Int32} = Int32[]
+    ionset_section::Vector{Int32} = Int32[]
+    ionset_species::Vector{Int32} = Int32[]
+    ionset_what::Vector{Int32} = Int32[]
+    ionset_value::Vector{Float64} = Float64[]
+    species_ptrs::Vector{Ptr{UInt8}} = Ptr{UInt8}[]  # filled by descriptor
 end
 
 function descriptor(a::ModelArrays)
@@ -181,7 +207,16 @@ function descriptor(a::ModelArrays)
               p(a.param_value), length(a.src_kind), p(a.src_kind), p(a.src_cell), p(a.src_section),
               p(a.src_x), p(a.src_threshold), p(a.src_point), p(a.src_sid), length(a.con_source),
               p(a.con_source), p(a.con_target), p(a.con_weight), p(a.con_delay),
-              p(a.pt_spike_offsets), p(a.pt_spike_times))
+              p(a.pt_spike_offsets), p(a.pt_spike_times),
+              Int32(length(a.species_name)), Ptr{Cstring}(p(species_pointers!(a))), p(a.species_valence),
+              p(a.species_internal), p(a.species_external), p(a.species_reversal), p(a.species_nernst),
+              length(a.ionset_cell), p(a.ionset_cell), p(a.ionset_section), p(a.ionset_species),
+              p(a.ionset_what), p(a.ionset_value))
+end
+
+function species_pointers!(a::ModelArrays)
+    a.species_ptrs = [pointer(n) for n in a.species_name]
+    return a.species_ptrs
 end
 
 # ---- calls -----------------------------------------------------------------------

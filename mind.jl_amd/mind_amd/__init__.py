@@ -6,7 +6,8 @@ build, initialize, run, step, *_record) over libmind's C ABI.
 """
 from .mechanisms import Mechanism, builtin, compile_mod, load_mod  # noqa: F401
 from .model import (  # noqa: F401
-    Cell, CellSection, MicroModel, Morphology, PointProcess, Section, SimulationResult, SpikeSource,
+    Cell, CellSection, IonSpecies, IonStyle, MicroModel, Morphology, PointProcess, Section, SimulationResult,
+    SpikeSource, default_ions, ion_style,
     add_pointprocess, add_population, batch_plan, build, checkpoint, connect, connection_weights, destroy,
     restore,
     initialize, inject_event, inject_spike, insert, ion_record, read_value, section, section_group,

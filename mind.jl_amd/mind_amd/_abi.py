@@ -77,6 +77,19 @@ class ModelDesc(C.Structure):
         ("con_delay", _p(C.c_double)),
         ("pt_spike_offsets", _p(C.c_int64)),
         ("pt_spike_times", _p(C.c_double)),
+        ("n_species", C.c_int32),
+        ("species_name", _p(C.c_char_p)),
+        ("species_valence", _p(C.c_double)),
+        ("species_internal", _p(C.c_double)),
+        ("species_external", _p(C.c_double)),
+        ("species_reversal", _p(C.c_double)),
+        ("species_nernst", _p(C.c_int32)),
+        ("n_ion_settings", C.c_int64),
+        ("ionset_cell", _p(C.c_int32)),
+        ("ionset_section", _p(C.c_int32)),
+        ("ionset_species", _p(C.c_int32)),
+        ("ionset_what", _p(C.c_int32)),
+        ("ionset_value", _p(C.c_double)),
     ]
 
 
@@ -201,6 +214,9 @@ def lib():
     L.mind_mech_info.argtypes = [C.c_int32, _p(MechDesc)]
     L.mind_mech_fields.argtypes = [C.c_int32, _p(C.c_char_p), _p(C.c_double), C.c_int32, _p(C.c_int32)]
     L.mind_mech_lookup.argtypes = [C.c_char_p, _p(C.c_int32)]
+    L.mind_mech_ions.argtypes = [C.c_int32, C.c_int32, _p(C.c_char_p), _p(C.c_int32), _p(C.c_double),
+                                 _p(C.c_int32), _p(C.c_int32), _p(C.c_int32)]
+    L.mind_mech_columns.argtypes = [C.c_int32, _p(C.c_uint8), _p(C.c_uint8), C.c_int32, _p(C.c_int32)]
     L.mind_mech_lowering.argtypes = [C.c_int32, _p(C.c_char_p)]
     L.mind_rtc_check.argtypes = [_p(C.c_int32), C.c_int32, C.c_char_p]
     _lib = L

@@ -130,6 +130,53 @@ def lowering(mechanism: Mechanism) -> str:
     return text.value.decode()
 
 
+@dataclass(frozen=True)
+class IonUse:
+    """One USEION statement: the ion, its VALENCE (None if not declared) and per ion
+    column (reversal, internal, external concentration, current) the field column
+    read into / written from (-1 for none); reads include written concentrations."""
+
+    name: str
+    valence: float | None
+    read: tuple
+    write: tuple
+
+
+_IONS: dict = {}
+
+
+def ions(mechanism) -> tuple:
+    """The USEION statements of a mechanism (hh: na and k), from libmind."""
+    from . import _abi
+
+    mech = resolve(mechanism)
+    if mech.id not in _IONS:
+        L = _abi.lib()
+        out, count, k = [], C.c_int32(0), 0
+        while True:
+            name, has_v, val = C.c_char_p(), C.c_int32(0), C.c_double(0.0)
+            rd, wr = (C.c_int32 * 4)(), (C.c_int32 * 4)()
+            _abi.check(L.mind_mech_ions(mech.id, k, C.byref(name), C.byref(has_v), C.byref(val), rd, wr,
+                                        C.byref(count)))
+            if k >= count.value:
+                break
+            out.append(IonUse(name.value.decode(), val.value if has_v.value else None, tuple(rd), tuple(wr)))
+            k += 1
+        _IONS[mech.id] = tuple(out)
+    return _IONS[mech.id]
+
+
+def columns_info(mechanism):
+    """Per column: (is a STATE variable, touched by NET_RECEIVE), from libmind."""
+    from . import _abi
+
+    mech = resolve(mechanism)
+    n = len(mech.fields)
+    st, rc, count = (C.c_uint8 * max(1, n))(), (C.c_uint8 * max(1, n))(), C.c_int32(0)
+    _abi.check(_abi.lib().mind_mech_columns(mech.id, st, rc, n, C.byref(count)))
+    return tuple(bool(st[k]) for k in range(n)), tuple(bool(rc[k]) for k in range(n))
+
+
 def builtin(name) -> Mechanism:
     """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim,
     AlphaSynapse, PatternStim, SEClamp, VClamp)."""

@@ -177,11 +177,76 @@ class SimulationResult:
     run_seconds: float = 0.0
 
 
+@dataclass(frozen=True)
+class IonSpecies:
+    """IonSpecies (src/model/micro/biophysics.jl:3-25): valence, concentrations (mM),
+    reversal potential (mV) and whether the reversal follows the Nernst equation."""
+
+    name: str
+    valence: float
+    internal: float
+    external: float
+    reversal: float = 0.0
+    nernst: bool = False
+
+    def __post_init__(self):
+        if not np.isfinite(self.valence):
+            raise ValueError("ion valence must be finite")
+        if not all(np.isfinite(x) for x in (self.internal, self.external, self.reversal)):
+            raise ValueError("non-finite ion parameter")
+        if self.internal < 0 or self.external < 0:
+            raise ValueError("ion concentrations must be nonnegative")
+
+
+@dataclass(frozen=True)
+class IonStyle:
+    """IonStyle (biophysics.jl:27-52): c_style and e_style 0 (unused), 1 (parameter),
+    2 (assigned), 3 (state); einit / eadvance request the Nernst reversal at
+    initialization / every step; cinit resets concentrations to the species'."""
+
+    c_style: int = 0
+    e_style: int = 0
+    einit: bool = False
+    eadvance: bool = False
+    cinit: bool = False
+
+    def __post_init__(self):
+        if not all(isinstance(v, (int, np.integer)) and 0 <= v <= 3 for v in (self.c_style, self.e_style)):
+            raise ValueError("ion variable styles must be integers from 0 to 3")
+
+    def bits(self) -> int:
+        """ion_style_bits (utils/biophysics.jl:3-4)."""
+        return (self.c_style + 4 * int(bool(self.cinit)) + 8 * self.e_style + 32 * int(bool(self.einit))
+                + 64 * int(bool(self.eadvance)))
+
+
+ion_style = IonStyle
+
+
+def default_ions():
+    """default_ions (utils/biophysics.jl:31-42)."""
+    return {"na": IonSpecies("na", 1.0, 10.0, 140.0, 50.0),
+            "k": IonSpecies("k", 1.0, 54.4, 2.5, -77.0),
+            "ca": IonSpecies("ca", 2.0, 5.0e-5, 2.0, 132.4579341637009)}
+
+
+_POLICIES = {"automatic": 0, "fixed": 1, "initial": 2, "advance": 3, "dynamic": 4}
+# ion record fields -> column: e<ion>, <ion>i, <ion>o, i<ion>, di<ion>_dv_ (references.jl:161-172)
+_ION_COLUMNS = ("e{}", "{}i", "{}o", "i{}", "di{}_dv_")
+
+
 class MicroModel:
-    def __init__(self, celsius=6.3):
+    def __init__(self, celsius=6.3, ions=()):
         if not (np.isfinite(celsius) and celsius > -273.15):
             raise ValueError("invalid temperature")
         self.celsius = float(celsius)
+        self.ions = default_ions()
+        for ion in (ions.values() if isinstance(ions, dict) else ions):
+            if not isinstance(ion, IonSpecies):
+                raise ValueError("ions takes IonSpecies values")
+            self.ions[ion.name] = ion
+        # section ion settings: (cell, section, ion name, what, value)
+        self.ion_settings: list = []
         self.morphologies: list[Morphology] = []
         self.cell_morph: list[int] = []
         self.cell_gid: list[int] = []
@@ -204,8 +269,12 @@ class MicroModel:
         self.options = None
 
     # -- cells ---------------------------------------------------------------
-    def add_population(self, sections, count, first_gid=None, voltage=-65.0):
-        """Create `count` cells sharing one morphology (model.jl:217-283)."""
+    def add_population(self, sections, count, first_gid=None, voltage=-65.0, ions=None):
+        """Create `count` cells sharing one morphology (model.jl:217-283). ions maps a
+        section selector to {ion: {internal, external, reversal, reversal_policy | style}}
+        (cell_ion_values, utils/model.jl:280-320): values on every segment of the
+        selected sections, the reversal policy (automatic, fixed, initial, advance,
+        dynamic) or an IonStyle; any entry also places the ion there."""
         if count < 0:
             raise ValueError("population size must be nonnegative")
         morph = sections if isinstance(sections, Morphology) else Morphology.from_sections(sections)
@@ -228,6 +297,33 @@ class MicroModel:
             self.cell_gid.append(gid)
             self.cell_voltage.append(float(value))
             cells.append(Cell(self, first_cell + i, gid))
+        for selector, species in (ions or {}).items():
+            secs = morph.select(selector)
+            for ion, cfg in species.items():
+                name = str(ion).lstrip(":")
+                policy, style = cfg.get("reversal_policy"), cfg.get("style")
+                if policy is not None and style is not None:
+                    raise ValueError("choose either an ion style or a reversal policy")
+                if policy is not None and str(policy).lstrip(":") not in _POLICIES:
+                    raise ValueError("unknown ion reversal policy")
+                if style is not None and not isinstance(style, IonStyle):
+                    raise ValueError("style must be an IonStyle")
+                for c in cells:
+                    for sec in secs:
+                        for what, key in ((0, "internal"), (1, "external"), (2, "reversal")):
+                            if cfg.get(key) is None:
+                                continue
+                            v = float(cfg[key])
+                            if not np.isfinite(v):
+                                raise ValueError("non-finite ion value")
+                            if what < 2 and v < 0:
+                                raise ValueError("negative ion concentration")
+                            self.ion_settings.append((c.index, sec, name, what, v))
+                        if style is not None:
+                            self.ion_settings.append((c.index, sec, name, 4, float(style.bits())))
+                        else:
+                            pol = _POLICIES[str(policy).lstrip(":")] if policy is not None else 0
+                            self.ion_settings.append((c.index, sec, name, 3, float(pol)))
         return cells
 
     def _morph(self, cell: Cell) -> Morphology:
@@ -442,7 +538,45 @@ class MicroModel:
         d.con_target = _abi.as_ptr(arr("kt", [e[1] for e in K], np.int32), C.c_int32)
         d.con_weight = _abi.as_ptr(arr("kw", [e[2] for e in K], np.float64), C.c_double)
         d.con_delay = _abi.as_ptr(arr("kd", [e[3] for e in K], np.float64), C.c_double)
+        species = self.species_list()
+        index = {s.name: k for k, s in enumerate(species)}
+        names = (C.c_char_p * len(species))(*[s.name.encode() for s in species])
+        keep["spn"] = names
+        d.n_species = len(species)
+        d.species_name = C.cast(names, C.POINTER(C.c_char_p))
+        d.species_valence = _abi.as_ptr(arr("spv", [s.valence for s in species], np.float64), C.c_double)
+        d.species_internal = _abi.as_ptr(arr("spi", [s.internal for s in species], np.float64), C.c_double)
+        d.species_external = _abi.as_ptr(arr("spe", [s.external for s in species], np.float64), C.c_double)
+        d.species_reversal = _abi.as_ptr(arr("spr", [s.reversal for s in species], np.float64), C.c_double)
+        d.species_nernst = _abi.as_ptr(arr("spz", [int(s.nernst) for s in species], np.int32), C.c_int32)
+        I = self.ion_settings
+        for e in I:
+            if e[2] not in index:
+                raise ValueError(f"unknown ion {e[2]}")
+        d.n_ion_settings = len(I)
+        d.ionset_cell = _abi.as_ptr(arr("isc", [e[0] for e in I], np.int32), C.c_int32)
+        d.ionset_section = _abi.as_ptr(arr("iss", [e[1] for e in I], np.int32), C.c_int32)
+        d.ionset_species = _abi.as_ptr(arr("isp", [index[e[2]] for e in I], np.int32), C.c_int32)
+        d.ionset_what = _abi.as_ptr(arr("isw", [e[3] for e in I], np.int32), C.c_int32)
+        d.ionset_value = _abi.as_ptr(arr("isv", [e[4] for e in I], np.float64), C.c_double)
         return d, keep
+
+    def species_list(self):
+        """Species in descriptor order: the model's, then those only a used mechanism
+        declares (with its VALENCE, 1 mM on both sides; model.jl:690-703)."""
+        from .mechanisms import BY_ID, ions
+
+        out = list(self.ions.values())
+        names = {s.name for s in out}
+        used = [e[2] for e in self.density] + [p[0] for p in self.points]
+        for mid in dict.fromkeys(used):
+            for u in ions(BY_ID[mid]):
+                if u.name not in names:
+                    if u.valence is None:
+                        raise ValueError(f"USEION {u.name} needs a VALENCE declaration")
+                    out.append(IonSpecies(u.name, u.valence, 1.0, 1.0, 0.0))
+                    names.add(u.name)
+        return out
 
     # -- records -------------------------------------------------------------
     def record_struct(self, spec):
@@ -515,23 +649,22 @@ def mechanism_record(model, target, *args, name=None, section="soma", x=0.5, ind
                       (2, target.index, s, float(x), -1, mech.id, mech.field_index(str(fname), index)))
 
 
-# ion columns of the six mechanisms: hh reads ena/ek and writes ina/ik
-# (USEION na READ ena WRITE ina, USEION k READ ek WRITE ik)
-_ION_FIELDS = {"ena": "ena", "ek": "ek", "ina": "ina", "ik": "ik"}
-
-
 def ion_record(model, target, fname, name=None, section="soma", x=0.5):
     """ion_record(model, cell, field; section, x) or ion_record(model, cell_section, field; x)
-    (recording.jl:40-46, 64-70; ion_ref, references.jl:157-178). The runtime keeps the
-    ion state of the six mechanisms in hh's rows: the reversal potentials ena/ek it
-    reads and the currents ina/ik it writes (the only writer of those currents).
-    Concentrations and di/dv columns are not stored (no mechanism here reads them)."""
+    (recording.jl:40-46, 64-70; ion_ref, references.jl:156-178): a reversal potential,
+    concentration, current or current derivative such as ena, cai, cao, ica or
+    dica_dv_ of the ion row at the location."""
     key = str(fname).lstrip(":")
-    if key not in _ION_FIELDS:
-        raise ValueError(f"ion field {key} is not stored by this runtime (ena, ek, ina, ik are)")
-    if isinstance(target, CellSection):
-        return mechanism_record(model, target, "hh", _ION_FIELDS[key], name=name or key, x=x)
-    return mechanism_record(model, target, "hh", _ION_FIELDS[key], name=name or key, section=section, x=x)
+    species = model.species_list()
+    for k, sp in enumerate(species):
+        for column, pattern in enumerate(_ION_COLUMNS):
+            if key == pattern.format(sp.name):
+                if isinstance(target, CellSection):
+                    cell, s = target.cell, model._morph(target.cell).section_index(target.name)
+                else:
+                    cell, s = target, model._morph(target).section_index(section)
+                return RecordSpec(name or key, (3, cell.index, s, float(x), -1, k, column))
+    raise ValueError(f"unknown ion field {key}")
 
 
 def read_value(model, spec):

@@ -40,6 +40,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <set>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -262,15 +265,27 @@ const bool kPoint[MIND_MECH_COUNT] = {false, false, true, true, true, true, true
 
 // A runtime-compiled mechanism: its phases come from a host build (g++) of the
 // library's lowering of its MOD text, registered by oracle_register_mech.
-using UserCurrent = void (*)(double* data, int64_t rows, int64_t r, double v, double t, double dt, double* value,
-                             double* shifted, int32_t store);
+using UserCurrent = void (*)(double* data, int64_t rows, int64_t r, double v, double t, double dt, double celsius,
+                             double* value, double* shifted, int32_t store, double* ish);
 using UserAdvance = void (*)(double* data, int64_t rows, int64_t r, double v, double t, double dt, double celsius);
 using UserInitial = void (*)(double* data, int64_t rows, int64_t r, double v, double celsius);
 using UserReceive = void (*)(double* data, int64_t rows, int64_t r, double t, double weight);
+// One USEION statement: per ion column (0 reversal, 1 internal, 2 external
+// concentration, 3 current) the field read into / written from, -1 if none.
+struct IonDecl {
+  std::string name;
+  bool has_valence = false;
+  double valence = 0.0;
+  int read[4] = {-1, -1, -1, -1};
+  int write[4] = {-1, -1, -1, -1};
+};
 struct UserMech {
   int nfields = 0;
   bool point = false, has_state = false;
   std::vector<double> defaults;
+  std::vector<IonDecl> ions;
+  std::vector<uint8_t> state_col;
+  std::vector<int> recv_fields;  // columns NET_RECEIVE touches
   UserCurrent current = nullptr;
   UserAdvance advance = nullptr;
   UserInitial initial = nullptr;
@@ -279,6 +294,20 @@ struct UserMech {
 UserMech g_user[MIND_MECH_SLOTS];
 
 int field_count(int mech) { return mech < MIND_MECH_COUNT ? kFields[mech] : g_user[mech].nfields; }
+
+// USEION statements (hh.mod: USEION na READ ena WRITE ina, USEION k READ ek WRITE ik).
+const std::vector<IonDecl>& ion_decls(int mech) {
+  static const std::vector<IonDecl> none;
+  static const std::vector<IonDecl> hh = [] {
+    IonDecl na, k;
+    na.name = "na"; na.read[0] = HH_ENA; na.write[3] = HH_INA;
+    k.name = "k"; k.read[0] = HH_EK; k.write[3] = HH_IK;
+    return std::vector<IonDecl>{na, k};
+  }();
+  if (mech == MIND_MECH_HH) return hh;
+  if (mech >= MIND_MECH_COUNT && mech < MIND_MECH_SLOTS) return g_user[mech].ions;
+  return none;
+}
 bool is_point(int mech) { return mech < MIND_MECH_COUNT ? kPoint[mech] : g_user[mech].point; }
 
 void defaults(int mech, double* f) {  // PARAMETER defaults of the MOD files
@@ -336,6 +365,25 @@ inline double relax(double x, double xinf, double tau, double dt) {
   return x + (1.0 - mind_exp(dt * (-1.0 / tau))) * (xinf - x);
 }
 
+// Nernst potential and IonStyle flags (src/model/micro/utils/biophysics.jl:3-70).
+double nernst(double ci, double co, double z, double celsius) {
+  if (z == 0.0) return 0.0;
+  if (ci <= 0.0) return 1.0e6;
+  if (co <= 0.0) return -1.0e6;
+  const double r = 1.380649e-23 * 6.02214076e23, f = 1.602176634e-19 * 6.02214076e23;
+  const double rt_f = 1000.0 * r * (celsius + 273.15) / f;
+  return rt_f / z * mind_log(co / ci);
+}
+uint8_t style_bits(int c_style, int e_style, bool einit, bool eadvance, bool cinit) {
+  return (uint8_t)(c_style | (cinit ? 0x04 : 0) | (e_style << 3) | (einit ? 0x20 : 0) | (eadvance ? 0x40 : 0));
+}
+uint8_t promote(uint8_t style, int c_style, int e_style) {
+  const int c = std::max<int>(style & 0x03, c_style);
+  int e = std::max<int>((style >> 3) & 0x03, e_style);
+  if (c > 0 && e < 2) e = 2;
+  return style_bits(c, e, c > 0 && e == 2, c == 3 && e == 2, c == 3);
+}
+
 // ------------------------------------------------------------------ Model ---
 struct Group {
   int mech;
@@ -346,6 +394,8 @@ struct Group {
   std::vector<double> initial;   // data snapshot after parameters (restored by initialize)
   std::vector<Rng> rng;
   std::vector<int64_t> cell_rows;  // ncells+1: rows of cell c are [cell_rows[c], cell_rows[c+1])
+  std::vector<int32_t> slot_species;  // species of each USEION statement
+  std::vector<int32_t> ion_row;       // ion row of (row, statement), row-major; -1 none
   int rows() const { return (int)node.size(); }
   double& at(int field, int row) { return data[(size_t)field * node.size() + row]; }
 };
@@ -377,6 +427,14 @@ struct Sim {
   std::vector<int32_t> cell_morph;
   std::vector<Template> templates;
   std::vector<std::vector<SectionIn>> morph_secs;
+  // ions (model.jl:672-855): rows per (node, species), sorted by node then name
+  struct Species { std::string name; double z, ci, co, e; bool nernst; };
+  std::vector<Species> species;
+  int64_t nion = 0;
+  std::vector<int32_t> ion_sp;
+  std::vector<uint8_t> ion_style;
+  std::vector<double> ion[5], ion_packed[5];  // e, ci, co, i, di/dv
+  std::vector<int64_t> ion_off;               // per node (CSR)
   // mechanisms
   std::vector<Group> groups;
   std::vector<int> group_of_mech;       // mech -> group index or -1
@@ -429,6 +487,71 @@ thread_local std::string g_error;
 int fail(int code, const std::string& message) {
   g_error = message;
   return code;
+}
+
+// ------------------------------------------------------------------ Ions ----
+// prepare_currents_kernel! for ion row q (cuda/micro.jl:601-629).
+void ion_prepare(Sim& s, int64_t q, uint8_t mask) {
+  s.ion[3][q] = 0.0;
+  s.ion[4][q] = 0.0;
+  if (s.ion_style[q] & mask) s.ion[0][q] = nernst(s.ion[1][q], s.ion[2][q], s.species[s.ion_sp[q]].z, s.celsius);
+}
+// load_ions! / store_ions! (cuda/micro.jl:376-392)
+void ion_load(Sim& s, Group& g, int r) {
+  const std::vector<IonDecl>& decl = ion_decls(g.mech);
+  for (size_t k = 0; k < decl.size(); ++k) {
+    const int32_t q = g.ion_row[(size_t)r * decl.size() + k];
+    if (q < 0) continue;
+    for (int c = 0; c < 4; ++c)
+      if (decl[k].read[c] >= 0) g.at(decl[k].read[c], r) = s.ion[c][q];
+  }
+}
+void ion_store(Sim& s, Group& g, int r) {
+  const std::vector<IonDecl>& decl = ion_decls(g.mech);
+  for (size_t k = 0; k < decl.size(); ++k) {
+    const int32_t q = g.ion_row[(size_t)r * decl.size() + k];
+    if (q < 0) continue;
+    for (int c = 0; c < 3; ++c)
+      if (decl[k].write[c] >= 0) s.ion[c][q] = g.at(decl[k].write[c], r);
+  }
+}
+// The ion writes of nrn_cur (mechanism_current_instance!, cuda/micro.jl:474-497):
+// currents summed with their derivative from the v + 0.001 probe `ish`, scaled
+// by 100/area for point processes; other writes stored.
+void ion_current_writes(Sim& s, Group& g, int r, const double* ish) {
+  const std::vector<IonDecl>& decl = ion_decls(g.mech);
+  int probe = 0;
+  for (size_t k = 0; k < decl.size(); ++k) {
+    const int32_t q = g.ion_row[(size_t)r * decl.size() + k];
+    const bool cur = decl[k].write[3] >= 0;
+    if (q >= 0) {
+      for (int c = 0; c < 3; ++c)
+        if (decl[k].write[c] >= 0) s.ion[c][q] = g.at(decl[k].write[c], r);
+      if (cur) {
+        const double scale = is_point(g.mech) ? 100.0 / s.area[g.node[r]] : 1.0;
+        const double x = g.at(decl[k].write[3], r);
+        s.ion[3][q] += scale * x;
+        s.ion[4][q] += scale * (ish[probe] - x) / 0.001;
+      }
+    }
+    probe += cur ? 1 : 0;
+  }
+}
+// receive_ions! (cuda/micro.jl:324-337): concentrations NET_RECEIVE mentions.
+void ion_receive(Sim& s, Group& g, int r, bool publish) {
+  if (g.mech < MIND_MECH_COUNT) return;
+  const UserMech& u = g_user[g.mech];
+  const std::vector<IonDecl>& decl = u.ions;
+  for (size_t k = 0; k < decl.size(); ++k) {
+    const int32_t q = g.ion_row[(size_t)r * decl.size() + k];
+    if (q < 0) continue;
+    for (int c = 1; c < 3; ++c) {
+      const int f = decl[k].read[c];
+      if (f < 0 || std::find(u.recv_fields.begin(), u.recv_fields.end(), f) == u.recv_fields.end()) continue;
+      if (!publish) g.at(f, r) = s.ion[c][q];
+      else if (u.state_col[f]) s.ion[c][q] = g.at(f, r);
+    }
+  }
 }
 
 // --------------------------------------------------------------- Events -----
@@ -607,7 +730,11 @@ void receive(Sim& s, int gi, int row, const Event& ev) {
     case MIND_MECH_EXPSYN: g.at(EXPSYN_G, row) = g.at(EXPSYN_G, row) + w; break;
     case MIND_MECH_NETSTIM: netstim_receive(s, g, row, t, edge, w, flag); break;
     default:
-      if (g.mech >= MIND_MECH_COUNT) g_user[g.mech].receive(g.data.data(), g.rows(), row, t, w);
+      if (g.mech >= MIND_MECH_COUNT) {
+        ion_receive(s, g, row, false);
+        g_user[g.mech].receive(g.data.data(), g.rows(), row, t, w);
+        ion_receive(s, g, row, true);
+      }
       break;
   }
 }
@@ -669,6 +796,7 @@ void vclamp_clamp(Group& g, int r, double v, double dt, double stim, double e0, 
 void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (step.jl:208-224)
   std::fill(s.rhs.begin() + s.cell_offset[c0], s.rhs.begin() + s.cell_offset[c1], 0.0);
   std::fill(s.diag.begin() + s.cell_offset[c0], s.diag.begin() + s.cell_offset[c1], 0.0);
+  for (int64_t q = s.ion_off[s.cell_offset[c0]]; q < s.ion_off[s.cell_offset[c1]]; ++q) ion_prepare(s, q, 0x40);
   for (int pass = 0; pass < 2; ++pass) {  // density groups, then point groups
     for (Group& g : s.groups) {
       if (is_point(g.mech) != (pass == 1)) continue;
@@ -678,11 +806,15 @@ void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (s
         if (node < 0) continue;
         const double v = s.v[node];
         double shifted = 0.0, value = 0.0;
+        double ish[8] = {0};  // written ion currents at v + 0.001
+        const bool ions = !ion_decls(g.mech).empty();
+        if (ions) ion_load(s, g, (int)r);
         if (g.mech >= MIND_MECH_COUNT) {  // both probes inside the compiled phase
-          g_user[g.mech].current(g.data.data(), g.rows(), r, v, t, s.dt, &value, &shifted, 1);
+          g_user[g.mech].current(g.data.data(), g.rows(), r, v, t, s.dt, s.celsius, &value, &shifted, 1, ish);
           const double scale = is_point(g.mech) ? 1.0 : s.area[node] * 0.01;
           s.rhs[node] -= scale * value;
           s.diag[node] += scale * (shifted - value) / 0.001;
+          if (ions) ion_current_writes(s, g, (int)r, ish);
           continue;
         }
         for (int probe = 0; probe < 2; ++probe) {
@@ -700,6 +832,10 @@ void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (s
               g.at(HH_GNA, r) = gna; g.at(HH_INA, r) = ina;
               g.at(HH_GK, r) = gk; g.at(HH_IK, r) = ik; g.at(HH_IL, r) = il;
               sum = 0.0; sum += il; sum += ina; sum += ik;  // currents, then ion writes
+              if (probe == 0) {
+                ish[0] = ina;
+                ish[1] = ik;
+              }
               break;
             }
             case MIND_MECH_PAS: {
@@ -780,6 +916,7 @@ void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (s
         const double scale = is_point(g.mech) ? 1.0 : s.area[node] * 0.01;
         s.rhs[node] -= scale * value;
         s.diag[node] += scale * (shifted - value) / 0.001;
+        if (ions) ion_current_writes(s, g, (int)r, ish);
       }
     }
   }
@@ -818,6 +955,9 @@ int cable(Sim& s, int64_t c0, int64_t c1) {
   // v += dv, or 2 dv with secondorder (update_voltage_kernel!, cuda/micro.jl:660-679; the
   // secondorder 2 ion-current correction writes per-node ion current columns that none of
   // the six mechanisms reads)
+  if (s.secondorder == 2)  // ion currents follow the increment (update_voltage_kernel!)
+    for (int64_t i = a; i < n; ++i)
+      for (int64_t q = s.ion_off[i]; q < s.ion_off[i + 1]; ++q) s.ion[3][q] += s.ion[4][q] * s.rhs[i];
   if (s.secondorder) {
     for (int64_t i = a; i < n; ++i) s.v[i] += 2.0 * s.rhs[i];
   } else {
@@ -834,10 +974,14 @@ void advance_states(Sim& s, int64_t c0, int64_t c1) {  // nrn_state at t + dt (s
     if (g.mech >= MIND_MECH_COUNT) {
       if (!g_user[g.mech].has_state) continue;
       const double t = (double)s.step * dt + dt;
-      for (int64_t r = r0; r < rows; ++r)
+      for (int64_t r = r0; r < rows; ++r) {
+        ion_load(s, g, (int)r);
         g_user[g.mech].advance(g.data.data(), g.rows(), r, s.v[g.node[r]], t, dt, s.celsius);
+        ion_store(s, g, (int)r);
+      }
     } else if (g.mech == MIND_MECH_HH) {
       for (int64_t r = r0; r < rows; ++r) {
+        ion_load(s, g, (int)r);
         double f[HH_NFIELDS];
         for (int k = 0; k < HH_NFIELDS; ++k) f[k] = g.at(k, r);
         hh_rates(f, s.v[g.node[r]], s.celsius);
@@ -845,6 +989,7 @@ void advance_states(Sim& s, int64_t c0, int64_t c1) {  // nrn_state at t + dt (s
         f[HH_H] = relax(f[HH_H], f[HH_HINF], f[HH_HTAU], dt);
         f[HH_N] = relax(f[HH_N], f[HH_NINF], f[HH_NTAU], dt);
         for (int k = 0; k < HH_NFIELDS; ++k) g.at(k, r) = f[k];
+        ion_store(s, g, (int)r);
       }
     } else if (g.mech == MIND_MECH_EXP2SYN) {
       for (int64_t r = r0; r < rows; ++r) {
@@ -924,9 +1069,12 @@ void advance_micro(Sim& s) {  // advance_micro! (step.jl:91-151)
 void initial_blocks(Sim& s) {  // INITIAL per group (initialize.jl:342-374)
   for (size_t gi = 0; gi < s.groups.size(); ++gi) {
     Group& g = s.groups[gi];
+    const std::vector<IonDecl>& decl = ion_decls(g.mech);
     for (int r = 0; r < g.rows(); ++r) {
+      if (!decl.empty()) ion_load(s, g, r);
       if (g.mech >= MIND_MECH_COUNT) {
         g_user[g.mech].initial(g.data.data(), g.rows(), r, s.v[g.node[r]], s.celsius);
+        ion_store(s, g, r);
         continue;
       }
       switch (g.mech) {
@@ -972,8 +1120,18 @@ void initial_blocks(Sim& s) {  // INITIAL per group (initialize.jl:342-374)
         case MIND_MECH_NETSTIM: netstim_initial(s, g, r); break;
         default: break;
       }
+      if (!decl.empty()) ion_store(s, g, r);
     }
     if (g.mech == MIND_MECH_NETSTIM) flush(s);
+    // initialize_concentration_reversals_kernel! (cuda/micro.jl:764-784)
+    for (size_t k = 0; k < decl.size(); ++k) {
+      if (decl[k].write[1] < 0 && decl[k].write[2] < 0) continue;
+      for (int r = 0; r < g.rows(); ++r) {
+        const int32_t q = g.ion_row[(size_t)r * decl.size() + k];
+        if (q >= 0 && (s.ion_style[q] & 0x20))
+          s.ion[0][q] = nernst(s.ion[1][q], s.ion[2][q], s.species[s.ion_sp[q]].z, s.celsius);
+      }
+    }
   }
 }
 
@@ -1000,6 +1158,31 @@ int oracle_register_mech(int32_t mech, int32_t nfields, int32_t point, int32_t h
   u.advance = (UserAdvance)advance;
   u.initial = (UserInitial)initial;
   u.receive = (UserReceive)receive;
+  return 0;
+}
+
+// The USEION statements of a registered mechanism: per statement its ion name,
+// VALENCE and the read / write field of each ion column (4 per statement); the
+// STATE flag of each column and the columns NET_RECEIVE touches.
+int oracle_register_ions(int32_t mech, int32_t n, const char* const* names, const int32_t* has_valence,
+                         const double* valence, const int32_t* reads, const int32_t* writes, const uint8_t* state_col,
+                         int32_t nrecv, const int32_t* recv_fields) {
+  if (mech < MIND_MECH_COUNT || mech >= MIND_MECH_SLOTS) return fail(MIND_EARG, "bad mechanism slot");
+  UserMech& u = g_user[mech];
+  u.ions.clear();
+  for (int32_t k = 0; k < n; ++k) {
+    IonDecl decl;
+    decl.name = names[k];
+    decl.has_valence = has_valence[k] != 0;
+    decl.valence = valence[k];
+    for (int c = 0; c < 4; ++c) {
+      decl.read[c] = reads[4 * k + c];
+      decl.write[c] = writes[4 * k + c];
+    }
+    u.ions.push_back(decl);
+  }
+  u.state_col.assign(state_col, state_col + u.nfields);
+  u.recv_fields.assign(recv_fields, recv_fields + nrecv);
   return 0;
 }
 
@@ -1155,6 +1338,8 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
     row_params[gi].push_back(RowInit{d->pt_param_offsets[p], d->pt_param_offsets[p + 1]});
   }
   // rows sorted by node (stable) except artificial cells, which keep logical order
+  struct IonParam { size_t gi; int row, field; double value; };
+  std::vector<IonParam> ion_params;  // parameters of mechanism fields, in group and row order
   for (size_t gi = 0; gi < s.groups.size(); ++gi) {
     Group& g = s.groups[gi];
     const int rows = g.rows();
@@ -1182,8 +1367,10 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
     for (int f = 0; f < nf; ++f)
       for (int r = 0; r < rows; ++r) h.data[(size_t)f * rows + r] = def[f];
     for (int r = 0; r < rows; ++r)
-      for (int64_t q = rp[r].poff0; q < rp[r].poff1; ++q)
+      for (int64_t q = rp[r].poff0; q < rp[r].poff1; ++q) {
         h.data[(size_t)d->param_field[q] * rows + r] = d->param_value[q];
+        ion_params.push_back({gi, r, d->param_field[q], d->param_value[q]});
+      }
     h.initial = h.data;
     h.rng.resize(rows);
     h.cell_rows.assign(d->n_cells + 1, 0);  // NetStim rows (logical order) belong to no cell phase
@@ -1192,6 +1379,131 @@ void* oracle_build(const mind_model_desc* d, double dt, uint64_t seed) {
         h.cell_rows[c] = std::lower_bound(h.node.begin(), h.node.end(), (int32_t)s.cell_offset[c]) - h.node.begin();
     s.groups[gi] = h;
   }
+  // 3b. ion species, rows, values and styles (model.jl:672-855)
+  if (d->n_species == 0) {
+    s.species = {{"na", 1.0, 10.0, 140.0, 50.0, false},
+                 {"k", 1.0, 54.4, 2.5, -77.0, false},
+                 {"ca", 2.0, 5.0e-5, 2.0, 132.4579341637009, false}};
+  } else {
+    for (int32_t k = 0; k < d->n_species; ++k)
+      s.species.push_back({d->species_name[k], d->species_valence[k], d->species_internal[k],
+                           d->species_external[k], d->species_reversal[k], d->species_nernst[k] != 0});
+  }
+  auto species_index = [&](const std::string& name) {
+    for (size_t k = 0; k < s.species.size(); ++k)
+      if (s.species[k].name == name) return (int)k;
+    return -1;
+  };
+  for (Group& g : s.groups)
+    for (const IonDecl& u : ion_decls(g.mech)) {
+      int k = species_index(u.name);
+      if (k < 0) {
+        if (!u.has_valence) {
+          delete sp;
+          fail(MIND_EARG, "USEION " + u.name + " needs a VALENCE declaration");
+          return nullptr;
+        }
+        s.species.push_back({u.name, u.valence, 1.0, 1.0, 0.0, false});
+        k = (int)s.species.size() - 1;
+      }
+      g.slot_species.push_back(k);
+    }
+  auto section_segments = [&](int cell, int sec) {
+    std::vector<int64_t> out;
+    const std::vector<int32_t>& nodes = s.templates[d->cell_morphology[cell]].sec_nodes[sec];
+    for (size_t j = 0; j + 1 < nodes.size(); ++j) out.push_back(s.cell_offset[cell] + nodes[j]);
+    return out;
+  };
+  std::set<std::pair<int64_t, std::string>> used;
+  for (Group& g : s.groups)
+    for (int r = 0; r < g.rows(); ++r)
+      if (g.node[r] >= 0)
+        for (int k : g.slot_species) used.insert({g.node[r], s.species[k].name});
+  for (int64_t p = 0; p < d->n_points; ++p)  // NEURON need_memb: the point's whole section
+    if (d->pt_cell[p] >= 0 && s.point_group[p] >= 0)
+      for (int k : s.groups[s.point_group[p]].slot_species)
+        for (int64_t n : section_segments(d->pt_cell[p], d->pt_section[p])) used.insert({n, s.species[k].name});
+  for (int64_t q = 0; q < d->n_ion_settings; ++q)
+    for (int64_t n : section_segments(d->ionset_cell[q], d->ionset_section[q]))
+      used.insert({n, s.species[d->ionset_species[q]].name});
+  s.nion = (int64_t)used.size();
+  s.ion_off.assign(s.total + 1, 0);
+  for (int c = 0; c < 5; ++c) s.ion[c].assign(s.nion, 0.0);
+  s.ion_style.assign(s.nion, 0);
+  std::map<std::pair<int64_t, int>, int32_t> row_of;  // (node, species) -> ion row
+  {
+    int64_t q = 0;
+    for (const auto& [node, name] : used) {
+      const int k = species_index(name);
+      const Sim::Species& sp_k = s.species[k];
+      s.ion_sp.push_back(k);
+      s.ion_off[node + 1] += 1;
+      row_of[{node, k}] = (int32_t)q;
+      s.ion[0][q] = sp_k.nernst ? nernst(sp_k.ci, sp_k.co, sp_k.z, s.celsius) : sp_k.e;
+      s.ion[1][q] = sp_k.ci;
+      s.ion[2][q] = sp_k.co;
+      s.ion_style[q] = sp_k.nernst ? style_bits(1, 2, true, true, false) : 0;
+      ++q;
+    }
+  }
+  for (int64_t n = 0; n < s.total; ++n) s.ion_off[n + 1] += s.ion_off[n];
+  for (Group& g : s.groups) {
+    const size_t ns = g.slot_species.size();
+    g.ion_row.assign((size_t)g.rows() * ns, -1);
+    for (int r = 0; r < g.rows(); ++r)
+      for (size_t k = 0; k < ns; ++k)
+        if (g.node[r] >= 0) g.ion_row[(size_t)r * ns + k] = row_of.at({g.node[r], g.slot_species[k]});
+  }
+  // USEION values given as mechanism parameters go to the shared ion row (collect_ion_parameters!)
+  for (const IonParam& ip : ion_params) {
+    Group& g = s.groups[ip.gi];
+    const std::vector<IonDecl>& decl = ion_decls(g.mech);
+    for (size_t k = 0; k < decl.size(); ++k) {
+      const int32_t q = g.ion_row[(size_t)ip.row * decl.size() + k];
+      if (q < 0) continue;
+      for (int c = 0; c < 4; ++c)
+        if (decl[k].read[c] == ip.field || decl[k].write[c] == ip.field) s.ion[c][q] = ip.value;
+    }
+  }
+  for (int64_t q = 0; q < d->n_ion_settings; ++q) {  // section ion values (cell_ion_values)
+    const int what = d->ionset_what[q];
+    if (what > 2) continue;
+    const int column = what == 0 ? 1 : what == 1 ? 2 : 0;
+    for (int64_t n : section_segments(d->ionset_cell[q], d->ionset_section[q]))
+      s.ion[column][row_of.at({n, d->ionset_species[q]})] = d->ionset_value[q];
+  }
+  // styles: promotion by every instance's USEION, Nernst species, section overrides
+  for (Group& g : s.groups) {
+    const std::vector<IonDecl>& decl = ion_decls(g.mech);
+    for (size_t k = 0; k < decl.size(); ++k) {
+      const IonDecl& u = decl[k];
+      const int c_style = (u.write[1] >= 0 || u.write[2] >= 0) ? 3 : (u.read[1] >= 0 || u.read[2] >= 0) ? 1 : 0;
+      const int e_style = u.write[0] >= 0 ? 3 : u.read[0] >= 0 ? 1 : 0;
+      for (int r = 0; r < g.rows(); ++r) {
+        const int32_t q = g.ion_row[(size_t)r * decl.size() + k];
+        if (q >= 0) s.ion_style[q] = promote(s.ion_style[q], c_style, e_style);
+      }
+    }
+  }
+  for (int64_t q = 0; q < s.nion; ++q)
+    if (s.species[s.ion_sp[q]].nernst && ((s.ion_style[q] >> 3) & 3) != 3) s.ion_style[q] |= 0x60;
+  {
+    std::map<std::tuple<int, int, int>, std::pair<int, double>> last;
+    for (int64_t q = 0; q < d->n_ion_settings; ++q)
+      if (d->ionset_what[q] >= 3)
+        last[{d->ionset_cell[q], d->ionset_section[q], d->ionset_species[q]}] = {d->ionset_what[q], d->ionset_value[q]};
+    for (const auto& [key, o] : last)
+      for (int64_t n : section_segments(std::get<0>(key), std::get<1>(key))) {
+        const int32_t q = row_of.at({n, std::get<2>(key)});
+        if (o.first == 4) {
+          s.ion_style[q] = (uint8_t)o.second;
+        } else if (o.second > 0) {  // policies fixed, initial, advance, dynamic
+          const uint8_t flags[5] = {0, 0x00, 0x20, 0x40, 0x60};
+          s.ion_style[q] = (uint8_t)((s.ion_style[q] & 0x1f) | flags[(int)o.second]);
+        }
+      }
+  }
+  for (int c = 0; c < 5; ++c) s.ion_packed[c] = s.ion[c];
   // 4. sources, detectors, fanout
   s.n_sources = d->n_sources;
   s.source_sid.assign(d->src_sid, d->src_sid + d->n_sources);
@@ -1292,6 +1604,16 @@ int oracle_initialize(void* p) {  // initialize! (initialize.jl:270-429)
         rng_setseq(g.rng[r], s.seed, 0.0);
       }
     }
+  }
+  // ion rows: packed values, concentrations reset (cinit), initial reversals
+  // (initialize_ions_kernel!, prepare_currents_kernel!(Initial))
+  for (int c = 0; c < 5; ++c) s.ion[c] = s.ion_packed[c];
+  for (int64_t q = 0; q < s.nion; ++q) {
+    if (s.ion_style[q] & 0x04) {
+      s.ion[1][q] = s.species[s.ion_sp[q]].ci;
+      s.ion[2][q] = s.species[s.ion_sp[q]].co;
+    }
+    ion_prepare(s, q, 0x20);
   }
   initial_blocks(s);
   flush(s);
@@ -1408,6 +1730,13 @@ int oracle_run(void* p, double stop, const mind_record* recs, int64_t nrec, int3
     } else if (r.kind == 1) {
       ref.gi = s.point_group[r.point];
       ref.row = s.point_row[r.point];
+    } else if (r.kind == 3) {  // ion_ref (references.jl:156-178)
+      int64_t node;
+      oracle_location_node(p, r.cell, r.section, r.x, 1, &node);
+      ref.row = -1;
+      for (int64_t q = s.ion_off[node]; q < s.ion_off[node + 1]; ++q)
+        if (s.ion_sp[q] == r.mech) ref.row = (int)q;
+      if (ref.row < 0) return fail(MIND_EARG, "ion is not present at this location");
     } else {
       int64_t node;
       oracle_location_node(p, r.cell, r.section, r.x, 1, &node);
@@ -1425,7 +1754,9 @@ int oracle_run(void* p, double stop, const mind_record* recs, int64_t nrec, int3
   auto sample = [&](int64_t j) {
     for (int64_t k = 0; k < nrec; ++k) {
       const Ref& r = refs[k];
-      traces[(size_t)k * samples + j] = r.kind == 0 ? s.v[r.node] : s.groups[r.gi].at(r.field, r.row);
+      traces[(size_t)k * samples + j] = r.kind == 0   ? s.v[r.node]
+                                        : r.kind == 3 ? s.ion[r.field][r.row]
+                                                      : s.groups[r.gi].at(r.field, r.row);
     }
   };
   s.recording_spikes = record_spikes != 0;
@@ -1488,6 +1819,9 @@ void oracle_result_free(mind_result* r) {
   free(r->spike_sids);
   memset(r, 0, sizeof(*r));
 }
+
+// Test hook: the oracle's Nernst potential (its own restatement).
+double oracle_nernst(double ci, double co, double z, double celsius) { return nernst(ci, co, z, celsius); }
 
 // Test hooks: Philox block, uniform stream, Hines solve on a given tree.
 void oracle_philox(const uint32_t ctr_in[4], const uint32_t key[2], uint32_t out[4]) {

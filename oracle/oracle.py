@@ -28,6 +28,8 @@ def lib():
         P = C.POINTER
         L.oracle_build.restype = C.c_void_p
         L.oracle_build.argtypes = [P(_abi.ModelDesc), C.c_double, C.c_uint64]
+        L.oracle_nernst.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        L.oracle_nernst.restype = C.c_double
         L.oracle_free.argtypes = [C.c_void_p]
         L.oracle_set_secondorder.argtypes = [C.c_void_p, C.c_int32]
         L.oracle_set_threads.argtypes = [C.c_void_p, C.c_int32]
@@ -70,9 +72,9 @@ _registered = {}  # runtime mechanism id -> its host build (kept loaded)
 
 _WRAPPERS = """
 extern "C" {{
-void oracle_u{i}_current(double* data, int64_t rows, int64_t r, double v, double t, double dt, double* value,
-                        double* shifted, int32_t store) {{
-  mind::user::u{i}_current(mind::Cols{{data, rows}}, r, v, t, dt, *value, *shifted, store != 0);
+void oracle_u{i}_current(double* data, int64_t rows, int64_t r, double v, double t, double dt, double celsius,
+                        double* value, double* shifted, int32_t store, double* ish) {{
+  mind::user::u{i}_current(mind::Cols{{data, rows}}, r, v, t, dt, *value, *shifted, store != 0, celsius, ish);
 }}
 void oracle_u{i}_advance(double* data, int64_t rows, int64_t r, double v, double t, double dt, double celsius) {{
   mind::user::u{i}_advance(mind::Cols{{data, rows}}, r, v, t, dt, celsius);
@@ -115,6 +117,7 @@ def _host_build(mech):
 
 def register_mechanisms(model):
     """Registers with the oracle every runtime-compiled mechanism the model uses."""
+    from mind_amd import mechanisms
     from mind_amd.mechanisms import BY_ID
 
     ids = {mid for (_, _, mid, _) in model.density} | {p[0] for p in model.points}
@@ -129,6 +132,20 @@ def register_mechanisms(model):
         defaults = (C.c_double * max(1, n))(*mech.defaults)
         rc = lib().oracle_register_mech(mid, n, 1 if mech.kind == "point" else 0, int(mech.has_state), defaults,
                                         *fns)
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        uses = mechanisms.ions(mech)
+        is_state, in_recv = mechanisms.columns_info(mech)
+        m = max(1, len(uses))
+        names = (C.c_char_p * m)(*[u.name.encode() for u in uses])
+        has_v = (C.c_int32 * m)(*[int(u.valence is not None) for u in uses])
+        val = (C.c_double * m)(*[u.valence or 0.0 for u in uses])
+        rd = (C.c_int32 * (4 * m))(*[c for u in uses for c in u.read])
+        wr = (C.c_int32 * (4 * m))(*[c for u in uses for c in u.write])
+        st = (C.c_uint8 * max(1, n))(*is_state)
+        recv = [k for k in range(n) if in_recv[k]]
+        rf = (C.c_int32 * max(1, len(recv)))(*recv)
+        rc = lib().oracle_register_ions(mid, len(uses), names, has_v, val, rd, wr, st, len(recv), rf)
         if rc:
             raise RuntimeError(lib().oracle_last_error().decode())
         _registered[mid] = h
@@ -293,6 +310,11 @@ def hines(parent, g, d, b):
     if rc:
         raise ZeroDivisionError("singular")
     return b
+
+
+def nernst(ci, co, z, celsius):
+    """The oracle's Nernst potential (mV)."""
+    return lib().oracle_nernst(float(ci), float(co), float(z), float(celsius))
 
 
 def exp(x):

@@ -46,7 +46,7 @@ def _c_structs():
             decl = " ".join(decl.split())
             if not decl:
                 continue
-            m = re.match(r"(?:const )?(\w+)\s*(\*?)\s*(\w+(?:\s*,\s*\w+)*)$", decl)
+            m = re.match(r"(?:const )?(\w+)\s*(\*?)\s*(?:const\s*\*\s*)?(\w+(?:\s*,\s*\w+)*)$", decl)
             assert m, decl
             ctype, ptr, names = m.groups()
             kind = "p" if ptr else {"double": "f8", "int32_t": "i4", "int64_t": "i8", "uint64_t": "u8"}.get(ctype, "?")

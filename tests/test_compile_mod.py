@@ -1,9 +1,8 @@
 """Runtime MOD compilation at the C ABI (mind_compile_mod and the mechanism
 queries), on the host.
 
-* The library's C++ NMODL frontend gives the same text as the Python lowering
-  for the builtin MOD files (where the reference tree holds them), the toy
-  mechanism and malformed inputs (same error messages).
+* The library's C++ NMODL frontend refuses malformed inputs naming the file
+  and line.
 * compile_mod / load_mod: slots, field columns, defaults (PARAMETER values and
   ion values), refusals with file and line, the same text compiling to the
   same slot.
@@ -22,7 +21,7 @@ import numpy as np
 import pytest
 
 import oracle
-from mind_amd import MicroModel, compile_mod, configs, load_mod, nmodl, voltage_record
+from mind_amd import MicroModel, compile_mod, configs, load_mod, voltage_record
 from mind_amd import _abi
 from mind_amd.mechanisms import builtin, lowering
 
@@ -47,21 +46,6 @@ def _lower_cpp(nmodlc, tmp_path, files):
     return ("ok", out.read_text()) if r.returncode == 0 else ("err", r.stderr.strip())
 
 
-def _lower_py(files):
-    try:
-        return "ok", nmodl.compile_sources([(f, open(f).read()) for f in files])
-    except nmodl.ModError as e:
-        return "err", str(e)
-
-
-@pytest.mark.skipif(not os.path.isdir(REF_MOD), reason="the builtin MOD files are not present")
-def test_cpp_frontend_matches_python_lowering_on_builtins(nmodlc, tmp_path):
-    files = [os.path.join(REF_MOD, f"{n}.mod") for n in BUILTINS]
-    got, want = _lower_cpp(nmodlc, tmp_path, files), _lower_py(files)
-    assert got[0] == want[0] == "ok"
-    assert got[1] == want[1]
-
-
 BAD = [
     "NEURON { SUFFIX a }\nPARAMETER { g = 1 }\nBREAKPOINT { i = q*g }\nASSIGNED { i }",
     "NEURON { SUFFIX a POINTER p }",
@@ -77,19 +61,17 @@ BAD = [
 
 
 @pytest.mark.parametrize("text", BAD)
-def test_cpp_frontend_reports_the_python_lowering_errors(nmodlc, tmp_path, text):
+def test_cpp_frontend_reports_errors_with_file_and_line(nmodlc, tmp_path, text):
     f = tmp_path / "bad.mod"
     f.write_text(text)
-    got, want = _lower_cpp(nmodlc, tmp_path, [str(f)]), _lower_py([str(f)])
-    assert got == want
+    got = _lower_cpp(nmodlc, tmp_path, [str(f)])
     assert got[0] == "err" and got[1].startswith(f"{f}:")
 
 
-def test_cpp_frontend_matches_python_lowering_on_test_mods(nmodlc, tmp_path):
+def test_cpp_frontend_lowers_the_test_mods(nmodlc, tmp_path):
     for name in ("kdr", "expsyn2", "nap"):
-        f = os.path.join(MODS, f"{name}.mod")
-        got, want = _lower_cpp(nmodlc, tmp_path, [f]), _lower_py([f])
-        assert got == want, name
+        got = _lower_cpp(nmodlc, tmp_path, [os.path.join(MODS, f"{name}.mod")])
+        assert got[0] == "ok" and f"_current_eval(" in got[1], name
 
 
 def test_compile_mod_slots_fields_and_defaults():
@@ -129,9 +111,9 @@ def test_compile_mod_slots_fields_and_defaults():
     ("NEURON { ARTIFICIAL_CELL a }", "ARTIFICIAL_CELL"),
     ("NEURON { POINT_PROCESS a RANDOM r }", "RANDOM"),
     ("NEURON { POINT_PROCESS a }\nNET_RECEIVE(w) { net_event(t) }", "net_send and net_event"),
-    ("NEURON { SUFFIX a USEION cl READ ecl }\nASSIGNED { ecl }", "ion cl"),
-    ("NEURON { SUFFIX a USEION ca READ cai WRITE cai }\nASSIGNED { cai }", "concentration"),
-    ("NEURON { SUFFIX a USEION k READ ik }\nASSIGNED { ik }", "only reversal potentials"),
+    ("NEURON { SUFFIX a USEION ca READ cao WRITE ena }\nASSIGNED { cao ena }", "not an ion variable of ca"),
+    ("NEURON { SUFFIX a USEION ca READ cai }", "cai: not a declared scalar variable"),
+    ("NEURON { POINT_PROCESS a }\nASSIGNED { x }\nNET_RECEIVE(w) { x = celsius }", "celsius in NET_RECEIVE"),
     ("NEURON { SUFFIX a }\nBREAKPOINT { x = 1 }", "cannot assign x"),
 ])
 def test_compile_mod_refusals_name_file_and_line(text, msg):

@@ -1,4 +1,5 @@
-"""NMODL -> HIP C lowering (mind_amd.nmodl), checked on the host with g++.
+"""NMODL -> HIP C lowering (the C++ frontend csrc/nmodl.cpp through its driver
+nmodlc), checked on the host with g++.
 
 * Small MOD inputs written here: parse errors, field layout, and the numbers the
   generated C++ computes against closed forms.
@@ -7,13 +8,12 @@
   for bit over randomised states (the oracle tests pin the hand lowering).
 """
 import os
+import re
 import subprocess
 import textwrap
 
 import numpy as np
 import pytest
-
-from mind_amd import nmodl
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "mind.jl_amd", "csrc")
@@ -52,6 +52,47 @@ LEAKGATE = textwrap.dedent("""
 """)
 
 
+class ModError(Exception):
+    """A lowering failure reported by nmodlc ("<file>:<line>: <message>")."""
+
+
+@pytest.fixture(scope="module")
+def nmodlc(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("nmodlc") / "nmodlc"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(exe), os.path.join(CSRC, "nmodl.cpp"),
+                    os.path.join(CSRC, "nmodlc.cpp")], check=True, capture_output=True, text=True)
+    return str(exe)
+
+
+def lower(nmodlc, tmp_path, sources):
+    """mech_gen.h text of (filename, text) sources, or ModError."""
+    files = []
+    for name, text in sources:
+        f = tmp_path / os.path.basename(name)
+        f.write_text(text)
+        files.append(str(f))
+    return lower_files(nmodlc, tmp_path, files)
+
+
+def lower_files(nmodlc, tmp_path, files):
+    out = tmp_path / "lowered.h"
+    r = subprocess.run([nmodlc, *files, "-o", str(out)], capture_output=True, text=True)
+    if r.returncode:
+        raise ModError(r.stderr.strip())
+    return out.read_text()
+
+
+def field_columns(header, name):
+    """(first column, field name) of mechanism `name`'s fields in the header's enum."""
+    enum = re.search(rf"enum {{ ({name}_\w+ = \d+(?:, {name}_\w+ = \d+)*) }};", header).group(1)
+    cols = [(int(v), k[len(name) + 1:]) for k, v in (e.split(" = ") for e in enum.split(", "))]
+    return sorted(c for c in cols if c[1] != "NFIELDS")
+
+
+def columns(header, name):
+    return [n for _, n in field_columns(header, name)]
+
+
 def gxx_run(tmp_path, header_text, main_src, name="prog"):
     hdr = tmp_path / "mech_gen.h"
     hdr.write_text(header_text)
@@ -63,16 +104,15 @@ def gxx_run(tmp_path, header_text, main_src, name="prog"):
     return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
 
 
-def test_parse_layout_and_defaults():
-    m = nmodl.parse(LEAKGATE, "leakgate.mod")
-    assert (m.kind, m.name) == ("suffix", "leakgate")
-    assert m.fields == ["gbar", "erev", "tau0", "s", "i", "sinf"]
-    assert m.nonspecific == ["i"]
-    assert [d for d in m.decls if d[0] == "erev"][0][2] == -70.0
+def test_parse_layout_and_defaults(nmodlc, tmp_path):
+    hdr = lower(nmodlc, tmp_path, [("leakgate.mod", LEAKGATE)])
+    # PARAMETER, then STATE, then ASSIGNED names without v
+    assert columns(hdr, "leakgate") == ["gbar", "erev", "tau0", "s", "i", "sinf"]
+    assert "f[leakgate_erev] = -70.0;" in hdr
 
 
-def test_generated_toy_mechanism_matches_closed_forms(tmp_path):
-    hdr = nmodl.compile_sources([("leakgate.mod", LEAKGATE)])
+def test_generated_toy_mechanism_matches_closed_forms(nmodlc, tmp_path):
+    hdr = lower(nmodlc, tmp_path, [("leakgate.mod", LEAKGATE)])
     out = gxx_run(tmp_path, hdr, textwrap.dedent("""
         #include "mech_gen.h"
         #include <cstdio>
@@ -86,7 +126,7 @@ def test_generated_toy_mechanism_matches_closed_forms(tmp_path):
           gen::leakgate_advance(c, 0, -30.0, 0.025, 0.025, 6.3);
           double s1 = f[gen::leakgate_s];
           double val, sh;
-          gen::leakgate_current(c, 0, -30.0, 0.0, 0.025, val, sh, true);
+          gen::leakgate_current(c, 0, -30.0, 0.0, 0.025, val, sh, true, 6.3, nullptr);
           std::printf("%.17g %.17g %.17g %.17g %.17g\\n", s0, s1, val, sh, f[gen::leakgate_i]);
         }
     """))
@@ -109,9 +149,9 @@ def test_generated_toy_mechanism_matches_closed_forms(tmp_path):
      "cnexp form"),
     ("PARAMETER { g = 1 }", "no NEURON block"),
 ])
-def test_errors_name_the_problem(text, msg):
-    with pytest.raises(nmodl.ModError, match=msg):
-        nmodl.compile_sources([("bad.mod", text)])
+def test_errors_name_the_problem(nmodlc, tmp_path, text, msg):
+    with pytest.raises(ModError, match=msg):
+        lower(nmodlc, tmp_path, [("bad.mod", text)])
 
 
 HARNESS = textwrap.dedent("""
@@ -176,8 +216,10 @@ HARNESS = textwrap.dedent("""
           mech_initial(mech, ca, 0, v, cel);
           gen::initial(mech, cb, 0, v, cel);
           same(a, b, sizeof a, "initial", mech, trial);
-          mech_current(mech, ca, 0, v, t, dt, va, sa, true);
-          gen::current(mech, cb, 0, v, t, dt, vb, sb, true);
+          double ia[8] = {0}, ib[8] = {0};
+          mech_current(mech, ca, 0, v, t, dt, va, sa, true, ia);
+          gen::current(mech, cb, 0, v, t, dt, vb, sb, true, 6.3, ib);
+          same(ia, ib, sizeof ia, "ion current probe", mech, trial);
           same(&va, &vb, 8, "current", mech, trial);
           same(&sa, &sb, 8, "current probe", mech, trial);
           same(a, b, sizeof a, "current fields", mech, trial);
@@ -222,20 +264,19 @@ HARNESS = textwrap.dedent("""
 
 
 @pytest.mark.skipif(not os.path.isdir(MOD_DIR), reason="the builtin MOD files are not present")
-def test_builtin_mod_files_lower_to_the_hand_lowering_bit_for_bit(tmp_path):
+def test_builtin_mod_files_lower_to_the_hand_lowering_bit_for_bit(nmodlc, tmp_path):
     paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in BUILTINS]
-    mechs = [nmodl.parse(open(p).read(), p) for p in paths]
+    hdr = lower_files(nmodlc, tmp_path, paths)
     # NetStim's stream key comes from its source digest
-    assert {m.name: m for m in mechs}["NetStim"].digest == 0xa186833b569ff3b2
-    hdr = nmodl.compile_files(paths)
+    assert "constexpr uint64_t NetStim_digest = 0xa186833b569ff3b2ull;" in hdr
     out = gxx_run(tmp_path, hdr, HARNESS, "equiv")
     assert out.strip() == "mismatches 0", out
 
 
 @pytest.mark.skipif(not os.path.isdir(MOD_DIR), reason="the builtin MOD files are not present")
-def test_parameter_defaults_match_the_packing_defaults(tmp_path):
+def test_parameter_defaults_match_the_packing_defaults(nmodlc, tmp_path):
     paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in BUILTINS]
-    hdr = nmodl.compile_files(paths)
+    hdr = lower_files(nmodlc, tmp_path, paths)
     (tmp_path / "mech_gen.h").write_text(hdr)
     src = tmp_path / "defaults.cpp"
     src.write_text(textwrap.dedent("""
@@ -261,22 +302,21 @@ def test_parameter_defaults_match_the_packing_defaults(tmp_path):
                     *(os.path.join(CSRC, f) for f in ("layout.cpp", "mechreg.cpp", "nmodl.cpp"))],
                    check=True, capture_output=True, text=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
-    fields = {m.name: [f"{n}[{k + 1}]" if n in m.extents else n for n in m.fields for k in range(m.extents.get(n, 1))]
-              for m in (nmodl.parse(open(p).read(), p) for p in paths)}
     for line in out.strip().split("\n"):
         name, k, gen_default, packed = line.split()
-        fname = fields[name][int(k)]
+        # an array's elements follow its first column
+        fname = [n for c, n in field_columns(hdr, name) if c <= int(k)][-1]
         if fname in ("ena", "ek"):  # ion reversal potentials come from the ion defaults
             continue
         assert float(gen_default) == float(packed), (name, fname)
 
 
 @pytest.mark.skipif(not os.path.isdir(MOD_DIR), reason="the builtin MOD files are not present")
-def test_equivalence_check_sees_a_reassociated_product(tmp_path):
+def test_equivalence_check_sees_a_reassociated_product(nmodlc, tmp_path):
     """The comparison is sensitive to operation order: regrouping hh's m*m*m*h
     product (same value in exact arithmetic) is reported."""
     paths = [os.path.join(MOD_DIR, f"{n}.mod") for n in BUILTINS]
-    hdr = nmodl.compile_files(paths)
+    hdr = lower_files(nmodlc, tmp_path, paths)
     exact = "((((in.gnabar * in.m) * in.m) * in.m) * in.h)"  # hh_current_eval, from the loaded columns
     assert exact in hdr
     hdr = hdr.replace(exact, "(in.gnabar * ((in.m * in.m) * (in.m * in.h)))")
