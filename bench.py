@@ -97,7 +97,7 @@ def main():
     bytes_per_launch = bytes_per_step * (ksteps / launches)
     avg_launch_s = kms / launches / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    kernel = {0: "k_window", 4: "k_window_lean_hbm", 5: "k_window_lean"}[b["plan"]["layout"]]
+    kernel = b["kernel"]
     traffic, traffic_note = measured_traffic(kernel, b["plan"], b["window"], ksteps / launches,
                                              bytes_per_launch)
     destroy(model)
@@ -182,7 +182,7 @@ def timed_leg(model, method, steps, warm, plan="", profile_windows=0):
     import ctypes as C
 
     from mind_amd import _abi, build, initialize, run
-    from mind_amd.model import (batch_plan, batch_work, event_counts, kernel_time, set_batch_plan,
+    from mind_amd.model import (batch_kernel, batch_plan, batch_work, event_counts, kernel_time, set_batch_plan,
                                 set_profiling, synchronize, window_steps)
 
     dt = 0.025
@@ -209,6 +209,7 @@ def timed_leg(model, method, steps, warm, plan="", profile_windows=0):
     out.update(nodes=n.value, compartments=s.value)
     if method == "batched":
         out["plan"] = batch_plan(model)
+        out["kernel"] = batch_kernel(model)
         out["work"] = batch_work(model)
     if profile_windows:
         set_profiling(model, True)

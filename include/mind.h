@@ -400,6 +400,11 @@ int mind_window_steps(const mind_state* state, int32_t* steps);
  * the autotuner. */
 int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int32_t* layout,
                     int64_t* groups);
+/* Name of the cell kernel the batched method launches for the workgroups within
+ * the budget (a static string: k_window, k_window_lean or k_window_lean_hbm, with
+ * the suffix _core when every mechanism group of the model has an inlined path).
+ * For measurement; no counterpart in the reference. */
+int mind_batch_kernel(const mind_state* state, const char** name);
 /* Replaces that plan (threads a multiple of 64 up to 1024, layout 0, 4 or 5);
  * mind_initialize must follow. */
 int mind_set_batch_plan(mind_state* state, int64_t budget, int32_t threads, int32_t layout);

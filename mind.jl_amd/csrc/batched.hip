@@ -920,7 +920,8 @@ int launch_cells(State& S, const BatchPlan& P, int32_t n) {
   if (side) {
     HIP_TRY(hipEventRecord(S.fork, S.stream));
     HIP_TRY(hipStreamWaitEvent(S.aux, S.fork, 0));
-    HIP_TRY(launch(S, K_WINDOW_LARGE, (const void*)k_window_large, dim3((unsigned)P.large.size()),
+    HIP_TRY(launch(S, K_WINDOW_LARGE, S.core ? (const void*)k_window_large_core : (const void*)k_window_large,
+                   dim3((unsigned)P.large.size()),
                    dim3(std::min(P.threads, kLargeThreads)), P.large_lds, S.aux, S.d, P.b,
                    (const int32_t*)P.dev_large, (const int8_t*)P.dev_large_mode, n));
     HIP_TRY(hipEventRecord(S.join, S.aux));
@@ -929,9 +930,9 @@ int launch_cells(State& S, const BatchPlan& P, int32_t n) {
   if (S.profiling && !P.lists[0].empty() && (rc = timer_start(S))) return rc;
   if (!P.lists[0].empty()) {
     const int id = P.layout == 5 ? K_WINDOW_LEAN : P.layout == 4 ? K_WINDOW_LEAN_HBM : K_WINDOW;
-    const void* fn = P.layout == 5   ? (const void*)k_window_lean
-                     : P.layout == 4 ? (const void*)k_window_lean_hbm
-                                     : (const void*)k_window;
+    const void* fn = P.layout == 5   ? (S.core ? (const void*)k_window_lean_core : (const void*)k_window_lean)
+                     : P.layout == 4 ? (S.core ? (const void*)k_window_lean_hbm_core : (const void*)k_window_lean_hbm)
+                                     : (S.core ? (const void*)k_window_core : (const void*)k_window);
     HIP_TRY(launch(S, id, fn, dim3((unsigned)P.lists[0].size()), dim3(P.threads), P.lds[0], S.stream, S.d, P.b,
                    (const int32_t*)P.dev_lists[0], n));
   }
@@ -1050,6 +1051,9 @@ int batched_setup(State& S, int32_t cells_per_group) {
                               (int)kLdsResident));
   HIP_TRY(hipFuncSetAttribute((const void*)k_window_lean_hbm, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kLdsResident));
+  for (const void* fn : {(const void*)k_window_core, (const void*)k_window_large_core, (const void*)k_window_lean_core,
+                         (const void*)k_window_lean_hbm_core})
+    HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsResident));
   int64_t max_cell = 1;
   for (int32_t c = 0; c < S.L.ncells; ++c) max_cell = std::max<int64_t>(max_cell, S.L.cell_off[c + 1] - S.L.cell_off[c]);
   struct Cand { int64_t budget; int threads; int layout; };
@@ -1198,6 +1202,17 @@ int batched_replan(State& S, int64_t budget, int32_t threads, int32_t layout) {
 extern "C" int mind_batch_plan(const mind_state* state, int64_t* budget, int32_t* threads, int32_t* layout,
                                int64_t* groups) {
   return mind::batched_plan_info(*reinterpret_cast<const mind::State*>(state), budget, threads, layout, groups);
+}
+
+extern "C" int mind_batch_kernel(const mind_state* state, const char** name) {
+  const mind::State& S = *reinterpret_cast<const mind::State*>(state);
+  if (!S.batch) return mind::fail(MIND_EARG, "state has no batched plan");
+  const int lay = S.batch->layout;
+  if (S.core)
+    *name = lay == 5 ? "k_window_lean_core" : lay == 4 ? "k_window_lean_hbm_core" : "k_window_core";
+  else
+    *name = lay == 5 ? "k_window_lean" : lay == 4 ? "k_window_lean_hbm" : "k_window";
+  return 0;
 }
 
 extern "C" int mind_set_batch_plan(mind_state* state, int64_t budget, int32_t threads, int32_t layout) {

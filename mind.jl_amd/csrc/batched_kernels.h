@@ -105,11 +105,22 @@ inline size_t window_bytes(int m, int64_t nn, int64_t slen) {
 }
 
 // Delivery of the window-list events due at delivery call `call` (thread 0).
+// CORE: the receivers are the builtin synapses without ion exchanges (ExpSyn,
+// Exp2Syn; IClamp has no NET_RECEIVE), so only their phases are compiled in.
+template <bool CORE>
 __device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_end, int64_t call) {
   while (seg < seg_end && d.win[seg].call == call) {
     const int32_t e = d.win[seg].edge;
     const int32_t target = d.edge_target[e];
-    receive_event(d, d.g[d.point_group[target]], d.point_row[target], d.win[seg].t, d.edge_weight[e]);
+    const GroupDev& G = d.g[d.point_group[target]];
+    const int64_t row = d.point_row[target];
+    if (!CORE) {
+      receive_event(d, G, row, d.win[seg].t, d.edge_weight[e]);
+    } else if (G.mech == MIND_MECH_EXP2SYN) {
+      phase_receive(MIND_MECH_EXP2SYN, cols(G), row, d.win[seg].t, d.edge_weight[e]);
+    } else if (G.mech == MIND_MECH_EXPSYN) {
+      phase_receive(MIND_MECH_EXPSYN, cols(G), row, d.win[seg].t, d.edge_weight[e]);
+    }
     ++seg;
   }
   return seg;
@@ -526,11 +537,159 @@ __device__ __forceinline__ void density_currents(const Dev& d, const Cols& col, 
   }
 }
 
+// nrn_cur of a density mechanism whose ion transfers are at most two USEION
+// statements, two reads and two written currents (GroupDev::ion_fast; hh's ena,
+// ek, ina, ik): a row's ion rows, columns, area and voltage are read together;
+// then the ion values its reads name and the current and di/dv sums of its ion
+// rows (a density mechanism has one row per node, so no other row of the group
+// adds to them); then both probes and the sums. The same operations as
+// ion_current: the read values are stored in their columns and replace the
+// loaded ones, and each sum adds s * i with s = 1.
+template <int MECH, typename C>
+__device__ __forceinline__ void density_ion_currents(const Dev& d, const GroupDev& G, const Cols& col, int64_t r0,
+                                                     int64_t r1, int64_t n0, double tc, const double* v, C rhs,
+                                                     C diag, int tid, int T) {
+  using Ops = CurOps<MECH>;
+  const int ns = G.nslot, nr = G.nread, nw = G.nwrite;
+  IonBind br[2], bw[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    br[k] = k < nr ? G.ion_bind[k] : IonBind{0, 0, 0, 0};
+    bw[k] = k < nw ? G.ion_bind[nr + k] : IonBind{0, 0, 0, 0};
+  }
+  const int64_t nion = d.nion;
+  for (int64_t r = r0 + tid; r < r1; r += T) {
+    const int32_t node = G.node[r];
+    const int32_t q0 = G.ion_row[r * ns], q1 = G.ion_row[r * ns + (ns > 1 ? 1 : 0)];
+    typename Ops::In in = Ops::load(col, r);
+    const double area = d.area[node];
+    const int li = node - (int)n0;
+    const double vn = v[li];
+    int32_t qr[2], qw[2];
+    double x[2], ci[2], cd[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      qr[k] = k < nr ? (br[k].slot ? q1 : q0) : -1;
+      qw[k] = k < nw ? (bw[k].slot ? q1 : q0) : -1;
+      x[k] = qr[k] >= 0 ? d.ion[br[k].column * nion + qr[k]] : 0.0;
+      ci[k] = qw[k] >= 0 ? d.ion[MIND_ION_I * nion + qw[k]] : 0.0;
+      cd[k] = qw[k] >= 0 ? d.ion[MIND_ION_DIDV * nion + qw[k]] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (qr[k] >= 0) {
+        col(br[k].field, r) = x[k];
+        Ops::set(in, br[k].field, x[k]);
+      }
+    double value, shifted, ish[2 * kMaxIonCurrents];
+    Ops::eval_ish(col, r, in, vn, tc, d.dt, value, shifted, true, d.celsius, ish);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (qw[k] >= 0) {
+        const double xs = bw[k].aux ? ish[1] : ish[0];
+        const double xi = bw[k].aux ? ish[kMaxIonCurrents + 1] : ish[kMaxIonCurrents];
+        d.ion[MIND_ION_I * nion + qw[k]] = ci[k] + 1.0 * xi;
+        d.ion[MIND_ION_DIDV * nion + qw[k]] = cd[k] + 1.0 * (xs - xi) / 0.001;
+      }
+    const double scale = area * 0.01;
+    double acc_r = rhs[li], acc_d = diag[li];
+    acc_r -= scale * value;
+    acc_d += scale * (shifted - value) / 0.001;
+    rhs[li] = acc_r;
+    diag[li] = acc_d;
+  }
+}
+
 // Mechanisms whose current-phase stores are outputs only: no other phase and
 // no NET_RECEIVE reads them.
 __device__ __forceinline__ bool mech_outputs_only(int32_t mech) {
   return mech == MIND_MECH_HH || mech == MIND_MECH_PAS || mech == MIND_MECH_EXPSYN || mech == MIND_MECH_EXP2SYN ||
          mech == MIND_MECH_ICLAMP || mech == MIND_MECH_ALPHASYNAPSE;
+}
+
+// nrn_cur of any group's rows [r0, r1) of a workgroup: runs of point rows on one
+// node in row order, ion transfers through ion_current.
+template <typename C>
+__device__ __forceinline__ void group_currents(const Dev& d, const GroupDev& G, int64_t r0, int64_t r1, int64_t n0,
+                                            double tc, const double* v, C rhs, C diag, bool store_outputs) {
+  const int tid = threadIdx.x, T = blockDim.x;
+  const Cols col = cols(G);
+  for (int64_t r = r0 + tid; r < r1; r += T) {
+    if (G.point && !G.run_start[r]) continue;
+    const int32_t node = G.node[r];
+    const int li = node - (int)n0;
+    double acc_r = rhs[li], acc_d = diag[li];
+    int64_t q = r;
+    do {  // a run of rows on one node, in row order
+      double value, shifted;
+      if (G.nslot)
+        ion_current(d, G, col, q, v[li], tc, d.area[node], value, shifted);
+      else
+        phase_current(G.mech, col, q, v[li], tc, d.dt, value, shifted, store_outputs || !mech_outputs_only(G.mech),
+                      d.celsius, nullptr);
+      const double scale = G.point ? 1.0 : d.area[node] * 0.01;
+      acc_r -= scale * value;
+      acc_d += scale * (shifted - value) / 0.001;
+      ++q;
+    } while (G.point && q < r1 && G.node[q] == node);
+    rhs[li] = acc_r;
+    diag[li] = acc_d;
+  }
+}
+
+// nrn_state at t of one group's rows [r0, r1) of a workgroup, rows sharing a
+// node in row order (the ion-ordered path).
+__device__ __forceinline__ void group_states(const Dev& d, const GroupDev& G, int64_t r0, int64_t r1, int64_t n0,
+                                          double t, const double* v) {
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    if (G.point && !G.run_start[r]) continue;
+    int64_t q = r;
+    do {
+      advance_row(d, G, q, v[G.node[q] - n0], t);
+      ++q;
+    } while (G.point && q < r1 && G.node[q] == G.node[r]);
+  }
+}
+
+// nrn_cur of a point-process group of builtin mechanism MECH without ion
+// transfers (the core kernels' ExpSyn, Exp2Syn, IClamp): the same runs of rows on
+// one node and operations as group_currents, with MECH's phase alone compiled in.
+template <int MECH, typename C>
+__device__ __forceinline__ void point_currents(const Dev& d, const GroupDev& G, const Cols& col, int64_t r0,
+                                               int64_t r1, int64_t n0, double tc, const double* v, C rhs, C diag,
+                                               int tid, int T, bool store_outputs) {
+  using Ops = CurOps<MECH>;
+  const bool store = store_outputs || !mech_outputs_only(MECH);
+  for (int64_t r = r0 + tid; r < r1; r += T) {
+    if (!G.run_start[r]) continue;
+    const int32_t node = G.node[r];
+    const int li = node - (int)n0;
+    double acc_r = rhs[li], acc_d = diag[li];
+    const double vn = v[li];
+    int64_t q = r;
+    do {  // a run of rows on one node, in row order
+      double value, shifted;
+      Ops::eval(col, q, Ops::load(col, q), vn, tc, d.dt, value, shifted, store, d.celsius);
+      acc_r -= 1.0 * value;
+      acc_d += 1.0 * (shifted - value) / 0.001;
+      ++q;
+    } while (q < r1 && G.node[q] == node);
+    rhs[li] = acc_r;
+    diag[li] = acc_d;
+  }
+}
+
+// nrn_state of row r of a core-kernel group (hh, ExpSyn, Exp2Syn; none of them
+// loads ion values here, GroupDev::ion_state being 0): the mechanism's phase
+// alone, as advance_row runs it.
+__device__ __forceinline__ void core_advance(const Dev& d, const GroupDev& G, int64_t r, double v, double t) {
+  const Cols c = cols(G);
+  if (G.mech == MIND_MECH_HH)
+    phase_advance(MIND_MECH_HH, c, r, v, t, d.dt, d.celsius);
+  else if (G.mech == MIND_MECH_EXP2SYN)
+    phase_advance(MIND_MECH_EXP2SYN, c, r, v, t, d.dt, d.celsius);
+  else if (G.mech == MIND_MECH_EXPSYN)
+    phase_advance(MIND_MECH_EXPSYN, c, r, v, t, d.dt, d.celsius);
 }
 
 // Mechanisms with a state update (nrn_state).
@@ -547,7 +706,12 @@ struct WinShared {
   double sdet_thr[kMaxDetLds];
 };
 
-template <int M>
+// One workgroup's window. CORE (the host's State::core) compiles in only the
+// inlined mechanism paths: pas and hh density currents (hh with its own ion
+// currents), ExpSyn / Exp2Syn / IClamp point currents, their state updates and
+// receivers, no ion orders or secondorder-2 correction. The general kernels run
+// every other model, with the same operations for these mechanisms.
+template <int M, bool CORE>
 __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, const int grp, const int32_t n,
                                             WinShared& sh, double* lds) {
   const int tid = threadIdx.x, T = blockDim.x;
@@ -617,6 +781,7 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
     sdet_thr[q] = d.det_thr[kk];
   }
   const int64_t rec0 = d.nrec ? d.cell_rec_off[c0] : 0, rec1 = d.nrec ? d.cell_rec_off[c1] : 0;
+  const int64_t iq0 = d.nion ? d.node_ion_off[n0] : 0, iq1 = d.nion ? d.node_ion_off[n1] : 0;  // ion rows
   if (d.ngroups > kMaxGroups) raise_error(d.error, MIND_EUNSUPPORTED);
   __syncthreads();
   for (int s = 0; s < n; ++s) {
@@ -638,13 +803,12 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         d.det_above[kk] = cur;
       }
     }
-    if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k);  // midpoint delivery
+    if (tid == 0) seg = deliver_group<CORE>(d, seg, seg_end, 2 * k);  // midpoint delivery
     for (int i = tid; i < nn; i += T) {
       rhs[i] = 0.0;
       diag[i] = 0.0;
     }
-    if (d.nion)
-      for (int64_t q = d.node_ion_off[n0] + tid; q < d.node_ion_off[n1]; q += T) ion_prepare(d, q, 0x40);
+    for (int64_t q = iq0 + tid; q < iq1; q += T) ion_prepare(d, q, 0x40);
     __syncthreads();
     // nrn_cur at t + dt/2, one mechanism group after another. The assigned
     // outputs of a current phase that no other phase reads (i, g of hh, pas,
@@ -669,27 +833,19 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         __syncthreads();
         continue;
       }
-      for (int64_t r = r0 + tid; r < r1; r += T) {
-        if (G.point && !G.run_start[r]) continue;
-        const int32_t node = G.node[r];
-        const int li = node - (int)n0;
-        double acc_r = rhs[li], acc_d = diag[li];
-        int64_t q = r;
-        do {  // a run of rows on one node, in row order
-          double value, shifted;
-          if (G.nslot)
-            ion_current(d, G, col, q, v[li], tc, d.area[node], value, shifted);
-          else
-            phase_current(G.mech, col, q, v[li], tc, dt, value, shifted, store_outputs || !mech_outputs_only(G.mech),
-                          d.celsius, nullptr);
-          const double scale = G.point ? 1.0 : d.area[node] * 0.01;
-          acc_r -= scale * value;
-          acc_d += scale * (shifted - value) / 0.001;
-          ++q;
-        } while (G.point && q < r1 && G.node[q] == node);
-        rhs[li] = acc_r;
-        diag[li] = acc_d;
+      if (G.mech == MIND_MECH_HH && G.ion_fast) {
+        density_ion_currents<MIND_MECH_HH>(d, G, col, r0, r1, n0, tc, v, rhs, diag, tid, T);
+        __syncthreads();
+        continue;
       }
+      if (!CORE)
+        group_currents(d, G, r0, r1, n0, tc, v, rhs, diag, store_outputs);
+      else if (G.mech == MIND_MECH_EXP2SYN)
+        point_currents<MIND_MECH_EXP2SYN>(d, G, col, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
+      else if (G.mech == MIND_MECH_EXPSYN)
+        point_currents<MIND_MECH_EXPSYN>(d, G, col, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
+      else if (G.mech == MIND_MECH_ICLAMP)
+        point_currents<MIND_MECH_ICLAMP>(d, G, col, r0, r1, n0, tc, v, rhs, diag, tid, T, store_outputs);
       __syncthreads();
     }
     // cable assembly: own axial current first, children in increasing index
@@ -739,25 +895,16 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       solve_waves<M>(d, b, n0, nn, sched, hn, oh, od, ax, rhs, diag, v);
     }
     __syncthreads();
-    if (d.ion_correct && d.nion) {  // secondorder 2: ion currents += di/dv * dv
+    if (!CORE && d.ion_correct && d.nion) {  // secondorder 2: ion currents += di/dv * dv
       for (int i = tid; i < nn; i += T) ion_correct_node(d, n0 + i, rhs[i]);
       __syncthreads();
     }
-    if (d.ion_ordered) {
+    if (!CORE && d.ion_ordered) {
       // nrn_state at t + dt group by group, as the reference's launches run: a
-      // mechanism reads what an earlier one wrote to the ion rows; rows sharing a
-      // node run in row order
+      // mechanism reads what an earlier one wrote to the ion rows
       for (int gi = 0; gi < ngr; ++gi) {
-        const GroupDev& G = sg[gi];
-        if (!stateful(G.mech)) continue;
-        for (int64_t r = sr0[gi] + tid; r < sr1[gi]; r += T) {
-          if (G.point && !G.run_start[r]) continue;
-          int64_t q = r;
-          do {
-            advance_row(d, G, q, v[G.node[q] - n0], t + dt);
-            ++q;
-          } while (G.point && q < sr1[gi] && G.node[q] == G.node[r]);
-        }
+        if (!stateful(sg[gi].mech)) continue;
+        group_states(d, sg[gi], sr0[gi], sr1[gi], n0, t + dt, v);
         __syncthreads();
       }
     } else {
@@ -778,7 +925,10 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         }
         const GroupDev& G = sg[gi];
         r += sr0[gi];
-        advance_row(d, G, r, v[G.node[r] - n0], t + dt);
+        if (CORE)
+          core_advance(d, G, r, v[G.node[r] - n0], t + dt);
+        else
+          advance_row(d, G, r, v[G.node[r] - n0], t + dt);
       }
     }
     }
@@ -800,7 +950,7 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       }
     }
     __syncthreads();
-    if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k + 1);  // end-of-step delivery
+    if (tid == 0) seg = deliver_group<CORE>(d, seg, seg_end, 2 * k + 1);  // end-of-step delivery
   }
   if (M == 0 || M == 5) {
     __syncthreads();
@@ -812,14 +962,14 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
 __global__ void __launch_bounds__(kMaxThreads) k_window(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
   extern __shared__ double lds[];
   __shared__ WinShared sh;
-  window_body<0>(d, b, groups[blockIdx.x], n, sh, lds);
+  window_body<0, false>(d, b, groups[blockIdx.x], n, sh, lds);
 }
 
 // Groups within the node budget in the lean layout, v in LDS.
 __global__ void __launch_bounds__(kMaxThreads) k_window_lean(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
   extern __shared__ double lds[];
   __shared__ WinShared sh;
-  window_body<5>(d, b, groups[blockIdx.x], n, sh, lds);
+  window_body<5, false>(d, b, groups[blockIdx.x], n, sh, lds);
 }
 
 // Groups within the node budget in the lean layout, v in HBM.
@@ -827,7 +977,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_window_lean_hbm(Dev d, BatchDev
                                                                  int32_t n) {
   extern __shared__ double lds[];
   __shared__ WinShared sh;
-  window_body<4>(d, b, groups[blockIdx.x], n, sh, lds);
+  window_body<4, false>(d, b, groups[blockIdx.x], n, sh, lds);
 }
 
 // Cells above the node budget, one per workgroup, each with the layout that
@@ -838,11 +988,45 @@ __global__ void __launch_bounds__(kLargeThreads) k_window_large(Dev d, BatchDev 
   __shared__ WinShared sh;
   const int grp = groups[blockIdx.x];
   switch (modes[blockIdx.x]) {
-    case 0: window_body<0>(d, b, grp, n, sh, lds); break;
-    case 4: window_body<4>(d, b, grp, n, sh, lds); break;
-    case 5: window_body<5>(d, b, grp, n, sh, lds); break;
-    default: window_body<3>(d, b, grp, n, sh, lds); break;
+    case 0: window_body<0, false>(d, b, grp, n, sh, lds); break;
+    case 4: window_body<4, false>(d, b, grp, n, sh, lds); break;
+    case 5: window_body<5, false>(d, b, grp, n, sh, lds); break;
+    default: window_body<3, false>(d, b, grp, n, sh, lds); break;
   }
 }
+
+#ifndef __HIPCC_RTC__
+// The same kernels for models the core paths cover (State::core). Models with
+// runtime-compiled mechanisms never take them, so the hipRTC build leaves them out.
+__global__ void __launch_bounds__(kMaxThreads) k_window_core(Dev d, BatchDev b, const int32_t* groups, int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  window_body<0, true>(d, b, groups[blockIdx.x], n, sh, lds);
+}
+__global__ void __launch_bounds__(kMaxThreads) k_window_lean_core(Dev d, BatchDev b, const int32_t* groups,
+                                                                  int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  window_body<5, true>(d, b, groups[blockIdx.x], n, sh, lds);
+}
+__global__ void __launch_bounds__(kMaxThreads) k_window_lean_hbm_core(Dev d, BatchDev b, const int32_t* groups,
+                                                                      int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  window_body<4, true>(d, b, groups[blockIdx.x], n, sh, lds);
+}
+__global__ void __launch_bounds__(kLargeThreads) k_window_large_core(Dev d, BatchDev b, const int32_t* groups,
+                                                                   const int8_t* modes, int32_t n) {
+  extern __shared__ double lds[];
+  __shared__ WinShared sh;
+  const int grp = groups[blockIdx.x];
+  switch (modes[blockIdx.x]) {
+    case 0: window_body<0, true>(d, b, grp, n, sh, lds); break;
+    case 4: window_body<4, true>(d, b, grp, n, sh, lds); break;
+    case 5: window_body<5, true>(d, b, grp, n, sh, lds); break;
+    default: window_body<3, true>(d, b, grp, n, sh, lds); break;
+  }
+}
+#endif
 
 }  // namespace mind

@@ -58,6 +58,14 @@ struct GroupDev {
   const int32_t* ion_row;
   const IonBind* ion_bind;
   int16_t nslot, nread, nwrite, nrecv;
+  // 1 if nrn_state loads the ion values and stores its writes: some mechanism
+  // writes a reversal or a concentration, or this one reads an ion current or
+  // di/dv. Otherwise the columns still hold what nrn_cur loaded in the same
+  // step, and there is nothing but currents to store.
+  int16_t ion_state;
+  // 1 for a density mechanism with at most two USEION statements, two reads and
+  // two written currents and no other writes (the batched kernels' hh path)
+  int16_t ion_fast;
 };
 
 struct Emit {  // one spike emission: source index and spike time
@@ -188,6 +196,7 @@ struct Dev {
   int64_t nion;
   double* ion;
   const uint8_t* ion_style;
+  int32_t ion_styles;            // the OR of every row's style
   const int32_t* ion_species;
   const double* ion_valence;     // per species
   const double* ion_conc0;       // per species: internal, external (cinit)
@@ -233,7 +242,7 @@ __device__ inline Cols cols(const GroupDev& g) { return Cols{g.data, g.rows}; }
 __device__ inline void ion_prepare(const Dev& d, int64_t q, uint8_t mask) {
   d.ion[MIND_ION_I * d.nion + q] = 0.0;
   d.ion[MIND_ION_DIDV * d.nion + q] = 0.0;
-  if (d.ion_style[q] & mask)
+  if ((d.ion_styles & mask) && (d.ion_style[q] & mask))
     d.ion[MIND_ION_E * d.nion + q] = mind_nernst(d.ion[MIND_ION_CI * d.nion + q], d.ion[MIND_ION_CO * d.nion + q],
                                                  d.ion_valence[d.ion_species[q]], d.celsius);
 }
@@ -262,7 +271,7 @@ __device__ inline void ion_store(const Dev& d, const GroupDev& G, const Cols& c,
 __device__ inline void ion_current(const Dev& d, const GroupDev& G, const Cols& c, int64_t r, double v, double t,
                                    double area, double& value, double& shifted) {
   ion_load(d, G, c, r);
-  double ish[kMaxIonCurrents];
+  double ish[2 * kMaxIonCurrents];
   phase_current(G.mech, c, r, v, t, d.dt, value, shifted, true, d.celsius, ish);
   for (int k = G.nread; k < G.nread + G.nwrite; ++k) {
     const IonBind b = G.ion_bind[k];
@@ -270,7 +279,7 @@ __device__ inline void ion_current(const Dev& d, const GroupDev& G, const Cols& 
     if (q < 0) continue;
     if (b.column == MIND_ION_I) {
       const double s = G.point ? 100.0 / area : 1.0;
-      const double x = c(b.field, r);
+      const double x = ish[kMaxIonCurrents + b.aux];  // the stored field's value
       d.ion[MIND_ION_I * d.nion + q] += s * x;
       d.ion[MIND_ION_DIDV * d.nion + q] += s * (ish[b.aux] - x) / 0.001;
     } else {
@@ -279,12 +288,13 @@ __device__ inline void ion_current(const Dev& d, const GroupDev& G, const Cols& 
   }
 }
 
-// nrn_state of a row: loads, SOLVE statements, stores.
+// nrn_state of a row: loads, SOLVE statements, stores (the loads and stores only
+// where they can change something, GroupDev::ion_state).
 __device__ inline void advance_row(const Dev& d, const GroupDev& G, int64_t r, double v, double t) {
   const Cols c = cols(G);
-  if (G.nslot) ion_load(d, G, c, r);
+  if (G.ion_state) ion_load(d, G, c, r);
   phase_advance(G.mech, c, r, v, t, d.dt, d.celsius);
-  if (G.nslot) ion_store(d, G, c, r);
+  if (G.ion_state) ion_store(d, G, c, r);
 }
 
 // NET_RECEIVE of a connection event: the concentrations NET_RECEIVE mentions are

@@ -89,8 +89,9 @@ MIND_HD void vclamp_amplifier(const Cols& c, int64_t r, double v, double dt, dou
 // v + 0.001; stores the assigned fields of the evaluation at v.
 // Current fields sum NONSPECIFIC currents first, then ion writes (hh: il, ina,
 // ik), and subtract ELECTRODE currents (src/runtime/initialize.jl:496-502).
-// ish (if not null) receives the written ion currents at v + 0.001 in USEION
-// order (hh: ina, ik), for the ion rows' di/dv.
+// ish (if not null) receives the written ion currents in USEION order (hh: ina,
+// ik) at v + 0.001 in ish[k], for the ion rows' di/dv, and at v in
+// ish[kMaxIonCurrents + k].
 MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t, double dt,
                           double& value, double& shifted, bool store, double* ish) {
   (void)dt;
@@ -108,9 +109,9 @@ MIND_HD void mech_current(int mech, const Cols& c, int64_t r, double v, double t
         sum += il;
         sum += ina;
         sum += ik;
-        if (probe == 0 && ish) {
-          ish[0] = ina;
-          ish[1] = ik;
+        if (ish) {
+          ish[probe == 0 ? 0 : kMaxIonCurrents] = ina;
+          ish[probe == 0 ? 1 : kMaxIonCurrents + 1] = ik;
         }
         if (probe == 1 && store) {
           c(HH_GNA, r) = gna; c(HH_INA, r) = ina; c(HH_GK, r) = gk; c(HH_IK, r) = ik;

@@ -86,6 +86,13 @@ struct CurOps {
     (void)celsius;
     mech_current(MECH, c, r, v, t, dt, value, shifted, store, nullptr);
   }
+  static MIND_HD void eval_ish(const Cols& c, int64_t r, const In&, double v, double t, double dt, double& value,
+                               double& shifted, bool store, double celsius, double* ish) {
+    (void)celsius;
+    mech_current(MECH, c, r, v, t, dt, value, shifted, store, ish);
+  }
+  // the evaluation reads the column itself, which the caller has stored
+  static MIND_HD void set(In&, int, double) {}
 };
 MIND_HD void phase_advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, double celsius) {
   MIND_USER_CASE(user::advance(mech, c, r, v, t, dt, celsius))

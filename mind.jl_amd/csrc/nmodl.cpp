@@ -1302,6 +1302,8 @@ class Gen {
           currents.push_back(w);
           ion_currents.push_back(w);
         }
+    if (ion_currents.size() > (size_t)kIshSlots)
+      err("a mechanism writes more than " + std::to_string(kIshSlots) + " ion currents");
     std::vector<std::string> all = currents;
     all.insert(all.end(), m_.electrode.begin(), m_.electrode.end());
     for (const std::string& n : all)
@@ -1334,6 +1336,12 @@ class Gen {
     }
     lines.push_back("    } else {");
     lines.push_back("      value = sum;");
+    if (!ion_currents.empty()) {
+      lines.push_back("      if (ish) {");
+      for (size_t k = 0; k < ion_currents.size(); ++k)
+        lines.push_back("        ish[" + std::to_string(kIshSlots + k) + "] = " + value_of(ion_currents[k]) + ";");
+      lines.push_back("      }");
+    }
     std::vector<std::string> stores;
     if (!written.empty()) {
       lines.push_back("      if (store) {");
@@ -1364,6 +1372,16 @@ class Gen {
       out.push_back("  (void)r;");
     }
     out.push_back("  return in;");
+    out.push_back("}");
+    // replaces one read column's value in <P>_cur_in (a USEION read loaded from
+    // the ion row after the column reads were issued)
+    out.push_back("MIND_HD void " + p_ + "_cur_set(" + p_ + "_cur_in& in, int f, double x) {");
+    out.push_back("  (void)in;");
+    out.push_back("  (void)x;");
+    out.push_back("  switch (f) {");
+    for (const std::string& n : reads) out.push_back("    case " + p_ + "_" + n + ": in." + n + " = x; break;");
+    out.push_back("    default: break;");
+    out.push_back("  }");
     out.push_back("}");
     out.push_back("MIND_HD void " + p_ + "_current_eval(const Cols& c, int64_t r, const " + p_ +
                   "_cur_in& in, double v, double t, double dt, double& value, double& shifted, bool store, "
@@ -1580,7 +1598,11 @@ std::string dispatch(const std::vector<Mechanism>& mechs) {
                   "_cur_in;\n  static MIND_HD In load(const Cols& c, int64_t r) { return " + n +
                   "_current_load(c, r); }\n  static MIND_HD void eval(const Cols& c, int64_t r, const In& in, double "
                   "v, double t, double dt, double& value, double& shifted, bool store, double celsius) {\n    " +
-                  n + "_current_eval(c, r, in, v, t, dt, value, shifted, store, celsius, nullptr);\n  }\n};");
+                  n + "_current_eval(c, r, in, v, t, dt, value, shifted, store, celsius, nullptr);\n  }\n" +
+                  "  static MIND_HD void eval_ish(const Cols& c, int64_t r, const In& in, double v, double t, double "
+                  "dt, double& value, double& shifted, bool store, double celsius, double* ish) {\n    " +
+                  n + "_current_eval(c, r, in, v, t, dt, value, shifted, store, celsius, ish);\n  }\n" +
+                  "  static MIND_HD void set(In& in, int f, double x) { " + n + "_cur_set(in, f, x); }\n};");
   }
   out.push_back("MIND_HD void advance(int mech, const Cols& c, int64_t r, double v, double t, double dt, "
                 "double celsius) {\n  switch (mech) {\n" +

@@ -14,6 +14,10 @@
 //       (src/runtime/cuda/micro.jl:447-472); the current sum takes NONSPECIFIC
 //       currents first, then ion WRITE currents in USEION order, minus
 //       ELECTRODE currents (src/runtime/initialize.jl:496-502)
+//       The `ish` argument, when not null, receives each written ion current
+//       k (USEION order) at v + 0.001 in ish[k] and at v in ish[kIshSlots + k]
+//       (the di/dv probe and the summed value, without reading the stored field
+//       back); a mechanism writes at most kIshSlots ion currents.
 //   <P>_advance   nrn_state: the SOLVE statements hoisted out of BREAKPOINT
 //       (codegen.jl:372-378): cnexp relaxations x + (1 - exp(dt*(-1/tau))) *
 //       (xinf - x) (codegen.jl:569-641), Euler, or a PROCEDURE
@@ -37,6 +41,9 @@
 
 namespace mind {
 namespace nmodl {
+
+// Written ion currents of one mechanism (the halves of the ish array).
+constexpr int kIshSlots = 8;
 
 // A lowering failure: "<file>:<line>: <message>".
 struct ModError : std::runtime_error {

@@ -519,7 +519,16 @@ int setup_state(State& S, const mind_build_options& o) {
     H.nrecv = (int16_t)G.nrecv;
     H.ion_row = nullptr;
     H.ion_bind = nullptr;
+    H.ion_state = 0;
+    H.ion_fast = 0;
     if (G.nslot) {
+      bool reads_sums = false, currents_only = G.nwrite <= 2;
+      for (int k = 0; k < G.nread; ++k)
+        reads_sums |= G.ion_bind[k].column == MIND_ION_I || G.ion_bind[k].column == MIND_ION_DIDV;
+      for (int k = G.nread; k < G.nread + G.nwrite; ++k)
+        currents_only &= G.ion_bind[k].column == MIND_ION_I && G.ion_bind[k].aux < 2;
+      H.ion_state = (L.ion_ordered || reads_sums) ? 1 : 0;
+      H.ion_fast = (!H.point && G.nslot <= 2 && G.nread <= 2 && currents_only) ? 1 : 0;
       IonBind* pb;
       if ((rc = S.upload(&p32, G.ion_row))) return rc; H.ion_row = p32;
       if ((rc = S.upload(&pb, G.ion_bind))) return rc; H.ion_bind = pb;
@@ -540,6 +549,8 @@ int setup_state(State& S, const mind_build_options& o) {
     if ((rc = S.alloc(&d.ion, std::max<size_t>(1, L.ion_data.size())))) return rc;
     if ((rc = S.upload(&S.ion_initial, L.ion_data))) return rc;
     if ((rc = S.upload(&p8, L.ion_style))) return rc; d.ion_style = p8;
+    d.ion_styles = 0;
+    for (uint8_t st : L.ion_style) d.ion_styles |= st;
     if ((rc = S.upload(&p32, L.ion_species))) return rc; d.ion_species = p32;
     if ((rc = S.upload(&pd, valence))) return rc; d.ion_valence = pd;
     if ((rc = S.upload(&pd, conc0))) return rc; d.ion_conc0 = pd;
@@ -549,6 +560,15 @@ int setup_state(State& S, const mind_build_options& o) {
     GroupDev* pg;
     if ((rc = S.upload(&pg, S.groups_host))) return rc;
     d.g = pg;
+  }
+  // the core window kernels take models whose groups all have inlined paths
+  S.core = !S.ks && !L.ion_ordered && !(L.secondorder == 2 && L.nion);
+  for (const GroupDev& H : S.groups_host) {
+    const int m = H.mech;
+    S.core = S.core && (m == MIND_MECH_PAS || m == MIND_MECH_HH || m == MIND_MECH_EXPSYN ||
+                        m == MIND_MECH_EXP2SYN || m == MIND_MECH_ICLAMP || m == MIND_MECH_NETSTIM);
+    if (H.nslot) S.core = S.core && m == MIND_MECH_HH && H.ion_fast && !H.ion_state && !H.nrecv;
+    if (m == MIND_MECH_PAS || m == MIND_MECH_HH) S.core = S.core && !H.point;
   }
   {
     int32_t* p32;

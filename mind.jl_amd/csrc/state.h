@@ -42,6 +42,10 @@ struct State {
   Dev d{};
   int device = 0;
   int method = MIND_METHOD_PLAIN;
+  // every mechanism group has an inlined path in the batched kernels (PAS, HH with
+  // at most its own ion currents, EXPSYN, EXP2SYN, ICLAMP, NETSTIM), no ion
+  // orders, no secondorder-2 ion correction: the *_core window kernels run it
+  bool core = false;
   hipStream_t stream = nullptr;
   hipStream_t aux = nullptr;          // concurrent launches of oversized cells
   hipEvent_t fork = nullptr, join = nullptr;

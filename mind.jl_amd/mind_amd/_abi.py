@@ -197,6 +197,7 @@ def lib():
     L.mind_connection_weights.argtypes = [C.c_void_p, C.c_int64, _p(C.c_double), C.c_int32, _p(C.c_int32)]
     L.mind_window_steps.argtypes = [C.c_void_p, _p(C.c_int32)]
     L.mind_batch_plan.argtypes = [C.c_void_p, _p(C.c_int64), _p(C.c_int32), _p(C.c_int32), _p(C.c_int64)]
+    L.mind_batch_kernel.argtypes = [C.c_void_p, _p(C.c_char_p)]
     L.mind_set_batch_plan.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32]
     L.mind_check_batch_plan.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_int32, C.c_int32, _p(C.c_int64)]
     L.mind_pack_nodes.argtypes = [_p(ModelDesc), C.c_double, _p(C.c_int64), _p(C.c_int64),

@@ -834,6 +834,15 @@ def batch_plan(model):
     return dict(budget=b.value, threads=t.value, layout=lay.value, groups=g.value)
 
 
+def batch_kernel(model):
+    """Name of the cell kernel the batched method launches for the workgroups within
+    the node budget (k_window, k_window_lean or k_window_lean_hbm; with _core when
+    every mechanism group of the model has an inlined path)."""
+    name = C.c_char_p()
+    _abi.check(_abi.lib().mind_batch_kernel(model.state, C.byref(name)))
+    return name.value.decode()
+
+
 def set_batch_plan(model, budget, threads=256, layout=0):
     """Replaces the autotuned plan; initialize(model) must follow."""
     _abi.check(_abi.lib().mind_set_batch_plan(model.state, int(budget), int(threads), int(layout)))

@@ -806,7 +806,9 @@ void currents(Sim& s, double t, int64_t c0, int64_t c1) {  // micro_currents! (s
         if (node < 0) continue;
         const double v = s.v[node];
         double shifted = 0.0, value = 0.0;
-        double ish[8] = {0};  // written ion currents at v + 0.001
+        // written ion currents at v + 0.001 (ish[k]); compiled phases also write
+        // them at v (ish[8 + k])
+        double ish[16] = {0};
         const bool ions = !ion_decls(g.mech).empty();
         if (ions) ion_load(s, g, (int)r);
         if (g.mech >= MIND_MECH_COUNT) {  // both probes inside the compiled phase

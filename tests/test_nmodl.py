@@ -216,7 +216,7 @@ HARNESS = textwrap.dedent("""
           mech_initial(mech, ca, 0, v, cel);
           gen::initial(mech, cb, 0, v, cel);
           same(a, b, sizeof a, "initial", mech, trial);
-          double ia[8] = {0}, ib[8] = {0};
+          double ia[16] = {0}, ib[16] = {0};  // written ion currents at v + 0.001, then at v
           mech_current(mech, ca, 0, v, t, dt, va, sa, true, ia);
           gen::current(mech, cb, 0, v, t, dt, vb, sb, true, 6.3, ib);
           same(ia, ib, sizeof ia, "ion current probe", mech, trial);
