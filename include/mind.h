@@ -316,6 +316,12 @@ int mind_mech_ions(int32_t mech, int32_t index, const char** name, int32_t* has_
  * NET_RECEIVE block touches it (0 for the builtins), up to `capacity`; *count is the
  * number of columns. */
 int mind_mech_columns(int32_t mech, uint8_t* is_state, uint8_t* in_receive, int32_t capacity, int32_t* count);
+/* Per column: 1 for a GLOBAL, one value every instance shares (global_ref,
+ * src/model/references.jl:147-154), up to `capacity`; *count is the number of
+ * columns. mind_compile_mod refuses a MOD that assigns a GLOBAL (the reference runs
+ * such instances one after another), so each instance's column holds the GLOBAL's
+ * value. The builtins declare none. */
+int mind_mech_globals(int32_t mech, uint8_t* is_global, int32_t capacity, int32_t* count);
 /* C++ text of a compiled mechanism's phases (namespace mind::user, functions
  * u<id>_current/_advance/_initial/_receive over Cols), as hipRTC compiles them;
  * for host builds of the same lowering. */

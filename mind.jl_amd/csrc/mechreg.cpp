@@ -224,6 +224,17 @@ extern "C" int mind_mech_ions(int32_t mech, int32_t index, const char** name, in
   return 0;
 }
 
+extern "C" int mind_mech_globals(int32_t mech, uint8_t* is_global, int32_t capacity, int32_t* count) {
+  if (!count) return fail(MIND_EARG, "mind_mech_globals: null count");
+  if (!mech_known(mech)) return fail(MIND_EARG, "unknown mechanism " + std::to_string(mech));
+  const int n = field_count(mech);
+  const UserSlot* s = user_slot(mech);  // the builtins declare no GLOBAL
+  for (int k = 0; k < n && k < capacity; ++k)
+    if (is_global) is_global[k] = s && k < (int)s->m.global_column.size() && s->m.global_column[k] ? 1 : 0;
+  *count = n;
+  return 0;
+}
+
 extern "C" int mind_mech_columns(int32_t mech, uint8_t* is_state, uint8_t* in_receive, int32_t capacity,
                                  int32_t* count) {
   if (!count) return fail(MIND_EARG, "mind_mech_columns: null count");

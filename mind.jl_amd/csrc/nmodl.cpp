@@ -362,6 +362,7 @@ struct Mechanism {
   std::vector<Binding> bindings;
   std::vector<Ion> ions;
   std::vector<std::string> nonspecific, electrode, ranges, randoms;
+  std::vector<std::string> globals;  // GLOBAL names: one value every instance shares
   std::vector<Decl> decls;
   std::vector<Block> blocks;  // in first-definition order; a redefinition replaces the body in place
   std::map<std::string, double> defines;
@@ -530,7 +531,9 @@ class Parser {
       } else if (k == "ELECTRODE_CURRENT") {
         append(m_.electrode, names_list());
       } else if (k == "RANGE" || k == "GLOBAL") {
-        append(m_.ranges, names_list());
+        const std::vector<std::string> names = names_list();
+        append(m_.ranges, names);
+        if (k == "GLOBAL") append(m_.globals, names);
       } else if (k == "RANDOM") {
         append(m_.randoms, names_list());
       } else if (k == "THREADSAFE") {
@@ -843,6 +846,14 @@ class Gen {
   int ncols() const { return ncols_; }
   const std::vector<std::string>& fields() const { return fields_; }
   int column(const std::string& n) const { return findex_.at(n); }
+  // Fields a block assigns, through the routines it calls.
+  std::vector<std::string> block_writes(const Block& b) const {
+    std::vector<std::string> acc;
+    std::set<std::string> local(b.args.begin(), b.args.end());
+    if (b.kind == "function") local.insert(b.name);
+    collect_writes(b.body, acc, {b.name}, local);
+    return acc;
+  }
 
   std::string emit() {
     std::vector<std::string> out;
@@ -1725,6 +1736,19 @@ UserMechanism lower_user(const std::string& filename, const std::string& text, c
     }
     uses.push_back(u);
   }
+  // A GLOBAL is one value every instance shares (the reference's shared storage,
+  // initialize.jl:97-98; global_ref, references.jl:147-154). Instances that assign
+  // one run one after another there (serial instances, initialize.jl:959-1086); the
+  // device phases run instances side by side, so a GLOBAL may be read, not assigned.
+  std::set<std::string> globals;
+  for (const std::string& n : m.globals)
+    if (std::count(m.ranges.begin(), m.ranges.end(), n) == 1) globals.insert(n);  // not also RANGE
+  for (const Block& b : m.blocks)
+    for (const std::string& n : g.block_writes(b))
+      if (globals.count(n))
+        raise_at(filename, b.line,
+                 "GLOBAL " + n + " is assigned: instances sharing a GLOBAL would run one after another; "
+                 "declare it RANGE");
   UserMechanism u;
   u.name = m.name;
   u.filename = filename;
@@ -1747,6 +1771,7 @@ UserMechanism lower_user(const std::string& filename, const std::string& text, c
       if (ion_value.count(n)) v = ion_value[n];
       u.defaults.push_back(v);
       u.state_column.push_back(states.count(n) ? 1 : 0);
+      u.global_column.push_back(globals.count(n) ? 1 : 0);
     }
   }
   // the fields NET_RECEIVE touches, from its lowered text

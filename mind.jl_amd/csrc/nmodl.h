@@ -80,6 +80,7 @@ struct UserMechanism {
   std::string code;                  // C++ phases named <prefix>_*, for namespace mind::user
   std::vector<IonUse> ions;          // USEION statements in declaration order
   std::vector<uint8_t> state_column; // per column: 1 for a STATE variable
+  std::vector<uint8_t> global_column; // per column: 1 for a GLOBAL (shared, never assigned)
   std::vector<int> receive_fields;   // columns NET_RECEIVE reads or writes
 };
 

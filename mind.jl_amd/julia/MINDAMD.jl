@@ -21,7 +21,7 @@ module MINDAMD
 export ModelArrays, MindError, build_state, initialize_state!, run_state!, step_state!,
        destroy_state!, read_voltage, voltage_record_spec, field_record_spec, read_value,
        connection_weights, inject_event!, inject_spike!, Mechanism, compile_mod, load_mod,
-       builtin, column, flatten, MacroArrays, flatten_macro, build_macro_state, run_macro_state!,
+       builtin, column, globals, flatten, MacroArrays, flatten_macro, build_macro_state, run_macro_state!,
        SurrogateArrays, build_surrogate, initialize_surrogate!, run_surrogate!, destroy_surrogate!
 
 const libmind = get(ENV, "MIND_LIBRARY", joinpath(@__DIR__, "..", "csrc", "libmind.so"))
@@ -406,6 +406,23 @@ function column(m::Mechanism, field, index::Integer = 1)
     return Int32(k - 1)
 end
 
+"""
+    globals(m::Mechanism) -> Vector{String}
+
+The GLOBAL columns of a mechanism (`mind_mech_globals`): one value every instance
+shares (global_ref, src/model/references.jl:147-154). `compile_mod` refuses a MOD
+that assigns one; `flatten`'s `globals` keyword sets their values for every
+instance. The builtins declare none.
+"""
+function globals(m::Mechanism)
+    n = length(m.fields)
+    flags = zeros(UInt8, max(n, 1))
+    count = Ref{Int32}(0)
+    check(ccall((:mind_mech_globals, libmind), Cint, (Int32, Ptr{UInt8}, Int32, Ptr{Int32}),
+                m.id, flags, Int32(n), count))
+    return [m.fields[k] for k in 1:n if flags[k] != 0]
+end
+
 # ---- flattening a MIND.jl model ----------------------------------------------------
 """
     flatten(model; mechanisms=Dict{Symbol, Mechanism}(), macro_inputs=false) -> ModelArrays
@@ -417,7 +434,9 @@ Flattens a MIND.jl `MicroModel` (src/model/micro/model.jl:15-50: morphologies of
 `mechanisms` (name => `Mechanism` from `compile_mod`/`load_mod` here), else to
 the mechanism of the same name (`mind_mech_lookup`: the most recently compiled
 one, else the builtin). Parameter values are numbers, or vectors for array
-fields. Connections from macro inputs (negative source index) are left out:
+fields. `globals` (mechanism name => field => value) sets GLOBAL values for every
+instance of a mechanism (the reference's global parameters, `pack_globals`,
+src/model/build.jl:388). Connections from macro inputs (negative source index) are left out:
 `flatten_macro` passes them as MACRO2MICRO bindings.
 
 Raises `ArgumentError` for what the device path does not take: per-section
@@ -425,7 +444,8 @@ initial voltages, per-cell ion settings, ion species other than the defaults,
 threshold sources on mechanism fields, NET_RECEIVE blocks with several
 arguments, parameter values that are functions.
 """
-function flatten(model; mechanisms::AbstractDict = Dict{Symbol, Mechanism}(), macro_inputs::Bool = false)
+function flatten(model; mechanisms::AbstractDict = Dict{Symbol, Mechanism}(), macro_inputs::Bool = false,
+                 globals::AbstractDict = Dict{Symbol, Any}())
     a = ModelArrays(celsius = Float64(model.celsius))
     known = Dict{Symbol, Mechanism}(Symbol(k) => v for (k, v) in mechanisms)
     function resolve(compiled)
@@ -437,7 +457,8 @@ function flatten(model; mechanisms::AbstractDict = Dict{Symbol, Mechanism}(), ma
         return known[name]
     end
     function params!(m::Mechanism, pairs)
-        for (key, value) in pairs
+        shared = get(globals, Symbol(m.name), Dict{Symbol, Any}())
+        for (key, value) in Iterators.flatten((pairs, shared))
             if value isa Real
                 push!(a.param_field, column(m, key))
                 push!(a.param_value, Float64(value))

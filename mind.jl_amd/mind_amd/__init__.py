@@ -10,8 +10,8 @@ from .model import (  # noqa: F401
     SpikeSource, default_ions, ion_style,
     add_pointprocess, add_population, batch_plan, build, checkpoint, connect, connection_weights, destroy,
     restore,
-    initialize, inject_event, inject_spike, insert, ion_record, read_value, section, section_group,
-    set_batch_plan,
+    global_record, initialize, inject_event, inject_spike, insert, ion_record, read_value, section, section_group,
+    set_batch_plan, set_global,
     mechanism_record, read_field, read_voltage, register_spike_source, run, step,
     voltage_record, window_steps,
 )

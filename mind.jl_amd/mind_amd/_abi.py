@@ -218,6 +218,7 @@ def lib():
     L.mind_mech_ions.argtypes = [C.c_int32, C.c_int32, _p(C.c_char_p), _p(C.c_int32), _p(C.c_double),
                                  _p(C.c_int32), _p(C.c_int32), _p(C.c_int32)]
     L.mind_mech_columns.argtypes = [C.c_int32, _p(C.c_uint8), _p(C.c_uint8), C.c_int32, _p(C.c_int32)]
+    L.mind_mech_globals.argtypes = [C.c_int32, _p(C.c_uint8), C.c_int32, _p(C.c_int32)]
     L.mind_mech_lowering.argtypes = [C.c_int32, _p(C.c_char_p)]
     L.mind_rtc_check.argtypes = [_p(C.c_int32), C.c_int32, C.c_char_p]
     _lib = L

@@ -177,6 +177,23 @@ def columns_info(mechanism):
     return tuple(bool(st[k]) for k in range(n)), tuple(bool(rc[k]) for k in range(n))
 
 
+def globals_of(mechanism) -> tuple:
+    """Names of a mechanism's GLOBAL fields (one value every instance shares), from
+    libmind; the builtins declare none."""
+    from . import _abi
+
+    mech = resolve(mechanism)
+    if mech.id not in _GLOBALS:
+        n = len(mech.fields)
+        flags, count = (C.c_uint8 * max(1, n))(), C.c_int32(0)
+        _abi.check(_abi.lib().mind_mech_globals(mech.id, flags, n, C.byref(count)))
+        _GLOBALS[mech.id] = tuple(f for k, f in enumerate(mech.fields) if flags[k])
+    return _GLOBALS[mech.id]
+
+
+_GLOBALS: dict = {}
+
+
 def builtin(name) -> Mechanism:
     """Select a packaged builtin mechanism by name (hh, pas, ExpSyn, Exp2Syn, IClamp, NetStim,
     AlphaSynapse, PatternStim, SEClamp, VClamp)."""

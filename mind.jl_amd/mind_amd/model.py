@@ -247,6 +247,8 @@ class MicroModel:
             self.ions[ion.name] = ion
         # section ion settings: (cell, section, ion name, what, value)
         self.ion_settings: list = []
+        # GLOBAL values set for every instance of a mechanism: mech id -> {column: value}
+        self.global_values: dict = {}
         self.morphologies: list[Morphology] = []
         self.cell_morph: list[int] = []
         self.cell_gid: list[int] = []
@@ -334,7 +336,10 @@ class MicroModel:
         """(column, value) overrides; an array field takes a sequence of its extent."""
         items = params.items() if isinstance(params, dict) else params
         out = []
+        shared = _global_names(mech)
         for k, v in items:
+            if str(k) in shared:
+                raise ValueError(f"{k} is a GLOBAL of {mech.name}: set it for every instance with set_global")
             if isinstance(v, (list, tuple, np.ndarray)):
                 if len(v) != mech.extent(str(k)) or mech.extent(str(k)) == 1 and str(k) in mech.fields:
                     raise ValueError(f"{mech.name}.{k} takes {mech.extent(str(k))} values")
@@ -342,6 +347,21 @@ class MicroModel:
             else:
                 out.append((mech.field_index(str(k)), float(v)))
         return out
+
+    def set_global(self, mechanism, fname, value, index=1):
+        """The value of a mechanism's GLOBAL for every instance (the reference's global
+        parameters, packed once per mechanism: pack_globals, src/model/build.jl:388)."""
+        mech = resolve(mechanism)
+        key = str(fname).lstrip(":")
+        if key not in _global_names(mech):
+            raise ValueError(f"{key} is not a shared GLOBAL of {mech.name}")
+        if not np.isfinite(value):
+            raise ValueError("a GLOBAL value must be finite")
+        self.global_values.setdefault(mech.id, {})[mech.field_index(key, index)] = float(value)
+
+    def _with_globals(self, mech_id, params):
+        g = self.global_values.get(mech_id)
+        return list(params) + sorted(g.items()) if g else params
 
     def insert(self, cell: Cell, mechanism, sections="all", params=None):
         """Insert a density mechanism where absent (model.jl:153-169)."""
@@ -508,14 +528,14 @@ class MicroModel:
         d.dens_section = _abi.as_ptr(arr("ds", [e[1] for e in self.density], np.int32), C.c_int32)
         d.dens_mech = _abi.as_ptr(arr("dm", [e[2] for e in self.density], np.int32), C.c_int32)
         d.dens_param_offsets = _abi.as_ptr(
-            arr("dpo", params_csr(self.density, lambda e: e[3]), np.int64), C.c_int64)
+            arr("dpo", params_csr(self.density, lambda e: self._with_globals(e[2], e[3])), np.int64), C.c_int64)
         d.n_points = len(self.points)
         d.pt_mech = _abi.as_ptr(arr("pm", [e[0] for e in self.points], np.int32), C.c_int32)
         d.pt_cell = _abi.as_ptr(arr("pc", [e[1] for e in self.points], np.int32), C.c_int32)
         d.pt_section = _abi.as_ptr(arr("ps", [e[2] for e in self.points], np.int32), C.c_int32)
         d.pt_x = _abi.as_ptr(arr("px", [e[3] for e in self.points], np.float64), C.c_double)
         d.pt_param_offsets = _abi.as_ptr(
-            arr("ppo", params_csr(self.points, lambda e: e[4]), np.int64), C.c_int64)
+            arr("ppo", params_csr(self.points, lambda e: self._with_globals(e[0], e[4])), np.int64), C.c_int64)
         if any(e[5] for e in self.points):
             spike_off = np.cumsum([0] + [len(e[5]) for e in self.points])
             d.pt_spike_offsets = _abi.as_ptr(arr("pso", spike_off, np.int64), C.c_int64)
@@ -647,6 +667,38 @@ def mechanism_record(model, target, *args, name=None, section="soma", x=0.5, ind
     s = model._morph(target).section_index(section)
     return RecordSpec(name or f"{target.gid}.{fname}",
                       (2, target.index, s, float(x), -1, mech.id, mech.field_index(str(fname), index)))
+
+
+def _global_names(mech) -> frozenset:
+    """Field names (array fields by their base name) of a mechanism's GLOBALs."""
+    if not mech.compiled:
+        return frozenset()  # the builtins declare none
+    from .mechanisms import globals_of
+    return frozenset(f.split("[")[0] for f in globals_of(mech))
+
+
+def set_global(model, mechanism, fname, value, index=1):
+    return model.set_global(mechanism, fname, value, index)
+
+
+def global_record(model, mechanism, fname, name=None, index=1):
+    """global_record(model, mechanism, field) (recording.jl:32-38; global_ref,
+    references.jl:147-154): a GLOBAL of a mechanism the model uses, one value every
+    instance shares. Its phases never assign it (compile_mod refuses that), so every
+    instance's column holds the value; the record reads the first instance's: the
+    first point process of the mechanism, else its first density insertion."""
+    mech = resolve(mechanism)
+    key = str(fname).lstrip(":")
+    if key not in _global_names(mech):
+        raise ValueError(f"{key} is not a shared GLOBAL of {mech.name}")
+    column = mech.field_index(key, index)
+    for k, point in enumerate(model.points):
+        if point[0] == mech.id:
+            return RecordSpec(name or key, (1, -1, -1, 0.0, k, mech.id, column))
+    for cell_index, s, mech_id, _ in model.density:
+        if mech_id == mech.id:
+            return RecordSpec(name or key, (2, cell_index, s, 0.5, -1, mech.id, column))
+    raise ValueError(f"the model has no instance of {mech.name}")
 
 
 def ion_record(model, target, fname, name=None, section="soma", x=0.5):

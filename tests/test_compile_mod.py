@@ -115,6 +115,10 @@ def test_compile_mod_slots_fields_and_defaults():
     ("NEURON { SUFFIX a USEION ca READ cai }", "cai: not a declared scalar variable"),
     ("NEURON { POINT_PROCESS a }\nASSIGNED { x }\nNET_RECEIVE(w) { x = celsius }", "celsius in NET_RECEIVE"),
     ("NEURON { SUFFIX a }\nBREAKPOINT { x = 1 }", "cannot assign x"),
+    ("NEURON { SUFFIX a GLOBAL g }\nASSIGNED { g }\nINITIAL { g = 2 }", "GLOBAL g is assigned"),
+    ("NEURON { SUFFIX a GLOBAL g }\nASSIGNED { g }\nPROCEDURE p() { g = 1 }\nINITIAL { p() }",
+     "GLOBAL g is assigned"),
+    ("NEURON { POINT_PROCESS a GLOBAL g }\nASSIGNED { g }\nNET_RECEIVE(w) { g = w }", "GLOBAL g is assigned"),
 ])
 def test_compile_mod_refusals_name_file_and_line(text, msg):
     with pytest.raises(ValueError, match=msg) as err:
@@ -162,3 +166,49 @@ def test_oracle_user_lowering_equals_its_builtin_restatement():
     for name in a.traces:
         assert np.array_equal(a.traces[name], b.traces[name]), name
     assert va.max() - va.min() > 15.0  # the clamp and the synapses move the membrane
+
+
+def _leak_network(user, ncells=5, g=3e-4, e=-61.0):
+    """Ball-and-stick cells with hh on the soma and a leak everywhere: the compiled
+    leakg with its GLOBALs set for every instance, or pas with the same values."""
+    m = MicroModel()
+    cells = m.add_population(configs.ball_and_stick_sections(10), ncells)
+    leak = load_mod(os.path.join(MODS, "leakg.mod")) if user else builtin("pas")
+    if user:
+        m.set_global(leak, "gl", g)
+        m.set_global(leak, "el", e)
+    for k, c in enumerate(cells):
+        m.insert(c, builtin("hh"), sections="soma")
+        m.insert(c, leak, sections="all", params={} if user else {"g": g, "e": e})
+        m.add_pointprocess(c, "IClamp", section="soma", params={"del": 1.0 + k, "dur": 20.0, "amp": 0.3})
+    return m, cells, leak
+
+
+def test_globals_are_shared_values_read_by_every_instance():
+    from mind_amd import global_record, mechanism_record
+    from mind_amd.mechanisms import globals_of
+
+    m, cells, leak = _leak_network(True)
+    assert globals_of(leak) == ("gl", "el") and globals_of(builtin("pas")) == ()
+    with pytest.raises(ValueError, match="GLOBAL of leakg"):
+        m.insert(cells[0], leak, params={"gl": 1e-3})
+    with pytest.raises(ValueError, match="not a shared GLOBAL"):
+        global_record(m, leak, "i")
+    with pytest.raises(ValueError, match="not a shared GLOBAL"):
+        m.set_global(leak, "i", 1.0)
+    recs = [global_record(m, leak, "gl", name="gl"), global_record(m, leak, "el", name="el"),
+            mechanism_record(m, cells[3], leak, "i", section="dend", name="i3")]
+    recs += [voltage_record(m, c, "soma", 0.5, name=f"v{k}") for k, c in enumerate(cells)]
+    sim = oracle.OracleSim(m).initialize()
+    a = sim.run(25.0, recs)
+    # every sample of a GLOBAL is the value set for every instance
+    assert np.all(a.traces["gl"] == 3e-4) and np.all(a.traces["el"] == -61.0)
+    assert np.any(a.traces["i3"] != 0.0)
+    # and the compiled leak with those GLOBALs is pas with the same values, bit for bit
+    p, pcells, _ = _leak_network(False)
+    psim = oracle.OracleSim(p).initialize()
+    b = psim.run(25.0, [voltage_record(p, c, "soma", 0.5, name=f"v{k}") for k, c in enumerate(pcells)])
+    assert np.array_equal(sim.voltage(), psim.voltage())
+    for k in range(len(cells)):
+        assert np.array_equal(a.traces[f"v{k}"], b.traces[f"v{k}"])
+    assert max(a.traces[f"v{k}"].max() for k in range(len(cells))) > 0.0  # the clamps fire the somata

@@ -138,3 +138,47 @@ def test_compiled_synapse_on_event_exact_path(gpu, method):
     assert np.array_equal(got.spike_times, ref.spike_times)
     for r in recs:
         assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
+def test_compiled_globals_and_global_record(gpu, method):
+    """A compiled leak whose conductance and reversal are GLOBALs set for every
+    instance: global_record reads the shared value at every sample, and voltages,
+    spikes and records equal the oracle's and the same model with pas, bit for bit."""
+    from mind_amd import global_record
+
+    def network(user, ncells=16, g=3e-4, e=-61.0):
+        m = MicroModel()
+        cells = m.add_population(configs.ball_and_stick_sections(10), ncells)
+        leak = _mod("leakg") if user else builtin("pas")
+        if user:
+            m.set_global(leak, "gl", g)
+            m.set_global(leak, "el", e)
+        for k, c in enumerate(cells):
+            m.insert(c, builtin("hh"), sections="soma")
+            m.insert(c, leak, sections="all", params={} if user else {"g": g, "e": e})
+            m.add_pointprocess(c, "IClamp", section="soma", params={"del": 1.0 + k % 4, "dur": 20.0, "amp": 0.3})
+            m.register_spike_source(c, section="soma", x=0.5, threshold=-20.0)
+        return m, cells, leak
+
+    m, cells, leak = network(True)
+    recs = [global_record(m, leak, "gl", name="gl"), global_record(m, leak, "el", name="el"),
+            mechanism_record(m, cells[5], leak, "i", section="dend", name="i5")]
+    recs += [voltage_record(m, cells[k], "soma", 0.5, name=f"v{k}") for k in range(4)]
+    print(f"case: compiled GLOBAL leak, {method}", flush=True)
+    got, v_got = _run_gpu(m, recs, method, 30.0)
+    destroy(m)
+    sim = oracle.OracleSim(m).initialize()
+    ref = sim.run(30.0, recs)
+    assert np.all(got.traces["gl"] == 3e-4) and np.all(got.traces["el"] == -61.0)
+    assert np.array_equal(v_got, sim.voltage())
+    assert np.array_equal(got.spike_times, ref.spike_times)
+    for r in recs:
+        assert np.array_equal(got.traces[r.name], ref.traces[r.name]), r.name
+    p, pcells, _ = network(False)
+    prec = [voltage_record(p, pcells[k], "soma", 0.5, name=f"v{k}") for k in range(4)]
+    pgot, pv = _run_gpu(p, prec, method, 30.0)
+    destroy(p)
+    assert np.array_equal(v_got, pv)
+    assert np.array_equal(got.spike_times, pgot.spike_times)
+    assert len(got.spike_times) > 10
