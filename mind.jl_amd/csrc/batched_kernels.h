@@ -909,20 +909,24 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       }
     } else {
     // nrn_state at t + dt: the rows of all stateful groups as one index space
-    // (rows are independent), so that the groups' rows run side by side
+    // (rows are independent), so that the groups' rows run side by side; each
+    // group's rows start at a multiple of 64, so that no wavefront runs two
+    // mechanisms' updates one after the other under divergent masks
     {
+      auto span = [](int64_t cnt) { return (cnt + 63) & ~(int64_t)63; };
       int64_t total = 0;
       for (int gi = 0; gi < ngr; ++gi)
-        if (stateful(sg[gi].mech)) total += sr1[gi] - sr0[gi];
+        if (stateful(sg[gi].mech)) total += span(sr1[gi] - sr0[gi]);
       for (int64_t f = tid; f < total; f += T) {
         int gi = 0;
         int64_t r = f;
         for (;; ++gi) {
           if (!stateful(sg[gi].mech)) continue;
-          const int64_t cnt = sr1[gi] - sr0[gi];
+          const int64_t cnt = span(sr1[gi] - sr0[gi]);
           if (r < cnt) break;
           r -= cnt;
         }
+        if (r >= sr1[gi] - sr0[gi]) continue;  // the padding of the group's last wavefront
         const GroupDev& G = sg[gi];
         r += sr0[gi];
         if (CORE)
