@@ -793,6 +793,20 @@ int stage_injections(State& S, int32_t n) {
   return 0;
 }
 
+// The longest window of at most n steps from S.step in which at most kInjCap
+// injected source spikes fall due (a window hands its exchange at most that many);
+// 0 if more than kInjCap fall due in the first step.
+int32_t injection_window(const State& S, int32_t n) {
+  if ((int64_t)S.pending_spikes.size() <= kInjCap) return n;
+  std::vector<double> t;
+  t.reserve(S.pending_spikes.size());
+  for (const InjSpike& q : S.pending_spikes) t.push_back(q.t);
+  std::nth_element(t.begin(), t.begin() + kInjCap, t.end());
+  const double first_over = t[kInjCap];  // the earliest time by which more than kInjCap are due
+  while (n > 0 && first_over <= call_cutoff(2 * (S.step + n) - 1, S.L.dt)) --n;
+  return n;
+}
+
 // The cell kernel of n steps (plain or batched).
 int cells_launch(State& S, int32_t n) {
   if (S.method == MIND_METHOD_BATCHED) return batched_launch(S, n);
@@ -1052,7 +1066,9 @@ int mind_run(mind_state* state, double stop, const mind_record* records, int64_t
   const auto t0 = std::chrono::steady_clock::now();
   int64_t left = count;
   while (left > 0) {
-    const int32_t n = (int32_t)std::min<int64_t>(left, S.wmax);
+    // a window short enough that the injected spikes due in it fit its exchange
+    const int32_t n = mind::injection_window(S, (int32_t)std::min<int64_t>(left, S.wmax));
+    if (n == 0) return fail(MIND_EQUEUE, "more injected spikes due in one step than a window takes");
     if ((rc = mind::stage_injections(S, n))) return rc;
     if (n == S.wmax && !S.profiling) {
       if (!S.graph) {

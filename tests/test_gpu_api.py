@@ -67,6 +67,37 @@ def test_injected_events_and_spikes_match_oracle(gpu, method):
 
 
 @pytest.mark.parametrize("method", ["plain", "batched"])
+def test_thousands_of_injected_spikes_in_one_window(gpu, method):
+    """3000 injected spikes due within 0.3 ms (more than one window's exchange takes):
+    the run shortens its windows so that each hands at most 1024 to the exchange, with
+    results identical to the oracle's; more than 1024 due in one step are refused."""
+    model, info, ext, conns = _network()
+    recs = [voltage_record(model, c, "soma", 0.5) for c in info["cells"][:4]]
+    print(f"case: injection burst {method}", flush=True)
+    build(model, dt_micro=0.025, seed=1, method=method)
+    initialize(model)
+    sim = oracle.OracleSim(model).initialize()
+    times = 20.0 + 1e-4 * np.arange(3000)
+    got, ref = [run(model, 15.0, recs)], [sim.run(15.0, recs)]
+    for t in times:
+        inject_spike(model, ext, float(t))
+        sim.inject_spike(ext, float(t))
+    got.append(run(model, 40.0, recs))
+    ref.append(sim.run(40.0, recs))
+    for g, r in zip(got, ref):
+        for rec in recs:
+            assert np.array_equal(g.traces[rec.name], r.traces[rec.name]), rec.name
+        assert np.array_equal(g.spike_times, r.spike_times)
+        assert np.array_equal(g.spike_sources, r.spike_sources)
+    assert np.array_equal(np.sort(got[1].spike_times[got[1].spike_sources == 900]), times)
+    for k in range(1100):  # all due within one step
+        inject_spike(model, ext, 50.0 + 1e-6 * k)
+    with pytest.raises(Exception):
+        run(model, 60.0, recs)
+    destroy(model)
+
+
+@pytest.mark.parametrize("method", ["plain", "batched"])
 def test_read_value_and_selectors_match_oracle(gpu, method):
     model, info, ext, conns = _network()
     cell = info["cells"][3]
