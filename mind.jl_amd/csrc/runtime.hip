@@ -357,6 +357,45 @@ __device__ inline bool win_less(const QEvent& a, const QEvent& b, const int32_t*
   return a.ticket < b.ticket;
 }
 
+// Restores the max-heap (by win_less) of w[0, m) below `root`.
+__device__ inline void sift_down(QEvent* w, int64_t root, int64_t m, const int32_t* edge_target) {
+  const QEvent x = w[root];
+  for (;;) {
+    int64_t c = 2 * root + 1;
+    if (c >= m) break;
+    if (c + 1 < m && win_less(w[c], w[c + 1], edge_target)) ++c;
+    if (!win_less(x, w[c], edge_target)) break;
+    w[root] = w[c];
+    root = c;
+  }
+  w[root] = x;
+}
+
+// Sorts one bucket's events w[0, m) by win_less: insertion sort for short lists,
+// heapsort (O(m log m), in place) for long ones. win_less is a strict total
+// order (tickets are unique), so both give the one sorted order.
+__device__ inline void sort_bucket(QEvent* w, int64_t m, const int32_t* edge_target) {
+  if (m <= 32) {
+    for (int64_t i = 1; i < m; ++i) {
+      const QEvent x = w[i];
+      int64_t j = i - 1;
+      while (j >= 0 && win_less(x, w[j], edge_target)) {
+        w[j + 1] = w[j];
+        --j;
+      }
+      w[j + 1] = x;
+    }
+    return;
+  }
+  for (int64_t i = m / 2 - 1; i >= 0; --i) sift_down(w, i, m, edge_target);
+  for (int64_t e = m - 1; e > 0; --e) {
+    const QEvent top = w[0];
+    w[0] = w[e];
+    w[e] = top;
+    sift_down(w, 0, e, edge_target);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
   __shared__ int64_t scan[kBlock / 64 + 1];
   const int tid = threadIdx.x;
@@ -411,18 +450,10 @@ __global__ void __launch_bounds__(kBlock) k_gather(Dev d, int32_t n) {
     }
   }
   __syncthreads();
-  // 4. deterministic order inside each bucket (insertion sort; lists are short)
+  // 4. deterministic order inside each bucket
   for (int64_t c = tid; c < nb; c += kBlock) {
-    const int64_t a = d.seg_off[c], z = d.seg_off[c + 1];
-    for (int64_t i = a + 1; i < z; ++i) {
-      const QEvent x = d.win[i];
-      int64_t j = i - 1;
-      while (j >= a && win_less(x, d.win[j], d.edge_target)) {
-        d.win[j + 1] = d.win[j];
-        --j;
-      }
-      d.win[j + 1] = x;
-    }
+    const int64_t a = d.seg_off[c];
+    sort_bucket(d.win + a, d.seg_off[c + 1] - a, d.edge_target);
   }
   __syncthreads();
   for (int s = tid; s < n; s += kBlock) d.ring_count[(k1 + s) % d.ring_slots] = 0;
