@@ -104,26 +104,46 @@ inline size_t window_bytes(int m, int64_t nn, int64_t slen) {
   return (size_t)(32 * nn + (2 * rtab_len(nn, slen) + 7) / 8 * 8);
 }
 
-// Delivery of the window-list events due at delivery call `call` (thread 0).
-// CORE: the receivers are the builtin synapses without ion exchanges (ExpSyn,
-// Exp2Syn; IClamp has no NET_RECEIVE), so only their phases are compiled in.
-template <bool CORE>
+// Delivery of the window-list events due at delivery call `call` (thread 0), for
+// any receiver.
 __device__ inline int64_t deliver_group(const Dev& d, int64_t seg, int64_t seg_end, int64_t call) {
   while (seg < seg_end && d.win[seg].call == call) {
     const int32_t e = d.win[seg].edge;
     const int32_t target = d.edge_target[e];
-    const GroupDev& G = d.g[d.point_group[target]];
-    const int64_t row = d.point_row[target];
-    if (!CORE) {
-      receive_event(d, G, row, d.win[seg].t, d.edge_weight[e]);
-    } else if (G.mech == MIND_MECH_EXP2SYN) {
-      phase_receive(MIND_MECH_EXP2SYN, cols(G), row, d.win[seg].t, d.edge_weight[e]);
-    } else if (G.mech == MIND_MECH_EXPSYN) {
-      phase_receive(MIND_MECH_EXPSYN, cols(G), row, d.win[seg].t, d.edge_weight[e]);
-    }
+    receive_event(d, d.g[d.point_group[target]], d.point_row[target], d.win[seg].t, d.edge_weight[e]);
     ++seg;
   }
   return seg;
+}
+
+// Delivery of the window-list events due at delivery call `call` by the whole
+// workgroup, for the core receivers (the builtin synapses without ion exchanges,
+// ExpSyn and Exp2Syn; IClamp has no NET_RECEIVE), whose NET_RECEIVE touches its
+// own row only, so only their phases are compiled in. Every thread finds the call's end by binary search (the list
+// is sorted by call, then target), then each thread delivers whole target runs,
+// a run's events in list order, so every row sees the order of the serial
+// delivery. Returns the call's end, the same in every thread.
+__device__ inline int64_t deliver_core_runs(const Dev& d, int64_t seg, int64_t seg_end, int64_t call, int tid,
+                                            int T) {
+  int64_t lo = seg, hi = seg_end;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (d.win[mid].call <= call) lo = mid + 1;
+    else hi = mid;
+  }
+  for (int64_t i = seg + tid; i < lo; i += T) {
+    const int32_t target = d.edge_target[d.win[i].edge];
+    if (i > seg && d.edge_target[d.win[i - 1].edge] == target) continue;  // inside a run
+    const GroupDev& G = d.g[d.point_group[target]];
+    const int64_t row = d.point_row[target];
+    for (int64_t j = i; j < lo; ++j) {
+      const int32_t e = d.win[j].edge;
+      if (d.edge_target[e] != target) break;
+      if (G.mech == MIND_MECH_EXP2SYN) phase_receive(MIND_MECH_EXP2SYN, cols(G), row, d.win[j].t, d.edge_weight[e]);
+      else if (G.mech == MIND_MECH_EXPSYN) phase_receive(MIND_MECH_EXPSYN, cols(G), row, d.win[j].t, d.edge_weight[e]);
+    }
+  }
+  return lo;
 }
 
 // M says where a workgroup's node state lives: 0 all in LDS (v, rhs, diag, axial
@@ -803,7 +823,9 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
         d.det_above[kk] = cur;
       }
     }
-    if (tid == 0) seg = deliver_group<CORE>(d, seg, seg_end, 2 * k);  // midpoint delivery
+    // midpoint delivery (nrn_cur follows a barrier)
+    if (CORE) seg = deliver_core_runs(d, seg, seg_end, 2 * k, tid, T);
+    else if (tid == 0) seg = deliver_group(d, seg, seg_end, 2 * k);
     for (int i = tid; i < nn; i += T) {
       rhs[i] = 0.0;
       diag[i] = 0.0;
@@ -954,7 +976,15 @@ __device__ __forceinline__ void window_body(const Dev& d, const BatchDev& b, con
       }
     }
     __syncthreads();
-    if (tid == 0) seg = deliver_group<CORE>(d, seg, seg_end, 2 * k + 1);  // end-of-step delivery
+    // end-of-step delivery; with events, a barrier before the next step's midpoint
+    // delivery, which may deliver to the same rows
+    if (CORE) {
+      const int64_t next = deliver_core_runs(d, seg, seg_end, 2 * k + 1, tid, T);
+      if (next != seg) __syncthreads();
+      seg = next;
+    } else if (tid == 0) {
+      seg = deliver_group(d, seg, seg_end, 2 * k + 1);
+    }
   }
   if (M == 0 || M == 5) {
     __syncthreads();
