@@ -239,39 +239,43 @@ __global__ void __launch_bounds__(kBlock) k_exchange(Dev d, int32_t n) {
       for (; sampled < k0 + n; ++sampled) record_artificial(d, sampled);
   }
   __syncthreads();
-  // B. Detector spikes of the window in (step, detector) order, compacted by
-  //    wavefront ballot and prefix sum in one pass over the flags of all steps
-  //    (row-major [step][detector]): each wave takes one contiguous slice (a
-  //    multiple of 64 flags) and counts it 64 flags per ballot; a block prefix
-  //    sum over the waves' counts gives each slice its first position; a second
-  //    walk writes every set flag at its position plus its rank in the ballot.
+  // B. Detector spikes of the window in (step, detector) order. The flags of all
+  //    steps (row-major [step][detector], one byte each, 0 or 1) are cut into one
+  //    contiguous chunk per thread, in thread order, a multiple of 8 bytes long:
+  //    each thread counts its set flags eight at a time, a block prefix sum gives
+  //    each chunk its first position, and a second walk writes every set flag
+  //    at its position.
   int64_t ndet_total = 0;
   {
-    const int lane = tid & 63, wave = tid >> 6;
     const int64_t nf = (int64_t)n * d.ndet;
-    const int64_t slice = ((nf + kBlock / 64 - 1) / (kBlock / 64) + 63) / 64 * 64;
-    const int64_t f0 = std::min<int64_t>((int64_t)wave * slice, nf), f1 = std::min<int64_t>(f0 + slice, nf);
+    const int64_t chunk = ((nf + kBlock - 1) / kBlock + 7) / 8 * 8;
+    const int64_t f0 = std::min<int64_t>((int64_t)tid * chunk, nf), f1 = std::min<int64_t>(f0 + chunk, nf);
     int64_t count = 0;
-    for (int64_t base = f0; base < f1; base += 64) {
-      const int64_t f = base + lane;
-      count += __popcll(__ballot(f < f1 && d.det_flags[f] != 0));
+    for (int64_t f = f0; f < f1; f += 8) {
+      if (f + 8 <= f1) {
+        count += __popcll(*reinterpret_cast<const uint64_t*>(d.det_flags + f));
+      } else {
+        for (int64_t g = f; g < f1; ++g) count += d.det_flags[g] != 0;
+      }
     }
     int64_t total;
-    const int64_t first = block_scan(lane == 0 ? count : 0, scan, &total);
-    int64_t pos = __shfl(first, 0, 64);
-    for (int64_t base = f0; base < f1; base += 64) {
-      const int64_t f = base + lane;
-      const bool flag = f < f1 && d.det_flags[f] != 0;
-      const unsigned long long ballot = __ballot(flag);
-      if (flag) {
-        const int64_t at = pos + __popcll(ballot & ((1ull << lane) - 1ull));
-        const int64_t st = f / d.ndet, q = f - st * d.ndet;
-        if (at < d.emit_cap)
-          d.emit_det[at] = Emit{3 * (k0 + st), (double)(k0 + st) * dt + 1.0e-10, d.det_source[q], 0};
+    int64_t pos = block_scan(count, scan, &total);
+    for (int64_t f = f0; f < f1 && count > 0; f += 8) {
+      uint64_t w = 0;
+      if (f + 8 <= f1) {
+        w = *reinterpret_cast<const uint64_t*>(d.det_flags + f);
+      } else {
+        for (int64_t g = f; g < f1; ++g) w |= (uint64_t)(d.det_flags[g] != 0) << (8 * (g - f));
+      }
+      for (; w != 0; w &= w - 1) {
+        const int64_t fi = f + (__builtin_ctzll(w) >> 3);  // flag bytes are 0 or 1
+        const int64_t st = fi / d.ndet, q = fi - st * d.ndet;
+        if (pos < d.emit_cap)
+          d.emit_det[pos] = Emit{3 * (k0 + st), (double)(k0 + st) * dt + 1.0e-10, d.det_source[q], 0};
         else
           raise_error(d.error, MIND_EQUEUE);
+        ++pos;
       }
-      pos += __popcll(ballot);
     }
     ndet_total = total;
   }
